@@ -1,0 +1,254 @@
+/*
+ * svx.h — C ABI of the MI355X (gfx950) linear-static hot path of Solvix:
+ * global stiffness assembly (CPS3 / CPS4 / C3D8 into CSR), homogeneous
+ * Dirichlet constraints, Jacobi-preconditioned conjugate gradient, stress
+ * recovery and reactions.
+ *
+ * Every entry point is plain C: opaque handles, caller-owned host buffers,
+ * library-owned device buffers behind the handles, integer status codes.
+ * No C++ exception crosses this boundary. One svx_ctx per host thread.
+ *
+ * Reference seam (typezo/Solvix, crates/solvix-core/src):
+ *   lib.rs:586        solve_with_method(&Model, Method) -> Solution
+ *   lib.rs:593-716    solve_with_thermal (the pipeline these calls replace)
+ *   assembly.rs:281   assemble(model, ndof, loads) -> (Csr, f, Vec<ElemInfo>)
+ *   assembly.rs:38-72 Csr::mat_vec / mat_vec_with
+ *   solver.rs:15      constrained_dofs
+ *   solver.rs:45      apply_dirichlet
+ *   solver.rs:66      pcg(k, f, tol, max_iter) -> (u, iterations, residual)
+ *   stress.rs:22,53,103  von_mises_at_centroids / nodal_von_mises / reactions
+ */
+#ifndef SVX_H
+#define SVX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Status codes. SVX_E_EMPTY_MESH .. SVX_E_NOT_CONVERGED mirror the variants of
+ * SolveError (lib.rs:515-521); the rest are boundary/runtime failures.
+ * ------------------------------------------------------------------------- */
+enum {
+    SVX_OK = 0,
+    SVX_E_EMPTY_MESH = 1,
+    SVX_E_NO_CONSTRAINTS = 2,
+    SVX_E_INVALID_MODEL = 3,
+    SVX_E_NOT_CONVERGED = 4,
+    SVX_E_DEVICE = 5,   /* HIP runtime failure; message holds the HIP string */
+    SVX_E_ARGUMENT = 6, /* bad handle, size, option or index */
+    SVX_E_CAPACITY = 7  /* batch / server budget or slot limit exceeded */
+};
+
+/* Element kinds on the path (lib.rs:53-73, subset CPS3 / CPS4 / C3D8). */
+enum { SVX_CPS3 = 0, SVX_CPS4 = 1, SVX_C3D8 = 2 };
+
+/* Element-level reasons for SVX_E_INVALID_MODEL, one per message of
+ * assembly.rs:145-175,194-196,259-261. The lowest failing element index wins
+ * (assembly.rs:354-362). */
+enum {
+    SVX_ELEM_OK = 0,
+    SVX_ELEM_NO_MATERIAL = 1,           /* "no material assigned" */
+    SVX_ELEM_BAD_MATERIAL = 2,          /* E <= 0 or |nu| >= 1 */
+    SVX_ELEM_BAD_POISSON_SOLID = 3,     /* nu >= 0.5 on a solid */
+    SVX_ELEM_DEGENERATE_TRIANGLE = 4,   /* CPS3 with zero signed area */
+    SVX_ELEM_NONPOSITIVE_JACOBIAN = 5   /* det J <= 0 at a Gauss point */
+};
+
+/* Linear solver selection (lib.rs:523-533). LDLT is not on the device path:
+ * SVX_METHOD_LDLT returns SVX_E_ARGUMENT and SVX_METHOD_AUTO reports
+ * SVX_E_NOT_CONVERGED where the reference would fall back to LDLT. */
+enum { SVX_METHOD_AUTO = 0, SVX_METHOD_PCG = 1, SVX_METHOD_LDLT = 2 };
+
+/* Arithmetic of the Krylov iterations.
+ * SVX_FP64:    plain FP64 Jacobi-PCG, the recurrence of solver.rs:82-149.
+ * SVX_FP32_IR: FP32 matrix/vectors inside the Krylov loop (dot products
+ *              accumulated in FP64) with FP64 iterative refinement: the true
+ *              residual f - K u, the update of u and the convergence test run
+ *              in FP64 on the FP64 matrix. */
+enum { SVX_FP64 = 0, SVX_FP32_IR = 1 };
+
+typedef struct svx_error {
+    int32_t code;       /* SVX_* status */
+    int32_t reason;     /* SVX_ELEM_* for element errors, else 0 */
+    int64_t index;      /* element index (position) for element errors, else -1 */
+    int64_t iterations; /* SVX_E_NOT_CONVERGED: iterations done */
+    double residual;    /* SVX_E_NOT_CONVERGED: relative residual reached */
+    char message[256];  /* human-readable text in the reference's wording */
+} svx_error;
+
+/* ---------------------------------------------------------------------------
+ * Flat model descriptor: the fields of Model (lib.rs:39-330) that the linear
+ * path reads, as parallel arrays. Ids are the deck's ids; positions are
+ * 0-based indices into the node / element arrays.
+ * ------------------------------------------------------------------------- */
+typedef struct svx_model_desc {
+    int64_t n_nodes;
+    const uint32_t* node_ids; /* [n_nodes] */
+    const double* coords;     /* [n_nodes * 3], x y z per node */
+
+    int64_t n_elem;
+    const uint32_t* elem_ids; /* [n_elem] */
+    const uint8_t* kinds;     /* [n_elem] SVX_CPS3 / SVX_CPS4 / SVX_C3D8 */
+    const int64_t* conn_ptr;  /* [n_elem + 1] */
+    const uint32_t* conn_ids; /* [conn_ptr[n_elem]] node ids */
+
+    int32_t n_materials;
+    const double* youngs;  /* [n_materials] */
+    const double* poisson; /* [n_materials] */
+
+    int32_t n_elsets;
+    const int64_t* elset_ptr;      /* [n_elsets + 1] */
+    const uint32_t* elset_members; /* element ids, duplicates allowed */
+
+    /* (elset, material) assignments in deck order (Model::mat_assignments);
+     * an index of -1 stands for a name that resolves to nothing. */
+    int32_t n_assign;
+    const int32_t* assign_elset;
+    const int32_t* assign_material;
+
+    double thickness; /* model-global plane thickness */
+
+    /* Effective boundary conditions (LoadState::bcs): each one constrains
+     * every listed dof (1-based) of every listed node. */
+    int32_t n_bcs;
+    const int64_t* bc_node_ptr; /* [n_bcs + 1] */
+    const uint32_t* bc_nodes;
+    const int64_t* bc_dof_ptr; /* [n_bcs + 1] */
+    const int32_t* bc_dofs;
+
+    /* Effective concentrated loads (LoadState::cloads). */
+    int64_t n_cloads;
+    const uint32_t* cload_node;
+    const int32_t* cload_dof; /* 1-based */
+    const double* cload_value;
+} svx_model_desc;
+
+/* Caller-owned output buffers for svx_solve_model (Solution, lib.rs:486-513).
+ * Array lengths: displacements n_nodes*3, von_mises n_elem, nodal_von_mises
+ * n_nodes, reactions svx_model_reaction_count(). A NULL array is skipped. */
+typedef struct svx_solution {
+    double* displacements;
+    double* von_mises;
+    double* nodal_von_mises;
+    uint32_t* reaction_node;
+    int32_t* reaction_dof;
+    double* reaction_value;
+    int64_t iterations;
+    double final_residual;
+    double cond_estimate;
+    int32_t solver_method; /* 0 = "pcg" */
+    int32_t ndof;          /* 2 for plane models, 3 for solid models */
+    double assemble_ms;    /* device time of assembly (pattern + values + f) */
+    double solve_ms;       /* Dirichlet + Krylov solve */
+    double recover_ms;     /* stress recovery + reactions */
+} svx_solution;
+
+typedef struct svx_ctx svx_ctx;
+typedef struct svx_mesh svx_mesh;
+typedef struct svx_system svx_system;
+typedef struct svx_csr svx_csr;
+typedef struct svx_batch svx_batch;
+typedef struct svx_server svx_server;
+
+/* ---------------------------------------------------------------------------
+ * Context
+ * ------------------------------------------------------------------------- */
+int svx_ctx_create(int device, svx_ctx** out);
+void svx_ctx_destroy(svx_ctx* ctx);
+/* Library version string and the device's name; never NULL. */
+const char* svx_version(void);
+int svx_device_name(svx_ctx* ctx, char* buf, size_t len);
+/* Message of the last failure on this host thread (also for calls that
+ * take no svx_error). No reference counterpart. */
+const char* svx_last_error(void);
+/* Launch-shape and refinement knobs of the Krylov engine for this context
+ * (0 / <= 0 keeps the automatic choice): CSR lanes per row, BSR lanes per
+ * block row, rows per workgroup tile, FP32-IR refresh period, iterations per
+ * captured graph, graph capture on/off, and the FP32-IR refresh threshold
+ * (replace the residual when ||r|| < ru_delta * ||r at last refresh||).
+ * No reference counterpart: the autotuner drives it. */
+int svx_set_tuning(svx_ctx* ctx, int lanes, int blanes, int rows_per_tile,
+                   int period, int chunk, int use_graph, double ru_delta);
+
+/* ---------------------------------------------------------------------------
+ * Whole pipeline — replaces solve_with_method(&Model, Method) (lib.rs:586),
+ * i.e. solve_with_thermal without thermal loads (lib.rs:593-716).
+ * ------------------------------------------------------------------------- */
+/* Number of Reaction entries the model produces (one per constrained
+ * node/dof pair, duplicates kept; solver.rs:15-41). */
+int64_t svx_model_reaction_count(const svx_model_desc* model);
+int svx_solve_model(svx_ctx* ctx, const svx_model_desc* model, int method,
+                    int precision, svx_solution* out, svx_error* err);
+
+/* ---------------------------------------------------------------------------
+ * Mesh: node coordinates and connectivity resident on the device, with the
+ * CSR pattern and DOF numbering (row = node_position * ndof + dof - 1,
+ * assembly.rs:2-3) built once per mesh.
+ * ------------------------------------------------------------------------- */
+int svx_mesh_create(svx_ctx* ctx, int ndof, int64_t n_nodes,
+                    const double* coords /* [n_nodes*3] */, int64_t n_elem,
+                    const uint8_t* kinds, const int64_t* conn_ptr,
+                    const int32_t* conn_idx /* node positions */,
+                    svx_mesh** out, svx_error* err);
+void svx_mesh_destroy(svx_mesh* mesh);
+int64_t svx_mesh_rows(const svx_mesh* mesh); /* n = n_nodes * ndof */
+int64_t svx_mesh_nnz(const svx_mesh* mesh);
+/* The pattern of the Csr built by csr_from_triplets (assembly.rs:87-116). */
+int svx_mesh_pattern(const svx_mesh* mesh, int32_t* row_ptr /* [n+1] */,
+                     int32_t* col_idx /* [nnz] */);
+
+/* ---------------------------------------------------------------------------
+ * Assembled system on a mesh — replaces assembly::assemble (assembly.rs:281).
+ * E / nu per element (NaN E = "no material assigned"); loads are (row, value)
+ * pairs accumulated into f in the given order (assembly.rs:365-380).
+ * ------------------------------------------------------------------------- */
+int svx_assemble(svx_mesh* mesh, const double* youngs, const double* poisson,
+                 double thickness, int64_t n_loads, const int64_t* load_rows,
+                 const double* load_values, svx_system** out, svx_error* err);
+void svx_system_destroy(svx_system* sys);
+/* Copy the current CSR values and load vector to the host (before
+ * svx_apply_dirichlet: the assembled K and f; after: the constrained ones). */
+int svx_system_values(const svx_system* sys, double* vals /* [nnz] */,
+                      double* f /* [n] or NULL */);
+/* Replace f (the assembled vector is kept for reactions only if this is
+ * called before svx_apply_dirichlet). */
+int svx_system_set_rhs(svx_system* sys, const double* f /* [n] */);
+/* solver::apply_dirichlet (solver.rs:45-62): rows may repeat. */
+int svx_apply_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows);
+/* solver::pcg (solver.rs:66) on the system's (constrained) K and f. */
+int svx_solve(svx_system* sys, double tol, int64_t max_iter, int precision,
+              double* u /* [n] or NULL */, int64_t* iterations,
+              double* residual, svx_error* err);
+/* Recovery from the last solution: centroid von Mises (stress.rs:22-42),
+ * nodal von Mises (stress.rs:53-98), reactions K_orig u - f_orig at the
+ * given rows (stress.rs:103-118) and the Jacobi conditioning indicator
+ * max|K_ii| / min|K_ii| of the constrained K (lib.rs:634-644). */
+int svx_recover(svx_system* sys, double* vm_elem /* [n_elem] */,
+                double* vm_node /* [n_nodes] */, int64_t n_rows,
+                const int64_t* rows, double* reactions /* [n_rows] */,
+                double* cond_estimate);
+
+/* ---------------------------------------------------------------------------
+ * Raw CSR operator — Csr::mat_vec (assembly.rs:38-72) and solver::pcg
+ * (solver.rs:66) on caller-provided matrices.
+ * ------------------------------------------------------------------------- */
+int svx_csr_create(svx_ctx* ctx, int64_t n, const int32_t* row_ptr,
+                   const int32_t* col_idx, const double* vals, svx_csr** out);
+void svx_csr_destroy(svx_csr* csr);
+/* y = K x. ordered != 0: one lane per row, serial column order, no fused
+ * multiply-add (bit-identical to the reference); ordered == 0: the tuned
+ * wavefront kernel. */
+int svx_csr_spmv(svx_csr* csr, const double* x, double* y, int ordered);
+int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter,
+                int precision, double* u, int64_t* iterations,
+                double* residual, svx_error* err);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVX_H */

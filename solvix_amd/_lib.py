@@ -1,0 +1,74 @@
+"""ctypes loader for libsolvix_hip.so (the HIP path). There is no fallback:
+if the library is missing or fails to load, every entry point raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsolvix_hip.so")
+_lib = None
+
+# (name, restype, argtypes) for every entry point of include/svx.h
+_vp = C.c_void_p
+_i64 = C.c_int64
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_errp = C.POINTER(_abi.SvxError)
+
+SIGNATURES = [
+    ("svx_ctx_create", C.c_int, [C.c_int, C.POINTER(_vp)]),
+    ("svx_ctx_destroy", None, [_vp]),
+    ("svx_version", C.c_char_p, []),
+    ("svx_device_name", C.c_int, [_vp, C.c_char_p, C.c_size_t]),
+    ("svx_last_error", C.c_char_p, []),
+    ("svx_set_tuning", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
+    ("svx_model_reaction_count", _i64, [C.POINTER(_abi.SvxModelDesc)]),
+    ("svx_solve_model", C.c_int, [_vp, C.POINTER(_abi.SvxModelDesc), C.c_int, C.c_int,
+                                  C.POINTER(_abi.SvxSolution), _errp]),
+    ("svx_mesh_create", C.c_int, [_vp, C.c_int, _i64, _dp, _i64, C.POINTER(C.c_uint8), _i64p, _i32p,
+                                  C.POINTER(_vp), _errp]),
+    ("svx_mesh_destroy", None, [_vp]),
+    ("svx_mesh_rows", _i64, [_vp]),
+    ("svx_mesh_nnz", _i64, [_vp]),
+    ("svx_mesh_pattern", C.c_int, [_vp, _i32p, _i32p]),
+    ("svx_assemble", C.c_int, [_vp, _dp, _dp, C.c_double, _i64, _i64p, _dp, C.POINTER(_vp), _errp]),
+    ("svx_system_destroy", None, [_vp]),
+    ("svx_system_values", C.c_int, [_vp, _dp, _dp]),
+    ("svx_system_set_rhs", C.c_int, [_vp, _dp]),
+    ("svx_apply_dirichlet", C.c_int, [_vp, _i64, _i64p]),
+    ("svx_solve", C.c_int, [_vp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _errp]),
+    ("svx_recover", C.c_int, [_vp, _dp, _dp, _i64, _i64p, _dp, _dp]),
+    ("svx_csr_create", C.c_int, [_vp, _i64, _i32p, _i32p, _dp, C.POINTER(_vp)]),
+    ("svx_csr_destroy", None, [_vp]),
+    ("svx_csr_spmv", C.c_int, [_vp, _dp, _dp, C.c_int]),
+    ("svx_csr_pcg", C.c_int, [_vp, _dp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _errp]),
+]
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with python -m solvix_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            if not hasattr(L, name):
+                continue
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Names of every function declared in include/svx.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "svx.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"^[A-Za-z_][\w\s\*]*?\b(svx_\w+)\s*\(", text, flags=re.M)))

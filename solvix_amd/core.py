@@ -1,0 +1,344 @@
+"""Host API of the device path, mirroring solvix-core's linear-static seam.
+
+    solve(model)                       lib.rs:556   (linear models)
+    solve_with_method(model, method)   lib.rs:586
+    Mesh.assemble(...)                 assembly.rs:281  assemble
+    System.apply_dirichlet(rows)       solver.rs:45     apply_dirichlet
+    System.solve / pcg(...)            solver.rs:66     pcg
+    System.recover(rows)               stress.rs:22,53,103
+    CsrMatrix.mat_vec(x)               assembly.rs:38   Csr::mat_vec
+    constrained_dofs(model)            solver.rs:15
+
+Everything runs through libsolvix_hip.so (include/svx.h); there is no CPU
+fallback. Arguments and errors keep the reference's meaning: errors raise
+SolveError with the reference's kinds and messages.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _abi
+from ._lib import lib
+from .model import Method, Model, Precision, Reaction, Solution, SolveError, error_from_code
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _raise(rc: int, err: Optional[_abi.SvxError] = None):
+    if rc == _abi.SVX_OK:
+        return
+    if err is not None and err.code:
+        raise error_from_code(err.code, err.message.decode(errors="replace"), err.iterations, err.residual,
+                              err.index, err.reason)
+    raise error_from_code(rc, lib().svx_last_error().decode(errors="replace"))
+
+
+class Context:
+    """One device context (svx_ctx): a device, a HIP stream, tuning knobs."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _raise(lib().svx_ctx_create(device, C.byref(self._h)))
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        lib().svx_device_name(self._h, buf, 256)
+        return buf.value.decode()
+
+    def set_tuning(self, lanes=0, blanes=0, rows_per_tile=0, period=0, chunk=0, use_graph=True, ru_delta=0.0):
+        _raise(lib().svx_set_tuning(self._h, lanes, blanes, rows_per_tile, period, chunk, int(use_graph), ru_delta))
+
+    def close(self):
+        if self._h:
+            lib().svx_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+# ------------------------------------------------------------ whole model
+def solve_with_method(model: Model, method: int = Method.AUTO, precision: int = Precision.FP64,
+                      ctx: Optional[Context] = None) -> Solution:
+    """solve_with_method (lib.rs:586) on the device."""
+    ctx = ctx or default_context()
+    desc, keep = model.to_desc()
+    nn, ne = model.n_nodes, model.n_elem
+    nr = max(0, int(lib().svx_model_reaction_count(C.byref(desc))))
+    disp = np.zeros((nn, 3))
+    vm = np.zeros(ne)
+    nvm = np.zeros(nn)
+    rn = np.zeros(nr, np.uint32)
+    rd = np.zeros(nr, np.int32)
+    rv = np.zeros(nr)
+    sol = _abi.SvxSolution(_d(disp), _d(vm), _d(nvm), rn.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           rd.ctypes.data_as(_i32p), _d(rv))
+    err = _abi.SvxError()
+    rc = lib().svx_solve_model(ctx.handle, C.byref(desc), method, precision, C.byref(sol), C.byref(err))
+    del keep
+    _raise(rc, err)
+    return Solution(
+        displacements=disp, von_mises=vm, nodal_von_mises=nvm,
+        reactions=[Reaction(int(a), int(b), float(c)) for a, b, c in zip(rn, rd, rv)],
+        iterations=int(sol.iterations), final_residual=float(sol.final_residual), solver_method="pcg",
+        cond_estimate=float(sol.cond_estimate),
+        timings_ms=dict(assemble=sol.assemble_ms, solve=sol.solve_ms, recover=sol.recover_ms))
+
+
+def solve(model: Model, precision: int = Precision.FP64, ctx: Optional[Context] = None) -> Solution:
+    """solve (lib.rs:556) for linear-static models."""
+    return solve_with_method(model, Method.AUTO, precision, ctx)
+
+
+def constrained_dofs(model: Model) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """constrained_dofs (solver.rs:15-41) -> (rows, node_ids, dofs), BC order."""
+    ndof = model.ndof()
+    pos = {}
+    for i, nid in enumerate(model.node_ids.tolist()):
+        pos.setdefault(nid, i)
+    rows, nodes, dofs = [], [], []
+    for nids, ds in model.bcs:
+        for nid in nids.tolist():
+            if nid not in pos:
+                raise SolveError("InvalidModel", f"boundary condition on unknown node {nid}")
+            for d in ds.tolist():
+                if d == 0 or d > ndof:
+                    raise SolveError("InvalidModel", f"boundary condition on node {nid} has unsupported dof {d}")
+                rows.append(pos[nid] * ndof + d - 1)
+                nodes.append(nid)
+                dofs.append(d)
+    return np.array(rows, np.int64), np.array(nodes, np.uint32), np.array(dofs, np.int32)
+
+
+def node_positions(model: Model) -> np.ndarray:
+    """Element connectivity as node positions (Mesh::node_index, lib.rs:106)."""
+    order = np.argsort(model.node_ids, kind="stable")
+    sorted_ids = model.node_ids[order]
+    idx = np.searchsorted(sorted_ids, model.conn_ids)
+    if np.any(idx >= sorted_ids.size) or np.any(sorted_ids[np.minimum(idx, sorted_ids.size - 1)] != model.conn_ids):
+        bad = model.conn_ids[(idx >= sorted_ids.size) | (sorted_ids[np.minimum(idx, sorted_ids.size - 1)] != model.conn_ids)][0]
+        raise SolveError("InvalidModel", f"UnknownNode({bad})")
+    return order[idx].astype(np.int32)
+
+
+def element_materials(model: Model) -> Tuple[np.ndarray, np.ndarray]:
+    """Per-element (E, nu) by material_for_element (lib.rs:357-367); NaN E
+    marks an element without material."""
+    E = np.full(model.n_elem, np.nan)
+    nu = np.zeros(model.n_elem)
+    done = np.zeros(model.n_elem, bool)
+    es = {name: ids for name, ids in model.elsets}
+    mats = {m[0]: (m[1], m[2]) for m in model.materials}
+    first_es = {}
+    for name, ids in model.elsets:
+        first_es.setdefault(name, ids)
+    for es_name, mat_name in model.assignments:
+        if es_name not in first_es:
+            continue
+        hit = np.isin(model.elem_ids, first_es[es_name]) & ~done
+        if mat_name in mats:
+            E[hit], nu[hit] = mats[mat_name]
+        done |= hit
+    return E, nu
+
+
+# --------------------------------------------------------------- the seam
+class Mesh:
+    """Device-resident mesh with its CSR pattern (built once per mesh)."""
+
+    def __init__(self, ndof: int, coords: np.ndarray, kinds: np.ndarray, conn_ptr: np.ndarray,
+                 conn_pos: np.ndarray, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.ndof = ndof
+        self._coords = np.ascontiguousarray(coords, np.float64).reshape(-1, 3)
+        kinds = np.ascontiguousarray(kinds, np.uint8)
+        conn_ptr = np.ascontiguousarray(conn_ptr, np.int64)
+        conn_pos = np.ascontiguousarray(conn_pos, np.int32)
+        self.n_nodes = self._coords.shape[0]
+        self.n_elem = kinds.shape[0]
+        self._h = C.c_void_p()
+        err = _abi.SvxError()
+        rc = lib().svx_mesh_create(self.ctx.handle, ndof, self.n_nodes, _d(self._coords), self.n_elem,
+                                   kinds.ctypes.data_as(C.POINTER(C.c_uint8)), conn_ptr.ctypes.data_as(_i64p),
+                                   conn_pos.ctypes.data_as(_i32p), C.byref(self._h), C.byref(err))
+        _raise(rc, err)
+        self.n = int(lib().svx_mesh_rows(self._h))
+        self.nnz = int(lib().svx_mesh_nnz(self._h))
+
+    @classmethod
+    def from_model(cls, model: Model, ctx: Optional[Context] = None) -> "Mesh":
+        return cls(model.ndof(), model.coords, model.kinds, model.conn_ptr, node_positions(model), ctx)
+
+    def pattern(self) -> Tuple[np.ndarray, np.ndarray]:
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(self.nnz, np.int32)
+        _raise(lib().svx_mesh_pattern(self._h, rp.ctypes.data_as(_i32p), ci.ctypes.data_as(_i32p)))
+        return rp, ci
+
+    def assemble(self, E, nu, thickness: float, load_rows=(), load_values=()) -> "System":
+        """assemble (assembly.rs:281): element pass + CSR values + load vector."""
+        E = np.ascontiguousarray(np.broadcast_to(np.asarray(E, np.float64), (self.n_elem,)))
+        nu = np.ascontiguousarray(np.broadcast_to(np.asarray(nu, np.float64), (self.n_elem,)))
+        lr = np.ascontiguousarray(load_rows, np.int64)
+        lv = np.ascontiguousarray(load_values, np.float64)
+        h = C.c_void_p()
+        err = _abi.SvxError()
+        rc = lib().svx_assemble(self._h, _d(E), _d(nu), float(thickness), lr.shape[0], lr.ctypes.data_as(_i64p),
+                                _d(lv), C.byref(h), C.byref(err))
+        _raise(rc, err)
+        return System(self, h)
+
+    def close(self):
+        if self._h:
+            lib().svx_mesh_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class System:
+    """An assembled K, f on a mesh (device resident)."""
+
+    def __init__(self, mesh: Mesh, handle):
+        self.mesh = mesh
+        self._h = handle
+        self.iterations = 0
+        self.residual = 0.0
+
+    @property
+    def handle(self):
+        return self._h
+
+    def values(self) -> Tuple[np.ndarray, np.ndarray]:
+        vals = np.zeros(self.mesh.nnz)
+        f = np.zeros(self.mesh.n)
+        _raise(lib().svx_system_values(self._h, _d(vals), _d(f)))
+        return vals, f
+
+    def set_rhs(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        _raise(lib().svx_system_set_rhs(self._h, _d(f)))
+
+    def apply_dirichlet(self, rows) -> None:
+        rows = np.ascontiguousarray(rows, np.int64)
+        _raise(lib().svx_apply_dirichlet(self._h, rows.shape[0], rows.ctypes.data_as(_i64p)))
+
+    def solve(self, tol: float = 1e-10, max_iter: int = 0, precision: int = Precision.FP64,
+              want_u: bool = True) -> Tuple[Optional[np.ndarray], int, float]:
+        """pcg (solver.rs:66) -> (u, iterations, residual); raises NotConverged."""
+        u = np.zeros(self.mesh.n) if want_u else None
+        its = C.c_int64()
+        res = C.c_double()
+        err = _abi.SvxError()
+        rc = lib().svx_solve(self._h, tol, max_iter, precision, _d(u) if want_u else None, C.byref(its),
+                             C.byref(res), C.byref(err))
+        self.iterations, self.residual = its.value, res.value
+        _raise(rc, err)
+        return u, its.value, res.value
+
+    def recover(self, rows=()) -> Tuple[np.ndarray, np.ndarray, np.ndarray, float]:
+        """(centroid von Mises, nodal von Mises, reactions at rows, cond_estimate)."""
+        rows = np.ascontiguousarray(rows, np.int64)
+        vm = np.zeros(self.mesh.n_elem)
+        vn = np.zeros(self.mesh.n_nodes)
+        re = np.zeros(rows.shape[0])
+        cond = C.c_double()
+        _raise(lib().svx_recover(self._h, _d(vm), _d(vn), rows.shape[0], rows.ctypes.data_as(_i64p), _d(re),
+                                 C.byref(cond)))
+        return vm, vn, re, cond.value
+
+    def close(self):
+        if self._h:
+            lib().svx_system_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def assemble(model: Model, ctx: Optional[Context] = None) -> Tuple[Mesh, System]:
+    """assemble (assembly.rs:281) for a whole model: (mesh, system)."""
+    mesh = Mesh.from_model(model, ctx)
+    E, nu = element_materials(model)
+    ndof = model.ndof()
+    pos = {}
+    for i, nid in enumerate(model.node_ids.tolist()):
+        pos.setdefault(nid, i)
+    rows = np.array([pos[int(n)] * ndof + int(d) - 1 for n, d in zip(model.cload_node, model.cload_dof)], np.int64)
+    return mesh, mesh.assemble(E, nu, model.thickness, rows, model.cload_value)
+
+
+class CsrMatrix:
+    """A caller-provided CSR matrix on the device (Csr, assembly.rs:29-35)."""
+
+    def __init__(self, row_ptr, col_idx, vals, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self._rp = np.ascontiguousarray(row_ptr, np.int32)
+        self._ci = np.ascontiguousarray(col_idx, np.int32)
+        self._v = np.ascontiguousarray(vals, np.float64)
+        self.n = self._rp.shape[0] - 1
+        self._h = C.c_void_p()
+        _raise(lib().svx_csr_create(self.ctx.handle, self.n, self._rp.ctypes.data_as(_i32p),
+                                    self._ci.ctypes.data_as(_i32p), _d(self._v), C.byref(self._h)))
+
+    def mat_vec(self, x, ordered: bool = False) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.n)
+        _raise(lib().svx_csr_spmv(self._h, _d(x), _d(y), int(ordered)))
+        return y
+
+    def pcg(self, f, tol: float = 1e-10, max_iter: int = 0, precision: int = Precision.FP64):
+        f = np.ascontiguousarray(f, np.float64)
+        u = np.zeros(self.n)
+        its = C.c_int64()
+        res = C.c_double()
+        err = _abi.SvxError()
+        rc = lib().svx_csr_pcg(self._h, _d(f), tol, max_iter, precision, _d(u), C.byref(its), C.byref(res),
+                               C.byref(err))
+        _raise(rc, err)
+        return u, its.value, res.value
+
+    def close(self):
+        if self._h:
+            lib().svx_csr_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

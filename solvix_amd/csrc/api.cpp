@@ -1,0 +1,777 @@
+// C ABI of the device path (include/svx.h): handles, the assemble / Dirichlet
+// / solve / recover seam, the raw CSR operator, and the whole-model pipeline
+// that mirrors solve_with_thermal (crates/solvix-core/src/lib.rs:593-716).
+// No C++ exception crosses this boundary.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <memory>
+#include <unordered_map>
+#include <unordered_set>
+
+#include "api_internal.hpp"
+
+using namespace svx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_err(svx_error* err, int code, const std::string& msg, int reason = 0, int64_t index = -1,
+             int64_t iterations = 0, double residual = 0.0) {
+    g_last_error = msg;
+    if (!err) return;
+    err->code = code;
+    err->reason = reason;
+    err->index = index;
+    err->iterations = iterations;
+    err->residual = residual;
+    snprintf(err->message, sizeof err->message, "%s", msg.c_str());
+}
+
+void clear_err(svx_error* err) {
+    if (!err) return;
+    std::memset(err, 0, sizeof *err);
+    err->index = -1;
+}
+
+template <typename F>
+int guarded(svx_error* err, F&& fn) {
+    try {
+        return fn();
+    } catch (const DeviceError& e) {
+        set_err(err, SVX_E_DEVICE, e.what());
+        return SVX_E_DEVICE;
+    } catch (const std::bad_alloc&) {
+        set_err(err, SVX_E_DEVICE, "host allocation failed");
+        return SVX_E_DEVICE;
+    } catch (const std::exception& e) {
+        set_err(err, SVX_E_ARGUMENT, e.what());
+        return SVX_E_ARGUMENT;
+    }
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+const char* reason_text(int reason) {
+    switch (reason) {
+        case SVX_ELEM_NO_MATERIAL: return "no material assigned";
+        case SVX_ELEM_BAD_MATERIAL: return "invalid material constants";
+        case SVX_ELEM_BAD_POISSON_SOLID: return "poisson ratio must be < 0.5 for solids and plane strain";
+        case SVX_ELEM_DEGENERATE_TRIANGLE: return "degenerate (zero-area) triangle";
+        case SVX_ELEM_NONPOSITIVE_JACOBIAN: return "non-positive Jacobian determinant at a Gauss point";
+        default: return "invalid element";
+    }
+}
+
+}  // namespace
+
+namespace svx {
+
+int nodes_of(uint8_t kind) { return kind == SVX_CPS3 ? 3 : kind == SVX_CPS4 ? 4 : 8; }
+
+// ------------------------------------------------------------------ mesh
+void mesh_build(svx_mesh* m, int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const uint8_t* kinds,
+                const int64_t* conn_ptr, const int32_t* conn_idx) {
+    hipStream_t s = m->ctx->stream;
+    m->ndof = ndof;
+    m->n_nodes = n_nodes;
+    m->n_elem = n_elem;
+    m->h_conn_ptr.assign(conn_ptr, conn_ptr + n_elem + 1);
+    m->h_conn.assign(conn_idx, conn_idx + conn_ptr[n_elem]);
+    m->h_kinds.assign(kinds, kinds + n_elem);
+    build_pattern(ndof, n_nodes, n_elem, conn_ptr, conn_idx, m->hp);
+    m->n = n_nodes * ndof;
+    m->nnz = m->hp.row_ptr[m->n];
+    m->nblk = m->hp.brow_ptr[n_nodes];
+    m->coords.alloc(n_nodes * 3);
+    m->coords.upload(coords, n_nodes * 3, s);
+    m->conn_ptr.alloc(n_elem + 1);
+    m->conn_ptr.upload(conn_ptr, n_elem + 1, s);
+    m->conn.alloc(m->h_conn.size());
+    m->conn.upload(m->h_conn.data(), m->h_conn.size(), s);
+    m->kinds.alloc(n_elem);
+    m->kinds.upload(kinds, n_elem, s);
+    m->row_ptr.alloc(m->n + 1);
+    m->row_ptr.upload(m->hp.row_ptr.data(), m->n + 1, s);
+    m->col.alloc(m->nnz);
+    m->col.upload(m->hp.col.data(), m->nnz, s);
+    m->brow_ptr.alloc(n_nodes + 1);
+    m->brow_ptr.upload(m->hp.brow_ptr.data(), n_nodes + 1, s);
+    m->bcol.alloc(m->nblk);
+    m->bcol.upload(m->hp.bcol.data(), m->nblk, s);
+    m->inc_ptr.alloc(n_nodes + 1);
+    m->inc_ptr.upload(m->hp.inc_ptr.data(), n_nodes + 1, s);
+    m->inc_elem.alloc(m->hp.inc_elem.size());
+    m->inc_elem.upload(m->hp.inc_elem.data(), m->hp.inc_elem.size(), s);
+    m->inc_slot.alloc(m->hp.inc_slot.size());
+    m->inc_slot.upload(m->hp.inc_slot.data(), m->hp.inc_slot.size(), s);
+    std::vector<int32_t> sel[3];
+    std::vector<int64_t> ke_off(n_elem), ns_off(n_elem);
+    int64_t ke = 0, ns = 0;
+    for (int64_t e = 0; e < n_elem; ++e) {
+        int nen = nodes_of(kinds[e]);
+        sel[kinds[e]].push_back((int32_t)e);
+        ke_off[e] = ke;
+        ns_off[e] = ns;
+        ke += (int64_t)nen * ndof * nen * ndof;
+        ns += (int64_t)nen * 6;
+    }
+    m->ke_total = ke;
+    m->ns_total = ns;
+    for (int k = 0; k < 3; ++k) {
+        m->nsel[k] = (int64_t)sel[k].size();
+        m->sel[k].alloc(sel[k].size());
+        m->sel[k].upload(sel[k].data(), sel[k].size(), s);
+    }
+    m->ke_off.alloc(n_elem);
+    m->ke_off.upload(ke_off.data(), n_elem, s);
+    m->ns_off.alloc(n_elem);
+    m->ns_off.upload(ns_off.data(), n_elem, s);
+    SVX_HIP(hipStreamSynchronize(s));
+}
+
+// Element pass + gather. Returns SVX_OK or the first bad element (index,
+// reason) through *bad_index / *bad_reason.
+void system_assemble(svx_system* sys, const double* E, const double* nu, double thickness, int64_t* bad_index,
+                     int* bad_reason) {
+    svx_mesh* m = sys->mesh;
+    hipStream_t s = m->ctx->stream;
+    sys->E.alloc(m->n_elem);
+    sys->E.upload(E, m->n_elem, s);
+    sys->nu.alloc(m->n_elem);
+    sys->nu.upload(nu, m->n_elem, s);
+    sys->thickness = thickness;
+    DevBuf<double> ke(std::max<int64_t>(1, m->ke_total));
+    DevBuf<uint8_t> reason(std::max<int64_t>(1, m->n_elem));
+    DevBuf<unsigned long long> first_bad(1);
+    unsigned long long init = ~0ull;
+    first_bad.upload(&init, 1, s);
+    for (int k = 0; k < 3; ++k)
+        launch_element_ke(k, m->nsel[k], m->sel[k].p, m->conn_ptr.p, m->conn.p, m->coords.p, sys->E.p, sys->nu.p,
+                          thickness, m->ndof, ke.p, m->ke_off.p, reason.p, first_bad.p, s);
+    unsigned long long fb = 0;
+    first_bad.download(&fb, 1, s);
+    SVX_HIP(hipStreamSynchronize(s));
+    *bad_index = -1;
+    *bad_reason = 0;
+    if (fb != ~0ull) {
+        uint8_t r = 0;
+        SVX_HIP(hipMemcpy(&r, reason.p + fb, 1, hipMemcpyDeviceToHost));
+        *bad_index = (int64_t)fb;
+        *bad_reason = r;
+        return;
+    }
+    sys->vals.alloc(m->nnz);
+    launch_gather_assembly(m->n, m->ndof, m->row_ptr.p, m->col.p, m->inc_ptr.p, m->inc_elem.p, m->inc_slot.p,
+                           m->conn_ptr.p, m->conn.p, m->kinds.p, ke.p, m->ke_off.p, sys->vals.p, s);
+    SVX_HIP(hipStreamSynchronize(s));
+}
+
+void system_set_f(svx_system* sys, const double* f) {
+    svx_mesh* m = sys->mesh;
+    sys->h_f.assign(f, f + m->n);
+    sys->f.alloc(m->n);
+    sys->f.upload(f, m->n, m->ctx->stream);
+    SVX_HIP(hipStreamSynchronize(m->ctx->stream));
+}
+
+void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {
+    svx_mesh* m = sys->mesh;
+    hipStream_t s = m->ctx->stream;
+    std::vector<uint8_t> mask(m->n, 0);
+    for (int64_t i = 0; i < n_rows; ++i) {
+        if (rows[i] < 0 || rows[i] >= m->n) throw std::invalid_argument("constrained row out of range");
+        mask[rows[i]] = 1;
+    }
+    sys->cons_rows.clear();
+    for (int64_t r = 0; r < m->n; ++r)
+        if (mask[r]) sys->cons_rows.push_back(r);
+    std::vector<int64_t> off(sys->cons_rows.size());
+    int64_t tot = 0;
+    sys->f_orig_cons.resize(sys->cons_rows.size());
+    for (size_t i = 0; i < sys->cons_rows.size(); ++i) {
+        int64_t r = sys->cons_rows[i];
+        off[i] = tot;
+        tot += m->hp.row_ptr[r + 1] - m->hp.row_ptr[r];
+        sys->f_orig_cons[i] = sys->h_f[r];
+    }
+    sys->d_cons_rows.alloc(sys->cons_rows.size());
+    sys->d_cons_rows.upload(sys->cons_rows.data(), sys->cons_rows.size(), s);
+    sys->d_cons_off.alloc(off.size());
+    sys->d_cons_off.upload(off.data(), off.size(), s);
+    sys->d_cons_vals.alloc(std::max<int64_t>(1, tot));
+    launch_gather_rows((int64_t)sys->cons_rows.size(), sys->d_cons_rows.p, m->row_ptr.p, sys->vals.p,
+                       sys->d_cons_off.p, sys->d_cons_vals.p, s);
+    DevBuf<uint8_t> dmask(m->n);
+    dmask.upload(mask.data(), m->n, s);
+    launch_dirichlet(m->n, m->row_ptr.p, m->col.p, dmask.p, sys->vals.p, sys->f.p, s);
+    for (int64_t r : sys->cons_rows) sys->h_f[r] = 0.0;
+    sys->dirichlet = true;
+    SVX_HIP(hipStreamSynchronize(s));
+}
+
+// Per-system preparation of the Krylov data for one precision.
+void system_prepare(svx_system* sys, int precision) {
+    svx_mesh* m = sys->mesh;
+    hipStream_t s = m->ctx->stream;
+    const int64_t n = m->n;
+    if (!sys->dinv.p) {
+        sys->dinv.alloc(n);
+        sys->dinv32.alloc(n);
+        DevBuf<unsigned long long> ext(2);
+        unsigned long long init[2] = {0ull, ~0ull};
+        ext.upload(init, 2, s);
+        launch_diag_inv(n, m->row_ptr.p, m->col.p, sys->vals.p, sys->dinv.p, sys->dinv32.p, ext.p, ext.p + 1, s);
+        unsigned long long h[2];
+        ext.download(h, 2, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        double hi, lo;
+        std::memcpy(&hi, &h[0], 8);
+        std::memcpy(&lo, &h[1], 8);
+        sys->cond = (h[1] == ~0ull) ? std::numeric_limits<double>::quiet_NaN() : hi / lo;
+    }
+    if (!sys->u.p) sys->u.alloc(n);
+    if (!sys->scal.p) sys->scal.alloc(1);
+    if (precision == SVX_FP64) {
+        if (!sys->r.p) {
+            sys->r.alloc(n), sys->p.alloc(n), sys->z.alloc(n), sys->q.alloc(n);
+        }
+    } else {
+        if (!sys->bval.p) {
+            sys->bval.alloc((size_t)m->nblk * m->ndof * m->ndof);
+            launch_fill_bsr(n, m->ndof, m->row_ptr.p, m->col.p, sys->vals.p, m->brow_ptr.p, sys->bval.p, s);
+        }
+        if (!sys->x32.p) {
+            sys->x32.alloc(n), sys->r32.alloc(n), sys->p32.alloc(n), sys->z32.alloc(n), sys->q32.alloc(n);
+        }
+    }
+    SVX_HIP(hipStreamSynchronize(s));
+}
+
+InstDesc system_desc(svx_system* sys) {
+    svx_mesh* m = sys->mesh;
+    InstDesc d{};
+    d.n = (int32_t)m->n;
+    d.bs = m->ndof;
+    d.row_ptr = m->row_ptr.p;
+    d.col = m->col.p;
+    d.val = sys->vals.p;
+    d.brow_ptr = m->brow_ptr.p;
+    d.bcol = m->bcol.p;
+    d.bval = sys->bval.p;
+    d.dinv = sys->dinv.p;
+    d.dinv32 = sys->dinv32.p;
+    d.f = sys->f.p;
+    d.u = sys->u.p;
+    d.r = sys->r.p;
+    d.p = sys->p.p;
+    d.z = sys->z.p;
+    d.q = sys->q.p;
+    d.x32 = sys->x32.p;
+    d.r32 = sys->r32.p;
+    d.p32 = sys->p32.p;
+    d.z32 = sys->z32.p;
+    d.q32 = sys->q32.p;
+    d.s = sys->scal.p;
+    return d;
+}
+
+InstScal initial_scal(double tol, int64_t max_iter, double ru_delta) {
+    InstScal sc{};
+    sc.tol = tol;
+    sc.max_iter = max_iter;
+    sc.status = ST_RUNNING;
+    sc.ru_delta = ru_delta;
+    return sc;
+}
+
+int status_to_code(const InstScal& sc) {
+    if (sc.status == ST_CONVERGED) return SVX_OK;
+    return SVX_E_NOT_CONVERGED;
+}
+
+void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,
+                  const TuneParams& tp) {
+    svx_mesh* m = sys->mesh;
+    svx_ctx* ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    system_prepare(sys, precision);
+    InstScal sc = initial_scal(tol, max_iter, tp.ru_delta);
+    sys->scal.upload(&sc, 1, s);
+    std::vector<InstDesc> descs{system_desc(sys)};
+    if (!sys->engine) sys->engine = std::make_unique<Engine>(s);
+    int rpt = tp.rows_per_tile > 0 ? tp.rows_per_tile : default_rows_per_tile(m->ndof);
+    rpt = std::max(m->ndof, (rpt / m->ndof) * m->ndof);
+    int lanes = tp.lanes > 0 ? tp.lanes : default_lanes((double)m->nnz / std::max<int64_t>(1, m->n));
+    int blanes = tp.blanes > 0 ? tp.blanes : default_lanes((double)m->nblk / std::max<int64_t>(1, m->n_nodes));
+    sys->engine->build(descs, {rpt}, lanes, blanes, m->ndof);
+    EngineOptions opt;
+    opt.precision = precision;
+    opt.period = tp.period;
+    opt.chunk = tp.chunk;
+    opt.use_graph = tp.use_graph;
+    sys->engine->run(opt, ctx->h_flag, ctx->d_flag);
+    sys->scal.download(out_scal, 1, s);
+    SVX_HIP(hipStreamSynchronize(s));
+    sys->solved = out_scal->status == ST_CONVERGED;
+    sys->has_u = true;
+}
+
+void system_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n_rows, const int64_t* rows,
+                    double* reactions) {
+    svx_mesh* m = sys->mesh;
+    hipStream_t s = m->ctx->stream;
+    if (!sys->has_u) throw std::invalid_argument("no solution to recover from");
+    if (vm_elem || vm_node) {
+        DevBuf<double> d_vm(std::max<int64_t>(1, m->n_elem));
+        DevBuf<double> d_ns(std::max<int64_t>(1, m->ns_total));
+        for (int k = 0; k < 3; ++k)
+            launch_element_stress(k, m->nsel[k], m->sel[k].p, m->conn_ptr.p, m->conn.p, m->coords.p, sys->E.p,
+                                  sys->nu.p, m->ndof, sys->u.p, d_vm.p, d_ns.p, m->ns_off.p, s);
+        if (vm_elem) d_vm.download(vm_elem, m->n_elem, s);
+        if (vm_node) {
+            DevBuf<double> d_vn(std::max<int64_t>(1, m->n_nodes));
+            launch_nodal_average(m->n_nodes, m->inc_ptr.p, m->inc_elem.p, m->inc_slot.p, m->ns_off.p, d_ns.p,
+                                 d_vn.p, s);
+            d_vn.download(vm_node, m->n_nodes, s);
+            SVX_HIP(hipStreamSynchronize(s));
+        }
+        SVX_HIP(hipStreamSynchronize(s));
+    }
+    if (n_rows > 0 && reactions) {
+        if (!sys->dirichlet) throw std::invalid_argument("reactions need svx_apply_dirichlet first");
+        const size_t nc = sys->cons_rows.size();
+        DevBuf<double> d_ku(std::max<size_t>(1, nc));
+        launch_reaction_rows((int64_t)nc, sys->d_cons_rows.p, m->row_ptr.p, m->col.p, sys->d_cons_off.p,
+                             sys->d_cons_vals.p, sys->u.p, d_ku.p, s);
+        std::vector<double> ku(nc);
+        d_ku.download(ku.data(), nc, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n_rows; ++i) {
+            auto it = std::lower_bound(sys->cons_rows.begin(), sys->cons_rows.end(), rows[i]);
+            if (it == sys->cons_rows.end() || *it != rows[i])
+                throw std::invalid_argument("reaction row is not constrained");
+            size_t q = it - sys->cons_rows.begin();
+            reactions[i] = ku[q] - sys->f_orig_cons[q];
+        }
+    }
+}
+
+}  // namespace svx
+
+// ======================================================================= ABI
+extern "C" {
+
+const char* svx_version(void) { return "solvix-mi355x 0.1 (gfx950)"; }
+
+int svx_ctx_create(int device, svx_ctx** out) {
+    if (!out) return SVX_E_ARGUMENT;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        auto c = std::make_unique<svx_ctx>();
+        c->device = device;
+        SVX_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        SVX_HIP(hipGetDeviceProperties(&prop, device));
+        c->name = prop.gcnArchName;
+        c->name += " / ";
+        c->name += prop.name;
+        SVX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        SVX_HIP(hipHostMalloc((void**)&c->h_flag, sizeof(int32_t) * 4, hipHostMallocDefault));
+        SVX_HIP(hipMalloc((void**)&c->d_flag, sizeof(int32_t) * 4));
+        *out = c.release();
+        return SVX_OK;
+    });
+}
+
+void svx_ctx_destroy(svx_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
+    if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int svx_device_name(svx_ctx* ctx, char* buf, size_t len) {
+    if (!ctx || !buf || len == 0) return SVX_E_ARGUMENT;
+    snprintf(buf, len, "%s", ctx->name.c_str());
+    return SVX_OK;
+}
+
+const char* svx_last_error(void) { return g_last_error.c_str(); }
+
+int svx_mesh_create(svx_ctx* ctx, int ndof, int64_t n_nodes, const double* coords, int64_t n_elem,
+                    const uint8_t* kinds, const int64_t* conn_ptr, const int32_t* conn_idx, svx_mesh** out,
+                    svx_error* err) {
+    clear_err(err);
+    if (!ctx || !out || (ndof != 2 && ndof != 3) || n_nodes <= 0 || n_elem < 0 || !coords ||
+        (n_elem > 0 && (!kinds || !conn_ptr || !conn_idx))) {
+        set_err(err, SVX_E_ARGUMENT, "svx_mesh_create: bad argument");
+        return SVX_E_ARGUMENT;
+    }
+    *out = nullptr;
+    return guarded(err, [&] {
+        if (conn_ptr[0] != 0) throw std::invalid_argument("conn_ptr[0] must be 0");
+        for (int64_t e = 0; e < n_elem; ++e) {
+            if (kinds[e] > SVX_C3D8) throw std::invalid_argument("unsupported element kind");
+            if ((kinds[e] == SVX_C3D8) != (ndof == 3)) throw std::invalid_argument("element kind does not match ndof");
+            if (conn_ptr[e + 1] - conn_ptr[e] != nodes_of(kinds[e]))
+                throw std::invalid_argument("connectivity length does not match the element kind");
+            for (int64_t k = conn_ptr[e]; k < conn_ptr[e + 1]; ++k)
+                if (conn_idx[k] < 0 || conn_idx[k] >= n_nodes) throw std::invalid_argument("node position out of range");
+        }
+        SVX_HIP(hipSetDevice(ctx->device));
+        auto m = std::make_unique<svx_mesh>();
+        m->ctx = ctx;
+        mesh_build(m.get(), ndof, n_nodes, coords, n_elem, kinds, conn_ptr, conn_idx);
+        *out = m.release();
+        return SVX_OK;
+    });
+}
+
+void svx_mesh_destroy(svx_mesh* mesh) { delete mesh; }
+int64_t svx_mesh_rows(const svx_mesh* mesh) { return mesh ? mesh->n : -1; }
+int64_t svx_mesh_nnz(const svx_mesh* mesh) { return mesh ? mesh->nnz : -1; }
+
+int svx_mesh_pattern(const svx_mesh* mesh, int32_t* row_ptr, int32_t* col_idx) {
+    if (!mesh) return SVX_E_ARGUMENT;
+    if (row_ptr) std::copy(mesh->hp.row_ptr.begin(), mesh->hp.row_ptr.end(), row_ptr);
+    if (col_idx) std::copy(mesh->hp.col.begin(), mesh->hp.col.end(), col_idx);
+    return SVX_OK;
+}
+
+int svx_assemble(svx_mesh* mesh, const double* youngs, const double* poisson, double thickness, int64_t n_loads,
+                 const int64_t* load_rows, const double* load_values, svx_system** out, svx_error* err) {
+    clear_err(err);
+    if (!mesh || !youngs || !poisson || !out || n_loads < 0 || (n_loads > 0 && (!load_rows || !load_values))) {
+        set_err(err, SVX_E_ARGUMENT, "svx_assemble: bad argument");
+        return SVX_E_ARGUMENT;
+    }
+    *out = nullptr;
+    return guarded(err, [&] {
+        SVX_HIP(hipSetDevice(mesh->ctx->device));
+        auto sys = std::make_unique<svx_system>();
+        sys->mesh = mesh;
+        int64_t bad = -1;
+        int why = 0;
+        system_assemble(sys.get(), youngs, poisson, thickness, &bad, &why);
+        if (bad >= 0) {
+            char buf[256];
+            snprintf(buf, sizeof buf, "element at position %lld: %s", (long long)bad, reason_text(why));
+            set_err(err, SVX_E_INVALID_MODEL, buf, why, bad);
+            return SVX_E_INVALID_MODEL;
+        }
+        std::vector<double> f(mesh->n, 0.0);
+        for (int64_t l = 0; l < n_loads; ++l) {
+            if (load_rows[l] < 0 || load_rows[l] >= mesh->n) throw std::invalid_argument("load row out of range");
+            f[load_rows[l]] += load_values[l];
+        }
+        system_set_f(sys.get(), f.data());
+        *out = sys.release();
+        return SVX_OK;
+    });
+}
+
+void svx_system_destroy(svx_system* sys) { delete sys; }
+
+int svx_system_values(const svx_system* sys, double* vals, double* f) {
+    if (!sys) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        hipStream_t s = sys->mesh->ctx->stream;
+        if (vals) sys->vals.download(vals, sys->mesh->nnz, s);
+        if (f) sys->f.download(f, sys->mesh->n, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        return SVX_OK;
+    });
+}
+
+int svx_system_set_rhs(svx_system* sys, const double* f) {
+    if (!sys || !f) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        system_set_f(sys, f);
+        return SVX_OK;
+    });
+}
+
+int svx_apply_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {
+    if (!sys || n_rows < 0 || (n_rows > 0 && !rows)) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        if (sys->dirichlet) throw std::invalid_argument("Dirichlet constraints already applied");
+        system_dirichlet(sys, n_rows, rows);
+        return SVX_OK;
+    });
+}
+
+int svx_solve(svx_system* sys, double tol, int64_t max_iter, int precision, double* u, int64_t* iterations,
+              double* residual, svx_error* err) {
+    clear_err(err);
+    if (!sys || (precision != SVX_FP64 && precision != SVX_FP32_IR) || !(tol > 0.0)) {
+        set_err(err, SVX_E_ARGUMENT, "svx_solve: bad argument");
+        return SVX_E_ARGUMENT;
+    }
+    return guarded(err, [&] {
+        svx_mesh* m = sys->mesh;
+        int64_t mi = max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * m->n);
+        InstScal sc{};
+        system_solve(sys, tol, mi, precision, &sc, m->ctx->tune);
+        if (u) {
+            sys->u.download(u, m->n, m->ctx->stream);
+            SVX_HIP(hipStreamSynchronize(m->ctx->stream));
+        }
+        if (iterations) *iterations = sc.it;
+        if (residual) *residual = sc.res;
+        sys->last_refreshes = sc.refreshes;
+        if (sc.status != ST_CONVERGED) {
+            char buf[256];
+            snprintf(buf, sizeof buf, "NotConverged { iterations: %lld, residual: %.6e }", (long long)sc.it, sc.res);
+            set_err(err, SVX_E_NOT_CONVERGED, buf, 0, -1, sc.it, sc.res);
+            return SVX_E_NOT_CONVERGED;
+        }
+        return SVX_OK;
+    });
+}
+
+int svx_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n_rows, const int64_t* rows,
+                double* reactions, double* cond_estimate) {
+    if (!sys || n_rows < 0 || (n_rows > 0 && (!rows || !reactions))) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        system_recover(sys, vm_elem, vm_node, n_rows, rows, reactions);
+        if (cond_estimate) *cond_estimate = sys->cond;
+        return SVX_OK;
+    });
+}
+
+// ----------------------------------------------------------------- raw CSR
+int svx_csr_create(svx_ctx* ctx, int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                   svx_csr** out) {
+    if (!ctx || !out || n <= 0 || n >= INT32_MAX || !row_ptr || !col_idx || !vals) return SVX_E_ARGUMENT;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        if (row_ptr[0] != 0) throw std::invalid_argument("row_ptr[0] must be 0");
+        for (int64_t r = 0; r < n; ++r)
+            if (row_ptr[r + 1] < row_ptr[r]) throw std::invalid_argument("row_ptr must be non-decreasing");
+        const int64_t nnz = row_ptr[n];
+        for (int64_t k = 0; k < nnz; ++k)
+            if (col_idx[k] < 0 || col_idx[k] >= n) throw std::invalid_argument("column index out of range");
+        SVX_HIP(hipSetDevice(ctx->device));
+        auto c = std::make_unique<svx_csr>();
+        c->ctx = ctx;
+        c->n = n;
+        c->nnz = nnz;
+        hipStream_t s = ctx->stream;
+        c->row_ptr.alloc(n + 1);
+        c->row_ptr.upload(row_ptr, n + 1, s);
+        c->col.alloc(std::max<int64_t>(1, nnz));
+        c->col.upload(col_idx, nnz, s);
+        c->vals.alloc(std::max<int64_t>(1, nnz));
+        c->vals.upload(vals, nnz, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        *out = c.release();
+        return SVX_OK;
+    });
+}
+
+void svx_csr_destroy(svx_csr* csr) { delete csr; }
+
+int svx_csr_spmv(svx_csr* csr, const double* x, double* y, int ordered) {
+    if (!csr || !x || !y) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        hipStream_t s = csr->ctx->stream;
+        DevBuf<double> dx(csr->n), dy(csr->n);
+        dx.upload(x, csr->n, s);
+        if (ordered)
+            launch_spmv_ordered(csr->n, csr->row_ptr.p, csr->col.p, csr->vals.p, dx.p, dy.p, s);
+        else
+            launch_spmv_csr64(csr->n, csr->row_ptr.p, csr->col.p, csr->vals.p, dx.p, dy.p,
+                              default_lanes((double)csr->nnz / csr->n), s);
+        dy.download(y, csr->n, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        return SVX_OK;
+    });
+}
+
+int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int precision, double* u,
+                int64_t* iterations, double* residual, svx_error* err) {
+    clear_err(err);
+    if (!csr || !f || (precision != SVX_FP64 && precision != SVX_FP32_IR) || !(tol > 0.0)) {
+        set_err(err, SVX_E_ARGUMENT, "svx_csr_pcg: bad argument");
+        return SVX_E_ARGUMENT;
+    }
+    return guarded(err, [&] {
+        svx_ctx* ctx = csr->ctx;
+        hipStream_t s = ctx->stream;
+        const int64_t n = csr->n;
+        int64_t mi = max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * n);
+        DevBuf<double> df(n), dinv(n), du(n), dr(n), dp(n), dz(n), dq(n);
+        DevBuf<float> dinv32(n), bval, x32, r32, p32, z32, q32;
+        df.upload(f, n, s);
+        launch_diag_inv(n, csr->row_ptr.p, csr->col.p, csr->vals.p, dinv.p, dinv32.p, nullptr, nullptr, s);
+        InstDesc d{};
+        d.n = (int32_t)n;
+        d.bs = 1;
+        d.row_ptr = csr->row_ptr.p;
+        d.col = csr->col.p;
+        d.val = csr->vals.p;
+        d.dinv = dinv.p;
+        d.dinv32 = dinv32.p;
+        d.f = df.p;
+        d.u = du.p;
+        d.r = dr.p, d.p = dp.p, d.z = dz.p, d.q = dq.p;
+        if (precision == SVX_FP32_IR) {
+            bval.alloc(std::max<int64_t>(1, csr->nnz));
+            launch_fill_bsr(n, 1, csr->row_ptr.p, csr->col.p, csr->vals.p, csr->row_ptr.p, bval.p, s);
+            x32.alloc(n), r32.alloc(n), p32.alloc(n), z32.alloc(n), q32.alloc(n);
+            d.brow_ptr = csr->row_ptr.p;
+            d.bcol = csr->col.p;
+            d.bval = bval.p;
+            d.x32 = x32.p, d.r32 = r32.p, d.p32 = p32.p, d.z32 = z32.p, d.q32 = q32.p;
+        }
+        DevBuf<InstScal> scal(1);
+        InstScal sc = initial_scal(tol, mi, ctx->tune.ru_delta);
+        scal.upload(&sc, 1, s);
+        d.s = scal.p;
+        std::vector<InstDesc> descs{d};
+        Engine eng(s);
+        const TuneParams& tp = ctx->tune;
+        double avg = (double)csr->nnz / n;
+        eng.build(descs, {tp.rows_per_tile > 0 ? tp.rows_per_tile : 256}, tp.lanes > 0 ? tp.lanes : default_lanes(avg),
+                  tp.blanes > 0 ? tp.blanes : default_lanes(avg), 1);
+        EngineOptions opt;
+        opt.precision = precision;
+        opt.period = tp.period;
+        opt.chunk = tp.chunk;
+        opt.use_graph = tp.use_graph;
+        eng.run(opt, ctx->h_flag, ctx->d_flag);
+        scal.download(&sc, 1, s);
+        if (u) du.download(u, n, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        if (iterations) *iterations = sc.it;
+        if (residual) *residual = sc.res;
+        if (sc.status != ST_CONVERGED) {
+            char buf[256];
+            snprintf(buf, sizeof buf, "NotConverged { iterations: %lld, residual: %.6e }", (long long)sc.it, sc.res);
+            set_err(err, SVX_E_NOT_CONVERGED, buf, 0, -1, sc.it, sc.res);
+            return SVX_E_NOT_CONVERGED;
+        }
+        return SVX_OK;
+    });
+}
+
+// --------------------------------------------------------------- tuning knobs
+int svx_set_tuning(svx_ctx* ctx, int lanes, int blanes, int rows_per_tile, int period, int chunk, int use_graph,
+                   double ru_delta) {
+    if (!ctx) return SVX_E_ARGUMENT;
+    TuneParams& t = ctx->tune;
+    t.lanes = lanes;
+    t.blanes = blanes;
+    t.rows_per_tile = rows_per_tile;
+    if (period > 0) t.period = period;
+    if (chunk > 0) t.chunk = chunk;
+    t.use_graph = use_graph != 0;
+    if (ru_delta > 0) t.ru_delta = ru_delta;
+    return SVX_OK;
+}
+
+// ------------------------------------------------------------ whole model
+int64_t svx_model_reaction_count(const svx_model_desc* m) {
+    if (!m) return -1;
+    int64_t c = 0;
+    for (int32_t b = 0; b < m->n_bcs; ++b)
+        c += (m->bc_node_ptr[b + 1] - m->bc_node_ptr[b]) * (m->bc_dof_ptr[b + 1] - m->bc_dof_ptr[b]);
+    return c;
+}
+
+int svx_solve_model(svx_ctx* ctx, const svx_model_desc* md, int method, int precision, svx_solution* out,
+                    svx_error* err) {
+    clear_err(err);
+    if (!ctx || !md || !out) {
+        set_err(err, SVX_E_ARGUMENT, "svx_solve_model: bad argument");
+        return SVX_E_ARGUMENT;
+    }
+    if (method == SVX_METHOD_LDLT) {
+        set_err(err, SVX_E_ARGUMENT, "Method::Ldlt is not on the device path");
+        return SVX_E_ARGUMENT;
+    }
+    if (precision != SVX_FP64 && precision != SVX_FP32_IR) {
+        set_err(err, SVX_E_ARGUMENT, "unknown precision");
+        return SVX_E_ARGUMENT;
+    }
+    return guarded(err, [&]() -> int {
+        SVX_HIP(hipSetDevice(ctx->device));
+        PreparedModel pm;
+        int rc = prepare_model(md, pm, err);
+        if (rc != SVX_OK) return rc;
+        double t0 = now_ms();
+        svx_mesh mesh;
+        mesh.ctx = ctx;
+        mesh_build(&mesh, pm.ndof, md->n_nodes, md->coords, md->n_elem, md->kinds, md->conn_ptr, pm.conn_pos.data());
+        svx_system sys;
+        sys.mesh = &mesh;
+        int64_t bad = -1;
+        int why = 0;
+        system_assemble(&sys, pm.E.data(), pm.nu.data(), md->thickness, &bad, &why);
+        if (bad >= 0) {
+            char buf[256];
+            snprintf(buf, sizeof buf, "element %u: %s", md->elem_ids[bad], reason_text(why));
+            set_err(err, SVX_E_INVALID_MODEL, buf, why, bad);
+            return SVX_E_INVALID_MODEL;
+        }
+        if (!pm.load_error.empty()) {
+            set_err(err, SVX_E_INVALID_MODEL, pm.load_error);
+            return SVX_E_INVALID_MODEL;
+        }
+        if (!pm.bc_error.empty()) {
+            set_err(err, SVX_E_INVALID_MODEL, pm.bc_error);
+            return SVX_E_INVALID_MODEL;
+        }
+        system_set_f(&sys, pm.f.data());
+        double t1 = now_ms();
+        system_dirichlet(&sys, (int64_t)pm.cons_rows.size(), pm.cons_rows.data());
+        const int64_t n = mesh.n;
+        InstScal sc{};
+        system_solve(&sys, 1e-10, std::max<int64_t>(1000, 10 * n), precision, &sc, ctx->tune);
+        double t2 = now_ms();
+        out->iterations = sc.it;
+        out->final_residual = sc.res;
+        out->ndof = pm.ndof;
+        out->solver_method = 0;
+        if (sc.status != ST_CONVERGED) {
+            char buf[256];
+            snprintf(buf, sizeof buf, "NotConverged { iterations: %lld, residual: %.6e }", (long long)sc.it, sc.res);
+            set_err(err, SVX_E_NOT_CONVERGED, buf, 0, -1, sc.it, sc.res);
+            return SVX_E_NOT_CONVERGED;
+        }
+        std::vector<double> u(n);
+        sys.u.download(u.data(), n, ctx->stream);
+        SVX_HIP(hipStreamSynchronize(ctx->stream));
+        if (out->displacements)
+            for (int64_t i = 0; i < md->n_nodes; ++i) {
+                out->displacements[i * 3] = u[i * pm.ndof];
+                out->displacements[i * 3 + 1] = u[i * pm.ndof + 1];
+                out->displacements[i * 3 + 2] = pm.ndof == 3 ? u[i * pm.ndof + 2] : 0.0;
+            }
+        std::vector<double> react(pm.cons_rows.size());
+        system_recover(&sys, out->von_mises, out->nodal_von_mises, (int64_t)pm.cons_rows.size(), pm.cons_rows.data(),
+                       react.data());
+        for (size_t i = 0; i < pm.cons_rows.size(); ++i) {
+            if (out->reaction_node) out->reaction_node[i] = pm.cons_node[i];
+            if (out->reaction_dof) out->reaction_dof[i] = pm.cons_dof[i];
+            if (out->reaction_value) out->reaction_value[i] = react[i];
+        }
+        out->cond_estimate = sys.cond;
+        double t3 = now_ms();
+        out->assemble_ms = t1 - t0;
+        out->solve_ms = t2 - t1;
+        out->recover_ms = t3 - t2;
+        return SVX_OK;
+    });
+}
+
+}  // extern "C"

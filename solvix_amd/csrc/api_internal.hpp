@@ -1,0 +1,120 @@
+// Handle types behind the C ABI and the host-side model preparation.
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "pattern.hpp"
+#include "svx_internal.hpp"
+
+namespace svx {
+
+// Launch-shape and refinement knobs (0 = automatic choice).
+struct TuneParams {
+    int lanes = 0;          // CSR lanes per row (FP64 SpMV)
+    int blanes = 0;         // BSR lanes per block row (FP32 SpMV)
+    int rows_per_tile = 0;  // rows per workgroup tile
+    int period = 8;         // FP32-IR refresh period (iterations)
+    int chunk = 32;         // iterations per captured graph
+    bool use_graph = true;
+    double ru_delta = 0.1;  // FP32-IR: refresh when ||r|| < ru_delta * ||r_last||
+};
+
+}  // namespace svx
+
+struct svx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    int32_t* h_flag = nullptr;  // pinned
+    int32_t* d_flag = nullptr;
+    svx::TuneParams tune;
+};
+
+struct svx_mesh {
+    svx_ctx* ctx = nullptr;
+    int ndof = 0;
+    int64_t n_nodes = 0, n_elem = 0, n = 0, nnz = 0, nblk = 0;
+    svx::HostPattern hp;
+    std::vector<int64_t> h_conn_ptr;
+    std::vector<int32_t> h_conn;
+    std::vector<uint8_t> h_kinds;
+    svx::DevBuf<double> coords;
+    svx::DevBuf<int64_t> conn_ptr;
+    svx::DevBuf<int32_t> conn;
+    svx::DevBuf<uint8_t> kinds;
+    svx::DevBuf<int32_t> row_ptr, col, brow_ptr, bcol;
+    svx::DevBuf<int64_t> inc_ptr;
+    svx::DevBuf<int32_t> inc_elem;
+    svx::DevBuf<int8_t> inc_slot;
+    svx::DevBuf<int32_t> sel[3];
+    int64_t nsel[3] = {0, 0, 0};
+    svx::DevBuf<int64_t> ke_off, ns_off;
+    int64_t ke_total = 0, ns_total = 0;
+};
+
+struct svx_system {
+    svx_mesh* mesh = nullptr;
+    svx::DevBuf<double> E, nu;
+    double thickness = 1.0;
+    svx::DevBuf<double> vals, f;
+    std::vector<double> h_f;  // current load vector (host copy)
+    bool dirichlet = false;
+    std::vector<int64_t> cons_rows;  // unique, ascending
+    std::vector<double> f_orig_cons;
+    svx::DevBuf<int64_t> d_cons_rows, d_cons_off;
+    svx::DevBuf<double> d_cons_vals;  // original values of the constrained rows
+    svx::DevBuf<double> dinv;
+    svx::DevBuf<float> dinv32, bval;
+    svx::DevBuf<double> u, r, p, z, q;
+    svx::DevBuf<float> x32, r32, p32, z32, q32;
+    svx::DevBuf<svx::InstScal> scal;
+    std::unique_ptr<svx::Engine> engine;
+    double cond = 0.0;
+    bool solved = false, has_u = false;
+    int64_t last_refreshes = 0;
+};
+
+struct svx_csr {
+    svx_ctx* ctx = nullptr;
+    int64_t n = 0, nnz = 0;
+    svx::DevBuf<int32_t> row_ptr, col;
+    svx::DevBuf<double> vals;
+};
+
+namespace svx {
+
+int nodes_of(uint8_t kind);
+
+// A model validated and resolved on the host (lib.rs:593-632 up to the
+// element pass): node positions, per-element material constants, the load
+// vector and the constrained rows, with deferred load / BC errors that the
+// reference reports only after the element pass.
+struct PreparedModel {
+    int ndof = 2;
+    std::vector<int32_t> conn_pos;
+    std::vector<double> E, nu;  // NaN E = no material
+    std::vector<double> f;
+    std::vector<int64_t> cons_rows;  // BC order, duplicates kept
+    std::vector<uint32_t> cons_node;
+    std::vector<int32_t> cons_dof;
+    std::string load_error, bc_error;
+};
+int prepare_model(const svx_model_desc* m, PreparedModel& out, svx_error* err);
+
+void mesh_build(svx_mesh* m, int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const uint8_t* kinds,
+                const int64_t* conn_ptr, const int32_t* conn_idx);
+void system_assemble(svx_system* sys, const double* E, const double* nu, double thickness, int64_t* bad_index,
+                     int* bad_reason);
+void system_set_f(svx_system* sys, const double* f);
+void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows);
+void system_prepare(svx_system* sys, int precision);
+InstDesc system_desc(svx_system* sys);
+InstScal initial_scal(double tol, int64_t max_iter, double ru_delta);
+void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,
+                  const TuneParams& tp);
+void system_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n_rows, const int64_t* rows,
+                    double* reactions);
+
+}  // namespace svx

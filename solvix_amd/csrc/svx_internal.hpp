@@ -1,0 +1,198 @@
+// Internal declarations shared by the HIP kernels and the C-ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/svx.h"
+
+namespace svx {
+
+// ----------------------------------------------------------------- errors
+struct DeviceError : std::runtime_error {
+    explicit DeviceError(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define SVX_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::svx::DeviceError(std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+// ------------------------------------------------------------ device memory
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p, n = o.n;
+            o.p = nullptr, o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) SVX_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr, n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        if (count) SVX_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* h, size_t count, hipStream_t s) const {
+        if (count) SVX_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+// ------------------------------------------------------- Krylov instances
+// Instance status values written by the device.
+enum : int32_t { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2, ST_MAXITER = 3, ST_IDLE = 4 };
+// Instance flags.
+enum : int32_t { FL_REFRESH = 1 };
+
+// Per-instance scalar state of a (preconditioned) CG solve, device resident.
+struct alignas(16) InstScal {
+    double bnorm;    // ||f||
+    double rz;       // r . z of the current iterate
+    double pap;      // p . K p
+    double alpha;
+    double beta;
+    double rr;       // r . r of the current (recursive) residual
+    double res;      // reported relative residual
+    double rmax;     // ||r|| at the last residual replacement (FP32-IR)
+    double tol;
+    int64_t it;      // iterations done
+    int64_t max_iter;
+    int32_t status;  // ST_*
+    int32_t flags;   // FL_*
+    uint32_t cnt_a;  // last-block-done counters (reset by the last block)
+    uint32_t cnt_b;
+    int64_t refreshes;  // residual replacements done (FP32-IR)
+    double ru_delta;    // replacement threshold: ||r|| < ru_delta * rmax
+};
+
+// Device pointers of one instance. FP64 CSR always; FP32 block-sparse
+// (node-block BSR, bs x bs blocks) for the FP32-IR inner iteration.
+struct InstDesc {
+    int32_t n;           // rows
+    int32_t tile_begin;  // first tile of this instance in the tile table
+    int32_t tile_count;
+    int32_t bs;          // block size of the BSR arrays (ndof), 0 = none
+    const int32_t* row_ptr;
+    const int32_t* col;
+    const double* val;
+    const int32_t* brow_ptr;  // [n/bs + 1]
+    const int32_t* bcol;      // block columns (node indices)
+    const float* bval;        // bs*bs per block, row-major
+    const double* dinv;
+    const float* dinv32;
+    const double* f;
+    double* u;   // FP64 solution
+    double* r;   // FP64 residual (FP64 CG) / true residual (FP32-IR)
+    double* p;
+    double* z;
+    double* q;   // K p
+    float* x32;  // FP32-IR: correction accumulated since the last replacement
+    float* r32;
+    float* p32;
+    float* z32;
+    float* q32;
+    InstScal* s;
+};
+
+// A tile is a contiguous row range of one instance handled by one workgroup.
+struct Tile {
+    int32_t inst;
+    int32_t row_begin;
+    int32_t row_end;
+    int32_t pad;
+};
+
+constexpr int kBlock = 256;
+
+// SpMV launch shape chosen per instance set: lanes per row (CSR) or per
+// block row (BSR) and rows per tile.
+struct SpmvShape {
+    int lanes = 16;
+    int rows_per_tile = 64;
+};
+
+// ------------------------------------------------------ kernel launchers
+// assembly.hip
+void launch_element_ke(int kind, int64_t n_sel, const int32_t* sel, const int64_t* conn_ptr,
+                       const int32_t* conn, const double* coords, const double* E, const double* nu,
+                       double thickness, int ndof, double* ke, const int64_t* ke_off, uint8_t* reason,
+                       unsigned long long* first_bad, hipStream_t s);
+void launch_gather_assembly(int64_t n_rows, int ndof, const int32_t* row_ptr, const int32_t* col,
+                            const int64_t* inc_ptr, const int32_t* inc_elem, const int8_t* inc_slot,
+                            const int64_t* conn_ptr, const int32_t* conn, const uint8_t* kinds,
+                            const double* ke, const int64_t* ke_off, double* vals, hipStream_t s);
+void launch_dirichlet(int64_t n_rows, const int32_t* row_ptr, const int32_t* col, const uint8_t* mask,
+                      double* vals, double* f, hipStream_t s);
+void launch_gather_rows(int64_t n_sel, const int64_t* rows, const int32_t* row_ptr, const double* vals,
+                        const int64_t* out_off, double* out, hipStream_t s);
+void launch_diag_inv(int64_t n_rows, const int32_t* row_ptr, const int32_t* col, const double* vals,
+                     double* dinv, float* dinv32, unsigned long long* absmax, unsigned long long* absmin,
+                     hipStream_t s);
+void launch_fill_bsr(int64_t n_rows, int bs, const int32_t* row_ptr, const int32_t* col, const double* vals,
+                     const int32_t* brow_ptr, float* bval, hipStream_t s);
+void launch_reaction_rows(int64_t n_sel, const int64_t* rows, const int32_t* row_ptr, const int32_t* col,
+                          const int64_t* off, const double* orig_vals, const double* u, double* out,
+                          hipStream_t s);
+
+// recover.hip
+void launch_element_stress(int kind, int64_t n_sel, const int32_t* sel, const int64_t* conn_ptr,
+                           const int32_t* conn, const double* coords, const double* E, const double* nu,
+                           int ndof, const double* u, double* vm_elem, double* nodal_stress,
+                           const int64_t* ns_off, hipStream_t s);
+void launch_nodal_average(int64_t n_nodes, const int64_t* inc_ptr, const int32_t* inc_elem,
+                          const int8_t* inc_slot, const int64_t* ns_off, const double* nodal_stress,
+                          double* vm_node, hipStream_t s);
+
+// krylov.hip
+void launch_spmv_ordered(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                         const double* x, double* y, hipStream_t s);
+void launch_spmv_csr64(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                       const double* x, double* y, int lanes, hipStream_t s);
+void launch_spmv_bsr32(int64_t nb, int bs, const int32_t* brow_ptr, const int32_t* bcol, const float* bval,
+                       const float* x, float* y, int lanes, hipStream_t s);
+
+// One CG iteration phase over a set of instances. `tiles` / `n_tiles`
+// describe the work; `descs` the instances; `partials` holds 2 doubles per
+// tile. Every launcher is graph-capturable (no allocation, no memset).
+struct KrylovSet {
+    const Tile* tiles = nullptr;
+    int32_t n_tiles = 0;
+    const InstDesc* descs = nullptr;
+    int32_t n_inst = 0;
+    double* partials = nullptr;  // [2 * n_tiles]
+    int lanes = 16;              // CSR lanes per row (FP64 SpMV)
+    int blanes = 8;              // BSR lanes per block row (FP32 SpMV)
+    int bs = 0;                  // common BSR block size of the set (2 or 3), 0 = mixed/none
+};
+void cg64_init(const KrylovSet& k, hipStream_t s);
+void cg64_iteration(const KrylovSet& k, hipStream_t s);
+void ir32_init(const KrylovSet& k, hipStream_t s);
+void ir32_iteration(const KrylovSet& k, hipStream_t s);
+void ir32_refresh(const KrylovSet& k, hipStream_t s);
+void launch_count_running(const InstDesc* descs, int32_t n_inst, int32_t* out, hipStream_t s);
+
+}  // namespace svx

@@ -1,0 +1,154 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Bars (north star): CSR pattern and DOF numbering bit-exact; assembled values
+within 1e-12 relative; displacements and von Mises within 1e-8 relative at
+rel. residual 1e-10; the measured error is printed per result.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from solvix_amd import _abi, decks
+from solvix_amd.core import CsrMatrix, Mesh, assemble, constrained_dofs, solve_with_method
+from solvix_amd.inp import read_inp
+from solvix_amd.model import Method, Precision, SolveError
+
+from helpers import dirichlet, rel_err, small_models
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _models():
+    ms = [read_inp(os.path.join(GOLDEN, f"{n}.inp")) for n in ("cantilever", "cantilever3d", "plate_hole")]
+    return ms + small_models()
+
+
+MODELS = _models()
+IDS = [m.name for m in MODELS]
+
+
+@pytest.mark.parametrize("model", MODELS, ids=IDS)
+def test_pattern_and_values(model, ctx):
+    r = oracle.solve(model, stop_after_assembly=True)
+    mesh, sys_ = assemble(model, ctx)
+    rp, ci = mesh.pattern()
+    assert np.array_equal(rp, r.row_ptr) and np.array_equal(ci, r.col_idx)
+    vals, f = sys_.values()
+    err = np.abs(vals - r.vals) / np.maximum(np.abs(r.vals), 1e-300 + np.abs(r.vals).max() * 1e-30)
+    exact = np.count_nonzero(vals == r.vals)
+    print(f"{model.name}: nnz {vals.size}, bit-exact {exact}, max rel {np.abs(vals - r.vals).max() / np.abs(r.vals).max():.2e}")
+    assert np.abs(vals - r.vals).max() <= 1e-12 * np.abs(r.vals).max()
+    assert np.array_equal(f, r.f)
+
+
+def test_spmv_ordered_is_bit_exact_and_tuned_is_close(ctx):
+    m = decks.hex_model(40, 12, 12, 2.0, 0.1, 0.1, E=1.0, nu=0.25, jitter=0.1, rng=decks.SplitMix64(3))
+    m.bcs = [(np.array([1], np.uint32), np.array([1, 2], np.int32))]
+    r = oracle.solve(m, stop_after_assembly=True)
+    A = CsrMatrix(r.row_ptr, r.col_idx, r.vals, ctx)
+    x = oracle.lcg_vector(A.n)
+    y_ref = oracle.spmv(r.row_ptr, r.col_idx, r.vals, x)
+    y_ord = A.mat_vec(x, ordered=True)
+    assert np.array_equal(y_ord.view(np.uint64), y_ref.view(np.uint64))
+    y = A.mat_vec(x)
+    absrow = oracle.spmv(r.row_ptr, r.col_idx, np.abs(r.vals), np.abs(x))
+    assert np.all(np.abs(y - y_ref) <= 1e-14 * absrow + 1e-300)
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+@pytest.mark.parametrize("model", MODELS, ids=IDS)
+def test_raw_pcg_matches_oracle(model, precision, ctx):
+    r = oracle.solve(model)
+    assert r.status == 0
+    v, f = dirichlet(r.row_ptr, r.col_idx, r.vals, r.f, r.reaction_rows)
+    A = CsrMatrix(r.row_ptr, r.col_idx, v, ctx)
+    u, its, res = A.pcg(f, 1e-10, 0, precision)
+    e = rel_err(u, r.u)
+    print(f"{model.name} [{precision}]: iterations {its} (oracle {r.iterations}), residual {res:.3e}, rel err u {e:.2e}")
+    assert res <= 1e-10
+    assert e <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+@pytest.mark.parametrize("model", MODELS, ids=IDS)
+def test_solve_model_matches_oracle(model, precision, ctx):
+    r = oracle.solve(model)
+    s = solve_with_method(model, Method.AUTO, precision, ctx)
+    eu = rel_err(s.displacements, r.displacements)
+    ev = rel_err(s.von_mises, r.von_mises)
+    en = rel_err(s.nodal_von_mises, r.nodal_von_mises)
+    er = rel_err([x.value for x in s.reactions], r.reaction_value)
+    print(f"{model.name} [{precision}]: its {s.iterations} (oracle {r.iterations}) res {s.final_residual:.2e} "
+          f"err u {eu:.1e} vm {ev:.1e} nodal {en:.1e} react {er:.1e} cond {s.cond_estimate:.3e}/{r.cond_estimate:.3e}")
+    assert s.final_residual <= 1e-10
+    assert eu <= 1e-8 and ev <= 1e-8 and en <= 1e-8
+    assert er <= 1e-6
+    assert [x.node_id for x in s.reactions] == list(r.reaction_node)
+    assert [x.dof for x in s.reactions] == list(r.reaction_dof)
+    assert s.cond_estimate == pytest.approx(r.cond_estimate, rel=1e-12)
+
+
+def test_known_answers_on_device(ctx):
+    import json
+    ka = json.load(open(os.path.join(GOLDEN, "known_answers.json")))
+    for name in ("cantilever", "cantilever3d", "plate_hole"):
+        m = read_inp(os.path.join(GOLDEN, ka[name]["deck"]))
+        for prec in (Precision.FP64, Precision.FP32_IR):
+            s = solve_with_method(m, Method.AUTO, prec, ctx)
+            assert float(f"{s.max_disp() * 1e3:.3f}") == ka[name]["max_disp_mm"]
+            assert float(f"{s.max_von_mises() / 1e6:.3f}") == ka[name]["max_von_mises_mpa"]
+
+
+def test_patch_tests_on_device(ctx):
+    m = decks.hex_model(1, 1, 1, 1, 1, 1, E=210e9, nu=0.3)
+    m.bcs = [(np.array([1, 2, 3, 4], np.uint32), np.array([3], np.int32)),
+             (np.array([1], np.uint32), np.array([1], np.int32)),
+             (np.array([1, 2], np.uint32), np.array([2], np.int32))]
+    m.cload_node, m.cload_dof, m.cload_value = np.array([5, 6, 7, 8], np.uint32), np.full(4, 3, np.int32), np.full(4, 2.5e5)
+    for prec in (Precision.FP64, Precision.FP32_IR):
+        s = solve_with_method(m, Method.AUTO, prec, ctx)
+        assert np.all(np.abs(s.von_mises - 1e6) < 1e-6 * 1e6)
+        assert np.all(np.abs(s.nodal_von_mises - 1e6) < 1e-6 * 1e6)
+        assert abs(sum(x.value for x in s.reactions if x.dof == 3) + 1e6) < 1.0
+
+
+def test_errors_match_reference(ctx):
+    q = decks.quad_model(1, 1, 1, 1)
+    q.cload_node, q.cload_dof, q.cload_value = np.array([2], np.uint32), np.array([1], np.int32), np.array([1.0])
+    with pytest.raises(SolveError) as e:
+        solve_with_method(q, Method.AUTO, Precision.FP64, ctx)
+    assert e.value.kind == "NoConstraints"
+    h = decks.hex_model(2, 1, 1, 2, 1, 1)
+    h.conn_ids = h.conn_ids.copy()
+    h.conn_ids[8 + 1], h.conn_ids[8 + 2] = h.conn_ids[8 + 2], h.conn_ids[8 + 1]  # element 2 inverted
+    h.bcs = [(np.array([1], np.uint32), np.array([1, 2, 3], np.int32))]
+    r = oracle.solve(h)
+    with pytest.raises(SolveError) as e:
+        solve_with_method(h, Method.AUTO, Precision.FP64, ctx)
+    assert e.value.kind == "InvalidModel" and str(e.value.message) == r.message
+    assert e.value.element_index == 1 and e.value.reason == _abi.SVX_ELEM_NONPOSITIVE_JACOBIAN
+    q = decks.quad_model(2, 1, 2, 1)
+    q.bcs = [(np.array([99], np.uint32), np.array([1], np.int32))]
+    with pytest.raises(SolveError) as e:
+        solve_with_method(q, Method.AUTO, Precision.FP64, ctx)
+    assert e.value.message == oracle.solve(q).message
+    q.bcs = [(np.array([1], np.uint32), np.array([1], np.int32))]
+    q.materials = [("MAT", 70e9, 1.2)]
+    with pytest.raises(SolveError) as e:
+        solve_with_method(q, Method.AUTO, Precision.FP64, ctx)
+    assert e.value.message == "element 1: invalid material constants"
+
+
+def test_not_converged_budget(ctx):
+    m = read_inp(os.path.join(GOLDEN, "cantilever.inp"))
+    mesh, sys_ = assemble(m, ctx)
+    rows, _, _ = constrained_dofs(m)
+    sys_.apply_dirichlet(rows)
+    with pytest.raises(SolveError) as e:
+        sys_.solve(1e-10, 25, Precision.FP64)
+    assert e.value.kind == "NotConverged" and e.value.iterations == 25
+    r = oracle.solve(m, max_iter=25)
+    assert e.value.residual == pytest.approx(r.residual, rel=1e-6)
