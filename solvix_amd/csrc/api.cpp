@@ -88,6 +88,9 @@ void mesh_build(svx_mesh* m, int ndof, int64_t n_nodes, const double* coords, in
     m->n = n_nodes * ndof;
     m->nnz = m->hp.row_ptr[m->n];
     m->nblk = m->hp.brow_ptr[n_nodes];
+    m->max_row_len = 0;
+    for (int64_t r = 0; r < m->n; ++r)
+        m->max_row_len = std::max<int64_t>(m->max_row_len, m->hp.row_ptr[r + 1] - m->hp.row_ptr[r]);
     m->coords.alloc(n_nodes * 3);
     m->coords.upload(coords, n_nodes * 3, s);
     m->conn_ptr.alloc(n_elem + 1);
@@ -281,12 +284,13 @@ InstDesc system_desc(svx_system* sys) {
     return d;
 }
 
-InstScal initial_scal(double tol, int64_t max_iter, double ru_delta) {
+InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len) {
     InstScal sc{};
     sc.tol = tol;
     sc.max_iter = max_iter;
     sc.status = ST_RUNNING;
     sc.ru_delta = ru_delta;
+    sc.floor_scale = std::sqrt((double)std::max<int64_t>(1, max_row_len)) * std::numeric_limits<double>::epsilon();
     return sc;
 }
 
@@ -301,7 +305,7 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     svx_ctx* ctx = m->ctx;
     hipStream_t s = ctx->stream;
     system_prepare(sys, precision);
-    InstScal sc = initial_scal(tol, max_iter, tp.ru_delta);
+    InstScal sc = initial_scal(tol, max_iter, tp.ru_delta, m->max_row_len);
     sys->scal.upload(&sc, 1, s);
     std::vector<InstDesc> descs{system_desc(sys)};
     if (!sys->engine) sys->engine = std::make_unique<Engine>(s);
@@ -555,8 +559,11 @@ int svx_csr_create(svx_ctx* ctx, int64_t n, const int32_t* row_ptr, const int32_
     *out = nullptr;
     return guarded(nullptr, [&] {
         if (row_ptr[0] != 0) throw std::invalid_argument("row_ptr[0] must be 0");
-        for (int64_t r = 0; r < n; ++r)
+        int64_t mrl = 0;
+        for (int64_t r = 0; r < n; ++r) {
             if (row_ptr[r + 1] < row_ptr[r]) throw std::invalid_argument("row_ptr must be non-decreasing");
+            mrl = std::max<int64_t>(mrl, row_ptr[r + 1] - row_ptr[r]);
+        }
         const int64_t nnz = row_ptr[n];
         for (int64_t k = 0; k < nnz; ++k)
             if (col_idx[k] < 0 || col_idx[k] >= n) throw std::invalid_argument("column index out of range");
@@ -565,6 +572,7 @@ int svx_csr_create(svx_ctx* ctx, int64_t n, const int32_t* row_ptr, const int32_
         c->ctx = ctx;
         c->n = n;
         c->nnz = nnz;
+        c->max_row_len = mrl;
         hipStream_t s = ctx->stream;
         c->row_ptr.alloc(n + 1);
         c->row_ptr.upload(row_ptr, n + 1, s);
@@ -634,7 +642,7 @@ int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int
             d.x32 = x32.p, d.r32 = r32.p, d.p32 = p32.p, d.z32 = z32.p, d.q32 = q32.p;
         }
         DevBuf<InstScal> scal(1);
-        InstScal sc = initial_scal(tol, mi, ctx->tune.ru_delta);
+        InstScal sc = initial_scal(tol, mi, ctx->tune.ru_delta, csr->max_row_len);
         scal.upload(&sc, 1, s);
         d.s = scal.p;
         std::vector<InstDesc> descs{d};

@@ -35,7 +35,7 @@ struct svx_ctx {
 struct svx_mesh {
     svx_ctx* ctx = nullptr;
     int ndof = 0;
-    int64_t n_nodes = 0, n_elem = 0, n = 0, nnz = 0, nblk = 0;
+    int64_t n_nodes = 0, n_elem = 0, n = 0, nnz = 0, nblk = 0, max_row_len = 0;
     svx::HostPattern hp;
     std::vector<int64_t> h_conn_ptr;
     std::vector<int32_t> h_conn;
@@ -78,7 +78,7 @@ struct svx_system {
 
 struct svx_csr {
     svx_ctx* ctx = nullptr;
-    int64_t n = 0, nnz = 0;
+    int64_t n = 0, nnz = 0, max_row_len = 0;
     svx::DevBuf<int32_t> row_ptr, col;
     svx::DevBuf<double> vals;
 };
@@ -111,7 +111,7 @@ void system_set_f(svx_system* sys, const double* f);
 void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows);
 void system_prepare(svx_system* sys, int precision);
 InstDesc system_desc(svx_system* sys);
-InstScal initial_scal(double tol, int64_t max_iter, double ru_delta);
+InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len);
 void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,
                   const TuneParams& tp);
 void system_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n_rows, const int64_t* rows,

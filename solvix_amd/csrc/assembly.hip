@@ -81,7 +81,7 @@ __device__ __forceinline__ double quad_b(const double (*x)[2], double xi, double
         j11 += deta[i] * x[i][1];
     }
     double detj = j00 * j11 - j01 * j10;
-    if (!(detj > 0.0)) return detj;
+    if (detj <= 0.0) return detj;
     double inv = 1.0 / detj;
 #pragma unroll
     for (int i = 0; i < 24; ++i) bm[i] = 0.0;
@@ -164,10 +164,10 @@ __global__ void ke_cps4_kernel(int64_t n_sel, const int32_t* __restrict__ sel, c
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) {
             double detj = quad_b(x, pts[a], pts[b], bm);
-            if (!(detj > 0.0)) return record_bad(reason, first_bad, e, SVX_ELEM_NONPOSITIVE_JACOBIAN);
+            if (detj <= 0.0) return record_bad(reason, first_bad, e, SVX_ELEM_NONPOSITIVE_JACOBIAN);
             btdb3<8>(ke, bm, d, t * detj);
         }
-    if (!(quad_b(x, 0.0, 0.0, bm) > 0.0)) return record_bad(reason, first_bad, e, SVX_ELEM_NONPOSITIVE_JACOBIAN);
+    if (quad_b(x, 0.0, 0.0, bm) <= 0.0) return record_bad(reason, first_bad, e, SVX_ELEM_NONPOSITIVE_JACOBIAN);
     double* out = ke_all + ke_off[e];
 #pragma unroll
     for (int i = 0; i < 64; ++i) out[i] = ke[i];
@@ -176,8 +176,13 @@ __global__ void ke_cps4_kernel(int64_t n_sel, const int32_t* __restrict__ sel, c
 
 constexpr int kHexPerBlock = 8;  // 8 elements x 24 lanes = 192 threads
 
-// Jacobian of the trilinear map at (xi, eta, zeta) and its adjugate inverse
-// (element.rs:440-521). Returns det J.
+// Jacobian of the trilinear map at (xi, eta, zeta) and its inverse as the
+// reference computes it (element.rs:440-521): out[i][j] = cof(j, i) / det
+// with cof(r, c) = (-1)^(r+c) times the cyclic-index minor. The cyclic minor
+// already carries the cofactor sign, so the four odd-parity entries come out
+// negated; for diagonal Jacobians (axis-aligned bricks) they vanish. The
+// device reproduces the reference bit for bit, sign pattern included.
+// Returns det J.
 __device__ __forceinline__ double hex_jac(const double (*x)[3], double xi, double eta, double zeta,
                                           double dn[3][8], double inv[3][3]) {
     const double ca[8] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0};
@@ -199,7 +204,7 @@ __device__ __forceinline__ double hex_jac(const double (*x)[3], double xi, doubl
     double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
                  m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
                  m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
-    if (!(det > 0.0)) return det;
+    if (det <= 0.0) return det;
     double id = 1.0 / det;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(kHexPerBlock * 24) void ke_c3d8_kernel(
         if (active) {
             double dn[3][8], inv[3][3];
             w = hex_jac(x, xi, eta, zeta, dn, inv);
-            if (!(w > 0.0)) {
+            if (w <= 0.0) {
                 active = false;  // every lane of the element sees the same det
                 if (a == 0) {
                     record_bad(reason, first_bad, e, SVX_ELEM_NONPOSITIVE_JACOBIAN);
@@ -325,7 +330,7 @@ __global__ __launch_bounds__(kHexPerBlock * 24) void ke_c3d8_kernel(
     }
     if (active) {
         double dn[3][8], inv[3][3];
-        if (!(hex_jac(x, 0.0, 0.0, 0.0, dn, inv) > 0.0)) {
+        if (hex_jac(x, 0.0, 0.0, 0.0, dn, inv) <= 0.0) {
             if (a == 0) record_bad(reason, first_bad, e, SVX_ELEM_NONPOSITIVE_JACOBIAN);
             return;
         }
@@ -417,7 +422,9 @@ __global__ void diag_inv_kernel(int64_t n, const int32_t* __restrict__ row_ptr, 
 }
 
 // FP32 node-block copy of a CSR matrix whose rows hold full bs x bs node
-// blocks (or a lone diagonal for a node outside every element).
+// blocks (or a lone diagonal for a node outside every element). Values are
+// stored component-major, bval[(a*bs + b) * nblk + block], so that lanes
+// walking consecutive blocks of a block row read consecutive words.
 __global__ void fill_bsr_kernel(int64_t n, int bs, const int32_t* __restrict__ row_ptr,
                                 const int32_t* __restrict__ col, const double* __restrict__ vals,
                                 const int32_t* __restrict__ brow_ptr, float* __restrict__ bval) {
@@ -425,15 +432,16 @@ __global__ void fill_bsr_kernel(int64_t n, int bs, const int32_t* __restrict__ r
     if (r >= n) return;
     int64_t node = r / bs;
     int a = (int)(r % bs);
+    const int64_t nblk = brow_ptr[n / bs];
     int32_t rb = row_ptr[r], re = row_ptr[r + 1];
-    int64_t base = (int64_t)brow_ptr[node] * bs * bs;
-    if (re - rb < bs) {  // isolated node: a 1-block row holding the diagonal
-        for (int b = 0; b < bs; ++b) bval[base + a * bs + b] = 0.0f;
+    int64_t first = brow_ptr[node];
+    if (bs > 1 && re - rb == 1) {  // isolated node: one block holding the diagonal
+        for (int b = 0; b < bs; ++b) bval[(int64_t)(a * bs + b) * nblk + first] = 0.0f;
     }
     for (int32_t k = rb; k < re; ++k) {
         int32_t nb = (k - rb) / bs;
         int b = col[k] % bs;
-        bval[base + (int64_t)nb * bs * bs + a * bs + b] = (float)vals[k];
+        bval[(int64_t)(a * bs + b) * nblk + first + nb] = (float)vals[k];
     }
 }
 

@@ -36,7 +36,7 @@ void Engine::build(std::vector<InstDesc>& descs, const std::vector<int>& rows_pe
     tiles_.upload(tiles.data(), tiles.size(), stream_);
     descs_.alloc(descs.size());
     descs_.upload(descs.data(), descs.size(), stream_);
-    partials_.alloc(2 * tiles.size());
+    partials_.alloc(4 * tiles.size());
     ks_.tiles = tiles_.p;
     ks_.n_tiles = (int32_t)tiles.size();
     ks_.descs = descs_.p;
@@ -53,7 +53,7 @@ static void enqueue_chunk(const KrylovSet& k, const EngineOptions& opt, int iter
         for (int i = 0; i < iters; ++i) cg64_iteration(k, s);
     } else {
         for (int i = 0; i < iters; i += opt.period) {
-            for (int j = 0; j < opt.period; ++j) ir32_iteration(k, s);
+            for (int j = 0; j < opt.period; ++j) ir32_iteration(k, s, j == opt.period - 1);
             ir32_refresh(k, s);
         }
     }
@@ -75,8 +75,15 @@ void Engine::run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag) {
         if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             graph_ok = false;
         } else {
-            enqueue_chunk(k, opt, chunk, stream_);
-            if (hipStreamEndCapture(stream_, &g) != hipSuccess || g == nullptr) {
+            bool enqueued = true;
+            try {
+                enqueue_chunk(k, opt, chunk, stream_);
+            } catch (const DeviceError&) {
+                enqueued = false;
+            }
+            if (hipStreamEndCapture(stream_, &g) != hipSuccess || g == nullptr || !enqueued) {
+                if (g) (void)hipGraphDestroy(g);
+                g = nullptr;
                 graph_ok = false;
             } else {
                 if (hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0) != hipSuccess) {

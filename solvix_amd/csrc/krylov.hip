@@ -28,30 +28,37 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Deterministic block sum of two values; result valid in thread 0.
-__device__ __forceinline__ void block_sum2(double& a, double& b) {
-    __shared__ double sa[kBlock / kWave], sb[kBlock / kWave];
+// Deterministic block sum of three values; results valid in thread 0.
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
+    __shared__ double sa[kBlock / kWave], sb[kBlock / kWave], sc[kBlock / kWave];
     a = wave_sum(a);
     b = wave_sum(b);
+    c = wave_sum(c);
     int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    if (lane == 0) sa[w] = a, sb[w] = b;
+    if (lane == 0) sa[w] = a, sb[w] = b, sc[w] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
-        a = sa[0], b = sb[0];
+        a = sa[0], b = sb[0], c = sc[0];
 #pragma unroll
-        for (int i = 1; i < kBlock / kWave; ++i) a += sa[i], b += sb[i];
+        for (int i = 1; i < kBlock / kWave; ++i) a += sa[i], b += sb[i], c += sc[i];
     }
     __syncthreads();
 }
 
-// Publish this tile's two partials; returns true in every thread of the last
-// tile of the instance to arrive, which then sees all partials.
-__device__ __forceinline__ bool publish_partials(double a, double b, double* partials, int tile, uint32_t* counter,
-                                                 int tile_count) {
+__device__ __forceinline__ void block_sum2(double& a, double& b) {
+    double c = 0.0;
+    block_sum3(a, b, c);
+}
+
+// Publish this tile's partials (4 slots per tile); returns true in every
+// thread of the last tile of the instance to arrive, which then sees all.
+__device__ __forceinline__ bool publish_partials(double a, double b, double c, double* partials, int tile,
+                                                 uint32_t* counter, int tile_count) {
     __shared__ int s_last;
     if (threadIdx.x == 0) {
-        partials[2 * tile] = a;
-        partials[2 * tile + 1] = b;
+        partials[4 * tile] = a;
+        partials[4 * tile + 1] = b;
+        partials[4 * tile + 2] = c;
         __threadfence();
         uint32_t prev = atomicAdd(counter, 1u);
         s_last = (prev == (uint32_t)(tile_count - 1));
@@ -61,16 +68,26 @@ __device__ __forceinline__ bool publish_partials(double a, double b, double* par
     if (last) __threadfence();
     return last;
 }
+__device__ __forceinline__ bool publish_partials(double a, double b, double* partials, int tile, uint32_t* counter,
+                                                 int tile_count) {
+    return publish_partials(a, b, 0.0, partials, tile, counter, tile_count);
+}
 
 // Sum of the instance's partials in tile order (strided, then block tree).
 __device__ __forceinline__ void reduce_partials(const double* partials, int tile_begin, int tile_count, double& a,
-                                                double& b) {
-    a = 0.0, b = 0.0;
+                                                double& b, double& c) {
+    a = 0.0, b = 0.0, c = 0.0;
     for (int k = threadIdx.x; k < tile_count; k += kBlock) {
-        a += partials[2 * (tile_begin + k)];
-        b += partials[2 * (tile_begin + k) + 1];
+        a += partials[4 * (tile_begin + k)];
+        b += partials[4 * (tile_begin + k) + 1];
+        c += partials[4 * (tile_begin + k) + 2];
     }
-    block_sum2(a, b);
+    block_sum3(a, b, c);
+}
+__device__ __forceinline__ void reduce_partials(const double* partials, int tile_begin, int tile_count, double& a,
+                                                double& b) {
+    double c;
+    reduce_partials(partials, tile_begin, tile_count, a, b, c);
 }
 
 template <int LANES>
@@ -269,6 +286,16 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
 }
 
 // ------------------------------------------------------------ FP32-IR
+// Reliable-update refinement (residual replacement): the FP32 loop runs the
+// PCG recurrence; every `period` iterations, an instance whose recursive
+// residual has fallen below ru_delta times its largest value since the last
+// replacement (or below tol) folds its FP32 correction into the FP64 u,
+// recomputes r = f - K u in FP64, and continues with p = z + beta p where z
+// and beta come from the replaced residual. The solve ends when the FP64
+// residual satisfies ||f - K u|| <= tol ||f|| + floor_scale || |K||u| + |f| ||,
+// i.e. reaches tol or the rounding level at which FP64 can still resolve
+// f - K u (the reference's own solutions sit at that level on ill-conditioned
+// meshes).
 enum : int32_t { FL_HOLD = 2 };
 
 __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restrict__ tiles,
@@ -299,10 +326,12 @@ __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restric
         S->bnorm = sqrt(ff);
         S->rr = ff;
         S->rz = rz;
+        S->rz_prev = rz;
         S->rmax = S->bnorm;
         S->it = 0;
         S->refreshes = 0;
         S->flags = 0;
+        S->floor = 0.0;
         if (S->bnorm == 0.0) {
             S->res = 0.0;
             S->status = ST_CONVERGED;
@@ -379,11 +408,10 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
         S->rr = rr;
         S->it += 1;
         double est = sqrt(rr);
+        S->rmax = fmax(S->rmax, est);
         S->beta = rz / S->rz;
+        S->rz_prev = S->rz;
         S->rz = rz;
-        // Only the FP64 residual may end the solve: an estimate below tol or
-        // below ru_delta * rmax requests a replacement at the next refresh;
-        // iterations continue meanwhile. The budget holds the instance.
         if (S->it >= S->max_iter) {
             S->flags |= FL_REFRESH | FL_HOLD;
         } else if (est / S->bnorm <= S->tol || est <= S->ru_delta * S->rmax) {
@@ -392,12 +420,15 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
     }
 }
 
+// p = z + beta p. At the last iteration of a period an instance flagged for
+// refresh skips this: its direction is rebuilt from the replaced residual.
 __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __restrict__ tiles,
-                                                              const InstDesc* __restrict__ descs) {
+                                                              const InstDesc* __restrict__ descs, int period_end) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & FL_HOLD)) return;
+    if (period_end && (S->flags & FL_REFRESH)) return;
     const float beta = (float)S->beta;
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + beta * D.p32[i];
 }
@@ -415,7 +446,8 @@ __global__ __launch_bounds__(kBlock) void ir32_fold_kernel(const Tile* __restric
     }
 }
 
-// Refresh step 2: true residual r = f - K u in FP64, replace r32/z32, test.
+// Refresh step 2: r = f - K u in FP64 with its evaluation floor, replace
+// r32 / z32, test convergence, set beta from the replaced residual.
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __restrict__ tiles,
                                                                const InstDesc* __restrict__ descs,
@@ -426,37 +458,75 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
-    double rr = 0.0, rz = 0.0;
+    double rr = 0.0, rz = 0.0, fl = 0.0;
     for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
-        double ku = csr_row<LANES>(D.row_ptr, D.col, D.val, D.u, row, lane);
+        double ku = 0.0, aku = 0.0;
+        for (int32_t k = D.row_ptr[row] + lane; k < D.row_ptr[row + 1]; k += LANES) {
+            double v = D.val[k] * D.u[D.col[k]];
+            ku += v;
+            aku += fabs(v);
+        }
+#pragma unroll
+        for (int off = LANES / 2; off > 0; off >>= 1) {
+            ku += __shfl_xor(ku, off, LANES);
+            aku += __shfl_xor(aku, off, LANES);
+        }
         if (lane == 0) {
-            double ri = D.f[row] - ku;
+            double fi = D.f[row];
+            double ri = fi - ku;
             float r32 = (float)ri;
             float z32 = r32 * D.dinv32[row];
             D.r32[row] = r32;
             D.z32[row] = z32;
             rr += ri * ri;
             rz += (double)r32 * (double)z32;
+            double a = aku + fabs(fi);
+            fl += a * a;
         }
     }
-    block_sum2(rr, rz);
-    if (!publish_partials(rr, rz, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, rr, rz);
+    block_sum3(rr, rz, fl);
+    if (!publish_partials(rr, rz, fl, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
+    reduce_partials(partials, D.tile_begin, D.tile_count, rr, rz, fl);
     if (threadIdx.x == 0) {
         S->cnt_a = 0;
         double res = sqrt(rr) / S->bnorm;
+        double floor = S->floor_scale * sqrt(fl) / S->bnorm;
         S->res = res;
+        S->floor = floor;
         S->rr = rr;
         S->refreshes += 1;
-        if (res <= S->tol) {
+        if (res <= S->tol + floor) {
             S->status = ST_CONVERGED;
         } else if (S->flags & FL_HOLD) {
             S->status = ST_MAXITER;
         } else {
             S->rmax = sqrt(rr);
+            S->beta = rz / S->rz_prev;
             S->rz = rz;
-            S->flags = 0;
         }
+    }
+}
+
+// Refresh step 3: p = z + beta p for the refreshed instances; the last tile
+// of each clears its refresh flag.
+__global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __restrict__ tiles,
+                                                              const InstDesc* __restrict__ descs) {
+    const Tile t = tiles[blockIdx.x];
+    const InstDesc& D = descs[t.inst];
+    InstScal* S = D.s;
+    if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
+    const float beta = (float)S->beta;
+    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + beta * D.p32[i];
+    __syncthreads();
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(&S->cnt_c, 1u) == (uint32_t)(D.tile_count - 1);
+    }
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+        S->cnt_c = 0;
+        S->flags = 0;
     }
 }
 
@@ -538,7 +608,7 @@ void ir32_init(const KrylovSet& k, hipStream_t s) {
     SVX_HIP(hipGetLastError());
 }
 
-void ir32_iteration(const KrylovSet& k, hipStream_t s) {
+void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end) {
     if (k.n_tiles <= 0) return;
     if (k.bs == 3) {
 #define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
@@ -554,7 +624,7 @@ void ir32_iteration(const KrylovSet& k, hipStream_t s) {
 #undef L
     }
     ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
-    ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
+    ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, period_end ? 1 : 0);
     SVX_HIP(hipGetLastError());
 }
 
@@ -564,6 +634,7 @@ void ir32_refresh(const KrylovSet& k, hipStream_t s) {
 #define L(LN) ir32_residual_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
     SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
+    ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
     SVX_HIP(hipGetLastError());
 }
 

@@ -67,7 +67,8 @@ __device__ __forceinline__ void quad_strain(const double (*x)[2], const double* 
     }
 }
 
-// Strain of the trilinear hex at a natural point (hex_b, element.rs:479-521).
+// Strain of the trilinear hex at a natural point (hex_b, element.rs:479-521),
+// with the reference's inverse (sign pattern of element.rs:447-462).
 __device__ __forceinline__ void hex_strain(const double (*x)[3], const double* ue, double xi, double eta, double zeta,
                                            double e[6]) {
     const double ca[8] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0};

@@ -86,7 +86,12 @@ struct alignas(16) InstScal {
     uint32_t cnt_a;  // last-block-done counters (reset by the last block)
     uint32_t cnt_b;
     int64_t refreshes;  // residual replacements done (FP32-IR)
-    double ru_delta;    // replacement threshold: ||r|| < ru_delta * rmax
+    double ru_delta;    // replacement threshold: ||r|| < ru_delta * max ||r|| since the last one
+    double rz_prev;     // r . z before the latest update (FP32-IR refresh beta)
+    double floor_scale; // sqrt(max row length) * eps64: FP64 evaluation floor of f - K u
+    double floor;       // floor_scale * || |K||u| + |f| || / ||f|| at the latest refresh
+    uint32_t cnt_c;
+    uint32_t pad_;
 };
 
 // Device pointers of one instance. FP64 CSR always; FP32 block-sparse
@@ -183,7 +188,7 @@ struct KrylovSet {
     int32_t n_tiles = 0;
     const InstDesc* descs = nullptr;
     int32_t n_inst = 0;
-    double* partials = nullptr;  // [2 * n_tiles]
+    double* partials = nullptr;  // [4 * n_tiles]
     int lanes = 16;              // CSR lanes per row (FP64 SpMV)
     int blanes = 8;              // BSR lanes per block row (FP32 SpMV)
     int bs = 0;                  // common BSR block size of the set (2 or 3), 0 = mixed/none
@@ -191,7 +196,7 @@ struct KrylovSet {
 void cg64_init(const KrylovSet& k, hipStream_t s);
 void cg64_iteration(const KrylovSet& k, hipStream_t s);
 void ir32_init(const KrylovSet& k, hipStream_t s);
-void ir32_iteration(const KrylovSet& k, hipStream_t s);
+void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end);
 void ir32_refresh(const KrylovSet& k, hipStream_t s);
 void launch_count_running(const InstDesc* descs, int32_t n_inst, int32_t* out, hipStream_t s);
 

@@ -46,3 +46,14 @@ def small_models():
     out.append(decks.cfg2(nx=16, ny=3, nz=3))
     out.append(decks.solid_instance(decks.SplitMix64(5), 3e3, 5e3, name="blk"))
     return [x for x in out if x is not None]
+
+
+def eval_floor(row_ptr, col_idx, vals, f, u):
+    """sqrt(m) * eps * || |K||u| + |f| || / ||f||: the rounding level of an FP64
+    evaluation of f - K u (m = longest row). The reference's own solutions of
+    ill-conditioned meshes have true residuals at this level."""
+    import scipy.sparse as sp
+    n = row_ptr.size - 1
+    A = sp.csr_matrix((np.abs(vals), col_idx, row_ptr), shape=(n, n))
+    m = np.diff(row_ptr).max()
+    return float(np.sqrt(m) * np.finfo(float).eps * np.linalg.norm(A @ np.abs(u) + np.abs(f)) / np.linalg.norm(f))

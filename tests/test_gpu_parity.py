@@ -15,7 +15,7 @@ from solvix_amd.core import CsrMatrix, Mesh, assemble, constrained_dofs, solve_w
 from solvix_amd.inp import read_inp
 from solvix_amd.model import Method, Precision, SolveError
 
-from helpers import dirichlet, rel_err, small_models
+from helpers import dirichlet, eval_floor, rel_err, small_models
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -67,8 +67,12 @@ def test_raw_pcg_matches_oracle(model, precision, ctx):
     A = CsrMatrix(r.row_ptr, r.col_idx, v, ctx)
     u, its, res = A.pcg(f, 1e-10, 0, precision)
     e = rel_err(u, r.u)
-    print(f"{model.name} [{precision}]: iterations {its} (oracle {r.iterations}), residual {res:.3e}, rel err u {e:.2e}")
-    assert res <= 1e-10
+    fl = eval_floor(r.row_ptr, r.col_idx, v, f, u)
+    print(f"{model.name} [{precision}]: iterations {its} (oracle {r.iterations}), residual {res:.3e} "
+          f"(floor {fl:.1e}), rel err u {e:.2e}")
+    # FP64 reports the recursive residual (solver.rs:131); FP32-IR the true
+    # FP64 residual, bounded by tol plus the FP64 evaluation floor.
+    assert res <= 1e-10 + (fl if precision == Precision.FP32_IR else 0.0)
     assert e <= 1e-8
 
 
@@ -83,7 +87,12 @@ def test_solve_model_matches_oracle(model, precision, ctx):
     er = rel_err([x.value for x in s.reactions], r.reaction_value)
     print(f"{model.name} [{precision}]: its {s.iterations} (oracle {r.iterations}) res {s.final_residual:.2e} "
           f"err u {eu:.1e} vm {ev:.1e} nodal {en:.1e} react {er:.1e} cond {s.cond_estimate:.3e}/{r.cond_estimate:.3e}")
-    assert s.final_residual <= 1e-10
+    fl = 0.0
+    if precision == Precision.FP32_IR:
+        v, f = dirichlet(r.row_ptr, r.col_idx, r.vals, r.f, r.reaction_rows)
+        u = s.displacements[:, :model.ndof()].ravel()
+        fl = eval_floor(r.row_ptr, r.col_idx, v, f, u)
+    assert s.final_residual <= 1e-10 + fl
     assert eu <= 1e-8 and ev <= 1e-8 and en <= 1e-8
     assert er <= 1e-6
     assert [x.node_id for x in s.reactions] == list(r.reaction_node)
