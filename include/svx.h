@@ -219,6 +219,9 @@ int svx_system_values(const svx_system* sys, double* vals /* [nnz] */,
 int svx_system_set_rhs(svx_system* sys, const double* f /* [n] */);
 /* solver::apply_dirichlet (solver.rs:45-62): rows may repeat. */
 int svx_apply_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows);
+/* Copy the system's current solution u (after svx_solve or a batched
+ * solve) to the host. */
+int svx_system_solution(const svx_system* sys, double* u /* [n] */);
 /* solver::pcg (solver.rs:66) on the system's (constrained) K and f. */
 int svx_solve(svx_system* sys, double tol, int64_t max_iter, int precision,
               double* u /* [n] or NULL */, int64_t* iterations,
@@ -246,6 +249,42 @@ int svx_csr_spmv(svx_csr* csr, const double* x, double* y, int ordered);
 int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter,
                 int precision, double* u, int64_t* iterations,
                 double* residual, svx_error* err);
+
+/* ---------------------------------------------------------------------------
+ * Batched solves (no reference counterpart: the reference solves one model
+ * per call, lib.rs:586). Systems of any size and element family, assembled
+ * and constrained, are solved together: each PCG phase is one launch over the
+ * row tiles of every running instance, chunks of iterations run as one
+ * captured hipGraph, every instance stops on its own convergence test. Each
+ * instance's result equals its solo solve bit for bit (per-instance
+ * reductions in a fixed order). The batch does not own the systems; after
+ * svx_batch_solve each system holds its solution for svx_recover.
+ * ------------------------------------------------------------------------- */
+int svx_batch_create(svx_ctx* ctx, svx_batch** out);
+void svx_batch_destroy(svx_batch* batch);
+/* tol > 0; max_iter <= 0 selects max(1000, 10 n) (lib.rs:632). */
+int svx_batch_add(svx_batch* batch, svx_system* sys, double tol,
+                  int64_t max_iter, int64_t* slot);
+int64_t svx_batch_size(const svx_batch* batch);
+/* SVX_E_NOT_CONVERGED if any instance failed; see svx_batch_result. */
+int svx_batch_solve(svx_batch* batch, int precision, svx_error* err);
+int svx_batch_result(const svx_batch* batch, int64_t slot,
+                     int64_t* iterations, double* residual,
+                     int32_t* converged);
+/* Average duration and algorithmic bytes of one SpMV phase over the whole
+ * batch (all instances), timed like svx_system_time_spmv. */
+int svx_batch_time_spmv(svx_batch* batch, int reps, double* avg_ms,
+                        double* bytes_per_launch);
+
+/* ---------------------------------------------------------------------------
+ * Measurement hook (no reference counterpart): average duration of the
+ * solver's own SpMV kernel for `precision` on this system's matrix over
+ * `reps` launches, timed with HIP events on the context's stream, and the
+ * algorithmic bytes one launch moves (matrix + indices + one read of x +
+ * one write of y). The system must have been solved with that precision.
+ * ------------------------------------------------------------------------- */
+int svx_system_time_spmv(svx_system* sys, int precision, int reps,
+                         double* avg_ms, double* bytes_per_launch);
 
 #ifdef __cplusplus
 }

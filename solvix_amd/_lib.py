@@ -39,6 +39,7 @@ SIGNATURES = [
     ("svx_system_destroy", None, [_vp]),
     ("svx_system_values", C.c_int, [_vp, _dp, _dp]),
     ("svx_system_set_rhs", C.c_int, [_vp, _dp]),
+    ("svx_system_solution", C.c_int, [_vp, _dp]),
     ("svx_apply_dirichlet", C.c_int, [_vp, _i64, _i64p]),
     ("svx_solve", C.c_int, [_vp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _errp]),
     ("svx_recover", C.c_int, [_vp, _dp, _dp, _i64, _i64p, _dp, _dp]),
@@ -46,6 +47,14 @@ SIGNATURES = [
     ("svx_csr_destroy", None, [_vp]),
     ("svx_csr_spmv", C.c_int, [_vp, _dp, _dp, C.c_int]),
     ("svx_csr_pcg", C.c_int, [_vp, _dp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _errp]),
+    ("svx_system_time_spmv", C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    ("svx_batch_create", C.c_int, [_vp, C.POINTER(_vp)]),
+    ("svx_batch_destroy", None, [_vp]),
+    ("svx_batch_add", C.c_int, [_vp, _vp, C.c_double, _i64, _i64p]),
+    ("svx_batch_size", _i64, [_vp]),
+    ("svx_batch_solve", C.c_int, [_vp, C.c_int, _errp]),
+    ("svx_batch_result", C.c_int, [_vp, _i64, _i64p, _dp, _i32p]),
+    ("svx_batch_time_spmv", C.c_int, [_vp, C.c_int, _dp, _dp]),
 ]
 
 

@@ -246,6 +246,12 @@ class System:
         _raise(lib().svx_system_values(self._h, _d(vals), _d(f)))
         return vals, f
 
+    def solution(self) -> np.ndarray:
+        """The current solution u (after solve() or a batched solve)."""
+        u = np.zeros(self.mesh.n)
+        _raise(lib().svx_system_solution(self._h, _d(u)))
+        return u
+
     def set_rhs(self, f):
         f = np.ascontiguousarray(f, np.float64)
         _raise(lib().svx_system_set_rhs(self._h, _d(f)))
@@ -278,6 +284,13 @@ class System:
                                  C.byref(cond)))
         return vm, vn, re, cond.value
 
+    def time_spmv(self, precision: int, reps: int = 50) -> Tuple[float, float]:
+        """(average ms, algorithmic bytes) of the solver's SpMV kernel."""
+        ms = C.c_double()
+        by = C.c_double()
+        _raise(lib().svx_system_time_spmv(self._h, precision, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
     def close(self):
         if self._h:
             lib().svx_system_destroy(self._h)
@@ -300,6 +313,54 @@ def assemble(model: Model, ctx: Optional[Context] = None) -> Tuple[Mesh, System]
         pos.setdefault(nid, i)
     rows = np.array([pos[int(n)] * ndof + int(d) - 1 for n, d in zip(model.cload_node, model.cload_dof)], np.int64)
     return mesh, mesh.assemble(E, nu, model.thickness, rows, model.cload_value)
+
+
+class Batch:
+    """Many systems solved together in one graph-captured launch sequence
+    (svx_batch_*). Systems must be assembled and constrained."""
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self._h = C.c_void_p()
+        _raise(lib().svx_batch_create(self.ctx.handle, C.byref(self._h)))
+        self.systems: List[System] = []
+
+    def add(self, system: System, tol: float = 1e-10, max_iter: int = 0) -> int:
+        slot = C.c_int64()
+        _raise(lib().svx_batch_add(self._h, system.handle, tol, max_iter, C.byref(slot)))
+        self.systems.append(system)
+        return slot.value
+
+    def solve(self, precision: int = Precision.FP64, raise_on_failure: bool = True):
+        err = _abi.SvxError()
+        rc = lib().svx_batch_solve(self._h, precision, C.byref(err))
+        if rc != _abi.SVX_OK and (raise_on_failure or rc != _abi.SVX_E_NOT_CONVERGED):
+            _raise(rc, err)
+        return [self.result(i) for i in range(len(self.systems))]
+
+    def result(self, slot: int) -> Tuple[int, float, bool]:
+        its = C.c_int64()
+        res = C.c_double()
+        ok = C.c_int32()
+        _raise(lib().svx_batch_result(self._h, slot, C.byref(its), C.byref(res), C.byref(ok)))
+        return its.value, res.value, bool(ok.value)
+
+    def time_spmv(self, reps: int = 20) -> Tuple[float, float]:
+        ms = C.c_double()
+        by = C.c_double()
+        _raise(lib().svx_batch_time_spmv(self._h, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def close(self):
+        if self._h:
+            lib().svx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class CsrMatrix:

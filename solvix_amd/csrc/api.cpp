@@ -289,6 +289,7 @@ InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max
     sc.tol = tol;
     sc.max_iter = max_iter;
     sc.status = ST_RUNNING;
+    sc.flags = FL_FRESH;
     sc.ru_delta = ru_delta;
     sc.floor_scale = std::sqrt((double)std::max<int64_t>(1, max_row_len)) * std::numeric_limits<double>::epsilon();
     return sc;
@@ -309,11 +310,13 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     sys->scal.upload(&sc, 1, s);
     std::vector<InstDesc> descs{system_desc(sys)};
     if (!sys->engine) sys->engine = std::make_unique<Engine>(s);
-    int rpt = tp.rows_per_tile > 0 ? tp.rows_per_tile : default_rows_per_tile(m->ndof);
-    rpt = std::max(m->ndof, (rpt / m->ndof) * m->ndof);
-    int lanes = tp.lanes > 0 ? tp.lanes : default_lanes((double)m->nnz / std::max<int64_t>(1, m->n));
-    int blanes = tp.blanes > 0 ? tp.blanes : default_lanes((double)m->nblk / std::max<int64_t>(1, m->n_nodes));
-    sys->engine->build(descs, {rpt}, lanes, blanes, m->ndof);
+    EngineShape shape;
+    const int bs = m->ndof;
+    shape.rows_per_tile[bs] = tp.rows_per_tile;
+    shape.lanes[bs] = tp.lanes > 0 ? tp.lanes : lanes_for_bs(bs);
+    shape.blanes[bs] = tp.blanes > 0 ? tp.blanes : blanes_for_bs(bs);
+    sys->engine->set_instances(descs, {1}, shape);
+    sys->engine_precision = precision;
     EngineOptions opt;
     opt.precision = precision;
     opt.period = tp.period;
@@ -496,6 +499,16 @@ int svx_system_values(const svx_system* sys, double* vals, double* f) {
     });
 }
 
+int svx_system_solution(const svx_system* sys, double* u) {
+    if (!sys || !u || !sys->has_u) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        hipStream_t s = sys->mesh->ctx->stream;
+        sys->u.download(u, sys->mesh->n, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        return SVX_OK;
+    });
+}
+
 int svx_system_set_rhs(svx_system* sys, const double* f) {
     if (!sys || !f) return SVX_E_ARGUMENT;
     return guarded(nullptr, [&] {
@@ -649,8 +662,11 @@ int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int
         Engine eng(s);
         const TuneParams& tp = ctx->tune;
         double avg = (double)csr->nnz / n;
-        eng.build(descs, {tp.rows_per_tile > 0 ? tp.rows_per_tile : 256}, tp.lanes > 0 ? tp.lanes : default_lanes(avg),
-                  tp.blanes > 0 ? tp.blanes : default_lanes(avg), 1);
+        EngineShape shape;
+        shape.rows_per_tile[1] = tp.rows_per_tile > 0 ? tp.rows_per_tile : 256;
+        shape.lanes[1] = tp.lanes > 0 ? tp.lanes : default_lanes(avg);
+        shape.blanes[1] = tp.blanes > 0 ? tp.blanes : default_lanes(avg);
+        eng.set_instances(descs, {1}, shape);
         EngineOptions opt;
         opt.precision = precision;
         opt.period = tp.period;
@@ -667,6 +683,25 @@ int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int
             snprintf(buf, sizeof buf, "NotConverged { iterations: %lld, residual: %.6e }", (long long)sc.it, sc.res);
             set_err(err, SVX_E_NOT_CONVERGED, buf, 0, -1, sc.it, sc.res);
             return SVX_E_NOT_CONVERGED;
+        }
+        return SVX_OK;
+    });
+}
+
+int svx_system_time_spmv(svx_system* sys, int precision, int reps, double* avg_ms, double* bytes_per_launch) {
+    if (!sys || reps <= 0 || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        if (!sys->engine || sys->engine_precision != precision)
+            throw std::invalid_argument("solve the system with this precision first");
+        svx_mesh* m = sys->mesh;
+        double ms = sys->engine->time_spmv(precision, reps);
+        if (avg_ms) *avg_ms = ms;
+        if (bytes_per_launch) {
+            const double n = (double)m->n, nnz = (double)m->nnz;
+            if (precision == SVX_FP64)  // vals + cols + row_ptr + x (once) + y
+                *bytes_per_launch = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;
+            else  // FP32 block values + block cols + block row_ptr + x (once) + y
+                *bytes_per_launch = 4.0 * nnz + 4.0 * (double)m->nblk + 4.0 * (m->n_nodes + 1) + 8.0 * n;
         }
         return SVX_OK;
     });

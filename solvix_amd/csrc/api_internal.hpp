@@ -71,6 +71,7 @@ struct svx_system {
     svx::DevBuf<float> x32, r32, p32, z32, q32;
     svx::DevBuf<svx::InstScal> scal;
     std::unique_ptr<svx::Engine> engine;
+    int engine_precision = -1;  // precision the engine's descriptors were built for
     double cond = 0.0;
     bool solved = false, has_u = false;
     int64_t last_refreshes = 0;

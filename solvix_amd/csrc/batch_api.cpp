@@ -1,0 +1,164 @@
+// Batched solves: many assembled, constrained systems (uneven sizes, 2D and
+// 3D together) solved by one engine — every phase of an iteration is one
+// launch over the tiles of all running instances, chunks of iterations are
+// captured in one hipGraph, and each instance stops on its own convergence
+// test (per-instance status masks). Results are per instance and identical to
+// solving the instance alone: every reduction is per instance and in a fixed
+// order.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+
+#include "api_internal.hpp"
+
+using namespace svx;
+
+struct svx_batch {
+    svx_ctx* ctx = nullptr;
+    std::vector<svx_system*> sys;
+    std::vector<double> tol;
+    std::vector<int64_t> max_iter;
+    DevBuf<InstScal> scal;
+    std::vector<InstScal> h_scal;
+    std::unique_ptr<Engine> engine;
+    int precision = -1;
+};
+
+namespace svx {
+
+// Launch shape for a set of systems: the per-block-size shape every solve
+// uses, so batched results equal solo ones bit for bit.
+EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp) {
+    (void)systems;
+    EngineShape sh;
+    for (int bs = 1; bs <= 3; ++bs) {
+        sh.rows_per_tile[bs] = tp.rows_per_tile;
+        sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : lanes_for_bs(bs);
+        sh.blanes[bs] = tp.blanes > 0 ? tp.blanes : blanes_for_bs(bs);
+    }
+    return sh;
+}
+
+}  // namespace svx
+
+namespace {
+template <typename F>
+int guarded_b(svx_error* err, F&& fn) {
+    try {
+        return fn();
+    } catch (const DeviceError& e) {
+        if (err) {
+            err->code = SVX_E_DEVICE;
+            snprintf(err->message, sizeof err->message, "%s", e.what());
+        }
+        return SVX_E_DEVICE;
+    } catch (const std::exception& e) {
+        if (err) {
+            err->code = SVX_E_ARGUMENT;
+            snprintf(err->message, sizeof err->message, "%s", e.what());
+        }
+        return SVX_E_ARGUMENT;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int svx_batch_create(svx_ctx* ctx, svx_batch** out) {
+    if (!ctx || !out) return SVX_E_ARGUMENT;
+    auto b = std::make_unique<svx_batch>();
+    b->ctx = ctx;
+    *out = b.release();
+    return SVX_OK;
+}
+
+void svx_batch_destroy(svx_batch* b) { delete b; }
+
+int svx_batch_add(svx_batch* b, svx_system* sys, double tol, int64_t max_iter, int64_t* slot) {
+    if (!b || !sys || !(tol > 0.0) || sys->mesh->ctx != b->ctx) return SVX_E_ARGUMENT;
+    if (!sys->dirichlet) return SVX_E_ARGUMENT;
+    b->sys.push_back(sys);
+    b->tol.push_back(tol);
+    b->max_iter.push_back(max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * sys->mesh->n));
+    if (slot) *slot = (int64_t)b->sys.size() - 1;
+    return SVX_OK;
+}
+
+int64_t svx_batch_size(const svx_batch* b) { return b ? (int64_t)b->sys.size() : -1; }
+
+int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
+    if (err) {
+        std::memset(err, 0, sizeof *err);
+        err->index = -1;
+    }
+    if (!b || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
+    return guarded_b(err, [&]() -> int {
+        hipStream_t s = b->ctx->stream;
+        const size_t n = b->sys.size();
+        if (n == 0) return SVX_OK;
+        for (svx_system* sy : b->sys) system_prepare(sy, precision);
+        b->scal.alloc(n);
+        std::vector<InstScal> init(n);
+        std::vector<InstDesc> descs(n);
+        for (size_t i = 0; i < n; ++i) {
+            svx_system* sy = b->sys[i];
+            init[i] = initial_scal(b->tol[i], b->max_iter[i], b->ctx->tune.ru_delta, sy->mesh->max_row_len);
+            descs[i] = system_desc(sy);
+            descs[i].s = b->scal.p + i;
+        }
+        b->scal.upload(init.data(), n, s);
+        if (!b->engine) b->engine = std::make_unique<Engine>(s);
+        b->engine->set_instances(descs, std::vector<char>(n, 1), batch_shape(b->sys, b->ctx->tune));
+        EngineOptions opt;
+        opt.precision = precision;
+        opt.period = b->ctx->tune.period;
+        opt.chunk = b->ctx->tune.chunk;
+        opt.use_graph = b->ctx->tune.use_graph;
+        b->engine->run(opt, b->ctx->h_flag, b->ctx->d_flag);
+        b->precision = precision;
+        b->h_scal.resize(n);
+        b->scal.download(b->h_scal.data(), n, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        int64_t failed = 0;
+        for (size_t i = 0; i < n; ++i) {
+            b->sys[i]->has_u = true;
+            b->sys[i]->solved = b->h_scal[i].status == ST_CONVERGED;
+            failed += !b->sys[i]->solved;
+        }
+        if (failed && err) {
+            err->code = SVX_E_NOT_CONVERGED;
+            snprintf(err->message, sizeof err->message, "%lld of %zu instances did not converge", (long long)failed, n);
+        }
+        return failed ? SVX_E_NOT_CONVERGED : SVX_OK;
+    });
+}
+
+int svx_batch_result(const svx_batch* b, int64_t slot, int64_t* iterations, double* residual, int32_t* converged) {
+    if (!b || slot < 0 || slot >= (int64_t)b->h_scal.size()) return SVX_E_ARGUMENT;
+    const InstScal& sc = b->h_scal[slot];
+    if (iterations) *iterations = sc.it;
+    if (residual) *residual = sc.res;
+    if (converged) *converged = sc.status == ST_CONVERGED;
+    return SVX_OK;
+}
+
+int svx_batch_time_spmv(svx_batch* b, int reps, double* avg_ms, double* bytes_per_launch) {
+    if (!b || !b->engine || reps <= 0 || b->precision < 0) return SVX_E_ARGUMENT;
+    return guarded_b(nullptr, [&] {
+        double ms = b->engine->time_spmv(b->precision, reps);
+        double bytes = 0.0;
+        for (svx_system* sy : b->sys) {
+            const svx_mesh* m = sy->mesh;
+            if (b->precision == SVX_FP64)
+                bytes += 12.0 * m->nnz + 4.0 * (m->n + 1) + 16.0 * m->n;
+            else
+                bytes += 4.0 * m->nnz + 4.0 * m->nblk + 4.0 * (m->n_nodes + 1) + 8.0 * m->n;
+        }
+        if (avg_ms) *avg_ms = ms;
+        if (bytes_per_launch) *bytes_per_launch = bytes;
+        return SVX_OK;
+    });
+}
+
+}  // extern "C"

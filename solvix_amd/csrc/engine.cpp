@@ -12,6 +12,9 @@ int default_lanes(double avg) {
 
 int default_rows_per_tile(int bs) { return bs == 3 ? 288 : 256; }
 
+int lanes_for_bs(int bs) { return bs == 3 ? 16 : 4; }
+int blanes_for_bs(int bs) { return bs == 3 ? 8 : 2; }
+
 Engine::~Engine() { release_graph(); }
 
 void Engine::release_graph() {
@@ -20,98 +23,158 @@ void Engine::release_graph() {
     exec_precision_ = exec_period_ = exec_chunk_ = -1;
 }
 
-void Engine::build(std::vector<InstDesc>& descs, const std::vector<int>& rows_per_tile, int lanes, int blanes,
-                   int bs) {
+void Engine::set_instances(const std::vector<InstDesc>& descs, const std::vector<char>& running,
+                           const EngineShape& shape) {
+    h_descs_ = descs;
+    shape_ = shape;
+    build_tiles(running);
+}
+
+void Engine::build_tiles(const std::vector<char>& running) {
     release_graph();
     std::vector<Tile> tiles;
-    for (size_t i = 0; i < descs.size(); ++i) {
-        InstDesc& d = descs[i];
-        int rpt = std::max(1, rows_per_tile[i]);
-        d.tile_begin = (int32_t)tiles.size();
-        for (int32_t r = 0; r < d.n; r += rpt) tiles.push_back(Tile{(int32_t)i, r, std::min(d.n, r + rpt), 0});
-        d.tile_count = (int32_t)tiles.size() - d.tile_begin;
-        if (d.tile_count == 0) throw std::runtime_error("instance with no rows");
+    int first[4] = {0, 0, 0, 0}, count[4] = {0, 0, 0, 0};
+    tiled_instances_ = 0;
+    for (int bs = 1; bs <= 3; ++bs) {
+        first[bs] = (int)tiles.size();
+        int rpt = shape_.rows_per_tile[bs] > 0 ? shape_.rows_per_tile[bs] : default_rows_per_tile(bs);
+        rpt = std::max(bs, (rpt / bs) * bs);
+        for (size_t i = 0; i < h_descs_.size(); ++i) {
+            InstDesc& d = h_descs_[i];
+            if (d.bs != bs || !running[i]) continue;
+            d.tile_begin = (int32_t)tiles.size() - first[bs];
+            for (int32_t r = 0; r < d.n; r += rpt) tiles.push_back(Tile{(int32_t)i, r, std::min(d.n, r + rpt), 0});
+            d.tile_count = (int32_t)tiles.size() - first[bs] - d.tile_begin;
+            tiled_instances_++;
+        }
+        count[bs] = (int)tiles.size() - first[bs];
     }
-    tiles_.alloc(tiles.size());
+    tiles_.alloc(std::max<size_t>(1, tiles.size()));
     tiles_.upload(tiles.data(), tiles.size(), stream_);
-    descs_.alloc(descs.size());
-    descs_.upload(descs.data(), descs.size(), stream_);
-    partials_.alloc(4 * tiles.size());
-    ks_.tiles = tiles_.p;
-    ks_.n_tiles = (int32_t)tiles.size();
-    ks_.descs = descs_.p;
-    ks_.n_inst = (int32_t)descs.size();
-    ks_.partials = partials_.p;
-    ks_.lanes = lanes;
-    ks_.blanes = blanes;
-    ks_.bs = bs;
+    descs_.alloc(std::max<size_t>(1, h_descs_.size()));
+    descs_.upload(h_descs_.data(), h_descs_.size(), stream_);
+    partials_.alloc(std::max<size_t>(4, 4 * tiles.size()));
+    for (int bs = 1; bs <= 3; ++bs) {
+        KrylovSet& k = groups_[bs];
+        k.tiles = tiles_.p + first[bs];
+        k.n_tiles = count[bs];
+        k.descs = descs_.p;
+        k.n_inst = (int32_t)h_descs_.size();
+        k.partials = partials_.p + 4 * (size_t)first[bs];
+        k.lanes = shape_.lanes[bs] > 0 ? shape_.lanes[bs] : 16;
+        k.blanes = shape_.blanes[bs] > 0 ? shape_.blanes[bs] : 8;
+        k.bs = bs;
+    }
     SVX_HIP(hipStreamSynchronize(stream_));
 }
 
-static void enqueue_chunk(const KrylovSet& k, const EngineOptions& opt, int iters, hipStream_t s) {
+void Engine::enqueue_init(const EngineOptions& opt) {
+    for (int bs = 1; bs <= 3; ++bs) {
+        if (opt.precision == SVX_FP64)
+            cg64_init(groups_[bs], stream_);
+        else
+            ir32_init(groups_[bs], stream_);
+    }
+}
+
+void Engine::enqueue_chunk(const EngineOptions& opt, int iters) {
     if (opt.precision == SVX_FP64) {
-        for (int i = 0; i < iters; ++i) cg64_iteration(k, s);
+        for (int i = 0; i < iters; ++i)
+            for (int bs = 1; bs <= 3; ++bs) cg64_iteration(groups_[bs], stream_);
     } else {
         for (int i = 0; i < iters; i += opt.period) {
-            for (int j = 0; j < opt.period; ++j) ir32_iteration(k, s, j == opt.period - 1);
-            ir32_refresh(k, s);
+            for (int j = 0; j < opt.period; ++j)
+                for (int bs = 1; bs <= 3; ++bs) ir32_iteration(groups_[bs], stream_, j == opt.period - 1);
+            for (int bs = 1; bs <= 3; ++bs) ir32_refresh(groups_[bs], stream_);
         }
     }
 }
 
-void Engine::run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag) {
-    const KrylovSet& k = ks_;
-    if (opt.precision == SVX_FP64)
-        cg64_init(k, stream_);
-    else
-        ir32_init(k, stream_);
+int Engine::run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
     int chunk = opt.chunk;
     if (opt.precision != SVX_FP64) chunk = std::max(opt.period, (chunk / opt.period) * opt.period);
-    bool graph_ok = opt.use_graph;
-    if (graph_ok && (exec_ == nullptr || exec_precision_ != opt.precision || exec_period_ != opt.period ||
-                     exec_chunk_ != chunk)) {
-        release_graph();
-        hipGraph_t g = nullptr;
-        if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            graph_ok = false;
-        } else {
-            bool enqueued = true;
-            try {
-                enqueue_chunk(k, opt, chunk, stream_);
-            } catch (const DeviceError&) {
-                enqueued = false;
-            }
-            if (hipStreamEndCapture(stream_, &g) != hipSuccess || g == nullptr || !enqueued) {
-                if (g) (void)hipGraphDestroy(g);
-                g = nullptr;
+    int done_chunks = 0;
+    for (;;) {
+        enqueue_init(opt);  // initialises FL_FRESH instances only
+        bool graph_ok = opt.use_graph;
+        if (graph_ok && (exec_ == nullptr || exec_precision_ != opt.precision || exec_period_ != opt.period ||
+                         exec_chunk_ != chunk)) {
+            release_graph();
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                 graph_ok = false;
             } else {
-                if (hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0) != hipSuccess) {
-                    exec_ = nullptr;
-                    graph_ok = false;
+                bool enqueued = true;
+                try {
+                    enqueue_chunk(opt, chunk);
+                } catch (const DeviceError&) {
+                    enqueued = false;
                 }
-                (void)hipGraphDestroy(g);
+                if (hipStreamEndCapture(stream_, &g) != hipSuccess || g == nullptr || !enqueued) {
+                    if (g) (void)hipGraphDestroy(g);
+                    graph_ok = false;
+                } else {
+                    if (hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0) != hipSuccess) {
+                        exec_ = nullptr;
+                        graph_ok = false;
+                    }
+                    (void)hipGraphDestroy(g);
+                }
+            }
+            (void)hipGetLastError();
+            if (graph_ok) {
+                exec_precision_ = opt.precision;
+                exec_period_ = opt.period;
+                exec_chunk_ = chunk;
             }
         }
-        (void)hipGetLastError();
-        if (graph_ok) {
-            exec_precision_ = opt.precision;
-            exec_period_ = opt.period;
-            exec_chunk_ = chunk;
-        }
-    }
-    for (;;) {
-        if (graph_ok) {
+        if (graph_ok)
             SVX_HIP(hipGraphLaunch(exec_, stream_));
-        } else {
-            enqueue_chunk(k, opt, chunk, stream_);
-        }
+        else
+            enqueue_chunk(opt, chunk);
         launches_++;
-        launch_count_running(k.descs, k.n_inst, d_flag, stream_);
+        done_chunks++;
+        launch_count_running(descs_.p, (int32_t)h_descs_.size(), d_flag, stream_);
         SVX_HIP(hipMemcpyAsync(h_flag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
         SVX_HIP(hipStreamSynchronize(stream_));
-        if (*h_flag == 0) break;
+        const int running = *h_flag;
+        if (running == 0) return 0;
+        if (max_chunks >= 0 && done_chunks >= max_chunks) return running;
+        if (opt.compact && 2 * running <= tiled_instances_ && tiled_instances_ > 1) {
+            std::vector<InstScal> sc;
+            read_scalars(sc);
+            std::vector<char> live(h_descs_.size());
+            for (size_t i = 0; i < sc.size(); ++i) live[i] = sc[i].status == ST_RUNNING;
+            build_tiles(live);
+        }
     }
+}
+
+void Engine::read_scalars(std::vector<InstScal>& out) const {
+    out.resize(h_descs_.size());
+    if (h_descs_.empty()) return;
+    DevBuf<InstScal> tmp(h_descs_.size());
+    launch_gather_scalars(descs_.p, (int32_t)h_descs_.size(), tmp.p, stream_);
+    tmp.download(out.data(), out.size(), stream_);
+    SVX_HIP(hipStreamSynchronize(stream_));
+}
+
+double Engine::time_spmv(int precision, int reps) {
+    hipEvent_t a, b;
+    SVX_HIP(hipEventCreate(&a));
+    SVX_HIP(hipEventCreate(&b));
+    launch_set_running(descs_.p, (int32_t)h_descs_.size(), stream_);
+    for (int bs = 1; bs <= 3; ++bs) spmv_phase(groups_[bs], precision, stream_);  // warm
+    SVX_HIP(hipEventRecord(a, stream_));
+    for (int i = 0; i < reps; ++i)
+        for (int bs = 1; bs <= 3; ++bs) spmv_phase(groups_[bs], precision, stream_);
+    SVX_HIP(hipEventRecord(b, stream_));
+    SVX_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    SVX_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return (double)ms / reps;
 }
 
 }  // namespace svx

@@ -1,6 +1,13 @@
 // Krylov engine: runs Jacobi-PCG (FP64 or FP32-IR) over a set of resident
 // instances with hipGraph-captured chunks of iterations and a device-side
 // convergence count polled once per chunk.
+//
+// Instances are grouped by node-block size (1, 2, 3); each group is a tile
+// table launched as one grid per phase, and the groups' launches are captured
+// into the same graph. Tiles of finished instances are dropped from the
+// tables (and the graph re-captured) once at most half of the tiled
+// instances still run. Instances flagged FL_FRESH are initialised by the init
+// phase at the start of the next chunk, so new work can join a running set.
 #pragma once
 #include "svx_internal.hpp"
 
@@ -11,36 +18,56 @@ struct EngineOptions {
     int period = 8;        // FP32-IR: iterations between residual refreshes
     int chunk = 32;        // iterations per captured graph
     bool use_graph = true;
+    bool compact = true;   // drop finished instances' tiles as they converge
+};
+
+struct EngineShape {
+    int lanes[4] = {0, 0, 0, 0};   // CSR lanes per row, per block size
+    int blanes[4] = {0, 0, 0, 0};  // BSR lanes per block row, per block size
+    int rows_per_tile[4] = {0, 0, 0, 0};
 };
 
 class Engine {
    public:
     explicit Engine(hipStream_t s) : stream_(s) {}
     ~Engine();
-    // Host descriptors (pointers already set; tile fields filled by build).
-    // rows_per_tile[i] must be a multiple of the instance's block size.
-    void build(std::vector<InstDesc>& descs, const std::vector<int>& rows_per_tile, int lanes, int blanes, int bs);
-    // Run until every instance leaves ST_RUNNING. Scalars must be initialised
-    // (status = ST_RUNNING, tol, max_iter, ru_delta) by the caller.
-    void run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag);
-    const KrylovSet& set() const { return ks_; }
+    // Replace the instance set. Descriptors must carry valid pointers for the
+    // precision that will run; tile fields are filled here. `running[i]`
+    // false leaves instance i out of the tile tables.
+    void set_instances(const std::vector<InstDesc>& descs, const std::vector<char>& running, const EngineShape& shape);
+    // Run chunks until no instance runs (max_chunks < 0) or max_chunks
+    // chunks ran. Returns the number of instances still running.
+    int run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks = -1);
+    // Download the per-instance scalars (status, iterations, residual).
+    void read_scalars(std::vector<InstScal>& out) const;
+    double time_spmv(int precision, int reps);
     int64_t launches() const { return launches_; }
+    int n_instances() const { return (int)h_descs_.size(); }
 
    private:
     void release_graph();
+    void build_tiles(const std::vector<char>& running);
+    void enqueue_init(const EngineOptions& opt);
+    void enqueue_chunk(const EngineOptions& opt, int iters);
     hipStream_t stream_;
+    std::vector<InstDesc> h_descs_;
+    EngineShape shape_;
     DevBuf<Tile> tiles_;
     DevBuf<InstDesc> descs_;
     DevBuf<double> partials_;
-    KrylovSet ks_{};
+    KrylovSet groups_[4]{};
+    int tiled_instances_ = 0;
     hipGraphExec_t exec_ = nullptr;
     int exec_precision_ = -1, exec_period_ = -1, exec_chunk_ = -1;
     int64_t launches_ = 0;
 };
 
-// Pick lanes-per-row for a CSR matrix / block-row for a BSR matrix from the
-// average row length (the autotuner overrides these when it has data).
 int default_lanes(double avg_row_len);
 int default_rows_per_tile(int bs);
+// Lanes per row (CSR) and per block row (BSR) for node-block size bs (2, 3);
+// a function of bs alone, so an instance gets the same launch shape — and the
+// same bits — alone and inside any batch.
+int lanes_for_bs(int bs);
+int blanes_for_bs(int bs);
 
 }  // namespace svx

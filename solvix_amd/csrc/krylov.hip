@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING) return;
+    if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
     double ff = 0.0, rz = 0.0;
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
         double fi = D.f[i];
@@ -195,6 +195,7 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
         S->rr = ff;
         S->rz = rz;
         S->it = 0;
+        S->flags = 0;
         if (S->bnorm == 0.0) {  // solver.rs:93-95
             S->res = 0.0;
             S->status = ST_CONVERGED;
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING) return;
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
     double pq = 0.0;
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING) return;
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const double alpha = S->alpha;
     double rr = 0.0, rz = 0.0;
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING) return;
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const double beta = S->beta;
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
 }
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restric
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING) return;
+    if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
     double ff = 0.0, rz = 0.0;
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
         double fi = D.f[i];
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & FL_HOLD)) return;
+    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
     const int32_t nblk = D.brow_ptr[D.n / BS];
@@ -387,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & FL_HOLD)) return;
+    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const float alpha = (float)S->alpha;
     double rr = 0.0, rz = 0.0;
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
@@ -427,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __rest
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & FL_HOLD)) return;
+    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     if (period_end && (S->flags & FL_REFRESH)) return;
     const float beta = (float)S->beta;
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + beta * D.p32[i];
@@ -541,6 +542,18 @@ __global__ void count_running_kernel(const InstDesc* __restrict__ descs, int32_t
     if (threadIdx.x == 0) *out = cnt;
 }
 
+__global__ void gather_scalars_kernel(const InstDesc* __restrict__ descs, int32_t n_inst, InstScal* out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) out[i] = *descs[i].s;
+}
+
+__global__ void set_running_kernel(const InstDesc* __restrict__ descs, int32_t n_inst) {
+    for (int32_t i = threadIdx.x; i < n_inst; i += blockDim.x) {
+        descs[i].s->status = ST_RUNNING;
+        descs[i].s->flags = 0;
+    }
+}
+
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 #define SVX_LANES_DISPATCH(lanes, MACRO) \
@@ -635,6 +648,39 @@ void ir32_refresh(const KrylovSet& k, hipStream_t s) {
     SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
+    SVX_HIP(hipGetLastError());
+}
+
+void launch_gather_scalars(const InstDesc* descs, int32_t n_inst, InstScal* out, hipStream_t s) {
+    if (n_inst <= 0) return;
+    gather_scalars_kernel<<<grid_for(n_inst, kBlock), kBlock, 0, s>>>(descs, n_inst, out);
+    SVX_HIP(hipGetLastError());
+}
+
+void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s) {
+    set_running_kernel<<<1, kBlock, 0, s>>>(descs, n_inst);
+    SVX_HIP(hipGetLastError());
+}
+
+void spmv_phase(const KrylovSet& k, int precision, hipStream_t s) {
+    if (k.n_tiles <= 0) return;
+    if (precision == SVX_FP64) {
+#define L(LN) cg64_spmv_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+        SVX_LANES_DISPATCH(k.lanes, L)
+#undef L
+    } else if (k.bs == 3) {
+#define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+        SVX_LANES_DISPATCH(k.blanes, L)
+#undef L
+    } else if (k.bs == 2) {
+#define L(LN) ir32_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+        SVX_LANES_DISPATCH(k.blanes, L)
+#undef L
+    } else {
+#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+        SVX_LANES_DISPATCH(k.blanes, L)
+#undef L
+    }
     SVX_HIP(hipGetLastError());
 }
 

@@ -65,8 +65,8 @@ struct DevBuf {
 // ------------------------------------------------------- Krylov instances
 // Instance status values written by the device.
 enum : int32_t { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2, ST_MAXITER = 3, ST_IDLE = 4 };
-// Instance flags.
-enum : int32_t { FL_REFRESH = 1 };
+// Instance flags. FL_FRESH marks work the next init phase must initialise.
+enum : int32_t { FL_REFRESH = 1, FL_FRESH = 4 };
 
 // Per-instance scalar state of a (preconditioned) CG solve, device resident.
 struct alignas(16) InstScal {
@@ -199,5 +199,11 @@ void ir32_init(const KrylovSet& k, hipStream_t s);
 void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end);
 void ir32_refresh(const KrylovSet& k, hipStream_t s);
 void launch_count_running(const InstDesc* descs, int32_t n_inst, int32_t* out, hipStream_t s);
+// The SpMV phase of one iteration alone (cg64 / ir32), for timing.
+void spmv_phase(const KrylovSet& k, int precision, hipStream_t s);
+// Set every instance of the set back to ST_RUNNING (timing helper).
+void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s);
+// Copy every instance's scalars into a contiguous array.
+void launch_gather_scalars(const InstDesc* descs, int32_t n_inst, InstScal* out, hipStream_t s);
 
 }  // namespace svx

@@ -1,0 +1,63 @@
+"""Batched solves on the device: every instance's result equals its solo
+solve bit for bit (alone vs in the mix), for FP64 and FP32-IR, and matches
+the oracle within the north-star tolerance."""
+import numpy as np
+import pytest
+
+import oracle
+from solvix_amd import decks
+from solvix_amd.core import Batch, assemble, constrained_dofs
+from solvix_amd.model import Precision
+
+from helpers import rel_err, small_models
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(model, ctx):
+    mesh, sys_ = assemble(model, ctx)
+    rows, _, _ = constrained_dofs(model)
+    sys_.apply_dirichlet(rows)
+    return mesh, sys_
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+def test_batch_equals_solo_bitwise(precision, ctx):
+    models = small_models() + [decks.plane_instance(decks.SplitMix64(0x53170003 ^ i), 2e3, 2e4) for i in range(6)]
+    solo = []
+    for m in models:
+        mesh, s = _system(m, ctx)
+        u, its, res = s.solve(1e-10, 0, precision)
+        solo.append((u, its, res))
+    keep = []
+    b = Batch(ctx)
+    for m in models:
+        mesh, s = _system(m, ctx)
+        keep.append((mesh, s))
+        b.add(s)
+    results = b.solve(precision)
+    for (mesh, s), (u0, its0, res0), (its, res, ok), m in zip(keep, solo, results, models):
+        assert ok, m.name
+        u = s.solution()
+        print(f"{m.name} [{precision}]: its {its} res {res:.2e} solo its {its0}")
+        assert its == its0 and res == res0, m.name
+        assert np.array_equal(u.view(np.uint64), u0.view(np.uint64)), m.name
+
+
+def test_batch_matches_oracle(ctx):
+    models = [decks.plane_instance(decks.SplitMix64(0x53170003 ^ i), 2e3, 1.5e4) for i in range(8)]
+    models.append(decks.cfg2(nx=12, ny=3, nz=3))
+    b = Batch(ctx)
+    keep = []
+    for m in models:
+        mesh, s = _system(m, ctx)
+        keep.append((mesh, s))
+        b.add(s)
+    for prec in (Precision.FP64, Precision.FP32_IR):
+        res = b.solve(prec)
+        for (mesh, s), m, (its, r, ok) in zip(keep, models, res):
+            ref = oracle.solve(m)
+            vm, vn, _, _ = s.recover([])
+            e = rel_err(vm, ref.von_mises)
+            print(f"{m.name} [{prec}] its {its} (oracle {ref.iterations}) res {r:.2e} err vm {e:.1e}")
+            assert ok and e <= 1e-8
