@@ -277,6 +277,22 @@ int svx_batch_time_spmv(svx_batch* batch, int reps, double* avg_ms,
                         double* bytes_per_launch);
 
 /* ---------------------------------------------------------------------------
+ * Multi-RHS solve (no reference counterpart: the reference solves one load
+ * vector per call). nrhs <= 64 load vectors F[k*n .. k*n+n) on one assembled,
+ * constrained system; constrained rows of every F are zeroed as
+ * apply_dirichlet does (solver.rs:59-61). Each load case follows the
+ * recurrence of solver.rs:82-149 with its own scalars and convergence test;
+ * the matrix is read once per iteration for all of them. U[k*n ..] receives
+ * the solutions; iterations / residuals / converged are per load case;
+ * elapsed_ms is device time from the upload of F to the download of U.
+ * FP64 only (SVX_FP32_IR returns SVX_E_ARGUMENT).
+ * ------------------------------------------------------------------------- */
+int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol,
+                    int64_t max_iter, int precision, double* U,
+                    int64_t* iterations, double* residuals, int32_t* converged,
+                    double* elapsed_ms, svx_error* err);
+
+/* ---------------------------------------------------------------------------
  * Measurement hook (no reference counterpart): average duration of the
  * solver's own SpMV kernel for `precision` on this system's matrix over
  * `reps` launches, timed with HIP events on the context's stream, and the

@@ -24,12 +24,13 @@ SOURCES = [
     ("assembly.hip", ["-ffp-contract=off"]),
     ("recover.hip", ["-ffp-contract=off"]),
     ("krylov.hip", []),
-    ("batch.hip", []),
+    ("block.hip", []),
     ("engine.cpp", []),
     ("pattern.cpp", []),
     ("model.cpp", []),
     ("api.cpp", []),
     ("batch_api.cpp", []),
+    ("block_api.cpp", []),
     ("server.cpp", []),
 ]
 

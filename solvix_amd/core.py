@@ -246,6 +246,24 @@ class System:
         _raise(lib().svx_system_values(self._h, _d(vals), _d(f)))
         return vals, f
 
+    def solve_multi(self, F: np.ndarray, tol: float = 1e-10, max_iter: int = 0,
+                    precision: int = Precision.FP64, raise_on_failure: bool = True):
+        """Solve for every row of F ([nrhs, n], nrhs <= 64) on this system.
+        Returns (U [nrhs, n], iterations, residuals, converged, device ms)."""
+        F = np.ascontiguousarray(F, np.float64).reshape(-1, self.mesh.n)
+        R = F.shape[0]
+        U = np.zeros_like(F)
+        its = np.zeros(R, np.int64)
+        res = np.zeros(R)
+        ok = np.zeros(R, np.int32)
+        ms = C.c_double()
+        err = _abi.SvxError()
+        rc = lib().svx_solve_multi(self._h, R, _d(F), tol, max_iter, precision, _d(U), its.ctypes.data_as(_i64p),
+                                   _d(res), ok.ctypes.data_as(_i32p), C.byref(ms), C.byref(err))
+        if rc != _abi.SVX_OK and (raise_on_failure or rc != _abi.SVX_E_NOT_CONVERGED):
+            _raise(rc, err)
+        return U, its, res, ok.astype(bool), ms.value
+
     def solution(self) -> np.ndarray:
         """The current solution u (after solve() or a batched solve)."""
         u = np.zeros(self.mesh.n)

@@ -240,11 +240,10 @@ void system_prepare(svx_system* sys, int precision) {
     }
     if (!sys->u.p) sys->u.alloc(n);
     if (!sys->scal.p) sys->scal.alloc(1);
-    if (precision == SVX_FP64) {
-        if (!sys->r.p) {
-            sys->r.alloc(n), sys->p.alloc(n), sys->z.alloc(n), sys->q.alloc(n);
-        }
-    } else {
+    if (!sys->r.p) {  // FP64 vectors: the FP64 arm, and the FP32-IR fallback
+        sys->r.alloc(n), sys->p.alloc(n), sys->z.alloc(n), sys->q.alloc(n);
+    }
+    if (precision != SVX_FP64) {
         if (!sys->bval.p) {
             sys->bval.alloc((size_t)m->nblk * m->ndof * m->ndof);
             launch_fill_bsr(n, m->ndof, m->row_ptr.p, m->col.p, sys->vals.p, m->brow_ptr.p, sys->bval.p, s);

@@ -1,0 +1,129 @@
+// svx_solve_multi: many load cases on one assembled, constrained system.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "api_internal.hpp"
+#include "block.hpp"
+
+using namespace svx;
+
+extern "C" {
+
+int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, int64_t max_iter, int precision,
+                    double* U, int64_t* iterations, double* residuals, int32_t* converged, double* elapsed_ms,
+                    svx_error* err) {
+    if (err) {
+        std::memset(err, 0, sizeof *err);
+        err->index = -1;
+    }
+    if (!sys || nrhs <= 0 || nrhs > kMaxRhs || !F || !(tol > 0.0) || !sys->dirichlet) return SVX_E_ARGUMENT;
+    if (precision != SVX_FP64) {
+        if (err) {
+            err->code = SVX_E_ARGUMENT;
+            snprintf(err->message, sizeof err->message, "multi-RHS solves run in FP64");
+        }
+        return SVX_E_ARGUMENT;
+    }
+    try {
+        svx_mesh* m = sys->mesh;
+        svx_ctx* ctx = m->ctx;
+        hipStream_t s = ctx->stream;
+        const int32_t n = (int32_t)m->n, R = nrhs;
+        hipEvent_t e0, e1;
+        SVX_HIP(hipEventCreate(&e0));
+        SVX_HIP(hipEventCreate(&e1));
+        system_prepare(sys, SVX_FP64);
+        const size_t nr = (size_t)n * R;
+        DevBuf<double> Fin(nr), Fp(nr), Uv(nr), Rv(nr), P(nr), Z(nr), Q(nr);
+        std::vector<uint8_t> mask(n, 0);
+        for (int64_t r : sys->cons_rows) mask[r] = 1;
+        DevBuf<uint8_t> dmask(n);
+        dmask.upload(mask.data(), n, s);
+        SVX_HIP(hipEventRecord(e0, s));
+        Fin.upload(F, nr, s);
+        blk_pack(Fin.p, Fp.p, n, R, dmask.p, s);
+        BlockArgs a{};
+        a.n = n;
+        a.R = R;
+        a.rows_per_tile = 64;
+        a.n_tiles = (n + a.rows_per_tile - 1) / a.rows_per_tile;
+        a.row_ptr = m->row_ptr.p;
+        a.col = m->col.p;
+        a.val = sys->vals.p;
+        a.dinv = sys->dinv.p;
+        a.F = Fp.p;
+        a.U = Uv.p, a.Rv = Rv.p, a.P = P.p, a.Z = Z.p, a.Q = Q.p;
+        DevBuf<double> partials((size_t)a.n_tiles * 2 * kMaxRhs);
+        DevBuf<BlockScal> S(1);
+        BlockScal hs{};
+        hs.tol = tol;
+        hs.max_iter = max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * (int64_t)n);
+        S.upload(&hs, 1, s);
+        a.partials = partials.p;
+        a.S = S.p;
+        blk_init(a, s);
+        const int chunk = ctx->tune.chunk;
+        hipGraphExec_t exec = nullptr;
+        if (ctx->tune.use_graph) {
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                bool ok = true;
+                try {
+                    for (int i = 0; i < chunk; ++i) blk_iteration(a, s);
+                } catch (const DeviceError&) {
+                    ok = false;
+                }
+                if (hipStreamEndCapture(s, &g) == hipSuccess && g && ok) {
+                    if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+                }
+                if (g) (void)hipGraphDestroy(g);
+            }
+            (void)hipGetLastError();
+        }
+        for (;;) {
+            if (exec)
+                SVX_HIP(hipGraphLaunch(exec, s));
+            else
+                for (int i = 0; i < chunk; ++i) blk_iteration(a, s);
+            blk_count_running(a, ctx->d_flag, s);
+            SVX_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            SVX_HIP(hipStreamSynchronize(s));
+            if (*ctx->h_flag == 0) break;
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        blk_unpack(Uv.p, Fin.p, n, R, s);
+        if (U) Fin.download(U, nr, s);
+        SVX_HIP(hipEventRecord(e1, s));
+        S.download(&hs, 1, s);
+        SVX_HIP(hipStreamSynchronize(s));
+        float ms = 0.0f;
+        SVX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (elapsed_ms) *elapsed_ms = ms;
+        int failed = 0;
+        for (int j = 0; j < R; ++j) {
+            if (iterations) iterations[j] = hs.it[j];
+            if (residuals) residuals[j] = hs.res[j];
+            if (converged) converged[j] = hs.status[j] == ST_CONVERGED;
+            failed += hs.status[j] != ST_CONVERGED;
+        }
+        if (failed) {
+            if (err) {
+                err->code = SVX_E_NOT_CONVERGED;
+                snprintf(err->message, sizeof err->message, "%d of %d load cases did not converge", failed, R);
+            }
+            return SVX_E_NOT_CONVERGED;
+        }
+        return SVX_OK;
+    } catch (const std::exception& e) {
+        if (err) {
+            err->code = SVX_E_DEVICE;
+            snprintf(err->message, sizeof err->message, "%s", e.what());
+        }
+        return SVX_E_DEVICE;
+    }
+}
+
+}  // extern "C"

@@ -91,6 +91,24 @@ void Engine::enqueue_chunk(const EngineOptions& opt, int iters) {
 }
 
 int Engine::run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
+    int left = run_phase(opt, h_flag, d_flag, max_chunks);
+    if (left != 0 || opt.precision == SVX_FP64) return left;
+    // FP32-IR instances that stopped improving finish as FP64 PCG from their
+    // current solution (their descriptors carry the FP64 vectors too).
+    std::vector<InstScal> sc;
+    read_scalars(sc);
+    std::vector<char> fb(sc.size());
+    bool any = false;
+    for (size_t i = 0; i < sc.size(); ++i) any |= (fb[i] = sc[i].status == ST_FALLBACK) != 0;
+    if (!any) return 0;
+    launch_promote_fallback(descs_.p, (int32_t)h_descs_.size(), stream_);
+    build_tiles(fb);
+    EngineOptions o64 = opt;
+    o64.precision = SVX_FP64;
+    return run_phase(o64, h_flag, d_flag, -1);
+}
+
+int Engine::run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
     int chunk = opt.chunk;
     if (opt.precision != SVX_FP64) chunk = std::max(opt.period, (chunk / opt.period) * opt.period);
     int done_chunks = 0;

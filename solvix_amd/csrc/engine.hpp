@@ -45,6 +45,7 @@ class Engine {
     int n_instances() const { return (int)h_descs_.size(); }
 
    private:
+    int run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks);
     void release_graph();
     void build_tiles(const std::vector<char>& running);
     void enqueue_init(const EngineOptions& opt);

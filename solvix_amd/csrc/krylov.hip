@@ -129,10 +129,11 @@ __device__ __forceinline__ void bsr_row(const int32_t* __restrict__ brp, const i
 __global__ void spmv_ordered_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                     const double* __restrict__ val, const double* __restrict__ x,
                                     double* __restrict__ y) {
+#pragma clang fp contract(off)
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     double o = 0.0;
-    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) o = __dadd_rn(o, __dmul_rn(val[k], x[col[k]]));
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) o += val[k] * x[col[k]];  // serial, unfused
     y[r] = o;
 }
 
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
+    if (S->status != ST_RUNNING || !(S->flags & FL_FRESH) || (S->flags & FL_WARM)) return;
     double ff = 0.0, rz = 0.0;
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
         double fi = D.f[i];
@@ -200,6 +201,44 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
             S->res = 0.0;
             S->status = ST_CONVERGED;
         }
+    }
+}
+
+// FP64 recurrence started from the current u: r = f - K u, z = r / diag,
+// p = z; the iteration count carries on from the FP32 phase.
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __restrict__ tiles,
+                                                                const InstDesc* __restrict__ descs,
+                                                                double* __restrict__ partials) {
+    const Tile t = tiles[blockIdx.x];
+    const InstDesc& D = descs[t.inst];
+    InstScal* S = D.s;
+    if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != (FL_FRESH | FL_WARM)) return;
+    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
+    constexpr int G = kBlock / LANES;
+    double rr = 0.0, rz = 0.0;
+    for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
+        double ku = csr_row<LANES>(D.row_ptr, D.col, D.val, D.u, row, lane);
+        if (lane == 0) {
+            double ri = D.f[row] - ku;
+            double zi = ri * D.dinv[row];
+            D.r[row] = ri;
+            D.z[row] = zi;
+            D.p[row] = zi;
+            rr += ri * ri;
+            rz += ri * zi;
+        }
+    }
+    block_sum2(rr, rz);
+    if (!publish_partials(rr, rz, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
+    reduce_partials(partials, D.tile_begin, D.tile_count, rr, rz);
+    if (threadIdx.x == 0) {
+        S->cnt_a = 0;
+        S->rr = rr;
+        S->rz = rz;
+        S->flags = 0;
+        S->res = sqrt(rr) / S->bnorm;
+        if (S->res <= S->tol) S->status = ST_CONVERGED;
     }
 }
 
@@ -298,6 +337,9 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
 // f - K u (the reference's own solutions sit at that level on ill-conditioned
 // meshes).
 enum : int32_t { FL_HOLD = 2 };
+// A refresh at least this often, so an instance whose FP32 estimate never
+// drops still gets its FP64 residual checked (and can fall back).
+constexpr int64_t kForcedRefresh = 256;
 
 __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
@@ -333,6 +375,10 @@ __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restric
         S->refreshes = 0;
         S->flags = 0;
         S->floor = 0.0;
+        S->res_best = 1.0;
+        S->stall = 0;
+        S->fp64_it = 0;
+        S->it_refresh = 0;
         if (S->bnorm == 0.0) {
             S->res = 0.0;
             S->status = ST_CONVERGED;
@@ -415,7 +461,8 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
         S->rz = rz;
         if (S->it >= S->max_iter) {
             S->flags |= FL_REFRESH | FL_HOLD;
-        } else if (est / S->bnorm <= S->tol || est <= S->ru_delta * S->rmax) {
+        } else if (est / S->bnorm <= S->tol || est <= S->ru_delta * S->rmax ||
+                   S->it - S->it_refresh >= kForcedRefresh) {
             S->flags |= FL_REFRESH;
         }
     }
@@ -496,10 +543,19 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
         S->floor = floor;
         S->rr = rr;
         S->refreshes += 1;
+        S->it_refresh = S->it;
+        if (res < 0.5 * S->res_best) {
+            S->res_best = res;
+            S->stall = 0;
+        } else {
+            S->stall += 1;
+        }
         if (res <= S->tol + floor) {
             S->status = ST_CONVERGED;
         } else if (S->flags & FL_HOLD) {
             S->status = ST_MAXITER;
+        } else if (S->stall >= 4) {
+            S->status = ST_FALLBACK;  // FP32 cannot resolve this system further
         } else {
             S->rmax = sqrt(rr);
             S->beta = rz / S->rz_prev;
@@ -545,6 +601,17 @@ __global__ void count_running_kernel(const InstDesc* __restrict__ descs, int32_t
 __global__ void gather_scalars_kernel(const InstDesc* __restrict__ descs, int32_t n_inst, InstScal* out) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_inst) out[i] = *descs[i].s;
+}
+
+__global__ void promote_fallback_kernel(const InstDesc* __restrict__ descs, int32_t n_inst) {
+    for (int32_t i = threadIdx.x; i < n_inst; i += blockDim.x) {
+        InstScal* S = descs[i].s;
+        if (S->status == ST_FALLBACK) {
+            S->status = ST_RUNNING;
+            S->flags = FL_FRESH | FL_WARM;
+            S->fp64_it = -S->it;  // completed when the FP64 phase ends
+        }
+    }
 }
 
 __global__ void set_running_kernel(const InstDesc* __restrict__ descs, int32_t n_inst) {
@@ -602,6 +669,9 @@ void launch_spmv_bsr32(int64_t nb, int bs, const int32_t* brow_ptr, const int32_
 void cg64_init(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     cg64_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
+#define L(LN) cg64_init_warm_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+    SVX_LANES_DISPATCH(k.lanes, L)
+#undef L
     SVX_HIP(hipGetLastError());
 }
 
@@ -654,6 +724,11 @@ void ir32_refresh(const KrylovSet& k, hipStream_t s) {
 void launch_gather_scalars(const InstDesc* descs, int32_t n_inst, InstScal* out, hipStream_t s) {
     if (n_inst <= 0) return;
     gather_scalars_kernel<<<grid_for(n_inst, kBlock), kBlock, 0, s>>>(descs, n_inst, out);
+    SVX_HIP(hipGetLastError());
+}
+
+void launch_promote_fallback(const InstDesc* descs, int32_t n_inst, hipStream_t s) {
+    promote_fallback_kernel<<<1, kBlock, 0, s>>>(descs, n_inst);
     SVX_HIP(hipGetLastError());
 }
 

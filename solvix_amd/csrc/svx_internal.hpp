@@ -64,9 +64,13 @@ struct DevBuf {
 
 // ------------------------------------------------------- Krylov instances
 // Instance status values written by the device.
-enum : int32_t { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2, ST_MAXITER = 3, ST_IDLE = 4 };
+// ST_FALLBACK: an FP32-IR instance whose FP64 residual stopped improving; it
+// continues as FP64 PCG warm-started from its current u.
+enum : int32_t { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2, ST_MAXITER = 3, ST_IDLE = 4, ST_FALLBACK = 5 };
 // Instance flags. FL_FRESH marks work the next init phase must initialise.
-enum : int32_t { FL_REFRESH = 1, FL_FRESH = 4 };
+// FL_WARM (with FL_FRESH): initialise the FP64 recurrence from the current u
+// (r = f - K u) instead of u = 0.
+enum : int32_t { FL_REFRESH = 1, FL_FRESH = 4, FL_WARM = 8 };
 
 // Per-instance scalar state of a (preconditioned) CG solve, device resident.
 struct alignas(16) InstScal {
@@ -91,7 +95,10 @@ struct alignas(16) InstScal {
     double floor_scale; // sqrt(max row length) * eps64: FP64 evaluation floor of f - K u
     double floor;       // floor_scale * || |K||u| + |f| || / ||f|| at the latest refresh
     uint32_t cnt_c;
-    uint32_t pad_;
+    int32_t stall;      // FP32-IR refreshes without halving the best FP64 residual
+    double res_best;    // best FP64 relative residual seen at a refresh
+    int64_t fp64_it;    // iterations run by the FP64 fallback
+    int64_t it_refresh; // iteration of the latest refresh (FP32-IR)
 };
 
 // Device pointers of one instance. FP64 CSR always; FP32 block-sparse
@@ -203,6 +210,8 @@ void launch_count_running(const InstDesc* descs, int32_t n_inst, int32_t* out, h
 void spmv_phase(const KrylovSet& k, int precision, hipStream_t s);
 // Set every instance of the set back to ST_RUNNING (timing helper).
 void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s);
+// ST_FALLBACK instances -> ST_RUNNING with FL_FRESH | FL_WARM.
+void launch_promote_fallback(const InstDesc* descs, int32_t n_inst, hipStream_t s);
 // Copy every instance's scalars into a contiguous array.
 void launch_gather_scalars(const InstDesc* descs, int32_t n_inst, InstScal* out, hipStream_t s);
 
