@@ -53,14 +53,14 @@ void Engine::build_tiles(const std::vector<char>& running) {
     tiles_.upload(tiles.data(), tiles.size(), stream_);
     descs_.alloc(std::max<size_t>(1, h_descs_.size()));
     descs_.upload(h_descs_.data(), h_descs_.size(), stream_);
-    partials_.alloc(std::max<size_t>(4, 4 * tiles.size()));
+    partials_.alloc(std::max<size_t>(kPartialSlots, kPartialSlots * tiles.size()));
     for (int bs = 1; bs <= 3; ++bs) {
         KrylovSet& k = groups_[bs];
         k.tiles = tiles_.p + first[bs];
         k.n_tiles = count[bs];
         k.descs = descs_.p;
         k.n_inst = (int32_t)h_descs_.size();
-        k.partials = partials_.p + 4 * (size_t)first[bs];
+        k.partials = partials_.p + kPartialSlots * (size_t)first[bs];
         k.lanes = shape_.lanes[bs] > 0 ? shape_.lanes[bs] : 16;
         k.blanes = shape_.blanes[bs] > 0 ? shape_.blanes[bs] : 8;
         k.bs = bs;

@@ -3,24 +3,42 @@
 // All Krylov kernels work on a *set* of instances (one system, or hundreds
 // of uneven ones): the launch grid is a tile table, each workgroup owns one
 // contiguous row range of one instance. A finished instance costs one scalar
-// load per tile. Dot products are reduced deterministically: a fixed
-// in-block tree, then the last workgroup of the instance (last-block-done
-// counter) sums the per-tile partials in tile order. Results therefore do
-// not depend on scheduling, nor on which other instances share the launch.
+// load per tile.
+//
+// Reductions use no atomics and no device-scope fences (on gfx950 an
+// agent-scope release fence writes the XCD's L2 back; one per workgroup per
+// phase costs more than the phase). A phase writes one partial sum per tile;
+// every tile of the *next* phase re-reduces its instance's partials in tile
+// order (fixed strided order, then a fixed in-block tree), so all tiles
+// derive bit-identical scalars, and the kernel boundary orders the data. The
+// instance's first tile (the leader) alone writes the live scalar state; the
+// other tiles read only fields the leader does not write in that phase
+// (InstScal::snap_*, configuration). Results therefore depend neither on
+// scheduling nor on which other instances share the launch.
 //
 // FP64 arm: the recurrence of solver.rs:82-149 (u, r, z, p, alpha, beta,
 // breakdown and budget tests), CSR SpMV with `lanes` lanes per row.
 // FP32-IR arm: the same recurrence on an FP32 node-block (BSR) copy of K and
-// FP32 vectors with FP64-accumulated dots; every `period` iterations the
-// correction is folded into the FP64 solution and the residual is replaced by
-// the true FP64 residual f - K u (reliable-update refinement). Convergence is
-// declared only on that FP64 residual.
+// FP32 vectors with FP64-accumulated dots, with reliable-update refinement
+// (residual replacement): every `period` iterations an instance whose
+// recursive residual has fallen below ru_delta times its largest value since
+// the last replacement (or below tol, or that has not been checked for
+// kForcedRefresh iterations) folds its FP32 correction into the FP64 u,
+// recomputes r = f - K u in FP64, and continues with p = z + beta p built from
+// the replaced residual. The solve ends when the FP64 residual satisfies
+// ||f - K u|| <= tol ||f|| + floor_scale || |K||u| + |f| ||, i.e. reaches tol
+// or the rounding level at which FP64 can still resolve f - K u. An instance
+// whose FP64 residual stops improving (four replacements without halving)
+// falls back to FP64 PCG from its current u.
 #include "svx_internal.hpp"
 
 namespace svx {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kWaves = kBlock / kWave;
+constexpr int32_t FL_HOLD = 2;          // FP32-IR: budget reached, awaiting the final FP64 check
+constexpr int64_t kForcedRefresh = 256; // FP32-IR: check the FP64 residual at least this often
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -28,66 +46,49 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Deterministic block sum of three values; results valid in thread 0.
-__device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
-    __shared__ double sa[kBlock / kWave], sb[kBlock / kWave], sc[kBlock / kWave];
-    a = wave_sum(a);
-    b = wave_sum(b);
-    c = wave_sum(c);
-    int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    if (lane == 0) sa[w] = a, sb[w] = b, sc[w] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        a = sa[0], b = sb[0], c = sc[0];
+// Deterministic block all-reduce of NV values: every thread receives the sums.
+template <int NV>
+__device__ __forceinline__ void block_allreduce(double (&v)[NV]) {
+    __shared__ double sh[NV][kWaves];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
 #pragma unroll
-        for (int i = 1; i < kBlock / kWave; ++i) a += sa[i], b += sb[i], c += sc[i];
+    for (int i = 0; i < NV; ++i) {
+        double s = wave_sum(v[i]);
+        if (lane == 0) sh[i][w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = sh[i][0];
+#pragma unroll
+        for (int k = 1; k < kWaves; ++k) s += sh[i][k];
+        v[i] = s;
     }
     __syncthreads();
 }
 
-__device__ __forceinline__ void block_sum2(double& a, double& b) {
-    double c = 0.0;
-    block_sum3(a, b, c);
-}
-
-// Publish this tile's partials (4 slots per tile); returns true in every
-// thread of the last tile of the instance to arrive, which then sees all.
-__device__ __forceinline__ bool publish_partials(double a, double b, double c, double* partials, int tile,
-                                                 uint32_t* counter, int tile_count) {
-    __shared__ int s_last;
+// This tile's partials (block-reduced) into its slots.
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], const int (&slot)[NV], double* partials, int tile) {
+    block_allreduce<NV>(v);
     if (threadIdx.x == 0) {
-        partials[4 * tile] = a;
-        partials[4 * tile + 1] = b;
-        partials[4 * tile + 2] = c;
-        __threadfence();
-        uint32_t prev = atomicAdd(counter, 1u);
-        s_last = (prev == (uint32_t)(tile_count - 1));
+#pragma unroll
+        for (int i = 0; i < NV; ++i) partials[(size_t)tile * kPartialSlots + slot[i]] = v[i];
     }
-    __syncthreads();
-    bool last = s_last != 0;
-    if (last) __threadfence();
-    return last;
-}
-__device__ __forceinline__ bool publish_partials(double a, double b, double* partials, int tile, uint32_t* counter,
-                                                 int tile_count) {
-    return publish_partials(a, b, 0.0, partials, tile, counter, tile_count);
 }
 
-// Sum of the instance's partials in tile order (strided, then block tree).
-__device__ __forceinline__ void reduce_partials(const double* partials, int tile_begin, int tile_count, double& a,
-                                                double& b, double& c) {
-    a = 0.0, b = 0.0, c = 0.0;
+// The instance's sums of the given slots over its tiles, in tile order.
+template <int NV>
+__device__ __forceinline__ void reduce_tiles(const double* __restrict__ partials, int tile_begin, int tile_count,
+                                             const int (&slot)[NV], double (&out)[NV]) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[i] = 0.0;
     for (int k = threadIdx.x; k < tile_count; k += kBlock) {
-        a += partials[4 * (tile_begin + k)];
-        b += partials[4 * (tile_begin + k) + 1];
-        c += partials[4 * (tile_begin + k) + 2];
+        const double* p = partials + (size_t)(tile_begin + k) * kPartialSlots;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) out[i] += p[slot[i]];
     }
-    block_sum3(a, b, c);
-}
-__device__ __forceinline__ void reduce_partials(const double* partials, int tile_begin, int tile_count, double& a,
-                                                double& b) {
-    double c;
-    reduce_partials(partials, tile_begin, tile_count, a, b, c);
+    block_allreduce<NV>(out);
 }
 
 template <int LANES>
@@ -95,7 +96,9 @@ __device__ __forceinline__ double csr_row(const int32_t* __restrict__ rp, const 
                                           const double* __restrict__ val, const double* __restrict__ x, int32_t row,
                                           int lane) {
     double acc = 0.0;
-    for (int32_t k = rp[row] + lane; k < rp[row + 1]; k += LANES) acc += val[k] * x[col[k]];
+    const int32_t kb = rp[row], ke = rp[row + 1];
+#pragma unroll 4
+    for (int32_t k = kb + lane; k < ke; k += LANES) acc += val[k] * x[col[k]];
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
     return acc;
@@ -106,10 +109,12 @@ __device__ __forceinline__ double csr_row(const int32_t* __restrict__ rp, const 
 template <int BS, int LANES>
 __device__ __forceinline__ void bsr_row(const int32_t* __restrict__ brp, const int32_t* __restrict__ bcol,
                                         const float* __restrict__ bval, int32_t nblk, const float* __restrict__ x,
-                                        int32_t br, int lane, double acc[BS]) {
+                                        int32_t br, int lane, double (&acc)[BS]) {
 #pragma unroll
     for (int a = 0; a < BS; ++a) acc[a] = 0.0;
-    for (int32_t k = brp[br] + lane; k < brp[br + 1]; k += LANES) {
+    const int32_t kb = brp[br], ke = brp[br + 1];
+#pragma unroll 2
+    for (int32_t k = kb + lane; k < ke; k += LANES) {
         int32_t j = bcol[k];
         double xj[BS];
 #pragma unroll
@@ -124,6 +129,8 @@ __device__ __forceinline__ void bsr_row(const int32_t* __restrict__ brp, const i
 #pragma unroll
         for (int off = LANES / 2; off > 0; off >>= 1) acc[a] += __shfl_xor(acc[a], off, LANES);
 }
+
+__device__ __forceinline__ bool is_leader(const Tile& t) { return t.row_begin == 0 && threadIdx.x == 0; }
 
 // ------------------------------------------------------------ plain SpMV
 __global__ void spmv_ordered_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
@@ -168,15 +175,16 @@ __global__ __launch_bounds__(kBlock) void spmv_bsr32_kernel(int64_t nb, const in
     }
 }
 
-// ------------------------------------------------------------ FP64 PCG
+// ------------------------------------------------------------ initialisation
+// Cold start (u = 0), FP64 vectors.
 __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || !(S->flags & FL_FRESH) || (S->flags & FL_WARM)) return;
-    double ff = 0.0, rz = 0.0;
+    const InstScal* S = D.s;
+    if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != FL_FRESH) return;
+    double v[2] = {0.0, 0.0};
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
         double fi = D.f[i];
         double zi = fi * D.dinv[i];
@@ -184,39 +192,25 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
         D.r[i] = fi;
         D.z[i] = zi;
         D.p[i] = zi;
-        ff += fi * fi;
-        rz += fi * zi;
+        v[0] += fi * fi;
+        v[1] += fi * zi;
     }
-    block_sum2(ff, rz);
-    if (!publish_partials(ff, rz, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, ff, rz);
-    if (threadIdx.x == 0) {
-        S->cnt_a = 0;
-        S->bnorm = sqrt(ff);
-        S->rr = ff;
-        S->rz = rz;
-        S->it = 0;
-        S->flags = 0;
-        if (S->bnorm == 0.0) {  // solver.rs:93-95
-            S->res = 0.0;
-            S->status = ST_CONVERGED;
-        }
-    }
+    const int slot[2] = {P_IFF, P_IRZ};
+    write_partials<2>(v, slot, partials, blockIdx.x);
 }
 
-// FP64 recurrence started from the current u: r = f - K u, z = r / diag,
-// p = z; the iteration count carries on from the FP32 phase.
+// Warm start: FP64 recurrence from the current u, r = f - K u.
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __restrict__ tiles,
                                                                 const InstDesc* __restrict__ descs,
                                                                 double* __restrict__ partials) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
+    const InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != (FL_FRESH | FL_WARM)) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
-    double rr = 0.0, rz = 0.0;
+    double v[2] = {0.0, 0.0};
     for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
         double ku = csr_row<LANES>(D.row_ptr, D.col, D.val, D.u, row, lane);
         if (lane == 0) {
@@ -225,130 +219,23 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
             D.r[row] = ri;
             D.z[row] = zi;
             D.p[row] = zi;
-            rr += ri * ri;
-            rz += ri * zi;
+            v[0] += ri * ri;
+            v[1] += ri * zi;
         }
     }
-    block_sum2(rr, rz);
-    if (!publish_partials(rr, rz, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, rr, rz);
-    if (threadIdx.x == 0) {
-        S->cnt_a = 0;
-        S->rr = rr;
-        S->rz = rz;
-        S->flags = 0;
-        S->res = sqrt(rr) / S->bnorm;
-        if (S->res <= S->tol) S->status = ST_CONVERGED;
-    }
+    const int slot[2] = {P_IFF, P_IRZ};
+    write_partials<2>(v, slot, partials, blockIdx.x);
 }
 
-template <int LANES>
-__global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restrict__ tiles,
-                                                           const InstDesc* __restrict__ descs,
-                                                           double* __restrict__ partials) {
-    const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
-    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
-    constexpr int G = kBlock / LANES;
-    double pq = 0.0;
-    for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
-        double acc = csr_row<LANES>(D.row_ptr, D.col, D.val, D.p, row, lane);
-        if (lane == 0) {
-            D.q[row] = acc;
-            pq += D.p[row] * acc;
-        }
-    }
-    double dummy = 0.0;
-    block_sum2(pq, dummy);
-    if (!publish_partials(pq, 0.0, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, pq, dummy);
-    if (threadIdx.x == 0) {
-        S->cnt_a = 0;
-        S->pap = pq;
-        if (pq <= 0.0 || pq != pq) {  // solver.rs:118-123
-            S->it += 1;
-            S->res = sqrt(S->rr) / S->bnorm;
-            S->status = ST_BREAKDOWN;
-        } else {
-            S->alpha = S->rz / pq;
-        }
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restrict__ tiles,
-                                                             const InstDesc* __restrict__ descs,
-                                                             double* __restrict__ partials) {
-    const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
-    const double alpha = S->alpha;
-    double rr = 0.0, rz = 0.0;
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
-        double pi = D.p[i];
-        D.u[i] += alpha * pi;
-        double ri = D.r[i] - alpha * D.q[i];
-        D.r[i] = ri;
-        double zi = ri * D.dinv[i];
-        D.z[i] = zi;
-        rr += ri * ri;
-        rz += ri * zi;
-    }
-    block_sum2(rr, rz);
-    if (!publish_partials(rr, rz, partials, blockIdx.x, &S->cnt_b, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, rr, rz);
-    if (threadIdx.x == 0) {
-        S->cnt_b = 0;
-        S->rr = rr;
-        S->it += 1;
-        double res = sqrt(rr) / S->bnorm;
-        S->res = res;
-        if (res <= S->tol) {
-            S->status = ST_CONVERGED;
-        } else {
-            S->beta = rz / S->rz;
-            S->rz = rz;
-            if (S->it >= S->max_iter) S->status = ST_MAXITER;
-        }
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __restrict__ tiles,
-                                                              const InstDesc* __restrict__ descs) {
-    const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
-    const double beta = S->beta;
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
-}
-
-// ------------------------------------------------------------ FP32-IR
-// Reliable-update refinement (residual replacement): the FP32 loop runs the
-// PCG recurrence; every `period` iterations, an instance whose recursive
-// residual has fallen below ru_delta times its largest value since the last
-// replacement (or below tol) folds its FP32 correction into the FP64 u,
-// recomputes r = f - K u in FP64, and continues with p = z + beta p where z
-// and beta come from the replaced residual. The solve ends when the FP64
-// residual satisfies ||f - K u|| <= tol ||f|| + floor_scale || |K||u| + |f| ||,
-// i.e. reaches tol or the rounding level at which FP64 can still resolve
-// f - K u (the reference's own solutions sit at that level on ill-conditioned
-// meshes).
-enum : int32_t { FL_HOLD = 2 };
-// A refresh at least this often, so an instance whose FP32 estimate never
-// drops still gets its FP64 residual checked (and can fall back).
-constexpr int64_t kForcedRefresh = 256;
-
+// Cold start (u = 0), FP32-IR vectors.
 __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
-    double ff = 0.0, rz = 0.0;
+    const InstScal* S = D.s;
+    if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != FL_FRESH) return;
+    double v[2] = {0.0, 0.0};
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
         double fi = D.f[i];
         float ri = (float)fi;
@@ -358,74 +245,174 @@ __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restric
         D.r32[i] = ri;
         D.z32[i] = zi;
         D.p32[i] = zi;
-        ff += fi * fi;
-        rz += (double)ri * (double)zi;
+        v[0] += fi * fi;
+        v[1] += (double)ri * (double)zi;
     }
-    block_sum2(ff, rz);
-    if (!publish_partials(ff, rz, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, ff, rz);
-    if (threadIdx.x == 0) {
-        S->cnt_a = 0;
-        S->bnorm = sqrt(ff);
-        S->rr = ff;
-        S->rz = rz;
-        S->rz_prev = rz;
-        S->rmax = S->bnorm;
-        S->it = 0;
-        S->refreshes = 0;
+    const int slot[2] = {P_IFF, P_IRZ};
+    write_partials<2>(v, slot, partials, blockIdx.x);
+}
+
+// Leader tile only: scalars of the freshly initialised instances.
+__global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restrict__ tiles,
+                                                             const InstDesc* __restrict__ descs,
+                                                             const double* __restrict__ partials) {
+    const Tile t = tiles[blockIdx.x];
+    if (t.row_begin != 0) return;
+    const InstDesc& D = descs[t.inst];
+    InstScal* S = D.s;
+    if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
+    const int slot[2] = {P_IFF, P_IRZ};
+    double v[2];
+    reduce_tiles<2>(partials, D.tile_begin, D.tile_count, slot, v);
+    if (threadIdx.x != 0) return;
+    if (S->flags & FL_WARM) {  // v[0] = r . r of the warm residual
+        S->rr = v[0];
+        S->rz = v[1];
+        S->res = sqrt(v[0]) / S->bnorm;
         S->flags = 0;
-        S->floor = 0.0;
-        S->res_best = 1.0;
-        S->stall = 0;
-        S->fp64_it = 0;
-        S->it_refresh = 0;
-        if (S->bnorm == 0.0) {
-            S->res = 0.0;
-            S->status = ST_CONVERGED;
-        }
+        if (S->res <= S->tol) S->status = ST_CONVERGED;
+        return;
+    }
+    S->bnorm = sqrt(v[0]);
+    S->rr = v[0];
+    S->rz = v[1];
+    S->it = 0;
+    S->rmax = S->bnorm;
+    S->res_best = 1.0;
+    S->stall = 0;
+    S->refreshes = 0;
+    S->it_refresh = 0;
+    S->fp64_it = 0;
+    S->floor = 0.0;
+    S->flags = 0;
+    if (S->bnorm == 0.0) {  // solver.rs:93-95
+        S->res = 0.0;
+        S->status = ST_CONVERGED;
     }
 }
 
+// ------------------------------------------------------------ FP64 PCG
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restrict__ tiles,
+                                                           const InstDesc* __restrict__ descs,
+                                                           double* __restrict__ partials) {
+    const Tile t = tiles[blockIdx.x];
+    const InstDesc& D = descs[t.inst];
+    const InstScal* S = D.s;
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
+    constexpr int G = kBlock / LANES;
+    double v[1] = {0.0};
+    for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
+        double acc = csr_row<LANES>(D.row_ptr, D.col, D.val, D.p, row, lane);
+        if (lane == 0) {
+            D.q[row] = acc;
+            v[0] += D.p[row] * acc;
+        }
+    }
+    const int slot[1] = {P_PQ};
+    write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restrict__ tiles,
+                                                             const InstDesc* __restrict__ descs,
+                                                             double* __restrict__ partials) {
+    const Tile t = tiles[blockIdx.x];
+    const InstDesc& D = descs[t.inst];
+    InstScal* S = D.s;
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const int sp[1] = {P_PQ};
+    double pq[1];
+    reduce_tiles<1>(partials, D.tile_begin, D.tile_count, sp, pq);
+    if (pq[0] <= 0.0 || pq[0] != pq[0]) {  // solver.rs:118-123
+        if (is_leader(t)) {
+            S->pap = pq[0];
+            S->it += 1;
+            S->res = sqrt(S->rr) / S->bnorm;
+            S->status = ST_BREAKDOWN;
+        }
+        return;
+    }
+    const double alpha = S->rz / pq[0];
+    if (is_leader(t)) {
+        S->pap = pq[0];
+        S->alpha = alpha;
+        S->snap_rz = S->rz;
+        S->snap_it = S->it;
+    }
+    double v[2] = {0.0, 0.0};
+    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
+        double pi = D.p[i];
+        D.u[i] += alpha * pi;
+        double ri = D.r[i] - alpha * D.q[i];
+        D.r[i] = ri;
+        double zi = ri * D.dinv[i];
+        D.z[i] = zi;
+        v[0] += ri * ri;
+        v[1] += ri * zi;
+    }
+    const int slot[2] = {P_RR, P_RZ};
+    write_partials<2>(v, slot, partials, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __restrict__ tiles,
+                                                              const InstDesc* __restrict__ descs,
+                                                              const double* __restrict__ partials) {
+    const Tile t = tiles[blockIdx.x];
+    const InstDesc& D = descs[t.inst];
+    InstScal* S = D.s;
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const int sl[2] = {P_RR, P_RZ};
+    double v[2];
+    reduce_tiles<2>(partials, D.tile_begin, D.tile_count, sl, v);
+    const double res = sqrt(v[0]) / S->bnorm;
+    const bool conv = res <= S->tol;
+    const double beta = v[1] / S->snap_rz;
+    if (is_leader(t)) {
+        const int64_t it = S->snap_it + 1;
+        S->it = it;
+        S->rr = v[0];
+        S->res = res;
+        if (conv) {
+            S->status = ST_CONVERGED;
+        } else {
+            S->beta = beta;
+            S->rz = v[1];
+            if (it >= S->max_iter) S->status = ST_MAXITER;
+        }
+    }
+    if (conv) return;
+    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
+}
+
+// ------------------------------------------------------------ FP32-IR
 template <int BS, int LANES>
 __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
+    const InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
     const int32_t nblk = D.brow_ptr[D.n / BS];
-    double pq = 0.0;
+    double v[1] = {0.0};
     for (int32_t br = t.row_begin / BS + grp; br < t.row_end / BS; br += G) {
         double acc[BS];
         bsr_row<BS, LANES>(D.brow_ptr, D.bcol, D.bval, nblk, D.p32, br, lane, acc);
         if (lane < BS) {
-            double v = acc[0];
+            double a = acc[0];
 #pragma unroll
-            for (int a = 1; a < BS; ++a)
-                if (lane == a) v = acc[a];
-            float qv = (float)v;
+            for (int c = 1; c < BS; ++c)
+                if (lane == c) a = acc[c];
+            float qv = (float)a;
             D.q32[br * BS + lane] = qv;
-            pq += (double)D.p32[br * BS + lane] * (double)qv;
+            v[0] += (double)D.p32[br * BS + lane] * (double)qv;
         }
     }
-    double dummy = 0.0;
-    block_sum2(pq, dummy);
-    if (!publish_partials(pq, 0.0, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, pq, dummy);
-    if (threadIdx.x == 0) {
-        S->cnt_a = 0;
-        S->pap = pq;
-        if (pq <= 0.0 || pq != pq) {
-            S->it += 1;
-            S->res = sqrt(S->rr) / S->bnorm;
-            S->status = ST_BREAKDOWN;
-        } else {
-            S->alpha = S->rz / pq;
-        }
-    }
+    const int slot[1] = {P_PQ};
+    write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restrict__ tiles,
@@ -435,8 +422,29 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
-    const float alpha = (float)S->alpha;
-    double rr = 0.0, rz = 0.0;
+    const int sp[1] = {P_PQ};
+    double pq[1];
+    reduce_tiles<1>(partials, D.tile_begin, D.tile_count, sp, pq);
+    if (pq[0] <= 0.0 || pq[0] != pq[0]) {
+        if (is_leader(t)) {
+            S->pap = pq[0];
+            S->it += 1;
+            S->res = sqrt(S->rr) / S->bnorm;
+            S->status = ST_BREAKDOWN;
+        }
+        return;
+    }
+    const double alpha_d = S->rz / pq[0];
+    if (is_leader(t)) {
+        S->pap = pq[0];
+        S->alpha = alpha_d;
+        S->snap_rz = S->rz;
+        S->snap_rmax = S->rmax;
+        S->snap_it = S->it;
+        S->snap_flags = S->flags;
+    }
+    const float alpha = (float)alpha_d;
+    double v[2] = {0.0, 0.0};
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
         float pi = D.p32[i];
         D.x32[i] += alpha * pi;
@@ -444,41 +452,44 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
         D.r32[i] = ri;
         float zi = ri * D.dinv32[i];
         D.z32[i] = zi;
-        rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
+        v[0] += (double)ri * (double)ri;
+        v[1] += (double)ri * (double)zi;
     }
-    block_sum2(rr, rz);
-    if (!publish_partials(rr, rz, partials, blockIdx.x, &S->cnt_b, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, rr, rz);
-    if (threadIdx.x == 0) {
-        S->cnt_b = 0;
-        S->rr = rr;
-        S->it += 1;
-        double est = sqrt(rr);
-        S->rmax = fmax(S->rmax, est);
-        S->beta = rz / S->rz;
-        S->rz_prev = S->rz;
-        S->rz = rz;
-        if (S->it >= S->max_iter) {
-            S->flags |= FL_REFRESH | FL_HOLD;
-        } else if (est / S->bnorm <= S->tol || est <= S->ru_delta * S->rmax ||
-                   S->it - S->it_refresh >= kForcedRefresh) {
-            S->flags |= FL_REFRESH;
-        }
-    }
+    const int slot[2] = {P_RR, P_RZ};
+    write_partials<2>(v, slot, partials, blockIdx.x);
 }
 
-// p = z + beta p. At the last iteration of a period an instance flagged for
-// refresh skips this: its direction is rebuilt from the replaced residual.
+// p = z + beta p and the refresh decision. At the last iteration of a period
+// an instance flagged for refresh skips the update: its direction is rebuilt
+// from the replaced residual.
 __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __restrict__ tiles,
-                                                              const InstDesc* __restrict__ descs, int period_end) {
+                                                              const InstDesc* __restrict__ descs,
+                                                              const double* __restrict__ partials, int period_end) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
-    if (period_end && (S->flags & FL_REFRESH)) return;
-    const float beta = (float)S->beta;
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + beta * D.p32[i];
+    const int sl[2] = {P_RR, P_RZ};
+    double v[2];
+    reduce_tiles<2>(partials, D.tile_begin, D.tile_count, sl, v);
+    const int64_t it = S->snap_it + 1;
+    const double est = sqrt(v[0]);
+    const double rmax = fmax(S->snap_rmax, est);
+    const double beta = v[1] / S->snap_rz;
+    const bool hold = it >= S->max_iter;
+    const bool refresh = hold || (S->snap_flags & FL_REFRESH) || est / S->bnorm <= S->tol ||
+                         est <= S->ru_delta * rmax || it - S->it_refresh >= kForcedRefresh;
+    if (is_leader(t)) {
+        S->it = it;
+        S->rr = v[0];
+        S->rz = v[1];
+        S->beta = beta;
+        S->rmax = rmax;
+        S->flags = S->snap_flags | (refresh ? FL_REFRESH : 0) | (hold ? FL_HOLD : 0);
+    }
+    if (hold || (period_end && refresh)) return;
+    const float b = (float)beta;
+    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + b * D.p32[i];
 }
 
 // Refresh step 1: fold the FP32 correction into the FP64 solution.
@@ -488,31 +499,35 @@ __global__ __launch_bounds__(kBlock) void ir32_fold_kernel(const Tile* __restric
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
+    if (is_leader(t)) {
+        S->snap_flags = S->flags;
+        S->snap_stall = S->stall;
+        S->snap_res_best = S->res_best;
+    }
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
         D.u[i] += (double)D.x32[i];
         D.x32[i] = 0.0f;
     }
 }
 
-// Refresh step 2: r = f - K u in FP64 with its evaluation floor, replace
-// r32 / z32, test convergence, set beta from the replaced residual.
+// Refresh step 2: r = f - K u in FP64 with its evaluation floor; r32, z32.
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __restrict__ tiles,
                                                                const InstDesc* __restrict__ descs,
                                                                double* __restrict__ partials) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
+    const InstScal* S = D.s;
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
-    double rr = 0.0, rz = 0.0, fl = 0.0;
+    double v[3] = {0.0, 0.0, 0.0};
     for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
         double ku = 0.0, aku = 0.0;
         for (int32_t k = D.row_ptr[row] + lane; k < D.row_ptr[row + 1]; k += LANES) {
-            double v = D.val[k] * D.u[D.col[k]];
-            ku += v;
-            aku += fabs(v);
+            double w = D.val[k] * D.u[D.col[k]];
+            ku += w;
+            aku += fabs(w);
         }
 #pragma unroll
         for (int off = LANES / 2; off > 0; off >>= 1) {
@@ -526,67 +541,64 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
             float z32 = r32 * D.dinv32[row];
             D.r32[row] = r32;
             D.z32[row] = z32;
-            rr += ri * ri;
-            rz += (double)r32 * (double)z32;
+            v[0] += ri * ri;
+            v[1] += (double)r32 * (double)z32;
             double a = aku + fabs(fi);
-            fl += a * a;
+            v[2] += a * a;
         }
     }
-    block_sum3(rr, rz, fl);
-    if (!publish_partials(rr, rz, fl, partials, blockIdx.x, &S->cnt_a, D.tile_count)) return;
-    reduce_partials(partials, D.tile_begin, D.tile_count, rr, rz, fl);
-    if (threadIdx.x == 0) {
-        S->cnt_a = 0;
-        double res = sqrt(rr) / S->bnorm;
-        double floor = S->floor_scale * sqrt(fl) / S->bnorm;
-        S->res = res;
-        S->floor = floor;
-        S->rr = rr;
-        S->refreshes += 1;
-        S->it_refresh = S->it;
-        if (res < 0.5 * S->res_best) {
-            S->res_best = res;
-            S->stall = 0;
-        } else {
-            S->stall += 1;
-        }
-        if (res <= S->tol + floor) {
-            S->status = ST_CONVERGED;
-        } else if (S->flags & FL_HOLD) {
-            S->status = ST_MAXITER;
-        } else if (S->stall >= 4) {
-            S->status = ST_FALLBACK;  // FP32 cannot resolve this system further
-        } else {
-            S->rmax = sqrt(rr);
-            S->beta = rz / S->rz_prev;
-            S->rz = rz;
-        }
-    }
+    const int slot[3] = {P_TRR, P_TRZ, P_TFL};
+    write_partials<3>(v, slot, partials, blockIdx.x);
 }
 
-// Refresh step 3: p = z + beta p for the refreshed instances; the last tile
-// of each clears its refresh flag.
+// Refresh step 3: convergence / budget / fallback decision, then
+// p = z + beta p from the replaced residual.
 __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __restrict__ tiles,
-                                                              const InstDesc* __restrict__ descs) {
+                                                              const InstDesc* __restrict__ descs,
+                                                              const double* __restrict__ partials) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
-    if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
-    const float beta = (float)S->beta;
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + beta * D.p32[i];
-    __syncthreads();
-    __shared__ int s_last;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        s_last = atomicAdd(&S->cnt_c, 1u) == (uint32_t)(D.tile_count - 1);
+    if (S->status != ST_RUNNING || !(S->snap_flags & FL_REFRESH)) return;
+    const int sl[3] = {P_TRR, P_TRZ, P_TFL};
+    double v[3];
+    reduce_tiles<3>(partials, D.tile_begin, D.tile_count, sl, v);
+    const double res = sqrt(v[0]) / S->bnorm;
+    const double floor = S->floor_scale * sqrt(v[2]) / S->bnorm;
+    const bool improved = res < 0.5 * S->snap_res_best;
+    const int32_t stall = improved ? 0 : S->snap_stall + 1;
+    const bool conv = res <= S->tol + floor;
+    const bool hold = (S->snap_flags & FL_HOLD) != 0;
+    const bool fallback = !conv && !hold && stall >= 4;
+    const bool cont = !conv && !hold && !fallback;
+    const double beta = v[1] / S->snap_rz;
+    if (is_leader(t)) {
+        S->res = res;
+        S->floor = floor;
+        S->rr = v[0];
+        S->refreshes += 1;
+        S->it_refresh = S->it;
+        S->res_best = improved ? res : S->snap_res_best;
+        S->stall = stall;
+        if (conv) {
+            S->status = ST_CONVERGED;
+        } else if (hold) {
+            S->status = ST_MAXITER;
+        } else if (fallback) {
+            S->status = ST_FALLBACK;  // FP32 cannot resolve this system further
+        } else {
+            S->rmax = sqrt(v[0]);
+            S->rz = v[1];
+            S->beta = beta;
+            S->flags = 0;
+        }
     }
-    __syncthreads();
-    if (s_last && threadIdx.x == 0) {
-        S->cnt_c = 0;
-        S->flags = 0;
-    }
+    if (!cont) return;
+    const float b = (float)beta;
+    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + b * D.p32[i];
 }
 
+// ------------------------------------------------------------ utilities
 __global__ void count_running_kernel(const InstDesc* __restrict__ descs, int32_t n_inst, int32_t* out) {
     __shared__ int32_t cnt;
     if (threadIdx.x == 0) cnt = 0;
@@ -598,20 +610,20 @@ __global__ void count_running_kernel(const InstDesc* __restrict__ descs, int32_t
     if (threadIdx.x == 0) *out = cnt;
 }
 
-__global__ void gather_scalars_kernel(const InstDesc* __restrict__ descs, int32_t n_inst, InstScal* out) {
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_inst) out[i] = *descs[i].s;
-}
-
 __global__ void promote_fallback_kernel(const InstDesc* __restrict__ descs, int32_t n_inst) {
     for (int32_t i = threadIdx.x; i < n_inst; i += blockDim.x) {
         InstScal* S = descs[i].s;
         if (S->status == ST_FALLBACK) {
             S->status = ST_RUNNING;
             S->flags = FL_FRESH | FL_WARM;
-            S->fp64_it = -S->it;  // completed when the FP64 phase ends
+            S->fp64_it = -S->it;
         }
     }
+}
+
+__global__ void gather_scalars_kernel(const InstDesc* __restrict__ descs, int32_t n_inst, InstScal* out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) out[i] = *descs[i].s;
 }
 
 __global__ void set_running_kernel(const InstDesc* __restrict__ descs, int32_t n_inst) {
@@ -672,6 +684,7 @@ void cg64_init(const KrylovSet& k, hipStream_t s) {
 #define L(LN) cg64_init_warm_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
     SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
+    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
     SVX_HIP(hipGetLastError());
 }
 
@@ -681,13 +694,14 @@ void cg64_iteration(const KrylovSet& k, hipStream_t s) {
     SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
-    cg64_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
+    cg64_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
     SVX_HIP(hipGetLastError());
 }
 
 void ir32_init(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     ir32_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
+    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
     SVX_HIP(hipGetLastError());
 }
 
@@ -707,7 +721,7 @@ void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end) {
 #undef L
     }
     ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
-    ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, period_end ? 1 : 0);
+    ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, period_end ? 1 : 0);
     SVX_HIP(hipGetLastError());
 }
 
@@ -717,18 +731,18 @@ void ir32_refresh(const KrylovSet& k, hipStream_t s) {
 #define L(LN) ir32_residual_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
     SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
-    ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
+    ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
+    SVX_HIP(hipGetLastError());
+}
+
+void launch_promote_fallback(const InstDesc* descs, int32_t n_inst, hipStream_t s) {
+    promote_fallback_kernel<<<1, kBlock, 0, s>>>(descs, n_inst);
     SVX_HIP(hipGetLastError());
 }
 
 void launch_gather_scalars(const InstDesc* descs, int32_t n_inst, InstScal* out, hipStream_t s) {
     if (n_inst <= 0) return;
     gather_scalars_kernel<<<grid_for(n_inst, kBlock), kBlock, 0, s>>>(descs, n_inst, out);
-    SVX_HIP(hipGetLastError());
-}
-
-void launch_promote_fallback(const InstDesc* descs, int32_t n_inst, hipStream_t s) {
-    promote_fallback_kernel<<<1, kBlock, 0, s>>>(descs, n_inst);
     SVX_HIP(hipGetLastError());
 }
 

@@ -73,32 +73,42 @@ enum : int32_t { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2, ST_MAXITER 
 enum : int32_t { FL_REFRESH = 1, FL_FRESH = 4, FL_WARM = 8 };
 
 // Per-instance scalar state of a (preconditioned) CG solve, device resident.
+// Within a kernel only the instance's leader tile (its first row tile)
+// writes the live fields; every tile re-derives the scalars it needs from the
+// per-tile partials of the previous phase (fixed order, identical in every
+// tile) and from the snap_* fields, which the leader writes one phase earlier
+// and nobody writes in the phase that reads them.
 struct alignas(16) InstScal {
-    double bnorm;    // ||f||
-    double rz;       // r . z of the current iterate
-    double pap;      // p . K p
+    double bnorm;       // ||f||
+    double rz;          // r . z of the current iterate
+    double pap;         // p . K p
     double alpha;
     double beta;
-    double rr;       // r . r of the current (recursive) residual
-    double res;      // reported relative residual
-    double rmax;     // ||r|| at the last residual replacement (FP32-IR)
+    double rr;          // r . r of the current (recursive) residual
+    double res;         // reported relative residual
+    double rmax;        // FP32-IR: max ||r|| since the latest refresh
+    double floor;       // FP32-IR: FP64 evaluation floor at the latest refresh
+    double res_best;    // FP32-IR: best FP64 relative residual at a refresh
+    int64_t it;         // iterations done
+    int64_t refreshes;  // FP32-IR residual replacements
+    int64_t it_refresh; // FP32-IR: iteration of the latest refresh
+    int64_t fp64_it;    // FP32-IR fallback: -(iterations before FP64 took over)
+    int32_t status;     // ST_*
+    int32_t flags;      // FL_*
+    int32_t stall;      // FP32-IR refreshes without halving res_best
+    int32_t pad0;
+    // snapshots
+    double snap_rz;
+    double snap_rmax;
+    double snap_res_best;
+    int64_t snap_it;
+    int32_t snap_flags;
+    int32_t snap_stall;
+    // configuration (host)
     double tol;
-    int64_t it;      // iterations done
+    double ru_delta;     // FP32-IR refresh when ||r|| < ru_delta * rmax
+    double floor_scale;  // sqrt(max row length) * eps64
     int64_t max_iter;
-    int32_t status;  // ST_*
-    int32_t flags;   // FL_*
-    uint32_t cnt_a;  // last-block-done counters (reset by the last block)
-    uint32_t cnt_b;
-    int64_t refreshes;  // residual replacements done (FP32-IR)
-    double ru_delta;    // replacement threshold: ||r|| < ru_delta * max ||r|| since the last one
-    double rz_prev;     // r . z before the latest update (FP32-IR refresh beta)
-    double floor_scale; // sqrt(max row length) * eps64: FP64 evaluation floor of f - K u
-    double floor;       // floor_scale * || |K||u| + |f| || / ||f|| at the latest refresh
-    uint32_t cnt_c;
-    int32_t stall;      // FP32-IR refreshes without halving the best FP64 residual
-    double res_best;    // best FP64 relative residual seen at a refresh
-    int64_t fp64_it;    // iterations run by the FP64 fallback
-    int64_t it_refresh; // iteration of the latest refresh (FP32-IR)
 };
 
 // Device pointers of one instance. FP64 CSR always; FP32 block-sparse
@@ -139,6 +149,8 @@ struct Tile {
 };
 
 constexpr int kBlock = 256;
+// Per-tile partial sums, one slot per reduced quantity.
+enum : int { P_PQ = 0, P_RR = 1, P_RZ = 2, P_TRR = 3, P_TRZ = 4, P_TFL = 5, P_IFF = 6, P_IRZ = 7, kPartialSlots = 8 };
 
 // SpMV launch shape chosen per instance set: lanes per row (CSR) or per
 // block row (BSR) and rows per tile.
@@ -188,14 +200,14 @@ void launch_spmv_bsr32(int64_t nb, int bs, const int32_t* brow_ptr, const int32_
                        const float* x, float* y, int lanes, hipStream_t s);
 
 // One CG iteration phase over a set of instances. `tiles` / `n_tiles`
-// describe the work; `descs` the instances; `partials` holds 2 doubles per
-// tile. Every launcher is graph-capturable (no allocation, no memset).
+// describe the work; `descs` the instances; `partials` holds kPartialSlots
+// doubles per tile. Every launcher is graph-capturable (no allocation, no memset).
 struct KrylovSet {
     const Tile* tiles = nullptr;
     int32_t n_tiles = 0;
     const InstDesc* descs = nullptr;
     int32_t n_inst = 0;
-    double* partials = nullptr;  // [4 * n_tiles]
+    double* partials = nullptr;  // [kPartialSlots * n_tiles]
     int lanes = 16;              // CSR lanes per row (FP64 SpMV)
     int blanes = 8;              // BSR lanes per block row (FP32 SpMV)
     int bs = 0;                  // common BSR block size of the set (2 or 3), 0 = mixed/none
