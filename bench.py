@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cap", type=int, default=150, help="PCG iterations in the CPU sample")
+    ap.add_argument("--arms", default="both", choices=["both", "fp64", "fp32_ir"],
+                    help="run one precision only (profiling); the JSON line needs both")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the solver path is a single-GPU "
@@ -98,6 +100,8 @@ def main():
 
     arms = {}
     for prec, label in ((Precision.FP64, "fp64"), (Precision.FP32_IR, "fp32_ir")):
+        if args.arms not in ("both", label):
+            continue
         for _ in range(args.warmup):
             step(prec)
         t0 = time.perf_counter()
@@ -110,6 +114,8 @@ def main():
         print(f"[bench] {label}: {args.steps} solves in {dt:.3f} s, {its} iterations, residual {res:.2e}, "
               f"SpMV {ms_kernel * 1e3:.1f} us ({bytes_kernel / ms_kernel / 1e6:.0f} GB/s)", file=sys.stderr, flush=True)
 
+    if args.arms != "both":
+        return
     a, b = arms["fp32_ir"], arms["fp64"]
     err = float(np.abs(a["u"] - b["u"]).max() / np.abs(b["u"]).max())
 

@@ -226,6 +226,23 @@ int svx_system_solution(const svx_system* sys, double* u /* [n] */);
 int svx_solve(svx_system* sys, double tol, int64_t max_iter, int precision,
               double* u /* [n] or NULL */, int64_t* iterations,
               double* residual, svx_error* err);
+/* What the last svx_solve or svx_batch_solve of this system did (no
+ * reference counterpart: the reference reports iterations and residual
+ * only, lib.rs:706-716). For SVX_FP32_IR, residual is the FP64 true
+ * residual ||f - K u|| / ||f|| at the last check and floor the FP64
+ * evaluation floor sqrt(max row length) * eps64 * || |K||u| + |f| || / ||f||
+ * at that check; the solve converges when residual <= tol + floor. */
+typedef struct svx_solve_stats {
+    int64_t iterations;      /* all Krylov iterations, FP64 fallback included */
+    int64_t refreshes;       /* FP32-IR: FP64 residual replacements */
+    int64_t fp64_iterations; /* FP32-IR: iterations run by the FP64 fallback */
+    double residual;
+    double floor;
+    int32_t status;    /* 0 converged, 1 breakdown, 2 iteration budget spent */
+    int32_t precision; /* SVX_FP64 / SVX_FP32_IR; -1 before any solve */
+} svx_solve_stats;
+int svx_system_stats(const svx_system* sys, svx_solve_stats* out);
+
 /* Recovery from the last solution: centroid von Mises (stress.rs:22-42),
  * nodal von Mises (stress.rs:53-98), reactions K_orig u - f_orig at the
  * given rows (stress.rs:103-118) and the Jacobi conditioning indicator

@@ -70,6 +70,18 @@ class SvxModelDesc(C.Structure):
     ]
 
 
+class SvxSolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("refreshes", C.c_int64),
+        ("fp64_iterations", C.c_int64),
+        ("residual", C.c_double),
+        ("floor", C.c_double),
+        ("status", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
 class SvxSolution(C.Structure):
     _fields_ = [
         ("displacements", _p(C.c_double)),

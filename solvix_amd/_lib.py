@@ -42,6 +42,7 @@ SIGNATURES = [
     ("svx_system_solution", C.c_int, [_vp, _dp]),
     ("svx_apply_dirichlet", C.c_int, [_vp, _i64, _i64p]),
     ("svx_solve", C.c_int, [_vp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _errp]),
+    ("svx_system_stats", C.c_int, [_vp, C.POINTER(_abi.SvxSolveStats)]),
     ("svx_recover", C.c_int, [_vp, _dp, _dp, _i64, _i64p, _dp, _dp]),
     ("svx_csr_create", C.c_int, [_vp, _i64, _i32p, _i32p, _dp, C.POINTER(_vp)]),
     ("svx_csr_destroy", None, [_vp]),

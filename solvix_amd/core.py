@@ -291,6 +291,13 @@ class System:
         _raise(rc, err)
         return u, its.value, res.value
 
+    def stats(self) -> dict:
+        """What the last solve did: iterations, FP64 refreshes and fallback
+        iterations (FP32-IR), final residual and its FP64 evaluation floor."""
+        st = _abi.SvxSolveStats()
+        _raise(lib().svx_system_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _abi.SvxSolveStats._fields_}
+
     def recover(self, rows=()) -> Tuple[np.ndarray, np.ndarray, np.ndarray, float]:
         """(centroid von Mises, nodal von Mises, reactions at rows, cond_estimate)."""
         rows = np.ascontiguousarray(rows, np.int64)

@@ -311,7 +311,7 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     if (!sys->engine) sys->engine = std::make_unique<Engine>(s);
     EngineShape shape;
     const int bs = m->ndof;
-    shape.rows_per_tile[bs] = tp.rows_per_tile;
+    shape.rows_per_tile[bs] = tp.rows_per_tile > 0 ? tp.rows_per_tile : default_rows_per_tile(bs, precision);
     shape.lanes[bs] = tp.lanes > 0 ? tp.lanes : lanes_for_bs(bs);
     shape.blanes[bs] = tp.blanes > 0 ? tp.blanes : blanes_for_bs(bs);
     sys->engine->set_instances(descs, {1}, shape);
@@ -324,6 +324,8 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     sys->engine->run(opt, ctx->h_flag, ctx->d_flag);
     sys->scal.download(out_scal, 1, s);
     SVX_HIP(hipStreamSynchronize(s));
+    sys->last = *out_scal;
+    sys->last_precision = precision;
     sys->solved = out_scal->status == ST_CONVERGED;
     sys->has_u = true;
 }
@@ -543,7 +545,6 @@ int svx_solve(svx_system* sys, double tol, int64_t max_iter, int precision, doub
         }
         if (iterations) *iterations = sc.it;
         if (residual) *residual = sc.res;
-        sys->last_refreshes = sc.refreshes;
         if (sc.status != ST_CONVERGED) {
             char buf[256];
             snprintf(buf, sizeof buf, "NotConverged { iterations: %lld, residual: %.6e }", (long long)sc.it, sc.res);
@@ -552,6 +553,21 @@ int svx_solve(svx_system* sys, double tol, int64_t max_iter, int precision, doub
         }
         return SVX_OK;
     });
+}
+
+int svx_system_stats(const svx_system* sys, svx_solve_stats* out) {
+    if (!sys || !out) return SVX_E_ARGUMENT;
+    const InstScal& sc = sys->last;
+    *out = svx_solve_stats{};
+    out->precision = sys->last_precision;
+    if (sys->last_precision < 0) return SVX_OK;
+    out->iterations = sc.it;
+    out->refreshes = sc.refreshes;
+    out->fp64_iterations = sc.fp64_it != 0 ? sc.it + sc.fp64_it : 0;
+    out->residual = sc.res;
+    out->floor = sc.floor;
+    out->status = sc.status == ST_CONVERGED ? 0 : sc.status == ST_BREAKDOWN ? 1 : 2;
+    return SVX_OK;
 }
 
 int svx_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n_rows, const int64_t* rows,

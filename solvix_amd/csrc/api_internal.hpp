@@ -74,7 +74,8 @@ struct svx_system {
     int engine_precision = -1;  // precision the engine's descriptors were built for
     double cond = 0.0;
     bool solved = false, has_u = false;
-    int64_t last_refreshes = 0;
+    svx::InstScal last{};    // scalars of the last solve
+    int last_precision = -1;
 };
 
 struct svx_csr {

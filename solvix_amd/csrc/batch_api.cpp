@@ -29,11 +29,11 @@ namespace svx {
 
 // Launch shape for a set of systems: the per-block-size shape every solve
 // uses, so batched results equal solo ones bit for bit.
-EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp) {
+EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp, int precision) {
     (void)systems;
     EngineShape sh;
     for (int bs = 1; bs <= 3; ++bs) {
-        sh.rows_per_tile[bs] = tp.rows_per_tile;
+        sh.rows_per_tile[bs] = tp.rows_per_tile > 0 ? tp.rows_per_tile : default_rows_per_tile(bs, precision);
         sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : lanes_for_bs(bs);
         sh.blanes[bs] = tp.blanes > 0 ? tp.blanes : blanes_for_bs(bs);
     }
@@ -109,7 +109,7 @@ int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
         }
         b->scal.upload(init.data(), n, s);
         if (!b->engine) b->engine = std::make_unique<Engine>(s);
-        b->engine->set_instances(descs, std::vector<char>(n, 1), batch_shape(b->sys, b->ctx->tune));
+        b->engine->set_instances(descs, std::vector<char>(n, 1), batch_shape(b->sys, b->ctx->tune, precision));
         EngineOptions opt;
         opt.precision = precision;
         opt.period = b->ctx->tune.period;
@@ -123,6 +123,8 @@ int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
         int64_t failed = 0;
         for (size_t i = 0; i < n; ++i) {
             b->sys[i]->has_u = true;
+            b->sys[i]->last = b->h_scal[i];
+            b->sys[i]->last_precision = precision;
             b->sys[i]->solved = b->h_scal[i].status == ST_CONVERGED;
             failed += !b->sys[i]->solved;
         }

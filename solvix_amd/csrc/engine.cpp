@@ -10,10 +10,16 @@ int default_lanes(double avg) {
     return l;
 }
 
-int default_rows_per_tile(int bs) { return bs == 3 ? 288 : 256; }
+// Launch shapes measured on the 265,923-DOF C3D8 block (FP64: 8 lanes per
+// CSR row, 192 rows per tile; FP32: 16 lanes per 3x3 block row, 144 rows per
+// tile) and on plane meshes for bs = 2.
+int default_rows_per_tile(int bs, int precision) {
+    if (bs == 3) return precision == SVX_FP64 ? 192 : 144;
+    return 256;
+}
 
-int lanes_for_bs(int bs) { return bs == 3 ? 16 : 4; }
-int blanes_for_bs(int bs) { return bs == 3 ? 8 : 2; }
+int lanes_for_bs(int bs) { return bs == 3 ? 8 : 4; }
+int blanes_for_bs(int bs) { return bs == 3 ? 16 : 2; }
 
 Engine::~Engine() { release_graph(); }
 
@@ -30,14 +36,27 @@ void Engine::set_instances(const std::vector<InstDesc>& descs, const std::vector
     build_tiles(running);
 }
 
-void Engine::build_tiles(const std::vector<char>& running) {
+void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     release_graph();
+    // previous layout, for carrying partials
+    std::vector<int32_t> old_begin(h_descs_.size(), -1);
+    int old_first[4] = {0, 0, 0, 0}, old_count[4] = {0, 0, 0, 0};
+    for (int bs = 1; bs <= 3; ++bs) {
+        old_first[bs] = groups_[bs].tiles ? (int)(groups_[bs].tiles - tiles_.p) : 0;
+        old_count[bs] = groups_[bs].n_tiles;
+    }
+    carry = carry && tiled_.size() == h_descs_.size() && partials_.p;
+    if (carry)
+        for (size_t i = 0; i < h_descs_.size(); ++i)
+            if (tiled_[i]) old_begin[i] = h_descs_[i].tile_begin;
+
     std::vector<Tile> tiles;
     int first[4] = {0, 0, 0, 0}, count[4] = {0, 0, 0, 0};
     tiled_instances_ = 0;
+    tiled_.assign(h_descs_.size(), 0);
     for (int bs = 1; bs <= 3; ++bs) {
         first[bs] = (int)tiles.size();
-        int rpt = shape_.rows_per_tile[bs] > 0 ? shape_.rows_per_tile[bs] : default_rows_per_tile(bs);
+        int rpt = shape_.rows_per_tile[bs] > 0 ? shape_.rows_per_tile[bs] : default_rows_per_tile(bs, SVX_FP64);
         rpt = std::max(bs, (rpt / bs) * bs);
         for (size_t i = 0; i < h_descs_.size(); ++i) {
             InstDesc& d = h_descs_[i];
@@ -46,14 +65,38 @@ void Engine::build_tiles(const std::vector<char>& running) {
             for (int32_t r = 0; r < d.n; r += rpt) tiles.push_back(Tile{(int32_t)i, r, std::min(d.n, r + rpt), 0});
             d.tile_count = (int32_t)tiles.size() - first[bs] - d.tile_begin;
             tiled_instances_++;
+            tiled_[i] = 1;
         }
         count[bs] = (int)tiles.size() - first[bs];
     }
+    // old local tile index feeding each new tile (-1: nothing to carry)
+    std::vector<int32_t> src(tiles.size(), -1);
+    bool any_src = false;
+    if (carry)
+        for (int bs = 1; bs <= 3; ++bs)
+            for (int g = 0; g < count[bs]; ++g) {
+                const Tile& t = tiles[first[bs] + g];
+                const InstDesc& d = h_descs_[t.inst];
+                if (old_begin[t.inst] < 0) continue;
+                src[first[bs] + g] = old_begin[t.inst] + (g - d.tile_begin);
+                any_src = true;
+            }
+    DevBuf<double> old_partials;
+    if (any_src) old_partials = std::move(partials_);
     tiles_.alloc(std::max<size_t>(1, tiles.size()));
     tiles_.upload(tiles.data(), tiles.size(), stream_);
     descs_.alloc(std::max<size_t>(1, h_descs_.size()));
     descs_.upload(h_descs_.data(), h_descs_.size(), stream_);
     partials_.alloc(std::max<size_t>(kPartialSlots, kPartialSlots * tiles.size()));
+    if (any_src) {
+        DevBuf<int32_t> d_src(src.size());
+        d_src.upload(src.data(), src.size(), stream_);
+        for (int bs = 1; bs <= 3; ++bs)
+            launch_carry_partials(old_partials.p + (size_t)kPartialSlots * old_first[bs], old_count[bs],
+                                  partials_.p + (size_t)kPartialSlots * first[bs], count[bs], d_src.p + first[bs],
+                                  count[bs], stream_);
+        SVX_HIP(hipStreamSynchronize(stream_));
+    }
     for (int bs = 1; bs <= 3; ++bs) {
         KrylovSet& k = groups_[bs];
         k.tiles = tiles_.p + first[bs];
@@ -163,7 +206,7 @@ int Engine::run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag
             read_scalars(sc);
             std::vector<char> live(h_descs_.size());
             for (size_t i = 0; i < sc.size(); ++i) live[i] = sc[i].status == ST_RUNNING;
-            build_tiles(live);
+            build_tiles(live, true);
         }
     }
 }

@@ -47,7 +47,10 @@ class Engine {
    private:
     int run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks);
     void release_graph();
-    void build_tiles(const std::vector<char>& running);
+    // Tile tables for the instances with running[i]. carry: keep the pending
+    // per-tile partials of instances that stay tiled (moved to their new
+    // tile positions), so a set can be re-tiled between chunks.
+    void build_tiles(const std::vector<char>& running, bool carry = false);
     void enqueue_init(const EngineOptions& opt);
     void enqueue_chunk(const EngineOptions& opt, int iters);
     hipStream_t stream_;
@@ -58,13 +61,14 @@ class Engine {
     DevBuf<double> partials_;
     KrylovSet groups_[4]{};
     int tiled_instances_ = 0;
+    std::vector<char> tiled_;  // instance i has tiles in the current tables
     hipGraphExec_t exec_ = nullptr;
     int exec_precision_ = -1, exec_period_ = -1, exec_chunk_ = -1;
     int64_t launches_ = 0;
 };
 
 int default_lanes(double avg_row_len);
-int default_rows_per_tile(int bs);
+int default_rows_per_tile(int bs, int precision);
 // Lanes per row (CSR) and per block row (BSR) for node-block size bs (2, 3);
 // a function of bs alone, so an instance gets the same launch shape — and the
 // same bits — alone and inside any batch.

@@ -67,26 +67,28 @@ __device__ __forceinline__ void block_allreduce(double (&v)[NV]) {
     __syncthreads();
 }
 
-// This tile's partials (block-reduced) into its slots.
+// This tile's partials (block-reduced) into its slots. Partials are stored
+// slot-major, partials[slot * stride + tile], so that one slot of an
+// instance is a contiguous run of its tiles.
 template <int NV>
-__device__ __forceinline__ void write_partials(double (&v)[NV], const int (&slot)[NV], double* partials, int tile) {
+__device__ __forceinline__ void write_partials(double (&v)[NV], const int (&slot)[NV], double* partials, int stride,
+                                               int tile) {
     block_allreduce<NV>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) partials[(size_t)tile * kPartialSlots + slot[i]] = v[i];
+        for (int i = 0; i < NV; ++i) partials[(size_t)slot[i] * stride + tile] = v[i];
     }
 }
 
 // The instance's sums of the given slots over its tiles, in tile order.
 template <int NV>
-__device__ __forceinline__ void reduce_tiles(const double* __restrict__ partials, int tile_begin, int tile_count,
-                                             const int (&slot)[NV], double (&out)[NV]) {
+__device__ __forceinline__ void reduce_tiles(const double* __restrict__ partials, int stride, int tile_begin,
+                                             int tile_count, const int (&slot)[NV], double (&out)[NV]) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) out[i] = 0.0;
     for (int k = threadIdx.x; k < tile_count; k += kBlock) {
-        const double* p = partials + (size_t)(tile_begin + k) * kPartialSlots;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) out[i] += p[slot[i]];
+        for (int i = 0; i < NV; ++i) out[i] += partials[(size_t)slot[i] * stride + tile_begin + k];
     }
     block_allreduce<NV>(out);
 }
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void spmv_bsr32_kernel(int64_t nb, const in
 // Cold start (u = 0), FP64 vectors.
 __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
-                                                           double* __restrict__ partials) {
+                                                           double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     const InstScal* S = D.s;
@@ -196,14 +198,14 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
         v[1] += fi * zi;
     }
     const int slot[2] = {P_IFF, P_IRZ};
-    write_partials<2>(v, slot, partials, blockIdx.x);
+    write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
 // Warm start: FP64 recurrence from the current u, r = f - K u.
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __restrict__ tiles,
                                                                 const InstDesc* __restrict__ descs,
-                                                                double* __restrict__ partials) {
+                                                                double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     const InstScal* S = D.s;
@@ -224,13 +226,13 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
         }
     }
     const int slot[2] = {P_IFF, P_IRZ};
-    write_partials<2>(v, slot, partials, blockIdx.x);
+    write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
 // Cold start (u = 0), FP32-IR vectors.
 __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
-                                                           double* __restrict__ partials) {
+                                                           double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     const InstScal* S = D.s;
@@ -249,13 +251,13 @@ __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restric
         v[1] += (double)ri * (double)zi;
     }
     const int slot[2] = {P_IFF, P_IRZ};
-    write_partials<2>(v, slot, partials, blockIdx.x);
+    write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
 // Leader tile only: scalars of the freshly initialised instances.
 __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
-                                                             const double* __restrict__ partials) {
+                                                             const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     if (t.row_begin != 0) return;
     const InstDesc& D = descs[t.inst];
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
     if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
     const int slot[2] = {P_IFF, P_IRZ};
     double v[2];
-    reduce_tiles<2>(partials, D.tile_begin, D.tile_count, slot, v);
+    reduce_tiles<2>(partials, pstride, D.tile_begin, D.tile_count, slot, v);
     if (threadIdx.x != 0) return;
     if (S->flags & FL_WARM) {  // v[0] = r . r of the warm residual
         S->rr = v[0];
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
-                                                           double* __restrict__ partials) {
+                                                           double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     const InstScal* S = D.s;
@@ -311,19 +313,19 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
         }
     }
     const int slot[1] = {P_PQ};
-    write_partials<1>(v, slot, partials, blockIdx.x);
+    write_partials<1>(v, slot, partials, pstride, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
-                                                             double* __restrict__ partials) {
+                                                             double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const int sp[1] = {P_PQ};
     double pq[1];
-    reduce_tiles<1>(partials, D.tile_begin, D.tile_count, sp, pq);
+    reduce_tiles<1>(partials, pstride, D.tile_begin, D.tile_count, sp, pq);
     if (pq[0] <= 0.0 || pq[0] != pq[0]) {  // solver.rs:118-123
         if (is_leader(t)) {
             S->pap = pq[0];
@@ -352,19 +354,19 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
         v[1] += ri * zi;
     }
     const int slot[2] = {P_RR, P_RZ};
-    write_partials<2>(v, slot, partials, blockIdx.x);
+    write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __restrict__ tiles,
                                                               const InstDesc* __restrict__ descs,
-                                                              const double* __restrict__ partials) {
+                                                              const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const int sl[2] = {P_RR, P_RZ};
     double v[2];
-    reduce_tiles<2>(partials, D.tile_begin, D.tile_count, sl, v);
+    reduce_tiles<2>(partials, pstride, D.tile_begin, D.tile_count, sl, v);
     const double res = sqrt(v[0]) / S->bnorm;
     const bool conv = res <= S->tol;
     const double beta = v[1] / S->snap_rz;
@@ -389,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
 template <int BS, int LANES>
 __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
-                                                           double* __restrict__ partials) {
+                                                           double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     const InstScal* S = D.s;
@@ -412,19 +414,19 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
         }
     }
     const int slot[1] = {P_PQ};
-    write_partials<1>(v, slot, partials, blockIdx.x);
+    write_partials<1>(v, slot, partials, pstride, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
-                                                             double* __restrict__ partials) {
+                                                             double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const int sp[1] = {P_PQ};
     double pq[1];
-    reduce_tiles<1>(partials, D.tile_begin, D.tile_count, sp, pq);
+    reduce_tiles<1>(partials, pstride, D.tile_begin, D.tile_count, sp, pq);
     if (pq[0] <= 0.0 || pq[0] != pq[0]) {
         if (is_leader(t)) {
             S->pap = pq[0];
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
         v[1] += (double)ri * (double)zi;
     }
     const int slot[2] = {P_RR, P_RZ};
-    write_partials<2>(v, slot, partials, blockIdx.x);
+    write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
 // p = z + beta p and the refresh decision. At the last iteration of a period
@@ -464,14 +466,15 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
 // from the replaced residual.
 __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __restrict__ tiles,
                                                               const InstDesc* __restrict__ descs,
-                                                              const double* __restrict__ partials, int period_end) {
+                                                              const double* __restrict__ partials, int pstride,
+                                                              int period_end) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const int sl[2] = {P_RR, P_RZ};
     double v[2];
-    reduce_tiles<2>(partials, D.tile_begin, D.tile_count, sl, v);
+    reduce_tiles<2>(partials, pstride, D.tile_begin, D.tile_count, sl, v);
     const int64_t it = S->snap_it + 1;
     const double est = sqrt(v[0]);
     const double rmax = fmax(S->snap_rmax, est);
@@ -514,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void ir32_fold_kernel(const Tile* __restric
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __restrict__ tiles,
                                                                const InstDesc* __restrict__ descs,
-                                                               double* __restrict__ partials) {
+                                                               double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     const InstScal* S = D.s;
@@ -548,21 +551,21 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
         }
     }
     const int slot[3] = {P_TRR, P_TRZ, P_TFL};
-    write_partials<3>(v, slot, partials, blockIdx.x);
+    write_partials<3>(v, slot, partials, pstride, blockIdx.x);
 }
 
 // Refresh step 3: convergence / budget / fallback decision, then
 // p = z + beta p from the replaced residual.
 __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __restrict__ tiles,
                                                               const InstDesc* __restrict__ descs,
-                                                              const double* __restrict__ partials) {
+                                                              const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
     InstScal* S = D.s;
     if (S->status != ST_RUNNING || !(S->snap_flags & FL_REFRESH)) return;
     const int sl[3] = {P_TRR, P_TRZ, P_TFL};
     double v[3];
-    reduce_tiles<3>(partials, D.tile_begin, D.tile_count, sl, v);
+    reduce_tiles<3>(partials, pstride, D.tile_begin, D.tile_count, sl, v);
     const double res = sqrt(v[0]) / S->bnorm;
     const double floor = S->floor_scale * sqrt(v[2]) / S->bnorm;
     const bool improved = res < 0.5 * S->snap_res_best;
@@ -633,6 +636,15 @@ __global__ void set_running_kernel(const InstDesc* __restrict__ descs, int32_t n
     }
 }
 
+__global__ void carry_partials_kernel(const double* __restrict__ old_p, int old_stride, double* __restrict__ new_p,
+                                      int new_stride, const int32_t* __restrict__ src, int n_new) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_new) return;
+    const int o = src[g];
+    if (o < 0 || o >= old_stride) return;
+    for (int k = 0; k < kPartialSlots; ++k) new_p[(size_t)k * new_stride + g] = old_p[(size_t)k * old_stride + o];
+}
+
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 #define SVX_LANES_DISPATCH(lanes, MACRO) \
@@ -680,58 +692,58 @@ void launch_spmv_bsr32(int64_t nb, int bs, const int32_t* brow_ptr, const int32_
 
 void cg64_init(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-    cg64_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
-#define L(LN) cg64_init_warm_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+    cg64_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+#define L(LN) cg64_init_warm_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
     SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
-    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
+    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
 
 void cg64_iteration(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-#define L(LN) cg64_spmv_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) cg64_spmv_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
     SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
-    cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
-    cg64_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
+    cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    cg64_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
 
 void ir32_init(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-    ir32_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
-    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
+    ir32_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
 
 void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end) {
     if (k.n_tiles <= 0) return;
     if (k.bs == 3) {
-#define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.blanes, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(LN) ir32_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) ir32_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.blanes, L)
 #undef L
     } else {
-#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.blanes, L)
 #undef L
     }
-    ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
-    ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, period_end ? 1 : 0);
+    ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, period_end ? 1 : 0);
     SVX_HIP(hipGetLastError());
 }
 
 void ir32_refresh(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     ir32_fold_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
-#define L(LN) ir32_residual_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) ir32_residual_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
     SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
-    ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials);
+    ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
 
@@ -754,22 +766,29 @@ void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s) {
 void spmv_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     if (precision == SVX_FP64) {
-#define L(LN) cg64_spmv_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) cg64_spmv_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     } else if (k.bs == 3) {
-#define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.blanes, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(LN) ir32_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) ir32_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.blanes, L)
 #undef L
     } else {
-#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials)
+#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.blanes, L)
 #undef L
     }
+    SVX_HIP(hipGetLastError());
+}
+
+void launch_carry_partials(const double* old_p, int old_stride, double* new_p, int new_stride, const int32_t* src,
+                           int n_new, hipStream_t s) {
+    if (n_new <= 0 || old_stride <= 0) return;
+    carry_partials_kernel<<<grid_for(n_new, kBlock), kBlock, 0, s>>>(old_p, old_stride, new_p, new_stride, src, n_new);
     SVX_HIP(hipGetLastError());
 }
 

@@ -35,8 +35,9 @@ def test_struct_layouts_match_the_c_header():
 #include <stddef.h>
 #include "svx.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(svx_error), sizeof(svx_model_desc), sizeof(svx_solution),
-         offsetof(svx_model_desc, thickness), offsetof(svx_model_desc, cload_value), offsetof(svx_solution, recover_ms));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(svx_error), sizeof(svx_model_desc), sizeof(svx_solution),
+         offsetof(svx_model_desc, thickness), offsetof(svx_model_desc, cload_value), offsetof(svx_solution, recover_ms),
+         sizeof(svx_solve_stats), offsetof(svx_solve_stats, status));
   return 0;
 }
 """
@@ -48,7 +49,7 @@ int main(void) {
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     got = [C.sizeof(_abi.SvxError), C.sizeof(_abi.SvxModelDesc), C.sizeof(_abi.SvxSolution),
            _abi.SvxModelDesc.thickness.offset, _abi.SvxModelDesc.cload_value.offset,
-           _abi.SvxSolution.recover_ms.offset]
+           _abi.SvxSolution.recover_ms.offset, C.sizeof(_abi.SvxSolveStats), _abi.SvxSolveStats.status.offset]
     assert [int(x) for x in out] == got
 
 
