@@ -169,8 +169,10 @@ const char* svx_last_error(void);
  * (0 / <= 0 keeps the automatic choice): CSR lanes per row, BSR lanes per
  * block row, rows per workgroup tile, FP32-IR refresh period, iterations per
  * captured graph, graph capture on/off, and the FP32-IR refresh threshold
- * (replace the residual when ||r|| < ru_delta * ||r at last refresh||).
- * No reference counterpart: the autotuner drives it. */
+ * (replace the residual when ||r|| < ru_delta * max ||r|| since the last
+ * refresh). Lane counts must be 0 or a power of two from 2 to 64, and
+ * 0 <= ru_delta < 1; anything else returns SVX_E_ARGUMENT. No reference
+ * counterpart: the autotuner drives it. */
 int svx_set_tuning(svx_ctx* ctx, int lanes, int blanes, int rows_per_tile,
                    int period, int chunk, int use_graph, double ru_delta);
 
@@ -318,6 +320,18 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol,
  * ------------------------------------------------------------------------- */
 int svx_system_time_spmv(svx_system* sys, int precision, int reps,
                          double* avg_ms, double* bytes_per_launch);
+
+/* Measurement hook (no reference counterpart): average device time of each
+ * phase of one solver iteration, each launched alone on this system with the
+ * solve's launch shape, reps times: avg_ms[0] SpMV, [1] update (alpha, u, r,
+ * z), [2] direction update (beta, p), [3] FP32-IR refresh (fold, FP64
+ * residual, repoint; 0 for FP64). bytes_per_launch[4] (or NULL) receives the
+ * algorithmic bytes of each phase. The system must have been solved with that
+ * precision; its solver state and solution are overwritten (solve again
+ * before reading either). */
+int svx_system_time_phases(svx_system* sys, int precision, int reps,
+                           double* avg_ms /* [4] */,
+                           double* bytes_per_launch /* [4] */);
 
 #ifdef __cplusplus
 }

@@ -316,6 +316,15 @@ class System:
         _raise(lib().svx_system_time_spmv(self._h, precision, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def time_phases(self, precision: int, reps: int = 20) -> Tuple[np.ndarray, np.ndarray]:
+        """(average ms, algorithmic bytes) of [SpMV, update, direction update,
+        FP32-IR refresh], each phase launched alone. Overwrites the solver
+        state and the solution: solve again before reading them."""
+        ms = np.zeros(4)
+        by = np.zeros(4)
+        _raise(lib().svx_system_time_phases(self._h, precision, reps, _d(ms), _d(by)))
+        return ms, by
+
     def close(self):
         if self._h:
             lib().svx_system_destroy(self._h)

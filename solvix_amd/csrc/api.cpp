@@ -283,6 +283,18 @@ InstDesc system_desc(svx_system* sys) {
     return d;
 }
 
+// Algorithmic HBM bytes of one solver SpMV launch on this mesh's matrix: the
+// matrix stream, its index arrays, one read of x and one write of y.
+double spmv_bytes(const svx_mesh* m, int precision) {
+    const double n = (double)m->n, nnz = (double)m->nnz, nodes = (double)m->n_nodes, nblk = (double)m->nblk;
+    if (precision == SVX_FP64) {
+        if (m->ndof > 1)  // FP64 values + block columns + block row_ptr + node row starts + x + y
+            return 8.0 * nnz + 4.0 * nblk + 4.0 * (nodes + 1) + 8.0 * nodes + 16.0 * n;
+        return 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;  // values + columns + row_ptr + x + y
+    }
+    return 4.0 * nnz + 4.0 * nblk + 4.0 * (nodes + 1) + 8.0 * n;  // FP32 blocks + block columns + row_ptr + x + y
+}
+
 InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len) {
     InstScal sc{};
     sc.tol = tol;
@@ -703,6 +715,7 @@ int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int
     });
 }
 
+
 int svx_system_time_spmv(svx_system* sys, int precision, int reps, double* avg_ms, double* bytes_per_launch) {
     if (!sys || reps <= 0 || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
     return guarded(nullptr, [&] {
@@ -711,12 +724,31 @@ int svx_system_time_spmv(svx_system* sys, int precision, int reps, double* avg_m
         svx_mesh* m = sys->mesh;
         double ms = sys->engine->time_spmv(precision, reps);
         if (avg_ms) *avg_ms = ms;
+        if (bytes_per_launch) *bytes_per_launch = spmv_bytes(m, precision);
+        return SVX_OK;
+    });
+}
+
+int svx_system_time_phases(svx_system* sys, int precision, int reps, double* avg_ms, double* bytes_per_launch) {
+    if (!sys || reps <= 0 || !avg_ms || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        if (!sys->engine || sys->engine_precision != precision)
+            throw std::invalid_argument("solve the system with this precision first");
+        svx_mesh* m = sys->mesh;
+        sys->engine->time_phases(precision, reps, avg_ms);
         if (bytes_per_launch) {
             const double n = (double)m->n, nnz = (double)m->nnz;
-            if (precision == SVX_FP64)  // vals + cols + row_ptr + x (once) + y
-                *bytes_per_launch = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;
-            else  // FP32 block values + block cols + block row_ptr + x (once) + y
-                *bytes_per_launch = 4.0 * nnz + 4.0 * (double)m->nblk + 4.0 * (m->n_nodes + 1) + 8.0 * n;
+            bytes_per_launch[0] = spmv_bytes(m, precision);
+            if (precision == SVX_FP64) {
+                bytes_per_launch[1] = 64.0 * n;  // p q r u dinv read, u r z written
+                bytes_per_launch[2] = 24.0 * n;  // z p read, p written
+                bytes_per_launch[3] = 0.0;
+            } else {
+                bytes_per_launch[1] = 32.0 * n;  // p q r x dinv read, x r z written (FP32)
+                bytes_per_launch[2] = 12.0 * n;
+                // fold (u, x32 read+write) + FP64 residual (CSR, u, f, dinv32; r32 z32 written) + repoint
+                bytes_per_launch[3] = 24.0 * n + (12.0 * nnz + 4.0 * (n + 1) + 28.0 * n) + 12.0 * n;
+            }
         }
         return SVX_OK;
     });
@@ -725,7 +757,10 @@ int svx_system_time_spmv(svx_system* sys, int precision, int reps, double* avg_m
 // --------------------------------------------------------------- tuning knobs
 int svx_set_tuning(svx_ctx* ctx, int lanes, int blanes, int rows_per_tile, int period, int chunk, int use_graph,
                    double ru_delta) {
-    if (!ctx) return SVX_E_ARGUMENT;
+    auto lanes_ok = [](int l) { return l == 0 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64; };
+    if (!ctx || !lanes_ok(lanes) || !lanes_ok(blanes) || rows_per_tile < 0 || period < 0 || chunk < 0 ||
+        ru_delta < 0.0 || ru_delta >= 1.0)
+        return SVX_E_ARGUMENT;
     TuneParams& t = ctx->tune;
     t.lanes = lanes;
     t.blanes = blanes;

@@ -113,6 +113,7 @@ void system_set_f(svx_system* sys, const double* f);
 void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows);
 void system_prepare(svx_system* sys, int precision);
 InstDesc system_desc(svx_system* sys);
+double spmv_bytes(const svx_mesh* m, int precision);
 InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len);
 void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,
                   const TuneParams& tp);

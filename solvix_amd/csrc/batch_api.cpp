@@ -150,13 +150,7 @@ int svx_batch_time_spmv(svx_batch* b, int reps, double* avg_ms, double* bytes_pe
     return guarded_b(nullptr, [&] {
         double ms = b->engine->time_spmv(b->precision, reps);
         double bytes = 0.0;
-        for (svx_system* sy : b->sys) {
-            const svx_mesh* m = sy->mesh;
-            if (b->precision == SVX_FP64)
-                bytes += 12.0 * m->nnz + 4.0 * (m->n + 1) + 16.0 * m->n;
-            else
-                bytes += 4.0 * m->nnz + 4.0 * m->nblk + 4.0 * (m->n_nodes + 1) + 8.0 * m->n;
-        }
+        for (svx_system* sy : b->sys) bytes += spmv_bytes(sy->mesh, b->precision);
         if (avg_ms) *avg_ms = ms;
         if (bytes_per_launch) *bytes_per_launch = bytes;
         return SVX_OK;

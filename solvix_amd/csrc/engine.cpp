@@ -10,16 +10,16 @@ int default_lanes(double avg) {
     return l;
 }
 
-// Launch shapes measured on the 265,923-DOF C3D8 block (FP64: 8 lanes per
-// CSR row, 192 rows per tile; FP32: 16 lanes per 3x3 block row, 144 rows per
-// tile) and on plane meshes for bs = 2.
+// Launch shapes measured on the 265,923-DOF C3D8 block (FP64: 16 lanes per
+// node row, 288 rows per tile; FP32: 16 lanes per 3x3 block row, 288 rows
+// per tile) and on 2e5-DOF plane meshes (4 lanes per node row, 256 rows).
 int default_rows_per_tile(int bs, int precision) {
-    if (bs == 3) return precision == SVX_FP64 ? 192 : 144;
-    return 256;
+    (void)precision;
+    return bs == 3 ? 288 : 256;
 }
 
-int lanes_for_bs(int bs) { return bs == 3 ? 8 : 4; }
-int blanes_for_bs(int bs) { return bs == 3 ? 16 : 2; }
+int lanes_for_bs(int bs) { return bs == 3 ? 16 : 4; }
+int blanes_for_bs(int bs) { return bs == 3 ? 16 : 4; }
 
 Engine::~Engine() { release_graph(); }
 
@@ -62,8 +62,9 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
             InstDesc& d = h_descs_[i];
             if (d.bs != bs || !running[i]) continue;
             d.tile_begin = (int32_t)tiles.size() - first[bs];
-            for (int32_t r = 0; r < d.n; r += rpt) tiles.push_back(Tile{(int32_t)i, r, std::min(d.n, r + rpt), 0});
-            d.tile_count = (int32_t)tiles.size() - first[bs] - d.tile_begin;
+            d.tile_count = (d.n + rpt - 1) / rpt;
+            for (int32_t r = 0; r < d.n; r += rpt)
+                tiles.push_back(Tile{(int32_t)i, r, std::min(d.n, r + rpt), d.tile_begin, d.tile_count, 0, d.s});
             tiled_instances_++;
             tiled_[i] = 1;
         }
@@ -218,6 +219,35 @@ void Engine::read_scalars(std::vector<InstScal>& out) const {
     launch_gather_scalars(descs_.p, (int32_t)h_descs_.size(), tmp.p, stream_);
     tmp.download(out.data(), out.size(), stream_);
     SVX_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::time_phases(int precision, int reps, double ms[4]) {
+    hipEvent_t a, b;
+    SVX_HIP(hipEventCreate(&a));
+    SVX_HIP(hipEventCreate(&b));
+    const int n_inst = (int32_t)h_descs_.size();
+    for (int ph = 0; ph < 4; ++ph) {
+        ms[ph] = 0.0;
+        if (ph == 3 && precision == SVX_FP64) continue;
+        for (int r = 0; r <= reps; ++r) {  // r == 0 warms up
+            launch_set_running(descs_.p, n_inst, stream_, ph == 3 ? FL_REFRESH : 0);
+            SVX_HIP(hipEventRecord(a, stream_));
+            for (int bs = 1; bs <= 3; ++bs) {
+                if (ph == 0) spmv_phase(groups_[bs], precision, stream_);
+                if (ph == 1) update_phase(groups_[bs], precision, stream_);
+                if (ph == 2) pupdate_phase(groups_[bs], precision, stream_);
+                if (ph == 3) ir32_refresh(groups_[bs], stream_);
+            }
+            SVX_HIP(hipEventRecord(b, stream_));
+            SVX_HIP(hipEventSynchronize(b));
+            float t = 0.0f;
+            SVX_HIP(hipEventElapsedTime(&t, a, b));
+            if (r > 0) ms[ph] += t;
+        }
+        ms[ph] /= reps;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
 }
 
 double Engine::time_spmv(int precision, int reps) {

@@ -41,6 +41,11 @@ class Engine {
     // Download the per-instance scalars (status, iterations, residual).
     void read_scalars(std::vector<InstScal>& out) const;
     double time_spmv(int precision, int reps);
+    // Average device time of each phase of an iteration, launched alone on
+    // the current set with every instance forced to run: [0] SpMV, [1] update,
+    // [2] direction update, [3] FP32-IR refresh (fold, FP64 residual,
+    // repoint; 0 for FP64). Leaves the instances' solver state undefined.
+    void time_phases(int precision, int reps, double ms[4]);
     int64_t launches() const { return launches_; }
     int n_instances() const { return (int)h_descs_.size(); }
 

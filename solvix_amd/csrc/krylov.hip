@@ -17,7 +17,9 @@
 // scheduling nor on which other instances share the launch.
 //
 // FP64 arm: the recurrence of solver.rs:82-149 (u, r, z, p, alpha, beta,
-// breakdown and budget tests), CSR SpMV with `lanes` lanes per row.
+// breakdown and budget tests). Plane and solid meshes multiply node rows (the
+// ndof rows of one node share their column structure) as node blocks, with
+// `lanes` lanes per node row; raw CSR input multiplies row by row.
 // FP32-IR arm: the same recurrence on an FP32 node-block (BSR) copy of K and
 // FP32 vectors with FP64-accumulated dots, with reliable-update refinement
 // (residual replacement): every `period` iterations an instance whose
@@ -27,9 +29,11 @@
 // recomputes r = f - K u in FP64, and continues with p = z + beta p built from
 // the replaced residual. The solve ends when the FP64 residual satisfies
 // ||f - K u|| <= tol ||f|| + floor_scale || |K||u| + |f| ||, i.e. reaches tol
-// or the rounding level at which FP64 can still resolve f - K u. An instance
-// whose FP64 residual stops improving (four replacements without halving)
-// falls back to FP64 PCG from its current u.
+// or the rounding level at which FP64 can still resolve f - K u (within
+// kFloorSlack of that level once two replacements in a row failed to halve
+// the residual). An instance whose FP64 residual stops improving above that
+// level (four replacements without halving) falls back to FP64 PCG from its
+// current u.
 #include "svx_internal.hpp"
 
 namespace svx {
@@ -39,6 +43,7 @@ constexpr int kWave = 64;
 constexpr int kWaves = kBlock / kWave;
 constexpr int32_t FL_HOLD = 2;          // FP32-IR: budget reached, awaiting the final FP64 check
 constexpr int64_t kForcedRefresh = 256; // FP32-IR: check the FP64 residual at least this often
+constexpr double kFloorSlack = 4.0;     // FP32-IR: floors accepted once the residual stagnates
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -80,13 +85,27 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], const int (&slot
     }
 }
 
-// The instance's sums of the given slots over its tiles, in tile order.
+// The instance's sums of the given slots over its tiles, in tile order
+// (thread k adds tiles k, k + kBlock, ... in turn; the first four loads of a
+// thread are issued together).
 template <int NV>
 __device__ __forceinline__ void reduce_tiles(const double* __restrict__ partials, int stride, int tile_begin,
                                              int tile_count, const int (&slot)[NV], double (&out)[NV]) {
+    constexpr int kAhead = 4;
+    double buf[kAhead][NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) out[i] = 0.0;
-    for (int k = threadIdx.x; k < tile_count; k += kBlock) {
+    for (int j = 0; j < kAhead; ++j) {
+        const int k = threadIdx.x + j * kBlock;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) buf[j][i] = k < tile_count ? partials[(size_t)slot[i] * stride + tile_begin + k] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        out[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < kAhead; ++j) out[i] += buf[j][i];
+    }
+    for (int k = threadIdx.x + kAhead * kBlock; k < tile_count; k += kBlock) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) out[i] += partials[(size_t)slot[i] * stride + tile_begin + k];
     }
@@ -106,6 +125,45 @@ __device__ __forceinline__ double csr_row(const int32_t* __restrict__ rp, const 
     return acc;
 }
 
+// One node row (BS consecutive rows of one node) of the FP64 CSR matrix,
+// read as node blocks: the rows of a node share their column structure (every
+// neighbour node contributes BS consecutive columns to each row), so the
+// block column list bcol replaces the per-entry column indices and row a of
+// the node holds block k's entries at val[rp[br*BS] + a*BS*nb + BS*k + b].
+// A node whose rows hold only their diagonal (no element touches it) is
+// multiplied entry by entry.
+template <int BS, int LANES>
+__device__ __forceinline__ void node_row64(const int32_t* __restrict__ rp, const int32_t* __restrict__ brp,
+                                           const int32_t* __restrict__ bcol, const double* __restrict__ val,
+                                           const double* __restrict__ x, int32_t br, int lane, double (&acc)[BS]) {
+#pragma unroll
+    for (int a = 0; a < BS; ++a) acc[a] = 0.0;
+    const int32_t kb = brp[br], nb = brp[br + 1] - kb;
+    const int32_t r0 = rp[br * BS];
+    if (rp[br * BS + BS] - r0 != BS * BS * nb) {  // isolated node: diagonal entries only
+        if (lane == 0)
+#pragma unroll
+            for (int a = 0; a < BS; ++a) acc[a] = val[rp[br * BS + a]] * x[br * BS + a];
+    } else {
+        const double* v0 = val + r0;
+#pragma unroll 1
+        for (int32_t k = lane; k < nb; k += LANES) {
+            const int32_t j = bcol[kb + k];
+            double xj[BS];
+#pragma unroll
+            for (int b = 0; b < BS; ++b) xj[b] = x[j * BS + b];
+#pragma unroll
+            for (int a = 0; a < BS; ++a)
+#pragma unroll
+                for (int b = 0; b < BS; ++b) acc[a] += v0[a * BS * nb + BS * k + b] * xj[b];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < BS; ++a)
+#pragma unroll
+        for (int off = LANES / 2; off > 0; off >>= 1) acc[a] += __shfl_xor(acc[a], off, LANES);
+}
+
 // One block row of the FP32 node-block matrix (values stored component-major:
 // bval[(a*BS+b)*nblk + k]) times x; FP64 accumulation.
 template <int BS, int LANES>
@@ -115,7 +173,7 @@ __device__ __forceinline__ void bsr_row(const int32_t* __restrict__ brp, const i
 #pragma unroll
     for (int a = 0; a < BS; ++a) acc[a] = 0.0;
     const int32_t kb = brp[br], ke = brp[br + 1];
-#pragma unroll 2
+#pragma unroll 1
     for (int32_t k = kb + lane; k < ke; k += LANES) {
         int32_t j = bcol[k];
         double xj[BS];
@@ -130,6 +188,15 @@ __device__ __forceinline__ void bsr_row(const int32_t* __restrict__ brp, const i
     for (int a = 0; a < BS; ++a)
 #pragma unroll
         for (int off = LANES / 2; off > 0; off >>= 1) acc[a] += __shfl_xor(acc[a], off, LANES);
+}
+
+template <int BS>
+__device__ __forceinline__ double pick(const double (&acc)[BS], int lane) {
+    double v = acc[0];
+#pragma unroll
+    for (int a = 1; a < BS; ++a)
+        if (lane == a) v = acc[a];
+    return v;
 }
 
 __device__ __forceinline__ bool is_leader(const Tile& t) { return t.row_begin == 0 && threadIdx.x == 0; }
@@ -168,12 +235,10 @@ __global__ __launch_bounds__(kBlock) void spmv_bsr32_kernel(int64_t nb, const in
     if (br >= nb) return;
     double acc[BS];
     bsr_row<BS, LANES>(brp, bcol, bval, brp[nb], x, (int32_t)br, lane, acc);
-    if (lane < BS) {
-        double v = acc[0];
-#pragma unroll
-        for (int a = 1; a < BS; ++a)
-            if (lane == a) v = acc[a];
-        y[br * BS + lane] = (float)v;
+    if constexpr (LANES >= BS) {
+        if (lane < BS) y[br * BS + lane] = (float)pick<BS>(acc, lane);
+    } else {
+        for (int a = lane; a < BS; a += LANES) y[br * BS + a] = (float)pick<BS>(acc, a);
     }
 }
 
@@ -184,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    const InstScal* S = D.s;
+    const InstScal* S = t.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != FL_FRESH) return;
     double v[2] = {0.0, 0.0};
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
@@ -208,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
                                                                 double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    const InstScal* S = D.s;
+    const InstScal* S = t.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != (FL_FRESH | FL_WARM)) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
@@ -235,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restric
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    const InstScal* S = D.s;
+    const InstScal* S = t.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != FL_FRESH) return;
     double v[2] = {0.0, 0.0};
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
@@ -261,11 +326,11 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
     const Tile t = tiles[blockIdx.x];
     if (t.row_begin != 0) return;
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
+    InstScal* S = t.s;
     if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
     const int slot[2] = {P_IFF, P_IRZ};
     double v[2];
-    reduce_tiles<2>(partials, pstride, D.tile_begin, D.tile_count, slot, v);
+    reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, slot, v);
     if (threadIdx.x != 0) return;
     if (S->flags & FL_WARM) {  // v[0] = r . r of the warm residual
         S->rr = v[0];
@@ -294,22 +359,46 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
 }
 
 // ------------------------------------------------------------ FP64 PCG
-template <int LANES>
+// q = K p and the partial p.q. BS = 1: one CSR row per LANES lanes;
+// BS = 2, 3: one node row (BS rows) per LANES lanes, read as node blocks.
+template <int BS, int LANES>
 __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    const InstScal* S = D.s;
+    const InstScal* S = t.s;
     if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
     double v[1] = {0.0};
-    for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
-        double acc = csr_row<LANES>(D.row_ptr, D.col, D.val, D.p, row, lane);
-        if (lane == 0) {
-            D.q[row] = acc;
-            v[0] += D.p[row] * acc;
+    if (BS == 1) {
+        for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
+            double acc = csr_row<LANES>(D.row_ptr, D.col, D.val, D.p, row, lane);
+            if (lane == 0) {
+                D.q[row] = acc;
+                v[0] += D.p[row] * acc;
+            }
+        }
+    } else {
+        for (int32_t br = t.row_begin / BS + grp; br < t.row_end / BS; br += G) {
+            double acc[BS];
+            node_row64<BS, LANES>(D.row_ptr, D.brow_ptr, D.bcol, D.val, D.p, br, lane, acc);
+            if constexpr (LANES >= BS) {
+                if (lane < BS) {
+                    const int32_t i = br * BS + lane;
+                    const double qi = pick<BS>(acc, lane);
+                    D.q[i] = qi;
+                    v[0] += D.p[i] * qi;
+                }
+            } else {
+                for (int a = lane; a < BS; a += LANES) {
+                    const int32_t i = br * BS + a;
+                    const double qi = pick<BS>(acc, a);
+                    D.q[i] = qi;
+                    v[0] += D.p[i] * qi;
+                }
+            }
         }
     }
     const int slot[1] = {P_PQ};
@@ -321,11 +410,11 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
                                                              double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    InstScal* S = t.s;
     const int sp[1] = {P_PQ};
     double pq[1];
-    reduce_tiles<1>(partials, pstride, D.tile_begin, D.tile_count, sp, pq);
+    reduce_tiles<1>(partials, pstride, t.tile_begin, t.tile_count, sp, pq);
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     if (pq[0] <= 0.0 || pq[0] != pq[0]) {  // solver.rs:118-123
         if (is_leader(t)) {
             S->pap = pq[0];
@@ -362,11 +451,11 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
                                                               const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    InstScal* S = t.s;
     const int sl[2] = {P_RR, P_RZ};
     double v[2];
-    reduce_tiles<2>(partials, pstride, D.tile_begin, D.tile_count, sl, v);
+    reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const double res = sqrt(v[0]) / S->bnorm;
     const bool conv = res <= S->tol;
     const double beta = v[1] / S->snap_rz;
@@ -394,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    const InstScal* S = D.s;
+    const InstScal* S = t.s;
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
@@ -403,14 +492,22 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
     for (int32_t br = t.row_begin / BS + grp; br < t.row_end / BS; br += G) {
         double acc[BS];
         bsr_row<BS, LANES>(D.brow_ptr, D.bcol, D.bval, nblk, D.p32, br, lane, acc);
-        if (lane < BS) {
-            double a = acc[0];
+        if constexpr (LANES >= BS) {
+            if (lane < BS) {
+                double a = acc[0];
 #pragma unroll
-            for (int c = 1; c < BS; ++c)
-                if (lane == c) a = acc[c];
-            float qv = (float)a;
-            D.q32[br * BS + lane] = qv;
-            v[0] += (double)D.p32[br * BS + lane] * (double)qv;
+                for (int c = 1; c < BS; ++c)
+                    if (lane == c) a = acc[c];
+                float qv = (float)a;
+                D.q32[br * BS + lane] = qv;
+                v[0] += (double)D.p32[br * BS + lane] * (double)qv;
+            }
+        } else {
+            for (int c = lane; c < BS; c += LANES) {
+                float qv = (float)pick<BS>(acc, c);
+                D.q32[br * BS + c] = qv;
+                v[0] += (double)D.p32[br * BS + c] * (double)qv;
+            }
         }
     }
     const int slot[1] = {P_PQ};
@@ -422,11 +519,11 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
                                                              double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
+    InstScal* S = t.s;
     const int sp[1] = {P_PQ};
     double pq[1];
-    reduce_tiles<1>(partials, pstride, D.tile_begin, D.tile_count, sp, pq);
+    reduce_tiles<1>(partials, pstride, t.tile_begin, t.tile_count, sp, pq);
+    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     if (pq[0] <= 0.0 || pq[0] != pq[0]) {
         if (is_leader(t)) {
             S->pap = pq[0];
@@ -470,11 +567,11 @@ __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __rest
                                                               int period_end) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
+    InstScal* S = t.s;
     const int sl[2] = {P_RR, P_RZ};
     double v[2];
-    reduce_tiles<2>(partials, pstride, D.tile_begin, D.tile_count, sl, v);
+    reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
+    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const int64_t it = S->snap_it + 1;
     const double est = sqrt(v[0]);
     const double rmax = fmax(S->snap_rmax, est);
@@ -500,7 +597,7 @@ __global__ __launch_bounds__(kBlock) void ir32_fold_kernel(const Tile* __restric
                                                            const InstDesc* __restrict__ descs) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
+    InstScal* S = t.s;
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
     if (is_leader(t)) {
         S->snap_flags = S->flags;
@@ -520,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
                                                                double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    const InstScal* S = D.s;
+    const InstScal* S = t.s;
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
@@ -561,16 +658,19 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
                                                               const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const InstDesc& D = descs[t.inst];
-    InstScal* S = D.s;
-    if (S->status != ST_RUNNING || !(S->snap_flags & FL_REFRESH)) return;
+    InstScal* S = t.s;
     const int sl[3] = {P_TRR, P_TRZ, P_TFL};
     double v[3];
-    reduce_tiles<3>(partials, pstride, D.tile_begin, D.tile_count, sl, v);
+    reduce_tiles<3>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
+    if (S->status != ST_RUNNING || !(S->snap_flags & FL_REFRESH)) return;
     const double res = sqrt(v[0]) / S->bnorm;
     const double floor = S->floor_scale * sqrt(v[2]) / S->bnorm;
     const bool improved = res < 0.5 * S->snap_res_best;
     const int32_t stall = improved ? 0 : S->snap_stall + 1;
-    const bool conv = res <= S->tol + floor;
+    // Converged at tol, or at the FP64 evaluation floor: within the floor, or
+    // within kFloorSlack floors once the residual has stopped halving (the
+    // FP64 product K u cannot resolve f - K u any further).
+    const bool conv = res <= S->tol + floor || (stall >= 2 && res <= S->tol + kFloorSlack * floor);
     const bool hold = (S->snap_flags & FL_HOLD) != 0;
     const bool fallback = !conv && !hold && stall >= 4;
     const bool cont = !conv && !hold && !fallback;
@@ -629,10 +729,11 @@ __global__ void gather_scalars_kernel(const InstDesc* __restrict__ descs, int32_
     if (i < n_inst) out[i] = *descs[i].s;
 }
 
-__global__ void set_running_kernel(const InstDesc* __restrict__ descs, int32_t n_inst) {
+__global__ void set_running_kernel(const InstDesc* __restrict__ descs, int32_t n_inst, int32_t flags) {
     for (int32_t i = threadIdx.x; i < n_inst; i += blockDim.x) {
         descs[i].s->status = ST_RUNNING;
-        descs[i].s->flags = 0;
+        descs[i].s->flags = flags;
+        descs[i].s->snap_flags = flags;
     }
 }
 
@@ -656,6 +757,22 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
         case 32: MACRO(32); break;       \
         default: MACRO(64); break;       \
     }
+
+void launch_cg64_spmv(const KrylovSet& k, hipStream_t s) {
+    if (k.bs == 3) {
+#define L(LN) cg64_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_LANES_DISPATCH(k.lanes, L)
+#undef L
+    } else if (k.bs == 2) {
+#define L(LN) cg64_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_LANES_DISPATCH(k.lanes, L)
+#undef L
+    } else {
+#define L(LN) cg64_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_LANES_DISPATCH(k.lanes, L)
+#undef L
+    }
+}
 
 }  // namespace
 
@@ -702,9 +819,7 @@ void cg64_init(const KrylovSet& k, hipStream_t s) {
 
 void cg64_iteration(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-#define L(LN) cg64_spmv_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-    SVX_LANES_DISPATCH(k.lanes, L)
-#undef L
+    launch_cg64_spmv(k, s);
     cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     cg64_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
@@ -758,17 +873,33 @@ void launch_gather_scalars(const InstDesc* descs, int32_t n_inst, InstScal* out,
     SVX_HIP(hipGetLastError());
 }
 
-void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s) {
-    set_running_kernel<<<1, kBlock, 0, s>>>(descs, n_inst);
+void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s, int32_t flags) {
+    set_running_kernel<<<1, kBlock, 0, s>>>(descs, n_inst, flags);
+    SVX_HIP(hipGetLastError());
+}
+
+void update_phase(const KrylovSet& k, int precision, hipStream_t s) {
+    if (k.n_tiles <= 0) return;
+    if (precision == SVX_FP64)
+        cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    else
+        ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    SVX_HIP(hipGetLastError());
+}
+
+void pupdate_phase(const KrylovSet& k, int precision, hipStream_t s) {
+    if (k.n_tiles <= 0) return;
+    if (precision == SVX_FP64)
+        cg64_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    else
+        ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, 0);
     SVX_HIP(hipGetLastError());
 }
 
 void spmv_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     if (precision == SVX_FP64) {
-#define L(LN) cg64_spmv_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.lanes, L)
-#undef L
+        launch_cg64_spmv(k, s);
     } else if (k.bs == 3) {
 #define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.blanes, L)

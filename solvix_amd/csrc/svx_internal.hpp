@@ -141,11 +141,17 @@ struct InstDesc {
 };
 
 // A tile is a contiguous row range of one instance handled by one workgroup.
-struct Tile {
+// It carries its instance's scalar-state pointer and tile range, so a kernel
+// can issue the loads of the instance state, of the partials and of the
+// descriptor together, one dependent step after the tile itself.
+struct alignas(16) Tile {
     int32_t inst;
     int32_t row_begin;
     int32_t row_end;
+    int32_t tile_begin;  // the instance's first tile (in its group)
+    int32_t tile_count;  // the instance's tiles
     int32_t pad;
+    InstScal* s;
 };
 
 constexpr int kBlock = 256;
@@ -225,7 +231,10 @@ void launch_carry_partials(const double* old_p, int old_stride, double* new_p, i
 // The SpMV phase of one iteration alone (cg64 / ir32), for timing.
 void spmv_phase(const KrylovSet& k, int precision, hipStream_t s);
 // Set every instance of the set back to ST_RUNNING (timing helper).
-void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s);
+void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s, int32_t flags = 0);
+// The update and direction phases of one iteration alone, for timing.
+void update_phase(const KrylovSet& k, int precision, hipStream_t s);
+void pupdate_phase(const KrylovSet& k, int precision, hipStream_t s);
 // ST_FALLBACK instances -> ST_RUNNING with FL_FRESH | FL_WARM.
 void launch_promote_fallback(const InstDesc* descs, int32_t n_inst, hipStream_t s);
 // Copy every instance's scalars into a contiguous array.

@@ -161,3 +161,24 @@ def test_not_converged_budget(ctx):
     assert e.value.kind == "NotConverged" and e.value.iterations == 25
     r = oracle.solve(m, max_iter=25)
     assert e.value.residual == pytest.approx(r.residual, rel=1e-6)
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+def test_every_launch_shape_matches_oracle(precision, ctx):
+    """Lanes per row / block row from 2 (fewer than the 3 rows of a node
+    block) to 64, with small and large tiles, on a 3D and a 2D model."""
+    models = [m for m in small_models() if m.name.startswith(("cfg2_c3d8", "cps4_24x6"))]
+    assert len(models) == 2
+    try:
+        for m in models:
+            r = oracle.solve(m)
+            for lanes in (2, 4, 16, 64):
+                for rpt in (48, 576):
+                    ctx.set_tuning(lanes=lanes, blanes=lanes, rows_per_tile=rpt)
+                    s = solve_with_method(m, Method.AUTO, precision, ctx)
+                    e = rel_err(s.displacements, r.displacements)
+                    print(f"{m.name} [{precision}] lanes {lanes} rows/tile {rpt}: its {s.iterations} "
+                          f"(oracle {r.iterations}) err u {e:.1e}")
+                    assert e <= 1e-8
+    finally:
+        ctx.set_tuning()
