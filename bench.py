@@ -26,6 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP64 SpMV on the 200x20x20 C3D8 block, from the
+# FETCH_SIZE / WRITE_SIZE counters of rocprofv3 scaled by a 1 GiB device copy
+# (profiles/cg64_spmv_c3d8_200x20x20.md); None where no counter run exists.
+TRAFFIC_BYTES = {("cfg2", "fp64"): 188.7e6}
 CONFIGS = {
     "cfg2": ("C3D8 200x20x20 block 20x2x2 m, jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20", "cfg2"),
     "cfg1": ("CPS4 400x40 cantilever 10x1 m, jitter 0.15h, -1e4 N tip load", "cfg1"),
@@ -59,6 +63,25 @@ def cpu_baseline(model, full_iterations, cap):
     }
 
 
+def copy_calibration(mib):
+    """Three device-to-device copies of `mib` MiB (torch's copy kernel): the
+    reference point that scales hardware byte counters to bytes."""
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(3):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / 3
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "avg_ms": ms,
+                      "GBps_read_plus_write": 2 * n / ms / 1e6}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +92,13 @@ def main():
     ap.add_argument("--cpu-cap", type=int, default=150, help="PCG iterations in the CPU sample")
     ap.add_argument("--arms", default="both", choices=["both", "fp64", "fp32_ir"],
                     help="run one precision only (profiling); the JSON line needs both")
+    ap.add_argument("--spmv-reps", type=int, default=100, help="launches of the timed SpMV kernel")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the solver path is a single-GPU "
               f"build; multi-GPU sharding is design-only.", file=sys.stderr)
@@ -104,11 +133,13 @@ def main():
             continue
         for _ in range(args.warmup):
             step(prec)
+        print(f"[bench] {label}: warmup done", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for k in range(args.steps):
             s, u, its, res = step(prec)
+            print(f"[bench] {label}: step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
         dt = time.perf_counter() - t0
-        ms_kernel, bytes_kernel = s.time_spmv(prec, 100)
+        ms_kernel, bytes_kernel = s.time_spmv(prec, args.spmv_reps)
         arms[label] = dict(sys=s, u=u, its=its, res=res, dt=dt, dofs_per_s=n_dofs * args.steps / dt,
                            spmv_ms=ms_kernel, spmv_bytes=bytes_kernel)
         print(f"[bench] {label}: {args.steps} solves in {dt:.3f} s, {its} iterations, residual {res:.2e}, "
@@ -119,10 +150,11 @@ def main():
     a, b = arms["fp32_ir"], arms["fp64"]
     err = float(np.abs(a["u"] - b["u"]).max() / np.abs(b["u"]).max())
 
-    def roof(arm, name):
+    def roof(arm, name, label):
         ach = arm["spmv_bytes"] / (arm["spmv_ms"] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES.get((args.config, label)),
+                "kernel": name,
                 "bytes_per_launch": arm["spmv_bytes"], "avg_launch_us": round(arm["spmv_ms"] * 1e3, 2)}
 
     spmv32 = "ir32_spmv_kernel<%d,...>" % ndof
@@ -142,12 +174,12 @@ def main():
         "config": {"workload": CONFIGS[args.config][0], "n_dof": n_dofs, "nnz": int(mesh.nnz),
                    "tolerance": 1e-10, "precision": "FP32 Krylov, FP64 refinement"},
         "fp64": {"value": round(b["dofs_per_s"], 1), "ms_per_step": round(b["dt"] / args.steps * 1e3, 3),
-                 "iterations": int(b["its"]), "residual": b["res"], "roofline": roof(b, "cg64_spmv_kernel")},
+                 "iterations": int(b["its"]), "residual": b["res"], "roofline": roof(b, "cg64_spmv_kernel<%d,...>" % ndof, "fp64")},
         "speedup_vs_fp64": round(a["dofs_per_s"] / b["dofs_per_s"], 3),
         "iterations": int(a["its"]),
         "residual": a["res"],
         "max_rel_err_u_vs_fp64": err,
-        "roofline": roof(a, spmv32),
+        "roofline": roof(a, spmv32, "fp32_ir"),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model, int(b["its"]), args.cpu_cap)
