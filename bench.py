@@ -33,6 +33,8 @@ TRAFFIC_BYTES = {("cfg2", "fp64"): 188.7e6}
 CONFIGS = {
     "cfg2": ("C3D8 200x20x20 block 20x2x2 m, jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20", "cfg2"),
     "cfg1": ("CPS4 400x40 cantilever 10x1 m, jitter 0.15h, -1e4 N tip load", "cfg1"),
+    "cfg3": ("512 uneven plane-stress instances, DOF log-uniform 2e3..2e5, CPS4 p=0.6 else CPS3, "
+             "left edge clamped, tip load; one batched launch vs one-by-one FP64", "cfg3"),
 }
 
 
@@ -82,6 +84,181 @@ def copy_calibration(mib):
                       "GBps_read_plus_write": 2 * n / ms / 1e6}), flush=True)
 
 
+def bench_batch(args):
+    """The batched workload: every instance's mesh (coordinates, connectivity,
+    CSR pattern) resident; a step assembles every instance, applies its
+    constraints, solves and recovers. Plain arm: FP64 PCG one instance after
+    another. Batched arms: all instances in one Krylov engine (FP32-IR, and
+    FP64 for reference), SpMV shape from the autotuner. Solved DOF counts
+    the DOF of converged instances."""
+    import numpy as np
+
+    from solvix_amd import decks
+    from solvix_amd.core import Batch, Context, Mesh, constrained_dofs, element_materials
+    from solvix_amd.model import Precision
+
+    models = decks.cfg3()
+    ctx = Context(0)
+    prep = []
+    for m in models:
+        E, nu = element_materials(m)
+        ndof = m.ndof()
+        pos = {int(nid): i for i, nid in enumerate(m.node_ids)}
+        lr = np.array([pos[int(nid)] * ndof + int(d) - 1 for nid, d in zip(m.cload_node, m.cload_dof)], np.int64)
+        prep.append((Mesh.from_model(m, ctx), E, nu, m.thickness, lr, m.cload_value, constrained_dofs(m)[0]))
+    dofs = np.array([m.n_dofs() for m in models], np.int64)
+    print(f"[bench] {len(models)} instances, {dofs.sum()} DOF, meshes resident", file=sys.stderr, flush=True)
+
+    def assemble(i):
+        mesh, E, nu, t, lr, lv, rows = prep[i]
+        sy = mesh.assemble(E, nu, t, lr, lv)
+        sy.apply_dirichlet(rows)
+        return sy
+
+    phase = {}
+
+    def clock(key, t0):
+        phase[key] = phase.get(key, 0.0) + time.perf_counter() - t0
+
+    def sequential():
+        us, its, ok = [], [], []
+        for i in range(len(prep)):
+            t0 = time.perf_counter()
+            sy = assemble(i)
+            clock("assemble", t0)
+            t0 = time.perf_counter()
+            try:
+                u, it, _ = sy.solve(1e-10, 0, Precision.FP64)
+                ok.append(True)
+            except Exception:
+                u, it = sy.solution(), sy.stats()["iterations"]
+                ok.append(False)
+            clock("solve", t0)
+            t0 = time.perf_counter()
+            sy.recover(prep[i][6])
+            clock("recover", t0)
+            us.append(u)
+            its.append(it)
+            sy.close()
+        return us, np.array(its), np.array(ok)
+
+    tuned = {}
+
+    def batched(prec):
+        t0 = time.perf_counter()
+        systems = [assemble(i) for i in range(len(prep))]
+        clock("assemble", t0)
+        t0 = time.perf_counter()
+        b = Batch(ctx)
+        for sy in systems:
+            b.add(sy)
+        t = tuned[prec]  # the autotuned shape of this (2D-only) batch, node-block size 2
+        if prec == Precision.FP64:
+            ctx.set_tuning(lanes=t["lanes"][2], rows_per_tile=t["rows_per_tile"][2])
+        else:
+            ctx.set_tuning(blanes=t["lanes"][2], rows_per_tile=t["rows_per_tile"][2])
+        try:
+            res = b.solve(prec, raise_on_failure=False)
+        finally:
+            ctx.set_tuning()
+        clock("solve", t0)
+        t0 = time.perf_counter()
+        for i, sy in enumerate(systems):
+            sy.recover(prep[i][6])
+        clock("recover", t0)
+        us = [sy.solution() for sy in systems]
+        its = np.array([r[0] for r in res])
+        ok = np.array([r[2] for r in res])
+        return b, systems, us, its, ok
+
+    # autotune once per precision (untimed; the tuned shape depends on the batch only)
+    for prec in (Precision.FP32_IR, Precision.FP64):
+        systems = [assemble(i) for i in range(len(prep))]
+        b = Batch(ctx)
+        for sy in systems:
+            b.add(sy)
+        tuned[prec] = b.autotune(prec, 3)
+        print(f"[bench] autotune precision {int(prec)}: {tuned[prec]}", file=sys.stderr, flush=True)
+        b.close()
+        for sy in systems:
+            sy.close()
+
+    out = {}
+    for label in ("fp64_sequential", "fp64_batched", "fp32_ir_batched"):
+        for k in range(args.warmup + args.steps):
+            phase.clear()
+            t0 = time.perf_counter()
+            if label == "fp64_sequential":
+                us, its, ok = sequential()
+                b = systems = None
+            else:
+                prec = Precision.FP64 if label == "fp64_batched" else Precision.FP32_IR
+                b, systems, us, its, ok = batched(prec)
+            dt = time.perf_counter() - t0
+            timed = k >= args.warmup
+            split = " ".join(f"{key} {v:.2f} s" for key, v in phase.items())
+            print(f"[bench] {label}: pass {k + 1}/{args.warmup + args.steps} {dt:.2f} s ({split}), converged "
+                  f"{int(ok.sum())}/{ok.size}, iterations max {int(its.max())} mean {its.mean():.0f}",
+                  file=sys.stderr, flush=True)
+            if timed:
+                rec = out.setdefault(label, {"dt": 0.0})
+                rec["dt"] += dt
+                rec.update(us=us, its=its, ok=ok)
+                if b is not None:
+                    ms, by = b.time_spmv(20)
+                    rec.update(spmv_ms=ms, spmv_bytes=by)
+            if b is not None:
+                b.close()
+                for sy in systems:
+                    sy.close()
+
+    def dps(rec):
+        return float(dofs[rec["ok"]].sum()) * args.steps / rec["dt"]
+
+    seq = out["fp64_sequential"]
+    mix = out["fp32_ir_batched"]
+    b64 = out["fp64_batched"]
+
+    def err(rec):
+        return float(max(np.abs(a - c).max() / max(np.abs(c).max(), 1e-300) for a, c in zip(rec["us"], seq["us"])))
+
+    def roof(rec, name):
+        ach = rec["spmv_bytes"] / (rec["spmv_ms"] * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                "bytes_per_launch": rec["spmv_bytes"], "avg_launch_us": round(rec["spmv_ms"] * 1e3, 2)}
+
+    line = {
+        "metric": "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 PCG",
+        "value": round(dps(mix), 1),
+        "unit": "solved DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mix["dt"] / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 Krylov + f64 refinement",
+        "data": "synthetic (seeded SplitMix64 deck generator)",
+        "config": {"workload": CONFIGS["cfg3"][0], "instances": len(models), "n_dof_total": int(dofs.sum()),
+                   "tolerance": 1e-10},
+        "converged": int(mix["ok"].sum()),
+        "iterations_max": int(mix["its"].max()),
+        "max_rel_err_u_vs_fp64": err(mix),
+        "roofline": roof(mix, "ir32_spmv_kernel<2,...> (whole batch)"),
+        "autotune": {"fp32_ir": tuned[Precision.FP32_IR], "fp64": tuned[Precision.FP64]},
+        "fp64_sequential": {"value": round(dps(seq), 1), "ms_per_step": round(seq["dt"] / args.steps * 1e3, 3),
+                            "converged": int(seq["ok"].sum()), "iterations_max": int(seq["its"].max())},
+        "fp64_batched": {"value": round(dps(b64), 1), "ms_per_step": round(b64["dt"] / args.steps * 1e3, 3),
+                         "converged": int(b64["ok"].sum()), "max_rel_err_u_vs_fp64": err(b64),
+                         "roofline": roof(b64, "cg64_spmv_kernel<2,...> (whole batch)")},
+        "speedup_vs_fp64": round(dps(mix) / dps(seq), 3),
+        "speedup_fp64_batched_vs_fp64": round(dps(b64) / dps(seq), 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +280,10 @@ def main():
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the solver path is a single-GPU "
               f"build; multi-GPU sharding is design-only.", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "cfg3":
+        bench_batch(args)
+        return
 
     import numpy as np
 

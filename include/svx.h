@@ -290,6 +290,17 @@ int svx_batch_solve(svx_batch* batch, int precision, svx_error* err);
 int svx_batch_result(const svx_batch* batch, int64_t slot,
                      int64_t* iterations, double* residual,
                      int32_t* converged);
+/* SpMV autotuner: times the solver's SpMV phase over the whole batch for
+ * lanes per row (block row) in {2, 4, 8, 16, 32} and rows per tile in
+ * {64 .. 512}, one block size at a time, and keeps the fastest shape for
+ * later svx_batch_solve calls with this precision (until svx_batch_add).
+ * lanes[bs] / rows_per_tile[bs] (arrays of 4, indexed by block size 1..3,
+ * or NULL) receive the choice, spmv_ms its time. A tuned shape changes the
+ * summation order, so results then differ from solo solves at rounding level
+ * instead of bit for bit. */
+int svx_batch_autotune(svx_batch* batch, int precision, int reps,
+                       int32_t* lanes /* [4] */, int32_t* rows_per_tile /* [4] */,
+                       double* spmv_ms);
 /* Average duration and algorithmic bytes of one SpMV phase over the whole
  * batch (all instances), timed like svx_system_time_spmv. */
 int svx_batch_time_spmv(svx_batch* batch, int reps, double* avg_ms,

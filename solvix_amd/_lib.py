@@ -57,6 +57,7 @@ SIGNATURES = [
     ("svx_batch_solve", C.c_int, [_vp, C.c_int, _errp]),
     ("svx_batch_result", C.c_int, [_vp, _i64, _i64p, _dp, _i32p]),
     ("svx_batch_time_spmv", C.c_int, [_vp, C.c_int, _dp, _dp]),
+    ("svx_batch_autotune", C.c_int, [_vp, C.c_int, C.c_int, _i32p, _i32p, _dp]),
     ("svx_solve_multi", C.c_int, [_vp, C.c_int32, _dp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _i32p, _dp,
                                   _errp]),
 ]

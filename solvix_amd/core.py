@@ -385,6 +385,16 @@ class Batch:
         _raise(lib().svx_batch_time_spmv(self._h, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def autotune(self, precision: int = Precision.FP64, reps: int = 5) -> dict:
+        """Pick the fastest SpMV launch shape for this batch (svx_batch_autotune);
+        later solves with this precision use it."""
+        lanes = np.zeros(4, np.int32)
+        rows = np.zeros(4, np.int32)
+        ms = C.c_double()
+        _raise(lib().svx_batch_autotune(self._h, precision, reps, lanes.ctypes.data_as(_i32p),
+                                        rows.ctypes.data_as(_i32p), C.byref(ms)))
+        return {"lanes": lanes.tolist(), "rows_per_tile": rows.tolist(), "spmv_ms": ms.value}
+
     def close(self):
         if self._h:
             lib().svx_batch_destroy(self._h)

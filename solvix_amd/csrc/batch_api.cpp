@@ -23,6 +23,8 @@ struct svx_batch {
     std::vector<InstScal> h_scal;
     std::unique_ptr<Engine> engine;
     int precision = -1;
+    EngineShape tuned_shape;   // from svx_batch_autotune
+    int tuned_precision = -1;  // precision the tuned shape belongs to, -1 = none
 };
 
 namespace svx {
@@ -79,6 +81,7 @@ int svx_batch_add(svx_batch* b, svx_system* sys, double tol, int64_t max_iter, i
     if (!b || !sys || !(tol > 0.0) || sys->mesh->ctx != b->ctx) return SVX_E_ARGUMENT;
     if (!sys->dirichlet) return SVX_E_ARGUMENT;
     b->sys.push_back(sys);
+    b->tuned_precision = -1;  // a new member invalidates a tuned shape
     b->tol.push_back(tol);
     b->max_iter.push_back(max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * sys->mesh->n));
     if (slot) *slot = (int64_t)b->sys.size() - 1;
@@ -86,6 +89,73 @@ int svx_batch_add(svx_batch* b, svx_system* sys, double tol, int64_t max_iter, i
 }
 
 int64_t svx_batch_size(const svx_batch* b) { return b ? (int64_t)b->sys.size() : -1; }
+
+}  // extern "C"
+
+namespace {
+// Prepare every system of the batch for `precision`, reset the instance
+// state and return the instance descriptors.
+std::vector<InstDesc> batch_setup(svx_batch* b, int precision) {
+    hipStream_t s = b->ctx->stream;
+    const size_t n = b->sys.size();
+    for (svx_system* sy : b->sys) system_prepare(sy, precision);
+    b->scal.alloc(n);
+    std::vector<InstScal> init(n);
+    std::vector<InstDesc> descs(n);
+    for (size_t i = 0; i < n; ++i) {
+        svx_system* sy = b->sys[i];
+        init[i] = initial_scal(b->tol[i], b->max_iter[i], b->ctx->tune.ru_delta, sy->mesh->max_row_len);
+        descs[i] = system_desc(sy);
+        descs[i].s = b->scal.p + i;
+    }
+    b->scal.upload(init.data(), n, s);
+    if (!b->engine) b->engine = std::make_unique<Engine>(s);
+    return descs;
+}
+}  // namespace
+
+extern "C" {
+
+int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* lanes, int32_t* rows_per_tile,
+                       double* spmv_ms) {
+    if (!b || reps <= 0 || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
+    return guarded_b(nullptr, [&]() -> int {
+        const size_t n = b->sys.size();
+        if (n == 0) return SVX_E_ARGUMENT;
+        std::vector<InstDesc> descs = batch_setup(b, precision);
+        const std::vector<char> all(n, 1);
+        bool present[4] = {false, false, false, false};
+        for (const InstDesc& d : descs) present[d.bs] = true;
+        EngineShape best = batch_shape(b->sys, b->ctx->tune, precision);
+        b->engine->set_instances(descs, all, best);
+        double best_ms = b->engine->time_spmv(precision, reps);
+        static const int kLanes[] = {2, 4, 8, 16, 32};
+        static const int kRows[] = {64, 128, 192, 256, 384, 512};
+        for (int bs = 1; bs <= 3; ++bs) {
+            if (!present[bs]) continue;
+            for (int l : kLanes)
+                for (int r : kRows) {
+                    EngineShape sh = best;
+                    if (precision == SVX_FP64)
+                        sh.lanes[bs] = l;
+                    else
+                        sh.blanes[bs] = l;
+                    sh.rows_per_tile[bs] = r;
+                    b->engine->set_instances(descs, all, sh);
+                    const double ms = b->engine->time_spmv(precision, reps);
+                    if (ms < best_ms) best_ms = ms, best = sh;
+                }
+        }
+        b->tuned_shape = best;
+        b->tuned_precision = precision;
+        for (int bs = 0; bs < 4; ++bs) {
+            if (lanes) lanes[bs] = precision == SVX_FP64 ? best.lanes[bs] : best.blanes[bs];
+            if (rows_per_tile) rows_per_tile[bs] = best.rows_per_tile[bs];
+        }
+        if (spmv_ms) *spmv_ms = best_ms;
+        return SVX_OK;
+    });
+}
 
 int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
     if (err) {
@@ -97,19 +167,10 @@ int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
         hipStream_t s = b->ctx->stream;
         const size_t n = b->sys.size();
         if (n == 0) return SVX_OK;
-        for (svx_system* sy : b->sys) system_prepare(sy, precision);
-        b->scal.alloc(n);
-        std::vector<InstScal> init(n);
-        std::vector<InstDesc> descs(n);
-        for (size_t i = 0; i < n; ++i) {
-            svx_system* sy = b->sys[i];
-            init[i] = initial_scal(b->tol[i], b->max_iter[i], b->ctx->tune.ru_delta, sy->mesh->max_row_len);
-            descs[i] = system_desc(sy);
-            descs[i].s = b->scal.p + i;
-        }
-        b->scal.upload(init.data(), n, s);
-        if (!b->engine) b->engine = std::make_unique<Engine>(s);
-        b->engine->set_instances(descs, std::vector<char>(n, 1), batch_shape(b->sys, b->ctx->tune, precision));
+        std::vector<InstDesc> descs = batch_setup(b, precision);
+        const EngineShape shape =
+            b->tuned_precision == precision ? b->tuned_shape : batch_shape(b->sys, b->ctx->tune, precision);
+        b->engine->set_instances(descs, std::vector<char>(n, 1), shape);
         EngineOptions opt;
         opt.precision = precision;
         opt.period = b->ctx->tune.period;
@@ -148,6 +209,7 @@ int svx_batch_result(const svx_batch* b, int64_t slot, int64_t* iterations, doub
 int svx_batch_time_spmv(svx_batch* b, int reps, double* avg_ms, double* bytes_per_launch) {
     if (!b || !b->engine || reps <= 0 || b->precision < 0) return SVX_E_ARGUMENT;
     return guarded_b(nullptr, [&] {
+        b->engine->tile_all();
         double ms = b->engine->time_spmv(b->precision, reps);
         double bytes = 0.0;
         for (svx_system* sy : b->sys) bytes += spmv_bytes(sy->mesh, b->precision);

@@ -70,6 +70,7 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         }
         count[bs] = (int)tiles.size() - first[bs];
     }
+    tiled_tiles_ = (int)tiles.size();
     // old local tile index feeding each new tile (-1: nothing to carry)
     std::vector<int32_t> src(tiles.size(), -1);
     bool any_src = false;
@@ -197,12 +198,14 @@ int Engine::run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag
         launches_++;
         done_chunks++;
         launch_count_running(descs_.p, (int32_t)h_descs_.size(), d_flag, stream_);
-        SVX_HIP(hipMemcpyAsync(h_flag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+        SVX_HIP(hipMemcpyAsync(h_flag, d_flag, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
         SVX_HIP(hipStreamSynchronize(stream_));
-        const int running = *h_flag;
+        const int running = h_flag[0], live_tiles = h_flag[1];
         if (running == 0) return 0;
         if (max_chunks >= 0 && done_chunks >= max_chunks) return running;
-        if (opt.compact && 2 * running <= tiled_instances_ && tiled_instances_ > 1) {
+        // Re-tile once finished instances hold a quarter of the tiles: each of
+        // their workgroups still costs a launch slot and a state load per phase.
+        if (opt.compact && tiled_instances_ > 1 && running < tiled_instances_ && 4 * live_tiles <= 3 * tiled_tiles_) {
             std::vector<InstScal> sc;
             read_scalars(sc);
             std::vector<char> live(h_descs_.size());

@@ -5,8 +5,8 @@
 // Instances are grouped by node-block size (1, 2, 3); each group is a tile
 // table launched as one grid per phase, and the groups' launches are captured
 // into the same graph. Tiles of finished instances are dropped from the
-// tables (and the graph re-captured) once at most half of the tiled
-// instances still run. Instances flagged FL_FRESH are initialised by the init
+// tables (and the graph re-captured) once they hold a quarter of all tiles;
+// pending per-tile partials move with the tiles. Instances flagged FL_FRESH are initialised by the init
 // phase at the start of the next chunk, so new work can join a running set.
 #pragma once
 #include "svx_internal.hpp"
@@ -41,6 +41,9 @@ class Engine {
     // Download the per-instance scalars (status, iterations, residual).
     void read_scalars(std::vector<InstScal>& out) const;
     double time_spmv(int precision, int reps);
+    // Put every instance back into the tile tables (after a solve compacted
+    // them), e.g. before timing the whole set.
+    void tile_all() { build_tiles(std::vector<char>(h_descs_.size(), 1)); }
     // Average device time of each phase of an iteration, launched alone on
     // the current set with every instance forced to run: [0] SpMV, [1] update,
     // [2] direction update, [3] FP32-IR refresh (fold, FP64 residual,
@@ -66,6 +69,7 @@ class Engine {
     DevBuf<double> partials_;
     KrylovSet groups_[4]{};
     int tiled_instances_ = 0;
+    int tiled_tiles_ = 0;
     std::vector<char> tiled_;  // instance i has tiles in the current tables
     hipGraphExec_t exec_ = nullptr;
     int exec_precision_ = -1, exec_period_ = -1, exec_chunk_ = -1;

@@ -702,15 +702,22 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
 }
 
 // ------------------------------------------------------------ utilities
+// out[0]: running instances; out[1]: tiles of running instances in the
+// current tile tables (tile_count of an instance describes its current tiling).
 __global__ void count_running_kernel(const InstDesc* __restrict__ descs, int32_t n_inst, int32_t* out) {
-    __shared__ int32_t cnt;
-    if (threadIdx.x == 0) cnt = 0;
+    __shared__ int32_t cnt, tiles;
+    if (threadIdx.x == 0) cnt = 0, tiles = 0;
     __syncthreads();
-    int32_t local = 0;
-    for (int32_t i = threadIdx.x; i < n_inst; i += blockDim.x) local += descs[i].s->status == ST_RUNNING;
+    int32_t local = 0, ltiles = 0;
+    for (int32_t i = threadIdx.x; i < n_inst; i += blockDim.x) {
+        const bool run = descs[i].s->status == ST_RUNNING;
+        local += run;
+        ltiles += run ? descs[i].tile_count : 0;
+    }
     atomicAdd(&cnt, local);
+    atomicAdd(&tiles, ltiles);
     __syncthreads();
-    if (threadIdx.x == 0) *out = cnt;
+    if (threadIdx.x == 0) out[0] = cnt, out[1] = tiles;
 }
 
 __global__ void promote_fallback_kernel(const InstDesc* __restrict__ descs, int32_t n_inst) {

@@ -223,7 +223,8 @@ void cg64_iteration(const KrylovSet& k, hipStream_t s);
 void ir32_init(const KrylovSet& k, hipStream_t s);
 void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end);
 void ir32_refresh(const KrylovSet& k, hipStream_t s);
-void launch_count_running(const InstDesc* descs, int32_t n_inst, int32_t* out, hipStream_t s);
+// out[0] = running instances, out[1] = their tiles in the current tables.
+void launch_count_running(const InstDesc* descs, int32_t n_inst, int32_t* out /* [2] */, hipStream_t s);
 // Move per-tile partials (all slots) to a new tile layout of the same group:
 // new tile g takes old tile src[g] (src[g] < 0: left untouched).
 void launch_carry_partials(const double* old_p, int old_stride, double* new_p, int new_stride, const int32_t* src,
