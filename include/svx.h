@@ -40,7 +40,8 @@ enum {
     SVX_E_NOT_CONVERGED = 4,
     SVX_E_DEVICE = 5,   /* HIP runtime failure; message holds the HIP string */
     SVX_E_ARGUMENT = 6, /* bad handle, size, option or index */
-    SVX_E_CAPACITY = 7  /* batch / server budget or slot limit exceeded */
+    SVX_E_CAPACITY = 7, /* batch / server budget or slot limit exceeded */
+    SVX_E_TIMEOUT = 8   /* svx_server_wait: the request is not done yet */
 };
 
 /* Element kinds on the path (lib.rs:53-73, subset CPS3 / CPS4 / C3D8). */
@@ -343,6 +344,60 @@ int svx_system_time_spmv(svx_system* sys, int precision, int reps,
 int svx_system_time_phases(svx_system* sys, int precision, int reps,
                            double* avg_ms /* [4] */,
                            double* bytes_per_launch /* [4] */);
+
+/* ---------------------------------------------------------------------------
+ * Request server (no reference counterpart: the reference solves one deck per
+ * process). Whole-model solves (the svx_solve_model pipeline, Method::Auto
+ * without the LDLT fallback) submitted from any host thread are validated and
+ * given their CSR pattern on a host worker, then admitted to the device while
+ * their estimated footprint fits hbm_budget and fewer than max_active are in
+ * flight; admitted systems join running Krylov engines at the next chunk
+ * boundary (continuous batching) and leave as they converge, freeing their
+ * memory. A request's result equals its svx_solve_model result with the same
+ * precision. The server owns ctx while it exists: no other calls on ctx.
+ * ------------------------------------------------------------------------- */
+typedef struct svx_server_config {
+    int32_t precision;  /* SVX_FP64 or SVX_FP32_IR */
+    int32_t max_active; /* requests in flight on the device at most */
+    int64_t hbm_budget; /* device bytes the admitted requests may hold */
+    double tol;         /* relative residual; 1e-10 is the reference's */
+    int32_t chunk;      /* iterations between admissions; 0 = context default */
+    int32_t pad;
+} svx_server_config;
+
+typedef struct svx_request_info {
+    double latency_ms; /* submit to result */
+    double queue_ms;   /* submit to admission */
+    double prepare_ms; /* submit to end of host preparation */
+    int64_t iterations, refreshes, fp64_iterations;
+    int64_t device_bytes; /* admitted footprint estimate */
+    int64_t n_dof;
+} svx_request_info;
+
+typedef struct svx_server_summary {
+    int64_t submitted, completed, failed, queued;
+    int64_t resident_bytes, peak_resident_bytes, peak_active;
+    int64_t solved_dof;        /* DOF of completed requests */
+    double p50_ms, p99_ms, max_ms; /* latency of completed requests */
+    double dof_per_s;          /* solved DOF / (last completion - first submit) */
+} svx_server_summary;
+
+int svx_server_create(svx_ctx* ctx, const svx_server_config* cfg,
+                      svx_server** out);
+/* Stops the server; requests still in flight are dropped. */
+void svx_server_destroy(svx_server* srv);
+/* Copies the deck; returns at once with a ticket. */
+int svx_server_submit(svx_server* srv, const svx_model_desc* model,
+                      int64_t* ticket);
+/* Waits for a ticket (timeout_ms < 0: no limit; SVX_E_TIMEOUT when it
+ * expires, the ticket stays valid). On completion fills out (arrays sized as
+ * for svx_solve_model) and info, returns the request's status (errors as
+ * svx_solve_model reports them, SVX_E_CAPACITY for a deck larger than the
+ * budget), and releases the ticket. */
+int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms,
+                    svx_solution* out, svx_request_info* info,
+                    svx_error* err);
+int svx_server_stats(svx_server* srv, svx_server_summary* out);
 
 #ifdef __cplusplus
 }

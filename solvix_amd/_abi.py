@@ -9,6 +9,7 @@ SVX_E_NOT_CONVERGED = 4
 SVX_E_DEVICE = 5
 SVX_E_ARGUMENT = 6
 SVX_E_CAPACITY = 7
+SVX_E_TIMEOUT = 8
 
 SVX_CPS3, SVX_CPS4, SVX_C3D8 = 0, 1, 2
 KIND_NAMES = {"CPS3": SVX_CPS3, "CPS4": SVX_CPS4, "C3D8": SVX_C3D8}
@@ -67,6 +68,47 @@ class SvxModelDesc(C.Structure):
         ("cload_node", _p(C.c_uint32)),
         ("cload_dof", _p(C.c_int32)),
         ("cload_value", _p(C.c_double)),
+    ]
+
+
+class SvxServerConfig(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("max_active", C.c_int32),
+        ("hbm_budget", C.c_int64),
+        ("tol", C.c_double),
+        ("chunk", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+class SvxRequestInfo(C.Structure):
+    _fields_ = [
+        ("latency_ms", C.c_double),
+        ("queue_ms", C.c_double),
+        ("prepare_ms", C.c_double),
+        ("iterations", C.c_int64),
+        ("refreshes", C.c_int64),
+        ("fp64_iterations", C.c_int64),
+        ("device_bytes", C.c_int64),
+        ("n_dof", C.c_int64),
+    ]
+
+
+class SvxServerSummary(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64),
+        ("completed", C.c_int64),
+        ("failed", C.c_int64),
+        ("queued", C.c_int64),
+        ("resident_bytes", C.c_int64),
+        ("peak_resident_bytes", C.c_int64),
+        ("peak_active", C.c_int64),
+        ("solved_dof", C.c_int64),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("dof_per_s", C.c_double),
     ]
 
 

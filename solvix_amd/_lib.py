@@ -58,6 +58,12 @@ SIGNATURES = [
     ("svx_batch_result", C.c_int, [_vp, _i64, _i64p, _dp, _i32p]),
     ("svx_batch_time_spmv", C.c_int, [_vp, C.c_int, _dp, _dp]),
     ("svx_batch_autotune", C.c_int, [_vp, C.c_int, C.c_int, _i32p, _i32p, _dp]),
+    ("svx_server_create", C.c_int, [_vp, C.POINTER(_abi.SvxServerConfig), C.POINTER(_vp)]),
+    ("svx_server_destroy", None, [_vp]),
+    ("svx_server_submit", C.c_int, [_vp, C.POINTER(_abi.SvxModelDesc), _i64p]),
+    ("svx_server_wait", C.c_int, [_vp, _i64, C.c_double, C.POINTER(_abi.SvxSolution),
+                                  C.POINTER(_abi.SvxRequestInfo), _errp]),
+    ("svx_server_stats", C.c_int, [_vp, C.POINTER(_abi.SvxServerSummary)]),
     ("svx_solve_multi", C.c_int, [_vp, C.c_int32, _dp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _i32p, _dp,
                                   _errp]),
 ]

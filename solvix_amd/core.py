@@ -447,3 +447,71 @@ class CsrMatrix:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ request server
+class Server:
+    """Continuous-batching request server (svx_server_*): whole-model solves
+    submitted at any time, admitted by device-memory footprint, solved
+    together, each result equal to its solo solve_with_method result. The
+    server owns its context: use a context of its own."""
+
+    def __init__(self, ctx: Context, precision: int = Precision.FP32_IR, hbm_budget: int = 200 << 30,
+                 max_active: int = 256, tol: float = 1e-10, chunk: int = 0):
+        self.ctx = ctx
+        cfg = _abi.SvxServerConfig(precision, max_active, hbm_budget, tol, chunk, 0)
+        self._h = C.c_void_p()
+        _raise(lib().svx_server_create(ctx.handle, C.byref(cfg), C.byref(self._h)))
+        self._pending = {}
+
+    def submit(self, model: Model) -> int:
+        desc, keep = model.to_desc()
+        t = C.c_int64()
+        _raise(lib().svx_server_submit(self._h, C.byref(desc), C.byref(t)))
+        del keep  # the server copied the deck
+        self._pending[t.value] = model
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0) -> Tuple[Solution, dict]:
+        """(Solution, request info); raises SolveError like solve_with_method
+        (kind "Timeout" if the request is not done within timeout_ms)."""
+        model = self._pending[ticket]
+        nn, ne = model.n_nodes, model.n_elem
+        nr = max(0, int(lib().svx_model_reaction_count(C.byref(model.to_desc()[0]))))
+        disp = np.zeros((nn, 3))
+        vm = np.zeros(ne)
+        nvm = np.zeros(nn)
+        rn = np.zeros(nr, np.uint32)
+        rd = np.zeros(nr, np.int32)
+        rv = np.zeros(nr)
+        sol = _abi.SvxSolution(_d(disp), _d(vm), _d(nvm), rn.ctypes.data_as(C.POINTER(C.c_uint32)),
+                               rd.ctypes.data_as(_i32p), _d(rv))
+        info = _abi.SvxRequestInfo()
+        err = _abi.SvxError()
+        rc = lib().svx_server_wait(self._h, ticket, timeout_ms, C.byref(sol), C.byref(info), C.byref(err))
+        if rc != _abi.SVX_E_TIMEOUT:
+            del self._pending[ticket]
+        _raise(rc, err)
+        out = Solution(
+            displacements=disp, von_mises=vm, nodal_von_mises=nvm,
+            reactions=[Reaction(int(a), int(b), float(c)) for a, b, c in zip(rn, rd, rv)],
+            iterations=int(sol.iterations), final_residual=float(sol.final_residual), solver_method="pcg",
+            cond_estimate=float(sol.cond_estimate), timings_ms=dict(solve=sol.solve_ms))
+        return out, {k: getattr(info, k) for k, _ in _abi.SvxRequestInfo._fields_}
+
+    def stats(self) -> dict:
+        st = _abi.SvxServerSummary()
+        _raise(lib().svx_server_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _abi.SvxServerSummary._fields_}
+
+    def close(self):
+        if self._h:
+            lib().svx_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -57,6 +57,12 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+}  // namespace
+
+namespace svx {
+
+int nodes_of(uint8_t kind) { return kind == SVX_CPS3 ? 3 : kind == SVX_CPS4 ? 4 : 8; }
+
 const char* reason_text(int reason) {
     switch (reason) {
         case SVX_ELEM_NO_MATERIAL: return "no material assigned";
@@ -68,16 +74,16 @@ const char* reason_text(int reason) {
     }
 }
 
-}  // namespace
-
-namespace svx {
-
-int nodes_of(uint8_t kind) { return kind == SVX_CPS3 ? 3 : kind == SVX_CPS4 ? 4 : 8; }
 
 // ------------------------------------------------------------------ mesh
 void mesh_build(svx_mesh* m, int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const uint8_t* kinds,
                 const int64_t* conn_ptr, const int32_t* conn_idx) {
-    hipStream_t s = m->ctx->stream;
+    mesh_build_host(m, ndof, n_nodes, n_elem, kinds, conn_ptr, conn_idx);
+    mesh_upload(m, coords);
+}
+
+void mesh_build_host(svx_mesh* m, int ndof, int64_t n_nodes, int64_t n_elem, const uint8_t* kinds,
+                     const int64_t* conn_ptr, const int32_t* conn_idx) {
     m->ndof = ndof;
     m->n_nodes = n_nodes;
     m->n_elem = n_elem;
@@ -91,6 +97,22 @@ void mesh_build(svx_mesh* m, int ndof, int64_t n_nodes, const double* coords, in
     m->max_row_len = 0;
     for (int64_t r = 0; r < m->n; ++r)
         m->max_row_len = std::max<int64_t>(m->max_row_len, m->hp.row_ptr[r + 1] - m->hp.row_ptr[r]);
+    int64_t ke = 0, ns = 0;
+    for (int64_t e = 0; e < n_elem; ++e) {
+        int nen = nodes_of(kinds[e]);
+        ke += (int64_t)nen * ndof * nen * ndof;
+        ns += (int64_t)nen * 6;
+    }
+    m->ke_total = ke;
+    m->ns_total = ns;
+}
+
+void mesh_upload(svx_mesh* m, const double* coords) {
+    hipStream_t s = m->ctx->stream;
+    const int64_t n_nodes = m->n_nodes, n_elem = m->n_elem;
+    const int ndof = m->ndof;
+    const int64_t* conn_ptr = m->h_conn_ptr.data();
+    const uint8_t* kinds = m->h_kinds.data();
     m->coords.alloc(n_nodes * 3);
     m->coords.upload(coords, n_nodes * 3, s);
     m->conn_ptr.alloc(n_elem + 1);
@@ -124,8 +146,6 @@ void mesh_build(svx_mesh* m, int ndof, int64_t n_nodes, const double* coords, in
         ke += (int64_t)nen * ndof * nen * ndof;
         ns += (int64_t)nen * 6;
     }
-    m->ke_total = ke;
-    m->ns_total = ns;
     for (int k = 0; k < 3; ++k) {
         m->nsel[k] = (int64_t)sel[k].size();
         m->sel[k].alloc(sel[k].size());

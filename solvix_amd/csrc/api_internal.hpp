@@ -88,6 +88,10 @@ struct svx_csr {
 namespace svx {
 
 int nodes_of(uint8_t kind);
+// Message text of an SVX_ELEM_* reason, in the reference's wording.
+const char* reason_text(int reason);
+// The per-block-size launch shape every batched solve uses by default.
+EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp, int precision);
 
 // A model validated and resolved on the host (lib.rs:593-632 up to the
 // element pass): node positions, per-element material constants, the load
@@ -107,6 +111,11 @@ int prepare_model(const svx_model_desc* m, PreparedModel& out, svx_error* err);
 
 void mesh_build(svx_mesh* m, int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const uint8_t* kinds,
                 const int64_t* conn_ptr, const int32_t* conn_idx);
+// The host half of mesh_build (pattern, incidence, offsets; no device calls)
+// and the device half (uploads; needs m->ctx).
+void mesh_build_host(svx_mesh* m, int ndof, int64_t n_nodes, int64_t n_elem, const uint8_t* kinds,
+                     const int64_t* conn_ptr, const int32_t* conn_idx);
+void mesh_upload(svx_mesh* m, const double* coords);
 void system_assemble(svx_system* sys, const double* E, const double* nu, double thickness, int64_t* bad_index,
                      int* bad_reason);
 void system_set_f(svx_system* sys, const double* f);

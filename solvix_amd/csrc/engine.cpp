@@ -218,9 +218,9 @@ int Engine::run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag
 void Engine::read_scalars(std::vector<InstScal>& out) const {
     out.resize(h_descs_.size());
     if (h_descs_.empty()) return;
-    DevBuf<InstScal> tmp(h_descs_.size());
-    launch_gather_scalars(descs_.p, (int32_t)h_descs_.size(), tmp.p, stream_);
-    tmp.download(out.data(), out.size(), stream_);
+    if (scal_tmp_.n < h_descs_.size()) scal_tmp_.alloc(h_descs_.size());
+    launch_gather_scalars(descs_.p, (int32_t)h_descs_.size(), scal_tmp_.p, stream_);
+    scal_tmp_.download(out.data(), out.size(), stream_);
     SVX_HIP(hipStreamSynchronize(stream_));
 }
 

@@ -52,6 +52,12 @@ class Engine {
     int64_t launches() const { return launches_; }
     int n_instances() const { return (int)h_descs_.size(); }
 
+    // Run chunks of opt.precision only (no FP64 fallback phase) until no
+    // instance runs or max_chunks chunks ran; returns the running count.
+    int run_chunks(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
+        return run_phase(opt, h_flag, d_flag, max_chunks);
+    }
+
    private:
     int run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks);
     void release_graph();
@@ -67,6 +73,7 @@ class Engine {
     DevBuf<Tile> tiles_;
     DevBuf<InstDesc> descs_;
     DevBuf<double> partials_;
+    mutable DevBuf<InstScal> scal_tmp_;  // read_scalars staging
     KrylovSet groups_[4]{};
     int tiled_instances_ = 0;
     int tiled_tiles_ = 0;

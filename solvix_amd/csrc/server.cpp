@@ -1,0 +1,549 @@
+// Request server: continuous batching of whole-model solves on one device.
+//
+// Decks arrive through svx_server_submit from any host thread. A host worker
+// validates each one and builds its CSR pattern (prepare_model,
+// mesh_build_host). The device thread admits prepared requests while their
+// estimated device footprint fits the HBM budget and an instance slot is
+// free: it uploads the mesh, assembles, applies the constraints and hands the
+// system to a running Krylov engine as a fresh instance, which joins at the
+// next chunk boundary. Between chunks (32 iterations) the thread retires
+// finished instances — recovery on the device, results copied to the
+// request, device memory freed — and admits new ones, so requests of any
+// size overlap on the device as slots and memory free up.
+//
+// Each instance runs exactly the operations of a solo svx_solve_model with
+// the same precision and launch shape (per-instance reductions, fixed lanes
+// per block size), so a result does not depend on what else is in flight.
+// FP32-IR instances that fall back to FP64 move to a second engine.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "api_internal.hpp"
+
+using namespace svx;
+
+namespace {
+
+double wall_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void put_err(svx_error* err, int code, const std::string& msg, int reason = 0, int64_t index = -1,
+             int64_t iterations = 0, double residual = 0.0) {
+    if (!err) return;
+    err->code = code;
+    err->reason = reason;
+    err->index = index;
+    err->iterations = iterations;
+    err->residual = residual;
+    snprintf(err->message, sizeof err->message, "%s", msg.c_str());
+}
+
+// A deck owned by the server (the caller's arrays may go away after submit).
+struct OwnedModel {
+    std::vector<uint32_t> node_ids, elem_ids, conn_ids, elset_members, bc_nodes, cload_node;
+    std::vector<double> coords, youngs, poisson, cload_value;
+    std::vector<uint8_t> kinds;
+    std::vector<int64_t> conn_ptr, elset_ptr, bc_node_ptr, bc_dof_ptr;
+    std::vector<int32_t> assign_elset, assign_material, bc_dofs, cload_dof;
+    svx_model_desc desc{};
+
+    template <typename T>
+    static void copy(std::vector<T>& v, const T* p, int64_t n) {
+        v.assign(p ? p : nullptr, p ? p + std::max<int64_t>(0, n) : nullptr);
+    }
+    explicit OwnedModel(const svx_model_desc& m) {
+        copy(node_ids, m.node_ids, m.n_nodes);
+        copy(coords, m.coords, m.n_nodes * 3);
+        copy(elem_ids, m.elem_ids, m.n_elem);
+        copy(kinds, m.kinds, m.n_elem);
+        copy(conn_ptr, m.conn_ptr, m.n_elem + 1);
+        copy(conn_ids, m.conn_ids, m.conn_ptr ? m.conn_ptr[m.n_elem] : 0);
+        copy(youngs, m.youngs, m.n_materials);
+        copy(poisson, m.poisson, m.n_materials);
+        copy(elset_ptr, m.elset_ptr, m.n_elsets + 1);
+        copy(elset_members, m.elset_members, m.elset_ptr ? m.elset_ptr[m.n_elsets] : 0);
+        copy(assign_elset, m.assign_elset, m.n_assign);
+        copy(assign_material, m.assign_material, m.n_assign);
+        copy(bc_node_ptr, m.bc_node_ptr, m.n_bcs + 1);
+        copy(bc_nodes, m.bc_nodes, m.bc_node_ptr ? m.bc_node_ptr[m.n_bcs] : 0);
+        copy(bc_dof_ptr, m.bc_dof_ptr, m.n_bcs + 1);
+        copy(bc_dofs, m.bc_dofs, m.bc_dof_ptr ? m.bc_dof_ptr[m.n_bcs] : 0);
+        copy(cload_node, m.cload_node, m.n_cloads);
+        copy(cload_dof, m.cload_dof, m.n_cloads);
+        copy(cload_value, m.cload_value, m.n_cloads);
+        desc = m;
+        desc.node_ids = node_ids.data();
+        desc.coords = coords.data();
+        desc.elem_ids = elem_ids.data();
+        desc.kinds = kinds.data();
+        desc.conn_ptr = conn_ptr.data();
+        desc.conn_ids = conn_ids.data();
+        desc.youngs = youngs.data();
+        desc.poisson = poisson.data();
+        desc.elset_ptr = elset_ptr.data();
+        desc.elset_members = elset_members.data();
+        desc.assign_elset = assign_elset.data();
+        desc.assign_material = assign_material.data();
+        desc.bc_node_ptr = bc_node_ptr.data();
+        desc.bc_nodes = bc_nodes.data();
+        desc.bc_dof_ptr = bc_dof_ptr.data();
+        desc.bc_dofs = bc_dofs.data();
+        desc.cload_node = cload_node.data();
+        desc.cload_dof = cload_dof.data();
+        desc.cload_value = cload_value.data();
+    }
+};
+
+enum class St { Queued, Prepared, Active, Done };
+
+struct Request {
+    int64_t ticket = 0;
+    std::unique_ptr<OwnedModel> model;
+    PreparedModel pm;
+    std::unique_ptr<svx_mesh> mesh;
+    std::unique_ptr<svx_system> sys;
+    St state = St::Queued;
+    int phase_precision = SVX_FP64;  // engine the instance runs in
+    int64_t bytes = 0;               // admitted device footprint (estimate)
+    double t_submit = 0, t_prepared = 0, t_admit = 0, t_done = 0;
+    // results
+    int rc = SVX_OK;
+    svx_error err{};
+    std::vector<double> disp, vm, nvm, react;
+    int64_t iterations = 0, refreshes = 0, fp64_iterations = 0;
+    double residual = 0, cond = 0;
+};
+
+// Device bytes a request holds while admitted: mesh arrays, CSR values and
+// vectors, the FP32 copy for FP32-IR, and the larger of the transient element
+// stiffness and nodal-stress buffers.
+int64_t footprint(const svx_mesh& m, int precision) {
+    const int64_t n = m.n, nn = m.n_nodes, ne = m.n_elem, nnz = m.nnz;
+    int64_t b = 24 * nn + 12 * ne + 4 * (int64_t)m.h_conn.size() + 8 * (ne + 1);  // coords, kinds/offsets, conn
+    b += 4 * (n + 1) + 4 * nnz + 4 * (nn + 1) + 4 * m.nblk;                        // CSR + node-block patterns
+    b += 8 * (nn + 1) + 5 * (int64_t)m.hp.inc_elem.size() + 16 * ne;              // incidence, element offsets
+    b += 8 * nnz + 8 * n * 9 + 4 * n;                                              // values, f, dinv, u r p z q, dinv32
+    if (precision == SVX_FP32_IR) b += 4 * nnz + 4 * n * 5;                        // FP32 blocks + vectors
+    b += 8 * std::max(m.ke_total, m.ns_total + nn * 6);                            // transient assembly / recovery
+    return b + (1 << 16);
+}
+
+}  // namespace
+
+struct svx_server {
+    svx_ctx* ctx = nullptr;
+    svx_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_prep, cv_dev, cv_done;
+    std::deque<Request*> prep_q, ready_q;
+    std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
+    std::vector<Request*> active;  // admitted, in an engine
+    int64_t next_ticket = 1;
+    bool stop = false;
+    std::thread prep_thread, dev_thread;
+    // statistics (guarded by mu)
+    int64_t submitted = 0, completed = 0, failed = 0, resident = 0, peak_resident = 0, peak_active = 0;
+    int64_t dof_done = 0;
+    double t_first = -1, t_last_done = 0;
+    std::vector<double> latencies;
+    std::string fatal;
+
+    void prep_loop();
+    void dev_loop();
+    void finish(Request* r);
+};
+
+void svx_server::prep_loop() {
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_prep.wait(lk, [&] { return stop || !prep_q.empty(); });
+            if (stop) return;
+            r = prep_q.front();
+            prep_q.pop_front();
+        }
+        svx_error err{};
+        err.index = -1;
+        int rc = SVX_OK;
+        try {
+            rc = prepare_model(&r->model->desc, r->pm, &err);
+            if (rc == SVX_OK) {
+                r->mesh = std::make_unique<svx_mesh>();
+                r->mesh->ctx = ctx;
+                const svx_model_desc& md = r->model->desc;
+                mesh_build_host(r->mesh.get(), r->pm.ndof, md.n_nodes, md.n_elem, md.kinds, md.conn_ptr,
+                                r->pm.conn_pos.data());
+                r->bytes = footprint(*r->mesh, cfg.precision);
+            }
+        } catch (const std::exception& e) {
+            rc = SVX_E_ARGUMENT;
+            put_err(&err, rc, e.what());
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        r->t_prepared = wall_ms();
+        if (rc != SVX_OK) {
+            r->rc = rc;
+            r->err = err;
+            r->state = St::Done;
+            r->t_done = r->t_prepared;
+            failed++;
+            cv_done.notify_all();
+        } else if (r->bytes > cfg.hbm_budget) {
+            r->rc = SVX_E_CAPACITY;
+            put_err(&r->err, SVX_E_CAPACITY, "deck exceeds the server's device memory budget");
+            r->state = St::Done;
+            r->t_done = r->t_prepared;
+            failed++;
+            cv_done.notify_all();
+        } else {
+            r->state = St::Prepared;
+            ready_q.push_back(r);
+            cv_dev.notify_all();
+        }
+    }
+}
+
+// Called on the device thread, without the lock, for a request that left
+// the engines (or failed during admission).
+void svx_server::finish(Request* r) {
+    r->mesh.reset();
+    r->sys.reset();
+    std::lock_guard<std::mutex> lk(mu);
+    r->t_done = wall_ms();
+    r->state = St::Done;
+    resident -= r->bytes;
+    if (r->rc == SVX_OK) {
+        completed++;
+        dof_done += (int64_t)r->model->desc.n_nodes * r->pm.ndof;
+        latencies.push_back(r->t_done - r->t_submit);
+        t_last_done = r->t_done;
+    } else {
+        failed++;
+    }
+    cv_done.notify_all();
+}
+
+void svx_server::dev_loop() {
+    try {
+        SVX_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        Engine eng32(s), eng64(s);
+        std::vector<Request*> in32, in64;  // instance order of each engine
+        bool dirty32 = false, dirty64 = false;
+        const int main_prec = cfg.precision;
+        EngineOptions opt;
+        opt.period = ctx->tune.period;
+        opt.chunk = cfg.chunk > 0 ? cfg.chunk : ctx->tune.chunk;
+        opt.use_graph = ctx->tune.use_graph;
+        for (;;) {
+            // admit
+            std::vector<Request*> admit;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                if (in32.empty() && in64.empty())
+                    cv_dev.wait(lk, [&] { return stop || !ready_q.empty(); });
+                if (stop) break;
+                while (!ready_q.empty()) {
+                    Request* r = ready_q.front();
+                    const int64_t n_active = (int64_t)(in32.size() + in64.size() + admit.size());
+                    if (n_active >= cfg.max_active || resident + r->bytes > cfg.hbm_budget) break;
+                    ready_q.pop_front();
+                    resident += r->bytes;
+                    peak_resident = std::max(peak_resident, resident);
+                    admit.push_back(r);
+                }
+                peak_active = std::max<int64_t>(peak_active, (int64_t)(in32.size() + in64.size() + admit.size()));
+            }
+            for (Request* r : admit) {
+                r->t_admit = wall_ms();
+                const svx_model_desc& md = r->model->desc;
+                mesh_upload(r->mesh.get(), md.coords);
+                r->sys = std::make_unique<svx_system>();
+                r->sys->mesh = r->mesh.get();
+                int64_t bad = -1;
+                int why = 0;
+                system_assemble(r->sys.get(), r->pm.E.data(), r->pm.nu.data(), md.thickness, &bad, &why);
+                if (bad >= 0) {
+                    char buf[256];
+                    snprintf(buf, sizeof buf, "element %u: %s", md.elem_ids[bad], reason_text(why));
+                    r->rc = SVX_E_INVALID_MODEL;
+                    put_err(&r->err, SVX_E_INVALID_MODEL, buf, why, bad);
+                    finish(r);
+                    continue;
+                }
+                const std::string& le = !r->pm.load_error.empty() ? r->pm.load_error : r->pm.bc_error;
+                if (!le.empty()) {
+                    r->rc = SVX_E_INVALID_MODEL;
+                    put_err(&r->err, SVX_E_INVALID_MODEL, le);
+                    finish(r);
+                    continue;
+                }
+                system_set_f(r->sys.get(), r->pm.f.data());
+                system_dirichlet(r->sys.get(), (int64_t)r->pm.cons_rows.size(), r->pm.cons_rows.data());
+                system_prepare(r->sys.get(), main_prec);
+                const int64_t n = r->mesh->n;
+                InstScal sc = initial_scal(cfg.tol, std::max<int64_t>(1000, 10 * n), ctx->tune.ru_delta,
+                                           r->mesh->max_row_len);
+                r->sys->scal.upload(&sc, 1, s);
+                SVX_HIP(hipStreamSynchronize(s));
+                r->phase_precision = main_prec;
+                (main_prec == SVX_FP64 ? in64 : in32).push_back(r);
+                (main_prec == SVX_FP64 ? dirty64 : dirty32) = true;
+            }
+            // one chunk per non-empty engine
+            auto step = [&](Engine& eng, std::vector<Request*>& in, bool& dirty, int prec) {
+                if (in.empty()) return;
+                if (dirty) {
+                    std::vector<InstDesc> descs;
+                    std::vector<svx_system*> systems;
+                    for (Request* r : in) {
+                        descs.push_back(system_desc(r->sys.get()));
+                        systems.push_back(r->sys.get());
+                    }
+                    eng.set_instances(descs, std::vector<char>(in.size(), 1), batch_shape(systems, ctx->tune, prec));
+                    dirty = false;
+                }
+                EngineOptions o = opt;
+                o.precision = prec;
+                o.compact = false;  // membership is rebuilt here as instances retire
+                eng.run_chunks(o, ctx->h_flag, ctx->d_flag, 1);
+            };
+            // Each engine steps over its current membership; the sweep below
+            // reads back exactly that membership (instance i of the engine is
+            // ran[i]). An FP32-IR instance that falls back joins in64 for the
+            // next round, after the FP64 engine's own sweep.
+            const std::vector<Request*> ran32 = in32, ran64 = in64;
+            step(eng32, in32, dirty32, SVX_FP32_IR);
+            step(eng64, in64, dirty64, SVX_FP64);
+            std::vector<Request*> moved;
+            // retire / move
+            auto sweep = [&](Engine& eng, const std::vector<Request*>& in, std::vector<Request*>& next, bool& dirty,
+                             bool fp32) {
+                if (in.empty()) return;
+                std::vector<InstScal> sc;
+                eng.read_scalars(sc);
+                if (sc.size() != in.size()) throw std::logic_error("server: engine membership out of step");
+                std::vector<Request*> keep;
+                for (size_t i = 0; i < in.size(); ++i) {
+                    Request* r = in[i];
+                    const InstScal& st = sc[i];
+                    if (st.status == ST_RUNNING) {
+                        keep.push_back(r);
+                        continue;
+                    }
+                    dirty = true;
+                    if (fp32 && st.status == ST_FALLBACK) {  // continue as FP64 from the current u
+                        InstScal moved_state = st;
+                        moved_state.status = ST_RUNNING;
+                        moved_state.flags = FL_FRESH | FL_WARM;
+                        moved_state.fp64_it = -st.it;
+                        r->sys->scal.upload(&moved_state, 1, s);
+                        SVX_HIP(hipStreamSynchronize(s));
+                        r->phase_precision = SVX_FP64;
+                        moved.push_back(r);
+                        continue;
+                    }
+                    r->iterations = st.it;
+                    r->refreshes = st.refreshes;
+                    r->fp64_iterations = st.fp64_it != 0 ? st.it + st.fp64_it : 0;
+                    r->residual = st.res;
+                    if (st.status != ST_CONVERGED) {
+                        char buf[256];
+                        snprintf(buf, sizeof buf, "NotConverged { iterations: %lld, residual: %.6e }",
+                                 (long long)st.it, st.res);
+                        r->rc = SVX_E_NOT_CONVERGED;
+                        put_err(&r->err, SVX_E_NOT_CONVERGED, buf, 0, -1, st.it, st.res);
+                        finish(r);
+                        continue;
+                    }
+                    r->sys->has_u = true;
+                    const svx_model_desc& md = r->model->desc;
+                    const int nd = r->pm.ndof;
+                    std::vector<double> u(r->mesh->n);
+                    r->sys->u.download(u.data(), u.size(), s);
+                    SVX_HIP(hipStreamSynchronize(s));
+                    r->disp.assign(md.n_nodes * 3, 0.0);
+                    for (int64_t k = 0; k < md.n_nodes; ++k)
+                        for (int c = 0; c < nd; ++c) r->disp[k * 3 + c] = u[k * nd + c];
+                    r->vm.resize(md.n_elem);
+                    r->nvm.resize(md.n_nodes);
+                    r->react.resize(r->pm.cons_rows.size());
+                    system_recover(r->sys.get(), r->vm.data(), r->nvm.data(), (int64_t)r->pm.cons_rows.size(),
+                                   r->pm.cons_rows.data(), r->react.data());
+                    r->cond = r->sys->cond;
+                    r->rc = SVX_OK;
+                    finish(r);
+                }
+                next.swap(keep);
+            };
+            sweep(eng32, ran32, in32, dirty32, true);
+            sweep(eng64, ran64, in64, dirty64, false);
+            if (!moved.empty()) {
+                in64.insert(in64.end(), moved.begin(), moved.end());
+                dirty64 = true;
+            }
+        }
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu);
+        fatal = e.what();
+        for (auto& kv : reqs)
+            if (kv.second->state != St::Done) {
+                kv.second->rc = SVX_E_DEVICE;
+                put_err(&kv.second->err, SVX_E_DEVICE, fatal);
+                kv.second->state = St::Done;
+            }
+        stop = true;
+        cv_done.notify_all();
+        cv_prep.notify_all();
+    }
+}
+
+extern "C" {
+
+int svx_server_create(svx_ctx* ctx, const svx_server_config* cfg, svx_server** out) {
+    if (!ctx || !cfg || !out || (cfg->precision != SVX_FP64 && cfg->precision != SVX_FP32_IR) || !(cfg->tol > 0.0) ||
+        cfg->hbm_budget <= 0 || cfg->max_active <= 0 || cfg->chunk < 0)
+        return SVX_E_ARGUMENT;
+    try {
+        auto s = std::make_unique<svx_server>();
+        s->ctx = ctx;
+        s->cfg = *cfg;
+        svx_server* p = s.get();
+        p->prep_thread = std::thread([p] { p->prep_loop(); });
+        p->dev_thread = std::thread([p] { p->dev_loop(); });
+        *out = s.release();
+        return SVX_OK;
+    } catch (const std::exception&) {
+        return SVX_E_DEVICE;
+    }
+}
+
+void svx_server_destroy(svx_server* srv) {
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+    }
+    srv->cv_prep.notify_all();
+    srv->cv_dev.notify_all();
+    if (srv->prep_thread.joinable()) srv->prep_thread.join();
+    if (srv->dev_thread.joinable()) srv->dev_thread.join();
+    delete srv;
+}
+
+int svx_server_submit(svx_server* srv, const svx_model_desc* model, int64_t* ticket) {
+    if (!srv || !model || !ticket) return SVX_E_ARGUMENT;
+    try {
+        auto r = std::make_unique<Request>();
+        r->model = std::make_unique<OwnedModel>(*model);
+        r->err.index = -1;
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (srv->stop) return SVX_E_DEVICE;
+        r->ticket = srv->next_ticket++;
+        r->t_submit = wall_ms();
+        if (srv->t_first < 0) srv->t_first = r->t_submit;
+        *ticket = r->ticket;
+        srv->prep_q.push_back(r.get());
+        srv->reqs[r->ticket] = std::move(r);
+        srv->submitted++;
+        srv->cv_prep.notify_one();
+        return SVX_OK;
+    } catch (const std::exception&) {
+        return SVX_E_DEVICE;
+    }
+}
+
+int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms, svx_solution* out, svx_request_info* info,
+                    svx_error* err) {
+    if (err) {
+        std::memset(err, 0, sizeof *err);
+        err->index = -1;
+    }
+    if (!srv) return SVX_E_ARGUMENT;
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        auto it = srv->reqs.find(ticket);
+        if (it == srv->reqs.end()) return SVX_E_ARGUMENT;
+        Request* q = it->second.get();
+        auto done = [&] { return q->state == St::Done; };
+        if (timeout_ms < 0)
+            srv->cv_done.wait(lk, done);
+        else if (!srv->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done))
+            return SVX_E_TIMEOUT;
+        r = std::move(it->second);
+        srv->reqs.erase(it);
+    }
+    if (info) {
+        info->latency_ms = r->t_done - r->t_submit;
+        info->queue_ms = r->t_admit > 0 ? r->t_admit - r->t_submit : r->t_done - r->t_submit;
+        info->prepare_ms = r->t_prepared - r->t_submit;
+        info->iterations = r->iterations;
+        info->refreshes = r->refreshes;
+        info->fp64_iterations = r->fp64_iterations;
+        info->device_bytes = r->bytes;
+        info->n_dof = (int64_t)r->model->desc.n_nodes * r->pm.ndof;
+    }
+    if (r->rc != SVX_OK) {
+        if (err) *err = r->err;
+        return r->rc;
+    }
+    if (out) {
+        const svx_model_desc& md = r->model->desc;
+        if (out->displacements) std::copy(r->disp.begin(), r->disp.end(), out->displacements);
+        if (out->von_mises) std::copy(r->vm.begin(), r->vm.end(), out->von_mises);
+        if (out->nodal_von_mises) std::copy(r->nvm.begin(), r->nvm.end(), out->nodal_von_mises);
+        for (size_t i = 0; i < r->pm.cons_rows.size(); ++i) {
+            if (out->reaction_node) out->reaction_node[i] = r->pm.cons_node[i];
+            if (out->reaction_dof) out->reaction_dof[i] = r->pm.cons_dof[i];
+            if (out->reaction_value) out->reaction_value[i] = r->react[i];
+        }
+        out->iterations = r->iterations;
+        out->final_residual = r->residual;
+        out->cond_estimate = r->cond;
+        out->solver_method = 0;
+        out->ndof = r->pm.ndof;
+        out->assemble_ms = 0.0;
+        out->solve_ms = r->t_done - r->t_admit;
+        out->recover_ms = 0.0;
+        (void)md;
+    }
+    return SVX_OK;
+}
+
+int svx_server_stats(svx_server* srv, svx_server_summary* out) {
+    if (!srv || !out) return SVX_E_ARGUMENT;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    *out = svx_server_summary{};
+    out->submitted = srv->submitted;
+    out->completed = srv->completed;
+    out->failed = srv->failed;
+    out->queued = (int64_t)(srv->prep_q.size() + srv->ready_q.size());
+    out->resident_bytes = srv->resident;
+    out->peak_resident_bytes = srv->peak_resident;
+    out->peak_active = srv->peak_active;
+    out->solved_dof = srv->dof_done;
+    if (!srv->latencies.empty()) {
+        std::vector<double> l = srv->latencies;
+        std::sort(l.begin(), l.end());
+        auto pct = [&](double q) { return l[std::min(l.size() - 1, (size_t)(q * (l.size() - 1) + 0.5))]; };
+        out->p50_ms = pct(0.50);
+        out->p99_ms = pct(0.99);
+        out->max_ms = l.back();
+        const double span = srv->t_last_done - srv->t_first;
+        out->dof_per_s = span > 0 ? srv->dof_done / (span * 1e-3) : 0.0;
+    }
+    return srv->fatal.empty() ? SVX_OK : SVX_E_DEVICE;
+}
+
+}  // extern "C"

@@ -40,6 +40,7 @@ _KIND_OF_CODE = {
     _abi.SVX_E_DEVICE: "Device",
     _abi.SVX_E_ARGUMENT: "Argument",
     _abi.SVX_E_CAPACITY: "Capacity",
+    _abi.SVX_E_TIMEOUT: "Timeout",
 }
 
 

@@ -38,6 +38,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(svx_error), sizeof(svx_model_desc), sizeof(svx_solution),
          offsetof(svx_model_desc, thickness), offsetof(svx_model_desc, cload_value), offsetof(svx_solution, recover_ms),
          sizeof(svx_solve_stats), offsetof(svx_solve_stats, status));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(svx_server_config), offsetof(svx_server_config, chunk),
+         sizeof(svx_request_info), sizeof(svx_server_summary), offsetof(svx_server_summary, p50_ms));
   return 0;
 }
 """
@@ -49,7 +51,9 @@ int main(void) {
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     got = [C.sizeof(_abi.SvxError), C.sizeof(_abi.SvxModelDesc), C.sizeof(_abi.SvxSolution),
            _abi.SvxModelDesc.thickness.offset, _abi.SvxModelDesc.cload_value.offset,
-           _abi.SvxSolution.recover_ms.offset, C.sizeof(_abi.SvxSolveStats), _abi.SvxSolveStats.status.offset]
+           _abi.SvxSolution.recover_ms.offset, C.sizeof(_abi.SvxSolveStats), _abi.SvxSolveStats.status.offset,
+           C.sizeof(_abi.SvxServerConfig), _abi.SvxServerConfig.chunk.offset, C.sizeof(_abi.SvxRequestInfo),
+           C.sizeof(_abi.SvxServerSummary), _abi.SvxServerSummary.p50_ms.offset]
     assert [int(x) for x in out] == got
 
 

@@ -1,0 +1,104 @@
+"""Request server: continuous batching with footprint admission.
+
+Each result must equal the same deck solved alone (solve_with_method with the
+same precision) within the displacement / stress bar (1e-8 relative); with the
+fixed per-block-size launch shape the engine runs every instance exactly as
+alone, so the results are expected to be bit-identical and the test reports
+whether they are. Requests arrive while others are on the device (staggered
+submission, a slot limit below the request count).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from solvix_amd import decks
+from solvix_amd.core import Context, Server, solve_with_method
+from solvix_amd.model import Method, Precision, SolveError
+
+from helpers import rel_err, small_models
+
+pytestmark = pytest.mark.gpu
+
+
+def _decks():
+    ms = small_models()
+    ms += [decks.solid_instance(decks.SplitMix64(0x5E + i), 3e3, 2e4, name=f"blk{i}") for i in range(3)]
+    ms += [decks.plane_instance(decks.SplitMix64(0x77 + i), 2e3, 1e4, name=f"pl{i}") for i in range(3)]
+    return ms
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+def test_results_alone_equal_results_in_the_mix(precision, ctx):
+    models = _decks()
+    alone = [solve_with_method(m, Method.AUTO, precision, ctx) for m in models]
+    sctx = Context(0)
+    srv = Server(sctx, precision, hbm_budget=1 << 30, max_active=4)
+    try:
+        tickets = [srv.submit(m) for m in models[:4]]
+        time.sleep(0.05)  # the rest arrives while the first ones are on the device
+        tickets += [srv.submit(m) for m in models[4:]]
+        bitwise = 0
+        for m, t, a in zip(models, tickets, alone):
+            s, info = srv.wait(t, 60000.0)
+            eu = rel_err(s.displacements, a.displacements)
+            ev = rel_err(s.von_mises, a.von_mises)
+            en = rel_err(s.nodal_von_mises, a.nodal_von_mises)
+            same = np.array_equal(s.displacements, a.displacements)
+            bitwise += same
+            print(f"{m.name} [{precision}]: its {s.iterations} (alone {a.iterations}) err u {eu:.1e} vm {ev:.1e} "
+                  f"nodal {en:.1e} bitwise {same} latency {info['latency_ms']:.1f} ms queue {info['queue_ms']:.1f} ms")
+            assert eu <= 1e-8 and ev <= 1e-8 and en <= 1e-8
+            assert s.iterations == a.iterations
+            assert [r.node_id for r in s.reactions] == [r.node_id for r in a.reactions]
+            assert rel_err([r.value for r in s.reactions], [r.value for r in a.reactions]) <= 1e-8
+        st = srv.stats()
+        print(f"bit-identical {bitwise}/{len(models)}; stats {st}")
+        assert st["completed"] == len(models) and st["failed"] == 0
+        assert 1 <= st["peak_active"] <= 4
+        assert st["p99_ms"] >= st["p50_ms"] > 0
+    finally:
+        srv.close()
+        sctx.close()
+
+
+def test_admission_respects_the_budget_and_reports_errors(ctx):
+    models = _decks()[:4]
+    sctx = Context(0)
+    # budget for about one deck at a time: requests queue and run one by one
+    srv = Server(sctx, Precision.FP64, hbm_budget=40 << 20, max_active=64)
+    try:
+        tickets = [srv.submit(m) for m in models]
+        for m, t in zip(models, tickets):
+            s, info = srv.wait(t, 60000.0)
+            a = solve_with_method(m, Method.AUTO, Precision.FP64, ctx)
+            assert rel_err(s.displacements, a.displacements) <= 1e-8
+            assert info["device_bytes"] <= 40 << 20
+        st = srv.stats()
+        assert st["peak_resident_bytes"] <= 40 << 20
+        # a deck larger than the whole budget is refused
+        big = decks.cfg2(nx=80, ny=16, nz=16)
+        t = srv.submit(big)
+        with pytest.raises(SolveError) as e:
+            srv.wait(t, 60000.0)
+        assert e.value.kind == "Capacity"
+        # model errors come back as solve_with_method reports them
+        q = decks.quad_model(2, 1, 2, 1)
+        q.cload_node, q.cload_dof, q.cload_value = np.array([2], np.uint32), np.array([1], np.int32), np.array([1.0])
+        t = srv.submit(q)
+        with pytest.raises(SolveError) as e:
+            srv.wait(t, 60000.0)
+        assert e.value.kind == "NoConstraints"
+        h = decks.hex_model(2, 1, 1, 2, 1, 1)
+        h.conn_ids = h.conn_ids.copy()
+        h.conn_ids[8 + 1], h.conn_ids[8 + 2] = h.conn_ids[8 + 2], h.conn_ids[8 + 1]
+        h.bcs = [(np.array([1], np.uint32), np.array([1, 2, 3], np.int32))]
+        with pytest.raises(SolveError) as e_alone:
+            solve_with_method(h, Method.AUTO, Precision.FP64, ctx)
+        t = srv.submit(h)
+        with pytest.raises(SolveError) as e:
+            srv.wait(t, 60000.0)
+        assert e.value.kind == "InvalidModel" and e.value.message == e_alone.value.message
+    finally:
+        srv.close()
+        sctx.close()
