@@ -33,6 +33,8 @@ TRAFFIC_BYTES = {("cfg2", "fp64"): 188.7e6}
 CONFIGS = {
     "cfg2": ("C3D8 200x20x20 block 20x2x2 m, jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20", "cfg2"),
     "cfg1": ("CPS4 400x40 cantilever 10x1 m, jitter 0.15h, -1e4 N tip load", "cfg1"),
+    "cfg5": ("served Poisson stream: 20% plane instances (DOF 2e3..2e5), 80% C3D8 blocks (DOF 1e4..3e5), "
+             "FP32 Krylov + FP64 refinement, 200 GB admission budget", "cfg5"),
     "cfg3": ("512 uneven plane-stress instances, DOF log-uniform 2e3..2e5, CPS4 p=0.6 else CPS3, "
              "left edge clamped, tip load; one batched launch vs one-by-one FP64", "cfg3"),
 }
@@ -259,6 +261,89 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_serve(args):
+    """Open-loop served stream: requests are submitted at their Poisson arrival
+    times from a client thread while the server solves; latency is submit to
+    result. Requests arriving in the first `lead` seconds are served but not
+    measured. Solved DOF/s counts the measured requests' DOF over the span
+    from the first measured submission to the last measured completion."""
+    import threading
+
+    import numpy as np
+
+    from solvix_amd import decks
+    from solvix_amd.core import Context, Server, solve_with_method
+    from solvix_amd.model import Method, Precision
+
+    stream = list(decks.cfg5_stream(duration_s=args.lead + args.duration, rate=args.rate))
+    print(f"[bench] {len(stream)} requests generated ({args.rate}/s over {args.lead + args.duration} s)",
+          file=sys.stderr, flush=True)
+    ctx = Context(0)
+    srv = Server(ctx, Precision.FP32_IR, hbm_budget=200 << 30, max_active=1024)
+    subs = [None] * len(stream)
+    t0 = time.perf_counter()
+
+    def client():
+        for i, req in enumerate(stream):
+            delay = t0 + req.arrival_s - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            subs[i] = (time.perf_counter(), srv.submit(req.model))
+
+    th = threading.Thread(target=client)
+    th.start()
+    done = []
+    last_print = time.perf_counter()
+    for i, req in enumerate(stream):
+        while subs[i] is None:
+            time.sleep(0.001)
+        t_sub, ticket = subs[i]
+        sol, info = srv.wait(ticket)
+        done.append((req.arrival_s, t_sub - t0, time.perf_counter() - t0, info, req.model.n_dofs(), sol))
+        if time.perf_counter() - last_print > 5.0:
+            st = srv.stats()
+            print(f"[bench] {i + 1}/{len(stream)} done at {time.perf_counter() - t0:.1f} s, active "
+                  f"{st['peak_active']} peak, queued {st['queued']}", file=sys.stderr, flush=True)
+            last_print = time.perf_counter()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    meas = [d for d in done if d[0] >= args.lead]
+    lat = np.array([d[3]["latency_ms"] for d in meas])
+    span = max(d[2] for d in meas) - min(d[1] for d in meas)
+    dof = float(sum(d[4] for d in meas))
+    # per-request error vs FP64: the smallest measured requests solved alone in FP64
+    sample = sorted(meas, key=lambda d: d[4])[:8]
+    errs = []
+    for d in sample:
+        req = stream[[r.arrival_s for r in stream].index(d[0])]
+        ref = solve_with_method(req.model, Method.AUTO, Precision.FP64, ctx)
+        e = np.abs(d[5].displacements - ref.displacements).max() / np.abs(ref.displacements).max()
+        errs.append(float(e))
+    line = {
+        "metric": "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 PCG",
+        "value": round(dof / span, 1),
+        "unit": "solved DOF/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(span * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 Krylov + f64 refinement",
+        "data": "synthetic (seeded SplitMix64 deck generator)",
+        "config": {"workload": CONFIGS["cfg5"][0], "rate_per_s": args.rate, "measured_s": args.duration,
+                   "lead_s": args.lead, "requests": len(stream), "measured_requests": len(meas),
+                   "offered_dof_per_s": round(sum(r.model.n_dofs() for r in stream) / (args.lead + args.duration), 1)},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "server": st,
+        "max_rel_err_u_vs_fp64_sample": max(errs) if errs else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +355,9 @@ def main():
     ap.add_argument("--arms", default="both", choices=["both", "fp64", "fp32_ir"],
                     help="run one precision only (profiling); the JSON line needs both")
     ap.add_argument("--spmv-reps", type=int, default=100, help="launches of the timed SpMV kernel")
+    ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="served stream: measured seconds")
+    ap.add_argument("--lead", type=float, default=10.0, help="served stream: unmeasured seconds before")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (counter calibration) and exit")
     args = ap.parse_args()
@@ -283,6 +371,9 @@ def main():
 
     if args.config == "cfg3":
         bench_batch(args)
+        return
+    if args.config == "cfg5":
+        bench_serve(args)
         return
 
     import numpy as np

@@ -1,7 +1,7 @@
 // Request server: continuous batching of whole-model solves on one device.
 //
-// Decks arrive through svx_server_submit from any host thread. A host worker
-// validates each one and builds its CSR pattern (prepare_model,
+// Decks arrive through svx_server_submit from any host thread. Host workers
+// (up to 4) validate each one and build its CSR pattern (prepare_model,
 // mesh_build_host). The device thread admits prepared requests while their
 // estimated device footprint fits the HBM budget and an instance slot is
 // free: it uploads the mesh, assembles, applies the constraints and hands the
@@ -149,7 +149,8 @@ struct svx_server {
     std::vector<Request*> active;  // admitted, in an engine
     int64_t next_ticket = 1;
     bool stop = false;
-    std::thread prep_thread, dev_thread;
+    std::vector<std::thread> prep_threads;
+    std::thread dev_thread;
     // statistics (guarded by mu)
     int64_t submitted = 0, completed = 0, failed = 0, resident = 0, peak_resident = 0, peak_active = 0;
     int64_t dof_done = 0;
@@ -419,7 +420,8 @@ int svx_server_create(svx_ctx* ctx, const svx_server_config* cfg, svx_server** o
         s->ctx = ctx;
         s->cfg = *cfg;
         svx_server* p = s.get();
-        p->prep_thread = std::thread([p] { p->prep_loop(); });
+        const int workers = std::max(1, std::min(4, (int)std::thread::hardware_concurrency() / 4));
+        for (int i = 0; i < workers; ++i) p->prep_threads.emplace_back([p] { p->prep_loop(); });
         p->dev_thread = std::thread([p] { p->dev_loop(); });
         *out = s.release();
         return SVX_OK;
@@ -436,7 +438,8 @@ void svx_server_destroy(svx_server* srv) {
     }
     srv->cv_prep.notify_all();
     srv->cv_dev.notify_all();
-    if (srv->prep_thread.joinable()) srv->prep_thread.join();
+    for (std::thread& t : srv->prep_threads)
+        if (t.joinable()) t.join();
     if (srv->dev_thread.joinable()) srv->dev_thread.join();
     delete srv;
 }
