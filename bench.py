@@ -33,6 +33,8 @@ TRAFFIC_BYTES = {("cfg2", "fp64"): 188.7e6}
 CONFIGS = {
     "cfg2": ("C3D8 200x20x20 block 20x2x2 m, jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20", "cfg2"),
     "cfg1": ("CPS4 400x40 cantilever 10x1 m, jitter 0.15h, -1e4 N tip load", "cfg1"),
+    "cfg4": ("64 load cases on one C3D8 120x12x12 cantilever (61,347 DOF), 1-8 point loads each; "
+             "one block solve vs 64 single solves", "cfg4"),
     "cfg5": ("served Poisson stream: 20% plane instances (DOF 2e3..2e5), 80% C3D8 blocks (DOF 1e4..3e5), "
              "FP32 Krylov + FP64 refinement, 200 GB admission budget", "cfg5"),
     "cfg3": ("512 uneven plane-stress instances, DOF log-uniform 2e3..2e5, CPS4 p=0.6 else CPS3, "
@@ -261,6 +263,79 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_multi(args):
+    """64 load cases on one assembled, constrained C3D8 system. A step solves
+    all of them to 1e-10: either one FP64 PCG per case (set the load vector,
+    solve) or one block PCG over all cases (the matrix read once per
+    iteration for all of them)."""
+    import numpy as np
+
+    from solvix_amd import decks
+    from solvix_amd.core import Context, assemble, constrained_dofs
+    from solvix_amd.model import Precision
+
+    model, cases = decks.cfg4()
+    ctx = Context(0)
+    mesh, sy = assemble(model, ctx)
+    sy.apply_dirichlet(constrained_dofs(model)[0])
+    n = mesh.n
+    pos = {int(nid): i for i, nid in enumerate(model.node_ids)}
+    F = np.zeros((len(cases), n))
+    for k, (nodes, dofs, vals) in enumerate(cases):
+        for nid, d, v in zip(nodes, dofs, vals):
+            F[k, pos[int(nid)] * 3 + int(d) - 1] += v
+    out = {}
+    arms = [a for a in ("single", "block") if args.multi_arm in ("both", a)]
+    for arm in arms:
+        for k in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            if arm == "single":
+                U = np.zeros_like(F)
+                its = np.zeros(len(cases), np.int64)
+                for j in range(len(cases)):
+                    sy.set_rhs(F[j])
+                    U[j], its[j], _ = sy.solve(1e-10, 0, Precision.FP64)
+                ok = np.ones(len(cases), bool)
+            else:
+                U, its, res, ok, dev_ms = sy.solve_multi(F, 1e-10, 0, Precision.FP64, raise_on_failure=False)
+            dt = time.perf_counter() - t0
+            print(f"[bench] {arm}: pass {k + 1}/{args.warmup + args.steps} {dt:.3f} s, converged "
+                  f"{int(ok.sum())}/{ok.size}, iterations max {int(its.max())} mean {its.mean():.0f}",
+                  file=sys.stderr, flush=True)
+            if k >= args.warmup:
+                rec = out.setdefault(arm, {"dt": 0.0})
+                rec["dt"] += dt
+                rec.update(U=U, its=its, ok=ok)
+    if len(arms) < 2:
+        return
+    a, b = out["block"], out["single"]
+    dof = float(n) * len(cases) * args.steps
+    err = float(max(np.abs(a["U"][j] - b["U"][j]).max() / np.abs(b["U"][j]).max() for j in range(len(cases))))
+    line = {
+        "metric": "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 PCG",
+        "value": round(dof / a["dt"], 1),
+        "unit": "solved DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(a["dt"] / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 deck generator)",
+        "config": {"workload": CONFIGS["cfg4"][0], "n_dof": n, "nnz": int(mesh.nnz), "load_cases": len(cases),
+                   "tolerance": 1e-10},
+        "iterations_max": int(a["its"].max()),
+        "converged": int(a["ok"].sum()),
+        "max_rel_err_u_vs_single": err,
+        "single_fp64": {"value": round(dof / b["dt"], 1), "ms_per_step": round(b["dt"] / args.steps * 1e3, 3),
+                        "iterations_max": int(b["its"].max())},
+        "speedup_vs_fp64": round(b["dt"] / a["dt"], 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_serve(args):
     """Open-loop served stream: requests are submitted at their Poisson arrival
     times from a client thread while the server solves; latency is submit to
@@ -355,6 +430,8 @@ def main():
     ap.add_argument("--arms", default="both", choices=["both", "fp64", "fp32_ir"],
                     help="run one precision only (profiling); the JSON line needs both")
     ap.add_argument("--spmv-reps", type=int, default=100, help="launches of the timed SpMV kernel")
+    ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block"],
+                    help="multi-RHS workload: run one arm only (profiling)")
     ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="served stream: measured seconds")
     ap.add_argument("--lead", type=float, default=10.0, help="served stream: unmeasured seconds before")
@@ -374,6 +451,9 @@ def main():
         return
     if args.config == "cfg5":
         bench_serve(args)
+        return
+    if args.config == "cfg4":
+        bench_multi(args)
         return
 
     import numpy as np

@@ -2,12 +2,17 @@
 //
 // Vectors are stored row-major [n][R]: one 64-wide wavefront owns one matrix
 // row and its lane j owns load case j. The SpMV reads each row's values and
-// column indices once (wave-uniform scalar loads) and multiplies them with R
+// column indices once (wave-uniform loads) and multiplies them with R
 // right-hand sides at once, so the matrix streams from HBM once per iteration
 // for all load cases (block SpMV). Every load case runs the recurrence of
-// solver.rs:82-149 with its own alpha, beta, residual and convergence mask;
-// dot products are reduced per load case in a fixed order (lane-local sums,
-// waves of the block in order, tiles in order).
+// solver.rs:82-149 with its own alpha, beta, residual and convergence mask.
+//
+// One iteration is five launches: SpMV (partial p.q per tile and load case),
+// a one-workgroup reduction (alpha, breakdown), the update of u, r, z
+// (partials r.r, r.z), a second reduction (residual, convergence, beta) and
+// the p update. Reductions are per load case in a fixed order (lane-local
+// sums, tiles strided over the reduction's waves, waves in order), with no
+// atomics or fences; the kernel boundaries order the data.
 #include "block.hpp"
 
 namespace svx {
@@ -15,9 +20,12 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kBlock / kWave;  // rows per block pass
+constexpr int kReduceThreads = 1024;
+constexpr int kReduceWaves = kReduceThreads / kWave;
 
-// Per-load-case partials of this block: lanes sum their waves in order.
-__device__ __forceinline__ void block_rhs_sum2(double& a, double& b) {
+// Per-load-case partials of this block (waves summed in order) into tile's slot.
+__device__ __forceinline__ void write_rhs_partials(double a, double b, double* partials, int tile, int n_tiles,
+                                                   int nv) {
     __shared__ double sa[kWavesPerBlock][kWave], sb[kWavesPerBlock][kWave];
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     sa[w][lane] = a;
@@ -27,36 +35,37 @@ __device__ __forceinline__ void block_rhs_sum2(double& a, double& b) {
         a = sa[0][lane], b = sb[0][lane];
 #pragma unroll
         for (int i = 1; i < kWavesPerBlock; ++i) a += sa[i][lane], b += sb[i][lane];
+        partials[(size_t)tile * kWave + lane] = a;
+        if (nv > 1) partials[((size_t)n_tiles + tile) * kWave + lane] = b;
     }
-    __syncthreads();
 }
 
-// Publish wave 0's per-lane partials for this tile; true in the last tile.
-__device__ __forceinline__ bool publish_rhs(double a, double b, double* partials, int tile, uint32_t* counter,
-                                            int n_tiles) {
-    __shared__ int s_last;
+// Sums over tiles of slot s (0 or 1) for every load case: wave w adds tiles
+// w, w + 16, ... in turn; the 16 wave sums are then added in order.
+__device__ __forceinline__ double reduce_slot(const double* __restrict__ partials, int n_tiles, int s) {
+    __shared__ double sh[kReduceWaves][kWave];
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    if (w == 0) {
-        partials[(size_t)tile * 2 * kWave + lane] = a;
-        partials[(size_t)tile * 2 * kWave + kWave + lane] = b;
+    const double* base = partials + (size_t)s * n_tiles * kWave;
+    double acc = 0.0;
+    int t = w;
+    for (; t + 3 * kReduceWaves < n_tiles; t += 4 * kReduceWaves) {  // four loads in flight
+        const double v0 = base[(size_t)t * kWave + lane];
+        const double v1 = base[(size_t)(t + kReduceWaves) * kWave + lane];
+        const double v2 = base[(size_t)(t + 2 * kReduceWaves) * kWave + lane];
+        const double v3 = base[(size_t)(t + 3 * kReduceWaves) * kWave + lane];
+        acc += v0;
+        acc += v1;
+        acc += v2;
+        acc += v3;
     }
-    __threadfence();
+    for (; t < n_tiles; t += kReduceWaves) acc += base[(size_t)t * kWave + lane];
+    sh[w][lane] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(counter, 1u) == (uint32_t)(n_tiles - 1);
+    double sum = sh[0][lane];
+#pragma unroll
+    for (int i = 1; i < kReduceWaves; ++i) sum += sh[i][lane];
     __syncthreads();
-    bool last = s_last != 0;
-    if (last) __threadfence();
-    return last;
-}
-
-// Sum over tiles (in order) for every load case, computed by wave 0.
-__device__ __forceinline__ void reduce_rhs(const double* partials, int n_tiles, double& a, double& b) {
-    const int lane = threadIdx.x % kWave;
-    a = 0.0, b = 0.0;
-    for (int t = 0; t < n_tiles; ++t) {
-        a += partials[(size_t)t * 2 * kWave + lane];
-        b += partials[(size_t)t * 2 * kWave + kWave + lane];
-    }
+    return sum;
 }
 
 __global__ __launch_bounds__(kBlock) void blk_init_kernel(BlockArgs A) {
@@ -77,26 +86,26 @@ __global__ __launch_bounds__(kBlock) void blk_init_kernel(BlockArgs A) {
         ff += fi * fi;
         rz += fi * zi;
     }
-    block_rhs_sum2(ff, rz);
-    if (!publish_rhs(ff, rz, A.partials, tile, &A.S->cnt_a, A.n_tiles)) return;
-    if (w == 0) {
-        reduce_rhs(A.partials, A.n_tiles, ff, rz);
-        if (on) {
-            A.S->bnorm[lane] = sqrt(ff);
-            A.S->rr[lane] = ff;
-            A.S->rz[lane] = rz;
-            A.S->it[lane] = 0;
-            A.S->status[lane] = (ff == 0.0) ? ST_CONVERGED : ST_RUNNING;
-            A.S->res[lane] = 0.0;
-        }
-        if (lane == 0) A.S->cnt_a = 0;
-    }
+    write_rhs_partials(ff, rz, A.partials, tile, A.n_tiles, 2);
 }
 
-__device__ __forceinline__ bool any_running(const BlockScal* S, int R) {
-    for (int j = 0; j < R; ++j)
-        if (S->status[j] == ST_RUNNING) return true;
-    return false;
+// One workgroup: initial scalars per load case.
+__global__ __launch_bounds__(kReduceThreads) void blk_init_finish_kernel(BlockArgs A) {
+    const double ff = reduce_slot(A.partials, A.n_tiles, 0);
+    const double rz = reduce_slot(A.partials, A.n_tiles, 1);
+    const int lane = threadIdx.x;
+    if (lane >= kWave) return;
+    const bool on = lane < A.R;
+    if (on) {
+        A.S->bnorm[lane] = sqrt(ff);
+        A.S->rr[lane] = ff;
+        A.S->rz[lane] = rz;
+        A.S->it[lane] = 0;
+        A.S->status[lane] = (ff == 0.0) ? ST_CONVERGED : ST_RUNNING;  // solver.rs:93-95
+        A.S->res[lane] = 0.0;
+    }
+    unsigned long long m = __ballot(on && ff != 0.0);
+    if (lane == 0) A.S->any_running = m != 0ull;
 }
 
 __global__ __launch_bounds__(kBlock) void blk_spmv_kernel(BlockArgs A) {
@@ -105,33 +114,45 @@ __global__ __launch_bounds__(kBlock) void blk_spmv_kernel(BlockArgs A) {
     const int tile = blockIdx.x;
     const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
     const bool on = lane < A.R;
+    const int R = A.R;
     double pq = 0.0;
     for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
         double acc = 0.0;
         const int32_t kb = A.row_ptr[i], ke = A.row_ptr[i + 1];
-        if (on)
-            for (int32_t k = kb; k < ke; ++k) acc += A.val[k] * A.P[(size_t)A.col[k] * A.R + lane];
         if (on) {
-            size_t o = (size_t)i * A.R + lane;
+            int32_t k = kb;
+            for (; k + 3 < ke; k += 4) {  // four gathers in flight
+                const int32_t c0 = A.col[k], c1 = A.col[k + 1], c2 = A.col[k + 2], c3 = A.col[k + 3];
+                const double v0 = A.val[k], v1 = A.val[k + 1], v2 = A.val[k + 2], v3 = A.val[k + 3];
+                const double p0 = A.P[(size_t)c0 * R + lane], p1 = A.P[(size_t)c1 * R + lane];
+                const double p2 = A.P[(size_t)c2 * R + lane], p3 = A.P[(size_t)c3 * R + lane];
+                acc += v0 * p0;
+                acc += v1 * p1;
+                acc += v2 * p2;
+                acc += v3 * p3;
+            }
+            for (; k < ke; ++k) acc += A.val[k] * A.P[(size_t)A.col[k] * R + lane];
+            size_t o = (size_t)i * R + lane;
             A.Q[o] = acc;
             pq += A.P[o] * acc;
         }
     }
-    double dummy = 0.0;
-    block_rhs_sum2(pq, dummy);
-    if (!publish_rhs(pq, 0.0, A.partials, tile, &A.S->cnt_a, A.n_tiles)) return;
-    if (w == 0) {
-        reduce_rhs(A.partials, A.n_tiles, pq, dummy);
-        if (on && A.S->status[lane] == ST_RUNNING) {
-            if (pq <= 0.0 || pq != pq) {
-                A.S->it[lane] += 1;
-                A.S->res[lane] = sqrt(A.S->rr[lane]) / A.S->bnorm[lane];
-                A.S->status[lane] = ST_BREAKDOWN;
-            } else {
-                A.S->alpha[lane] = A.S->rz[lane] / pq;
-            }
-        }
-        if (lane == 0) A.S->cnt_a = 0;
+    write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
+}
+
+// One workgroup: alpha per load case, breakdown test (solver.rs:118-123).
+__global__ __launch_bounds__(kReduceThreads) void blk_alpha_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
+    const double pq = reduce_slot(A.partials, A.n_tiles, 0);
+    const int lane = threadIdx.x;
+    if (lane >= kWave || lane >= A.R || A.S->status[lane] != ST_RUNNING) return;
+    if (pq <= 0.0 || pq != pq) {
+        A.S->it[lane] += 1;
+        A.S->res[lane] = sqrt(A.S->rr[lane]) / A.S->bnorm[lane];
+        A.S->status[lane] = ST_BREAKDOWN;
+        A.S->alpha[lane] = 0.0;
+    } else {
+        A.S->alpha[lane] = A.S->rz[lane] / pq;
     }
 }
 
@@ -154,24 +175,41 @@ __global__ __launch_bounds__(kBlock) void blk_update_kernel(BlockArgs A) {
             rr += ri * ri;
             rz += ri * zi;
         }
-    block_rhs_sum2(rr, rz);
-    if (!publish_rhs(rr, rz, A.partials, tile, &A.S->cnt_b, A.n_tiles)) return;
-    if (w == 0) {
-        reduce_rhs(A.partials, A.n_tiles, rr, rz);
-        if (lane < A.R && A.S->status[lane] == ST_RUNNING) {
-            A.S->rr[lane] = rr;
-            A.S->it[lane] += 1;
-            double res = sqrt(rr) / A.S->bnorm[lane];
-            A.S->res[lane] = res;
-            if (res <= A.S->tol) {
-                A.S->status[lane] = ST_CONVERGED;
-            } else {
-                A.S->beta[lane] = rz / A.S->rz[lane];
-                A.S->rz[lane] = rz;
-                if (A.S->it[lane] >= A.S->max_iter) A.S->status[lane] = ST_MAXITER;
-            }
+    write_rhs_partials(rr, rz, A.partials, tile, A.n_tiles, 2);
+}
+
+// One workgroup: residual, convergence, budget and beta per load case; the
+// running flag every kernel reads.
+__global__ __launch_bounds__(kReduceThreads) void blk_beta_kernel(BlockArgs A, int32_t* running_out) {
+    if (!A.S->any_running) {
+        if (threadIdx.x == 0 && running_out) *running_out = 0;
+        return;
+    }
+    const double rr = reduce_slot(A.partials, A.n_tiles, 0);
+    const double rz = reduce_slot(A.partials, A.n_tiles, 1);
+    const int lane = threadIdx.x;
+    if (lane >= kWave) return;
+    bool run = false;
+    if (lane < A.R && A.S->status[lane] == ST_RUNNING) {
+        A.S->rr[lane] = rr;
+        A.S->it[lane] += 1;
+        double res = sqrt(rr) / A.S->bnorm[lane];
+        A.S->res[lane] = res;
+        if (res <= A.S->tol) {  // solver.rs:131-134
+            A.S->status[lane] = ST_CONVERGED;
+        } else {
+            A.S->beta[lane] = rz / A.S->rz[lane];
+            A.S->rz[lane] = rz;
+            if (A.S->it[lane] >= A.S->max_iter)
+                A.S->status[lane] = ST_MAXITER;
+            else
+                run = true;
         }
-        if (lane == 0) A.S->cnt_b = 0;
+    }
+    unsigned long long m = __ballot(run);
+    if (lane == 0) {
+        A.S->any_running = m != 0ull;
+        if (running_out) *running_out = __popcll(m);
     }
 }
 
@@ -189,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void blk_pupdate_kernel(BlockArgs A) {
     }
 }
 
-// One lane per load case: recompute the running flag read by every kernel.
+// One lane per load case: the number of running load cases.
 __global__ void blk_flag_kernel(BlockArgs A, int32_t* running_out) {
     const int lane = threadIdx.x;
     int run = lane < A.R && A.S->status[lane] == ST_RUNNING;
@@ -232,15 +270,16 @@ void blk_unpack(const double* in, double* out, int32_t n, int32_t R, hipStream_t
 
 void blk_init(const BlockArgs& a, hipStream_t s) {
     blk_init_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
-    blk_flag_kernel<<<1, kWave, 0, s>>>(a, nullptr);
+    blk_init_finish_kernel<<<1, kReduceThreads, 0, s>>>(a);
     SVX_HIP(hipGetLastError());
 }
 
 void blk_iteration(const BlockArgs& a, hipStream_t s) {
     blk_spmv_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
+    blk_alpha_kernel<<<1, kReduceThreads, 0, s>>>(a);
     blk_update_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
+    blk_beta_kernel<<<1, kReduceThreads, 0, s>>>(a, nullptr);
     blk_pupdate_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
-    blk_flag_kernel<<<1, kWave, 0, s>>>(a, nullptr);
     SVX_HIP(hipGetLastError());
 }
 

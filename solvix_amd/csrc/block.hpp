@@ -17,7 +17,6 @@ struct alignas(16) BlockScal {
     int32_t status[kMaxRhs];
     double tol;
     int64_t max_iter;
-    uint32_t cnt_a, cnt_b;
     int32_t any_running;
     int32_t pad_;
 };
@@ -37,7 +36,7 @@ struct BlockArgs {
     double* P;
     double* Z;
     double* Q;
-    double* partials;  // [n_tiles][2][64]
+    double* partials;  // [2][n_tiles][64]: slot-major per-tile, per-load-case sums
     BlockScal* S;
 };
 
