@@ -61,3 +61,27 @@ def test_batch_matches_oracle(ctx):
             e = rel_err(vm, ref.von_mises)
             print(f"{m.name} [{prec}] its {its} (oracle {ref.iterations}) res {r:.2e} err vm {e:.1e}")
             assert ok and e <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+def test_autotuned_batch_matches_oracle(precision, ctx):
+    """The autotuner's shape changes summation orders only: results stay
+    within the tolerance of the oracle, 2D and 3D in one batch."""
+    models = [decks.plane_instance(decks.SplitMix64(0x53170003 ^ i), 2e3, 1.5e4) for i in range(5)]
+    models.append(decks.cfg2(nx=12, ny=3, nz=3))
+    b = Batch(ctx)
+    keep = []
+    for m in models:
+        mesh, s = _system(m, ctx)
+        keep.append((mesh, s))
+        b.add(s)
+    tuned = b.autotune(precision, 2)
+    print(f"[{precision}] tuned {tuned}")
+    assert tuned["spmv_ms"] > 0
+    assert tuned["lanes"][2] in (2, 4, 8, 16, 32) and tuned["lanes"][3] in (2, 4, 8, 16, 32)
+    res = b.solve(precision)
+    for (mesh, s), m, (its, r, ok) in zip(keep, models, res):
+        ref = oracle.solve(m)
+        e = rel_err(s.solution(), ref.u)
+        print(f"{m.name} [{precision}] its {its} (oracle {ref.iterations}) err u {e:.1e}")
+        assert ok and e <= 1e-8

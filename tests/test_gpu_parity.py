@@ -15,7 +15,7 @@ from solvix_amd.core import CsrMatrix, Mesh, assemble, constrained_dofs, solve_w
 from solvix_amd.inp import read_inp
 from solvix_amd.model import Method, Precision, SolveError
 
-from helpers import dirichlet, eval_floor, rel_err, small_models
+from helpers import FLOOR_SLACK, dirichlet, eval_floor, rel_err, small_models
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -72,7 +72,7 @@ def test_raw_pcg_matches_oracle(model, precision, ctx):
           f"(floor {fl:.1e}), rel err u {e:.2e}")
     # FP64 reports the recursive residual (solver.rs:131); FP32-IR the true
     # FP64 residual, bounded by tol plus the FP64 evaluation floor.
-    assert res <= 1e-10 + (fl if precision == Precision.FP32_IR else 0.0)
+    assert res <= 1e-10 + (FLOOR_SLACK * fl if precision == Precision.FP32_IR else 0.0)
     assert e <= 1e-8
 
 
@@ -92,7 +92,7 @@ def test_solve_model_matches_oracle(model, precision, ctx):
         v, f = dirichlet(r.row_ptr, r.col_idx, r.vals, r.f, r.reaction_rows)
         u = s.displacements[:, :model.ndof()].ravel()
         fl = eval_floor(r.row_ptr, r.col_idx, v, f, u)
-    assert s.final_residual <= 1e-10 + fl
+    assert s.final_residual <= 1e-10 + FLOOR_SLACK * fl
     assert eu <= 1e-8 and ev <= 1e-8 and en <= 1e-8
     assert er <= 1e-6
     assert [x.node_id for x in s.reactions] == list(r.reaction_node)
