@@ -7,6 +7,8 @@
 // for all load cases (block SpMV). Every load case runs the recurrence of
 // solver.rs:82-149 with its own alpha, beta, residual and convergence mask.
 //
+// Plane and solid meshes use a node-row SpMV (one wavefront per node, each
+// gathered row of P serving the node's ndof rows).
 // One iteration is five launches: SpMV (partial p.q per tile and load case),
 // a one-workgroup reduction (alpha, breakdown), the update of u, r, z
 // (partials r.r, r.z), a second reduction (residual, convergence, beta) and
@@ -135,6 +137,57 @@ __global__ __launch_bounds__(kBlock) void blk_spmv_kernel(BlockArgs A) {
             size_t o = (size_t)i * R + lane;
             A.Q[o] = acc;
             pq += A.P[o] * acc;
+        }
+    }
+    write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
+}
+
+// Node-row form of the block SpMV for plane / solid meshes: one wavefront
+// owns the BS rows of one node, which share their columns, so each gathered
+// row of P (one load case per lane) serves all BS rows; the values are read
+// as node blocks of the CSR storage (row a of the node holds block k's
+// entries at val[row_ptr[node*BS] + a*BS*nb + BS*k + b]). Each row still sums
+// its entries in column order, so the results equal the row form's.
+template <int BS>
+__global__ __launch_bounds__(kBlock) void blk_spmv_node_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
+    const bool on = lane < A.R;
+    const int R = A.R;
+    double pq = 0.0;
+    for (int32_t node = r0 / BS + w; node < r1 / BS; node += kWavesPerBlock) {
+        double acc[BS];
+#pragma unroll
+        for (int a = 0; a < BS; ++a) acc[a] = 0.0;
+        const int32_t kb = A.brow_ptr[node], nb = A.brow_ptr[node + 1] - kb;
+        const int32_t rs = A.row_ptr[node * BS];
+        if (on) {
+            if (A.row_ptr[node * BS + BS] - rs != BS * BS * nb) {  // isolated node: diagonal entries only
+#pragma unroll
+                for (int a = 0; a < BS; ++a)
+                    acc[a] = A.val[A.row_ptr[node * BS + a]] * A.P[(size_t)(node * BS + a) * R + lane];
+            } else {
+                const double* v0 = A.val + rs;
+#pragma unroll 2
+                for (int32_t k = 0; k < nb; ++k) {
+                    const int32_t j = A.bcol[kb + k];
+                    double p[BS];
+#pragma unroll
+                    for (int b = 0; b < BS; ++b) p[b] = A.P[(size_t)(j * BS + b) * R + lane];
+#pragma unroll
+                    for (int a = 0; a < BS; ++a)
+#pragma unroll
+                        for (int b = 0; b < BS; ++b) acc[a] += v0[a * BS * nb + BS * k + b] * p[b];
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < BS; ++a) {
+                const size_t o = (size_t)(node * BS + a) * R + lane;
+                A.Q[o] = acc[a];
+                pq += A.P[o] * acc[a];
+            }
         }
     }
     write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
@@ -274,8 +327,17 @@ void blk_init(const BlockArgs& a, hipStream_t s) {
     SVX_HIP(hipGetLastError());
 }
 
+void launch_blk_spmv(const BlockArgs& a, hipStream_t s) {
+    if (a.bs == 3)
+        blk_spmv_node_kernel<3><<<a.n_tiles, kBlock, 0, s>>>(a);
+    else if (a.bs == 2)
+        blk_spmv_node_kernel<2><<<a.n_tiles, kBlock, 0, s>>>(a);
+    else
+        blk_spmv_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
+}
+
 void blk_iteration(const BlockArgs& a, hipStream_t s) {
-    blk_spmv_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
+    launch_blk_spmv(a, s);
     blk_alpha_kernel<<<1, kReduceThreads, 0, s>>>(a);
     blk_update_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
     blk_beta_kernel<<<1, kReduceThreads, 0, s>>>(a, nullptr);
@@ -289,7 +351,7 @@ void blk_count_running(const BlockArgs& a, int32_t* out, hipStream_t s) {
 }
 
 void blk_spmv_only(const BlockArgs& a, hipStream_t s) {
-    blk_spmv_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
+    launch_blk_spmv(a, s);
     SVX_HIP(hipGetLastError());
 }
 

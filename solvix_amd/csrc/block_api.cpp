@@ -46,9 +46,12 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
         BlockArgs a{};
         a.n = n;
         a.R = R;
-        a.rows_per_tile = 64;
+        a.bs = m->ndof > 1 ? m->ndof : 1;
+        a.rows_per_tile = a.bs > 1 ? 96 : 64;  // 96 rows = 32 / 48 nodes, 8 / 12 per wavefront
         a.n_tiles = (n + a.rows_per_tile - 1) / a.rows_per_tile;
         a.row_ptr = m->row_ptr.p;
+        a.brow_ptr = m->brow_ptr.p;
+        a.bcol = m->bcol.p;
         a.col = m->col.p;
         a.val = sys->vals.p;
         a.dinv = sys->dinv.p;
