@@ -239,6 +239,7 @@ typedef struct svx_solve_stats {
     int64_t iterations;      /* all Krylov iterations, FP64 fallback included */
     int64_t refreshes;       /* FP32-IR: FP64 residual replacements */
     int64_t fp64_iterations; /* FP32-IR: iterations run by the FP64 fallback */
+    int64_t residual_evals;  /* FP32-IR: FP64 residuals f - K u evaluated */
     double residual;
     double floor;
     int32_t status;    /* 0 converged, 1 breakdown, 2 iteration budget spent */

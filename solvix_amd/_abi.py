@@ -117,6 +117,7 @@ class SvxSolveStats(C.Structure):
         ("iterations", C.c_int64),
         ("refreshes", C.c_int64),
         ("fp64_iterations", C.c_int64),
+        ("residual_evals", C.c_int64),
         ("residual", C.c_double),
         ("floor", C.c_double),
         ("status", C.c_int32),

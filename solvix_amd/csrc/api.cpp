@@ -596,6 +596,7 @@ int svx_system_stats(const svx_system* sys, svx_solve_stats* out) {
     out->iterations = sc.it;
     out->refreshes = sc.refreshes;
     out->fp64_iterations = sc.fp64_it != 0 ? sc.it + sc.fp64_it : 0;
+    out->residual_evals = sc.evals;
     out->residual = sc.res;
     out->floor = sc.floor;
     out->status = sc.status == ST_CONVERGED ? 0 : sc.status == ST_BREAKDOWN ? 1 : 2;

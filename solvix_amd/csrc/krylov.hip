@@ -44,6 +44,7 @@ constexpr int kWaves = kBlock / kWave;
 constexpr int32_t FL_HOLD = 2;          // FP32-IR: budget reached, awaiting the final FP64 check
 constexpr int64_t kForcedRefresh = 256; // FP32-IR: check the FP64 residual at least this often
 constexpr double kFloorSlack = 4.0;     // FP32-IR: floors accepted once the residual stagnates
+constexpr int kStallLimit = 4;          // FP32-IR: refreshes without halving the best residual before FP64 takes over
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -345,11 +346,12 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
     S->rz = v[1];
     S->it = 0;
     S->rmax = S->bnorm;
-    S->res_best = 1.0;
+    S->res_best = HUGE_VAL;  // the first refresh sets the reference (CG residuals may first grow above 1)
     S->stall = 0;
     S->refreshes = 0;
     S->it_refresh = 0;
     S->fp64_it = 0;
+    S->evals = 0;
     S->floor = 0.0;
     S->flags = 0;
     if (S->bnorm == 0.0) {  // solver.rs:93-95
@@ -619,6 +621,7 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
     const InstDesc& D = descs[t.inst];
     const InstScal* S = t.s;
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
+    if (is_leader(t)) const_cast<InstScal*>(S)->evals += 1;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
     double v[3] = {0.0, 0.0, 0.0};
@@ -672,7 +675,7 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
     // FP64 product K u cannot resolve f - K u any further).
     const bool conv = res <= S->tol + floor || (stall >= 2 && res <= S->tol + kFloorSlack * floor);
     const bool hold = (S->snap_flags & FL_HOLD) != 0;
-    const bool fallback = !conv && !hold && stall >= 4;
+    const bool fallback = !conv && !hold && stall >= kStallLimit;
     const bool cont = !conv && !hold && !fallback;
     const double beta = v[1] / S->snap_rz;
     if (is_leader(t)) {

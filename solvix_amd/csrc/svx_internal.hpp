@@ -93,6 +93,7 @@ struct alignas(16) InstScal {
     int64_t refreshes;  // FP32-IR residual replacements
     int64_t it_refresh; // FP32-IR: iteration of the latest refresh
     int64_t fp64_it;    // FP32-IR fallback: -(iterations before FP64 took over)
+    int64_t evals;      // FP32-IR: FP64 residual evaluations (refresh steps run)
     int32_t status;     // ST_*
     int32_t flags;      // FL_*
     int32_t stall;      // FP32-IR refreshes without halving res_best
