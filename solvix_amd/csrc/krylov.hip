@@ -46,6 +46,54 @@ constexpr int64_t kForcedRefresh = 256; // FP32-IR: check the FP64 residual at l
 constexpr double kFloorSlack = 4.0;     // FP32-IR: floors accepted once the residual stagnates
 constexpr int kStallLimit = 4;          // FP32-IR: refreshes without halving the best residual before FP64 takes over
 
+// Instance arrays typed as global memory. The pointers come out of the
+// descriptor table, so the compiler cannot infer their address space: as
+// plain pointers they compile to flat loads, whose waits also cover LDS
+// traffic and cannot retire one load ahead of the others (every column index
+// would wait for the value stream issued beside it). Typed global, they
+// compile to global loads with partial waits.
+#define SVX_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+using gptr = SVX_GLOBAL T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> global(T* p) {
+    return (gptr<T>)p;
+}
+
+struct Dev {
+    int32_t n;
+    gptr<const int32_t> row_ptr, col, brow_ptr, bcol;
+    gptr<const double> val, dinv, f;
+    gptr<const float> bval, dinv32;
+    gptr<double> u, r, p, z, q;
+    gptr<float> x32, r32, p32, z32, q32;
+};
+
+__device__ __forceinline__ Dev view(const InstDesc& d) {
+    Dev v;
+    v.n = d.n;
+    v.row_ptr = global(d.row_ptr);
+    v.col = global(d.col);
+    v.brow_ptr = global(d.brow_ptr);
+    v.bcol = global(d.bcol);
+    v.val = global(d.val);
+    v.dinv = global(d.dinv);
+    v.f = global(d.f);
+    v.bval = global(d.bval);
+    v.dinv32 = global(d.dinv32);
+    v.u = global(d.u);
+    v.r = global(d.r);
+    v.p = global(d.p);
+    v.z = global(d.z);
+    v.q = global(d.q);
+    v.x32 = global(d.x32);
+    v.r32 = global(d.r32);
+    v.p32 = global(d.p32);
+    v.z32 = global(d.z32);
+    v.q32 = global(d.q32);
+    return v;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
@@ -114,9 +162,8 @@ __device__ __forceinline__ void reduce_tiles(const double* __restrict__ partials
 }
 
 template <int LANES>
-__device__ __forceinline__ double csr_row(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                          const double* __restrict__ val, const double* __restrict__ x, int32_t row,
-                                          int lane) {
+__device__ __forceinline__ double csr_row(gptr<const int32_t> rp, gptr<const int32_t> col, gptr<const double> val,
+                                          gptr<const double> x, int32_t row, int lane) {
     double acc = 0.0;
     const int32_t kb = rp[row], ke = rp[row + 1];
 #pragma unroll 4
@@ -134,9 +181,9 @@ __device__ __forceinline__ double csr_row(const int32_t* __restrict__ rp, const 
 // A node whose rows hold only their diagonal (no element touches it) is
 // multiplied entry by entry.
 template <int BS, int LANES>
-__device__ __forceinline__ void node_row64(const int32_t* __restrict__ rp, const int32_t* __restrict__ brp,
-                                           const int32_t* __restrict__ bcol, const double* __restrict__ val,
-                                           const double* __restrict__ x, int32_t br, int lane, double (&acc)[BS]) {
+__device__ __forceinline__ void node_row64(gptr<const int32_t> rp, gptr<const int32_t> brp, gptr<const int32_t> bcol,
+                                           gptr<const double> val, gptr<const double> x, int32_t br, int lane,
+                                           double (&acc)[BS]) {
 #pragma unroll
     for (int a = 0; a < BS; ++a) acc[a] = 0.0;
     const int32_t kb = brp[br], nb = brp[br + 1] - kb;
@@ -146,7 +193,7 @@ __device__ __forceinline__ void node_row64(const int32_t* __restrict__ rp, const
 #pragma unroll
             for (int a = 0; a < BS; ++a) acc[a] = val[rp[br * BS + a]] * x[br * BS + a];
     } else {
-        const double* v0 = val + r0;
+        const gptr<const double> v0 = val + r0;
 #pragma unroll 1
         for (int32_t k = lane; k < nb; k += LANES) {
             const int32_t j = bcol[kb + k];
@@ -168,9 +215,8 @@ __device__ __forceinline__ void node_row64(const int32_t* __restrict__ rp, const
 // One block row of the FP32 node-block matrix (values stored component-major:
 // bval[(a*BS+b)*nblk + k]) times x; FP64 accumulation.
 template <int BS, int LANES>
-__device__ __forceinline__ void bsr_row(const int32_t* __restrict__ brp, const int32_t* __restrict__ bcol,
-                                        const float* __restrict__ bval, int32_t nblk, const float* __restrict__ x,
-                                        int32_t br, int lane, double (&acc)[BS]) {
+__device__ __forceinline__ void bsr_row(gptr<const int32_t> brp, gptr<const int32_t> bcol, gptr<const float> bval,
+                                        int32_t nblk, gptr<const float> x, int32_t br, int lane, double (&acc)[BS]) {
 #pragma unroll
     for (int a = 0; a < BS; ++a) acc[a] = 0.0;
     const int32_t kb = brp[br], ke = brp[br + 1];
@@ -191,13 +237,13 @@ __device__ __forceinline__ void bsr_row(const int32_t* __restrict__ brp, const i
         for (int off = LANES / 2; off > 0; off >>= 1) acc[a] += __shfl_xor(acc[a], off, LANES);
 }
 
+// acc[a] for a < BS, as selects: an indexed read would move acc to LDS.
 template <int BS>
-__device__ __forceinline__ double pick(const double (&acc)[BS], int lane) {
-    double v = acc[0];
-#pragma unroll
-    for (int a = 1; a < BS; ++a)
-        if (lane == a) v = acc[a];
-    return v;
+__device__ __forceinline__ double pick(const double (&acc)[BS], int a) {
+    static_assert(BS >= 1 && BS <= 3, "node blocks of 1, 2 or 3 dofs");
+    if constexpr (BS == 1) return acc[0];
+    else if constexpr (BS == 2) return a == 0 ? acc[0] : acc[1];
+    else return a == 0 ? acc[0] : (a == 1 ? acc[1] : acc[2]);
 }
 
 __device__ __forceinline__ bool is_leader(const Tile& t) { return t.row_begin == 0 && threadIdx.x == 0; }
@@ -222,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void spmv_csr64_kernel(int64_t n, const int
     const int lane = threadIdx.x % LANES;
     int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / LANES;
     if (row >= n) return;
-    double acc = csr_row<LANES>(rp, col, val, x, (int32_t)row, lane);
+    double acc = csr_row<LANES>(global(rp), global(col), global(val), global(x), (int32_t)row, lane);
     if (lane == 0) y[row] = acc;
 }
 
@@ -235,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void spmv_bsr32_kernel(int64_t nb, const in
     int64_t br = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / LANES;
     if (br >= nb) return;
     double acc[BS];
-    bsr_row<BS, LANES>(brp, bcol, bval, brp[nb], x, (int32_t)br, lane, acc);
+    bsr_row<BS, LANES>(global(brp), global(bcol), global(bval), brp[nb], global(x), (int32_t)br, lane, acc);
     if constexpr (LANES >= BS) {
         if (lane < BS) y[br * BS + lane] = (float)pick<BS>(acc, lane);
     } else {
@@ -249,8 +295,8 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    const InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != FL_FRESH) return;
     double v[2] = {0.0, 0.0};
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
@@ -273,8 +319,8 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
                                                                 const InstDesc* __restrict__ descs,
                                                                 double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    const InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != (FL_FRESH | FL_WARM)) return;
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
@@ -300,8 +346,8 @@ __global__ __launch_bounds__(kBlock) void ir32_init_kernel(const Tile* __restric
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    const InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != FL_FRESH) return;
     double v[2] = {0.0, 0.0};
     for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
@@ -326,8 +372,8 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
                                                              const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     if (t.row_begin != 0) return;
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
     const int slot[2] = {P_IFF, P_IRZ};
     double v[2];
@@ -361,6 +407,57 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
 }
 
 // ------------------------------------------------------------ FP64 PCG
+// The instance arrays reach the row loops as restrict-qualified parameters:
+// q, r, u, z and p are distinct arrays, and saying so lets the compiler issue
+// every load of a row before its stores and keep stores out of the waits.
+
+// q = K p over node rows [b0, b1) (BS = 1: CSR rows); returns this thread's
+// share of p.q.
+template <int BS, int LANES>
+__device__ __forceinline__ double spmv64_rows(int32_t b0, int32_t b1, gptr<const int32_t> __restrict__ rp,
+                                              gptr<const int32_t> __restrict__ brp,
+                                              gptr<const int32_t> __restrict__ bcol,
+                                              gptr<const double> __restrict__ val, gptr<const double> __restrict__ p,
+                                              gptr<double> __restrict__ q) {
+    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
+    constexpr int G = kBlock / LANES;
+    double dot = 0.0;
+    if constexpr (BS == 1) {
+        for (int32_t row = b0 + grp; row < b1; row += G) {
+            const double pr = p[row];  // dot operand, issued with the row's loads
+            const double acc = csr_row<LANES>(rp, bcol, val, p, row, lane);
+            if (lane == 0) {
+                q[row] = acc;
+                dot += pr * acc;
+            }
+        }
+    } else {
+        for (int32_t br = b0 + grp; br < b1; br += G) {
+            double acc[BS];
+            if constexpr (LANES >= BS) {
+                const int32_t i = br * BS + (lane < BS ? lane : 0);
+                const double pi = p[i];
+                node_row64<BS, LANES>(rp, brp, bcol, val, p, br, lane, acc);
+                if (lane < BS) {
+                    const double qi = pick<BS>(acc, lane);
+                    q[i] = qi;
+                    dot += pi * qi;
+                }
+            } else {
+                node_row64<BS, LANES>(rp, brp, bcol, val, p, br, lane, acc);
+                for (int a = lane; a < BS; a += LANES) {
+                    const int32_t i = br * BS + a;
+                    const double qi = pick<BS>(acc, a);
+                    const double pi = p[i];
+                    q[i] = qi;
+                    dot += pi * qi;
+                }
+            }
+        }
+    }
+    return dot;
+}
+
 // q = K p and the partial p.q. BS = 1: one CSR row per LANES lanes;
 // BS = 2, 3: one node row (BS rows) per LANES lanes, read as node blocks.
 template <int BS, int LANES>
@@ -368,51 +465,51 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    const InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
-    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
-    constexpr int G = kBlock / LANES;
-    double v[1] = {0.0};
-    if (BS == 1) {
-        for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
-            double acc = csr_row<LANES>(D.row_ptr, D.col, D.val, D.p, row, lane);
-            if (lane == 0) {
-                D.q[row] = acc;
-                v[0] += D.p[row] * acc;
-            }
-        }
-    } else {
-        for (int32_t br = t.row_begin / BS + grp; br < t.row_end / BS; br += G) {
-            double acc[BS];
-            node_row64<BS, LANES>(D.row_ptr, D.brow_ptr, D.bcol, D.val, D.p, br, lane, acc);
-            if constexpr (LANES >= BS) {
-                if (lane < BS) {
-                    const int32_t i = br * BS + lane;
-                    const double qi = pick<BS>(acc, lane);
-                    D.q[i] = qi;
-                    v[0] += D.p[i] * qi;
-                }
-            } else {
-                for (int a = lane; a < BS; a += LANES) {
-                    const int32_t i = br * BS + a;
-                    const double qi = pick<BS>(acc, a);
-                    D.q[i] = qi;
-                    v[0] += D.p[i] * qi;
-                }
-            }
-        }
-    }
+    double v[1];
+    if constexpr (BS == 1)
+        v[0] = spmv64_rows<1, LANES>(t.row_begin, t.row_end, D.row_ptr, D.brow_ptr, D.col, D.val, D.p, D.q);
+    else
+        v[0] = spmv64_rows<BS, LANES>(t.row_begin / BS, t.row_end / BS, D.row_ptr, D.brow_ptr, D.bcol, D.val, D.p,
+                                      D.q);
     const int slot[1] = {P_PQ};
     write_partials<1>(v, slot, partials, pstride, blockIdx.x);
+}
+
+// u += alpha p, r -= alpha q, z = D^-1 r over [b, e); this thread's r.r, r.z.
+__device__ __forceinline__ void cg64_update_rows(int32_t b, int32_t e, double alpha, gptr<const double> __restrict__ p,
+                                                 gptr<const double> __restrict__ q,
+                                                 gptr<const double> __restrict__ dinv, gptr<double> __restrict__ u,
+                                                 gptr<double> __restrict__ r, gptr<double> __restrict__ z,
+                                                 double (&v)[2]) {
+    for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
+        const double pi = p[i], qi = q[i], di = dinv[i], ui = u[i], r0 = r[i];
+        u[i] = ui + alpha * pi;
+        const double ri = r0 - alpha * qi;
+        r[i] = ri;
+        const double zi = ri * di;
+        z[i] = zi;
+        v[0] += ri * ri;
+        v[1] += ri * zi;
+    }
+}
+
+__device__ __forceinline__ void direction_rows(int32_t b, int32_t e, double beta, gptr<const double> __restrict__ z,
+                                               gptr<double> __restrict__ p) {
+    for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
+        const double zi = z[i], pi = p[i];
+        p[i] = zi + beta * pi;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
                                                              double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     const int sp[1] = {P_PQ};
     double pq[1];
     reduce_tiles<1>(partials, pstride, t.tile_begin, t.tile_count, sp, pq);
@@ -434,16 +531,7 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
         S->snap_it = S->it;
     }
     double v[2] = {0.0, 0.0};
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
-        double pi = D.p[i];
-        D.u[i] += alpha * pi;
-        double ri = D.r[i] - alpha * D.q[i];
-        D.r[i] = ri;
-        double zi = ri * D.dinv[i];
-        D.z[i] = zi;
-        v[0] += ri * ri;
-        v[1] += ri * zi;
-    }
+    cg64_update_rows(t.row_begin, t.row_end, alpha, D.p, D.q, D.dinv, D.u, D.r, D.z, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -452,8 +540,8 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
                                                               const InstDesc* __restrict__ descs,
                                                               const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     const int sl[2] = {P_RR, P_RZ};
     double v[2];
     reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
@@ -475,53 +563,100 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
         }
     }
     if (conv) return;
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p[i] = D.z[i] + beta * D.p[i];
+    direction_rows(t.row_begin, t.row_end, beta, D.z, D.p);
 }
 
 // ------------------------------------------------------------ FP32-IR
+// q32 = K32 p32 over block rows [b0, b1); this thread's share of p.q.
+template <int BS, int LANES>
+__device__ __forceinline__ double spmv32_rows(int32_t b0, int32_t b1, gptr<const int32_t> __restrict__ brp,
+                                              gptr<const int32_t> __restrict__ bcol,
+                                              gptr<const float> __restrict__ bval, int32_t nblk,
+                                              gptr<const float> __restrict__ p, gptr<float> __restrict__ q) {
+    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
+    constexpr int G = kBlock / LANES;
+    double dot = 0.0;
+    for (int32_t br = b0 + grp; br < b1; br += G) {
+        double acc[BS];
+        if constexpr (LANES >= BS) {
+            const int32_t i = br * BS + (lane < BS ? lane : 0);
+            const float pi = p[i];
+            bsr_row<BS, LANES>(brp, bcol, bval, nblk, p, br, lane, acc);
+            if (lane < BS) {
+                const float qv = (float)pick<BS>(acc, lane);
+                q[i] = qv;
+                dot += (double)pi * (double)qv;
+            }
+        } else {
+            bsr_row<BS, LANES>(brp, bcol, bval, nblk, p, br, lane, acc);
+            for (int c = lane; c < BS; c += LANES) {
+                const int32_t i = br * BS + c;
+                const float qv = (float)pick<BS>(acc, c);
+                const float pi = p[i];
+                q[i] = qv;
+                dot += (double)pi * (double)qv;
+            }
+        }
+    }
+    return dot;
+}
+
 template <int BS, int LANES>
 __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    const InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
-    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
-    constexpr int G = kBlock / LANES;
     const int32_t nblk = D.brow_ptr[D.n / BS];
-    double v[1] = {0.0};
-    for (int32_t br = t.row_begin / BS + grp; br < t.row_end / BS; br += G) {
-        double acc[BS];
-        bsr_row<BS, LANES>(D.brow_ptr, D.bcol, D.bval, nblk, D.p32, br, lane, acc);
-        if constexpr (LANES >= BS) {
-            if (lane < BS) {
-                double a = acc[0];
-#pragma unroll
-                for (int c = 1; c < BS; ++c)
-                    if (lane == c) a = acc[c];
-                float qv = (float)a;
-                D.q32[br * BS + lane] = qv;
-                v[0] += (double)D.p32[br * BS + lane] * (double)qv;
-            }
-        } else {
-            for (int c = lane; c < BS; c += LANES) {
-                float qv = (float)pick<BS>(acc, c);
-                D.q32[br * BS + c] = qv;
-                v[0] += (double)D.p32[br * BS + c] * (double)qv;
-            }
-        }
-    }
+    double v[1];
+    v[0] = spmv32_rows<BS, LANES>(t.row_begin / BS, t.row_end / BS, D.brow_ptr, D.bcol, D.bval, nblk, D.p32, D.q32);
     const int slot[1] = {P_PQ};
     write_partials<1>(v, slot, partials, pstride, blockIdx.x);
+}
+
+__device__ __forceinline__ void ir32_update_rows(int32_t b, int32_t e, float alpha, gptr<const float> __restrict__ p,
+                                                 gptr<const float> __restrict__ q,
+                                                 gptr<const float> __restrict__ dinv, gptr<float> __restrict__ x,
+                                                 gptr<float> __restrict__ r, gptr<float> __restrict__ z,
+                                                 double (&v)[2]) {
+    for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
+        const float pi = p[i], qi = q[i], di = dinv[i], xi = x[i], r0 = r[i];
+        x[i] = xi + alpha * pi;
+        const float ri = r0 - alpha * qi;
+        r[i] = ri;
+        const float zi = ri * di;
+        z[i] = zi;
+        v[0] += (double)ri * (double)ri;
+        v[1] += (double)ri * (double)zi;
+    }
+}
+
+__device__ __forceinline__ void direction32_rows(int32_t b, int32_t e, float beta, gptr<const float> __restrict__ z,
+                                                 gptr<float> __restrict__ p) {
+    for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
+        const float zi = z[i], pi = p[i];
+        p[i] = zi + beta * pi;
+    }
+}
+
+__device__ __forceinline__ void fold_rows(int32_t b, int32_t e, gptr<float> __restrict__ x,
+                                          gptr<double> __restrict__ u) {
+    for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
+        const float xi = x[i];
+        const double ui = u[i];
+        u[i] = ui + (double)xi;
+        x[i] = 0.0f;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
                                                              double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     const int sp[1] = {P_PQ};
     double pq[1];
     reduce_tiles<1>(partials, pstride, t.tile_begin, t.tile_count, sp, pq);
@@ -546,16 +681,7 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
     }
     const float alpha = (float)alpha_d;
     double v[2] = {0.0, 0.0};
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
-        float pi = D.p32[i];
-        D.x32[i] += alpha * pi;
-        float ri = D.r32[i] - alpha * D.q32[i];
-        D.r32[i] = ri;
-        float zi = ri * D.dinv32[i];
-        D.z32[i] = zi;
-        v[0] += (double)ri * (double)ri;
-        v[1] += (double)ri * (double)zi;
-    }
+    ir32_update_rows(t.row_begin, t.row_end, alpha, D.p32, D.q32, D.dinv32, D.x32, D.r32, D.z32, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -568,8 +694,8 @@ __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __rest
                                                               const double* __restrict__ partials, int pstride,
                                                               int period_end) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     const int sl[2] = {P_RR, P_RZ};
     double v[2];
     reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
@@ -590,45 +716,40 @@ __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __rest
         S->flags = S->snap_flags | (refresh ? FL_REFRESH : 0) | (hold ? FL_HOLD : 0);
     }
     if (hold || (period_end && refresh)) return;
-    const float b = (float)beta;
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + b * D.p32[i];
+    direction32_rows(t.row_begin, t.row_end, (float)beta, D.z32, D.p32);
 }
 
 // Refresh step 1: fold the FP32 correction into the FP64 solution.
 __global__ __launch_bounds__(kBlock) void ir32_fold_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
     if (is_leader(t)) {
         S->snap_flags = S->flags;
         S->snap_stall = S->stall;
         S->snap_res_best = S->res_best;
     }
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) {
-        D.u[i] += (double)D.x32[i];
-        D.x32[i] = 0.0f;
-    }
+    fold_rows(t.row_begin, t.row_end, D.x32, D.u);
 }
 
 // Refresh step 2: r = f - K u in FP64 with its evaluation floor; r32, z32.
 template <int LANES>
-__global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __restrict__ tiles,
-                                                               const InstDesc* __restrict__ descs,
-                                                               double* __restrict__ partials, int pstride) {
-    const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    const InstScal* S = t.s;
-    if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
-    if (is_leader(t)) const_cast<InstScal*>(S)->evals += 1;
+__device__ __forceinline__ void residual_rows(int32_t b, int32_t e, gptr<const int32_t> __restrict__ rp,
+                                              gptr<const int32_t> __restrict__ col,
+                                              gptr<const double> __restrict__ val, gptr<const double> __restrict__ u,
+                                              gptr<const double> __restrict__ f,
+                                              gptr<const float> __restrict__ dinv32, gptr<float> __restrict__ r32,
+                                              gptr<float> __restrict__ z32, double (&v)[3]) {
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
-    double v[3] = {0.0, 0.0, 0.0};
-    for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
+    for (int32_t row = b + grp; row < e; row += G) {
+        const double fi = f[row];
+        const float di = dinv32[row];
         double ku = 0.0, aku = 0.0;
-        for (int32_t k = D.row_ptr[row] + lane; k < D.row_ptr[row + 1]; k += LANES) {
-            double w = D.val[k] * D.u[D.col[k]];
+        for (int32_t k = rp[row] + lane; k < rp[row + 1]; k += LANES) {
+            const double w = val[k] * u[col[k]];
             ku += w;
             aku += fabs(w);
         }
@@ -638,18 +759,30 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
             aku += __shfl_xor(aku, off, LANES);
         }
         if (lane == 0) {
-            double fi = D.f[row];
-            double ri = fi - ku;
-            float r32 = (float)ri;
-            float z32 = r32 * D.dinv32[row];
-            D.r32[row] = r32;
-            D.z32[row] = z32;
+            const double ri = fi - ku;
+            const float rs = (float)ri;
+            const float zs = rs * di;
+            r32[row] = rs;
+            z32[row] = zs;
             v[0] += ri * ri;
-            v[1] += (double)r32 * (double)z32;
-            double a = aku + fabs(fi);
+            v[1] += (double)rs * (double)zs;
+            const double a = aku + fabs(fi);
             v[2] += a * a;
         }
     }
+}
+
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __restrict__ tiles,
+                                                               const InstDesc* __restrict__ descs,
+                                                               double* __restrict__ partials, int pstride) {
+    const Tile t = tiles[blockIdx.x];
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
+    if (is_leader(t)) S->evals += 1;
+    double v[3] = {0.0, 0.0, 0.0};
+    residual_rows<LANES>(t.row_begin, t.row_end, D.row_ptr, D.col, D.val, D.u, D.f, D.dinv32, D.r32, D.z32, v);
     const int slot[3] = {P_TRR, P_TRZ, P_TFL};
     write_partials<3>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -660,8 +793,8 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
                                                               const InstDesc* __restrict__ descs,
                                                               const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const InstDesc& D = descs[t.inst];
-    InstScal* S = t.s;
+    const Dev D = view(descs[t.inst]);
+    const gptr<InstScal> S = global(t.s);
     const int sl[3] = {P_TRR, P_TRZ, P_TFL};
     double v[3];
     reduce_tiles<3>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
@@ -700,8 +833,7 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
         }
     }
     if (!cont) return;
-    const float b = (float)beta;
-    for (int32_t i = t.row_begin + threadIdx.x; i < t.row_end; i += kBlock) D.p32[i] = D.z32[i] + b * D.p32[i];
+    direction32_rows(t.row_begin, t.row_end, (float)beta, D.z32, D.p32);
 }
 
 // ------------------------------------------------------------ utilities
