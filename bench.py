@@ -157,10 +157,7 @@ def bench_batch(args):
         for sy in systems:
             b.add(sy)
         t = tuned[prec]  # the autotuned shape of this (2D-only) batch, node-block size 2
-        if prec == Precision.FP64:
-            ctx.set_tuning(lanes=t["lanes"][2], rows_per_tile=t["rows_per_tile"][2])
-        else:
-            ctx.set_tuning(blanes=t["lanes"][2], rows_per_tile=t["rows_per_tile"][2])
+        ctx.set_tuning(unroll=t["unroll"][2], rows_per_tile=t["rows_per_tile"][2])
         try:
             res = b.solve(prec, raise_on_failure=False)
         finally:

@@ -167,14 +167,18 @@ int svx_device_name(svx_ctx* ctx, char* buf, size_t len);
  * take no svx_error). No reference counterpart. */
 const char* svx_last_error(void);
 /* Launch-shape and refinement knobs of the Krylov engine for this context
- * (0 / <= 0 keeps the automatic choice): CSR lanes per row, BSR lanes per
- * block row, rows per workgroup tile, FP32-IR refresh period, iterations per
- * captured graph, graph capture on/off, and the FP32-IR refresh threshold
- * (replace the residual when ||r|| < ru_delta * max ||r|| since the last
- * refresh). Lane counts must be 0 or a power of two from 2 to 64, and
+ * (0 / <= 0 keeps the automatic choice): lanes per CSR row (raw CSR systems,
+ * and the CSR passes of node-block systems: FP64 residual, warm start),
+ * blocks in flight per lane in the SpMV of node-block systems (their matrix
+ * is multiplied in a node-row sliced ELL layout, 64 node rows per
+ * wavefront), rows per workgroup tile (node-block systems: rounded up to
+ * whole 64-node-row chunks), FP32-IR refresh period, iterations per captured
+ * graph, graph capture on/off, and the FP32-IR refresh threshold (replace the
+ * residual when ||r|| < ru_delta * max ||r|| since the last refresh). Lane
+ * counts must be 0 or a power of two from 2 to 64, unroll 0..4, and
  * 0 <= ru_delta < 1; anything else returns SVX_E_ARGUMENT. No reference
  * counterpart: the autotuner drives it. */
-int svx_set_tuning(svx_ctx* ctx, int lanes, int blanes, int rows_per_tile,
+int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile,
                    int period, int chunk, int use_graph, double ru_delta);
 
 /* ---------------------------------------------------------------------------
@@ -293,15 +297,16 @@ int svx_batch_result(const svx_batch* batch, int64_t slot,
                      int64_t* iterations, double* residual,
                      int32_t* converged);
 /* SpMV autotuner: times the solver's SpMV phase over the whole batch for
- * lanes per row (block row) in {2, 4, 8, 16, 32} and rows per tile in
- * {64 .. 512}, one block size at a time, and keeps the fastest shape for
- * later svx_batch_solve calls with this precision (until svx_batch_add).
- * lanes[bs] / rows_per_tile[bs] (arrays of 4, indexed by block size 1..3,
- * or NULL) receive the choice, spmv_ms its time. A tuned shape changes the
- * summation order, so results then differ from solo solves at rounding level
- * instead of bit for bit. */
+ * blocks in flight per lane in {1, 2, 3, 4} and 4, 8 or 16 ELL chunks
+ * (64 node rows each) per workgroup tile, one node-block size at a time, and
+ * keeps the fastest shape for later svx_batch_solve calls with this precision
+ * (until svx_batch_add). unroll[bs] / rows_per_tile[bs] (arrays of 4,
+ * indexed by block size 1..3, or NULL) receive the choice, spmv_ms its time.
+ * The unroll does not change results; a tile size other than the default
+ * changes the order of the partial sums of the dot products, so results then
+ * differ from solo solves at rounding level instead of bit for bit. */
 int svx_batch_autotune(svx_batch* batch, int precision, int reps,
-                       int32_t* lanes /* [4] */, int32_t* rows_per_tile /* [4] */,
+                       int32_t* unroll /* [4] */, int32_t* rows_per_tile /* [4] */,
                        double* spmv_ms);
 /* Average duration and algorithmic bytes of one SpMV phase over the whole
  * batch (all instances), timed like svx_system_time_spmv. */

@@ -59,8 +59,8 @@ class Context:
         lib().svx_device_name(self._h, buf, 256)
         return buf.value.decode()
 
-    def set_tuning(self, lanes=0, blanes=0, rows_per_tile=0, period=0, chunk=0, use_graph=True, ru_delta=0.0):
-        _raise(lib().svx_set_tuning(self._h, lanes, blanes, rows_per_tile, period, chunk, int(use_graph), ru_delta))
+    def set_tuning(self, lanes=0, unroll=0, rows_per_tile=0, period=0, chunk=0, use_graph=True, ru_delta=0.0):
+        _raise(lib().svx_set_tuning(self._h, lanes, unroll, rows_per_tile, period, chunk, int(use_graph), ru_delta))
 
     def close(self):
         if self._h:
@@ -388,12 +388,12 @@ class Batch:
     def autotune(self, precision: int = Precision.FP64, reps: int = 5) -> dict:
         """Pick the fastest SpMV launch shape for this batch (svx_batch_autotune);
         later solves with this precision use it."""
-        lanes = np.zeros(4, np.int32)
+        unroll = np.zeros(4, np.int32)
         rows = np.zeros(4, np.int32)
         ms = C.c_double()
-        _raise(lib().svx_batch_autotune(self._h, precision, reps, lanes.ctypes.data_as(_i32p),
+        _raise(lib().svx_batch_autotune(self._h, precision, reps, unroll.ctypes.data_as(_i32p),
                                         rows.ctypes.data_as(_i32p), C.byref(ms)))
-        return {"lanes": lanes.tolist(), "rows_per_tile": rows.tolist(), "spmv_ms": ms.value}
+        return {"unroll": unroll.tolist(), "rows_per_tile": rows.tolist(), "spmv_ms": ms.value}
 
     def close(self):
         if self._h:

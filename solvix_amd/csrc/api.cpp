@@ -129,6 +129,14 @@ void mesh_upload(svx_mesh* m, const double* coords) {
     m->brow_ptr.upload(m->hp.brow_ptr.data(), n_nodes + 1, s);
     m->bcol.alloc(m->nblk);
     m->bcol.upload(m->hp.bcol.data(), m->nblk, s);
+    m->n_chunks = (int64_t)m->hp.ell_nb.size();
+    m->ell_blocks = m->hp.ell_off.empty() ? 0 : m->hp.ell_off.back();
+    m->ell_off.alloc(m->n_chunks + 1);
+    m->ell_off.upload(m->hp.ell_off.data(), m->n_chunks + 1, s);
+    m->ell_nb.alloc(std::max<int64_t>(1, m->n_chunks));
+    m->ell_nb.upload(m->hp.ell_nb.data(), m->n_chunks, s);
+    m->ell_col.alloc(std::max<int64_t>(1, m->ell_blocks));
+    m->ell_col.upload(m->hp.ell_col.data(), m->ell_blocks, s);
     m->inc_ptr.alloc(n_nodes + 1);
     m->inc_ptr.upload(m->hp.inc_ptr.data(), n_nodes + 1, s);
     m->inc_elem.alloc(m->hp.inc_elem.size());
@@ -235,6 +243,11 @@ void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {
     launch_dirichlet(m->n, m->row_ptr.p, m->col.p, dmask.p, sys->vals.p, sys->f.p, s);
     for (int64_t r : sys->cons_rows) sys->h_f[r] = 0.0;
     sys->dirichlet = true;
+    // K changed: the preconditioner and the ELL copies are rebuilt on the next solve
+    sys->dinv.release();
+    sys->dinv32.release();
+    sys->ell64.release();
+    sys->ell32.release();
     SVX_HIP(hipStreamSynchronize(s));
 }
 
@@ -263,10 +276,17 @@ void system_prepare(svx_system* sys, int precision) {
     if (!sys->r.p) {  // FP64 vectors: the FP64 arm, and the FP32-IR fallback
         sys->r.alloc(n), sys->p.alloc(n), sys->z.alloc(n), sys->q.alloc(n);
     }
+    const size_t ell_vals = (size_t)std::max<int64_t>(1, m->ell_blocks) * m->ndof * m->ndof;
+    if (!sys->ell64.p) {  // FP64 PCG, and the FP64 fallback of FP32-IR
+        sys->ell64.alloc(ell_vals);
+        launch_ell_pack(m->ndof, m->n_nodes, m->ell_off.p, m->ell_nb.p, m->row_ptr.p, m->brow_ptr.p, sys->vals.p,
+                        sys->ell64.p, nullptr, s);
+    }
     if (precision != SVX_FP64) {
-        if (!sys->bval.p) {
-            sys->bval.alloc((size_t)m->nblk * m->ndof * m->ndof);
-            launch_fill_bsr(n, m->ndof, m->row_ptr.p, m->col.p, sys->vals.p, m->brow_ptr.p, sys->bval.p, s);
+        if (!sys->ell32.p) {
+            sys->ell32.alloc(ell_vals);
+            launch_ell_pack(m->ndof, m->n_nodes, m->ell_off.p, m->ell_nb.p, m->row_ptr.p, m->brow_ptr.p, sys->vals.p,
+                            nullptr, sys->ell32.p, s);
         }
         if (!sys->x32.p) {
             sys->x32.alloc(n), sys->r32.alloc(n), sys->p32.alloc(n), sys->z32.alloc(n), sys->q32.alloc(n);
@@ -285,7 +305,11 @@ InstDesc system_desc(svx_system* sys) {
     d.val = sys->vals.p;
     d.brow_ptr = m->brow_ptr.p;
     d.bcol = m->bcol.p;
-    d.bval = sys->bval.p;
+    d.ell_off = m->ell_off.p;
+    d.ell_nb = m->ell_nb.p;
+    d.ell_col = m->ell_col.p;
+    d.ell_val = sys->ell64.p;
+    d.ell_val32 = sys->ell32.p;
     d.dinv = sys->dinv.p;
     d.dinv32 = sys->dinv32.p;
     d.f = sys->f.p;
@@ -304,15 +328,12 @@ InstDesc system_desc(svx_system* sys) {
 }
 
 // Algorithmic HBM bytes of one solver SpMV launch on this mesh's matrix: the
-// matrix stream, its index arrays, one read of x and one write of y.
+// matrix values (nnz, without the ELL padding), one block column per block,
+// the chunk table, one read of x and one write of y.
 double spmv_bytes(const svx_mesh* m, int precision) {
-    const double n = (double)m->n, nnz = (double)m->nnz, nodes = (double)m->n_nodes, nblk = (double)m->nblk;
-    if (precision == SVX_FP64) {
-        if (m->ndof > 1)  // FP64 values + block columns + block row_ptr + node row starts + x + y
-            return 8.0 * nnz + 4.0 * nblk + 4.0 * (nodes + 1) + 8.0 * nodes + 16.0 * n;
-        return 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;  // values + columns + row_ptr + x + y
-    }
-    return 4.0 * nnz + 4.0 * nblk + 4.0 * (nodes + 1) + 8.0 * n;  // FP32 blocks + block columns + row_ptr + x + y
+    const double n = (double)m->n, nnz = (double)m->nnz, nblk = (double)m->nblk, ch = (double)m->n_chunks;
+    if (precision == SVX_FP64) return 8.0 * nnz + 4.0 * nblk + 8.0 * ch + 16.0 * n;
+    return 4.0 * nnz + 4.0 * nblk + 8.0 * ch + 8.0 * n;
 }
 
 InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len) {
@@ -344,8 +365,7 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     EngineShape shape;
     const int bs = m->ndof;
     shape.rows_per_tile[bs] = tp.rows_per_tile > 0 ? tp.rows_per_tile : default_rows_per_tile(bs, precision);
-    shape.lanes[bs] = tp.lanes > 0 ? tp.lanes : lanes_for_bs(bs);
-    shape.blanes[bs] = tp.blanes > 0 ? tp.blanes : blanes_for_bs(bs);
+    shape.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;
     EngineOptions opt;
@@ -713,7 +733,6 @@ int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int
         EngineShape shape;
         shape.rows_per_tile[1] = tp.rows_per_tile > 0 ? tp.rows_per_tile : 256;
         shape.lanes[1] = tp.lanes > 0 ? tp.lanes : default_lanes(avg);
-        shape.blanes[1] = tp.blanes > 0 ? tp.blanes : default_lanes(avg);
         eng.set_instances(descs, {1}, shape);
         EngineOptions opt;
         opt.precision = precision;
@@ -776,15 +795,15 @@ int svx_system_time_phases(svx_system* sys, int precision, int reps, double* avg
 }
 
 // --------------------------------------------------------------- tuning knobs
-int svx_set_tuning(svx_ctx* ctx, int lanes, int blanes, int rows_per_tile, int period, int chunk, int use_graph,
+int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile, int period, int chunk, int use_graph,
                    double ru_delta) {
     auto lanes_ok = [](int l) { return l == 0 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64; };
-    if (!ctx || !lanes_ok(lanes) || !lanes_ok(blanes) || rows_per_tile < 0 || period < 0 || chunk < 0 ||
-        ru_delta < 0.0 || ru_delta >= 1.0)
+    if (!ctx || !lanes_ok(lanes) || unroll < 0 || unroll > kMaxUnroll || rows_per_tile < 0 || period < 0 ||
+        chunk < 0 || ru_delta < 0.0 || ru_delta >= 1.0)
         return SVX_E_ARGUMENT;
     TuneParams& t = ctx->tune;
     t.lanes = lanes;
-    t.blanes = blanes;
+    t.unroll = unroll;
     t.rows_per_tile = rows_per_tile;
     if (period > 0) t.period = period;
     if (chunk > 0) t.chunk = chunk;

@@ -12,8 +12,8 @@ namespace svx {
 
 // Launch-shape and refinement knobs (0 = automatic choice).
 struct TuneParams {
-    int lanes = 0;          // CSR lanes per row (FP64 SpMV)
-    int blanes = 0;         // BSR lanes per block row (FP32 SpMV)
+    int lanes = 0;          // lanes per row, raw CSR systems
+    int unroll = 0;         // blocks in flight per lane, node-block (ELL) systems
     int rows_per_tile = 0;  // rows per workgroup tile
     int period = 8;         // FP32-IR refresh period (iterations)
     int chunk = 32;         // iterations per captured graph
@@ -45,6 +45,8 @@ struct svx_mesh {
     svx::DevBuf<int32_t> conn;
     svx::DevBuf<uint8_t> kinds;
     svx::DevBuf<int32_t> row_ptr, col, brow_ptr, bcol;
+    svx::DevBuf<int32_t> ell_off, ell_nb, ell_col;  // node-row sliced ELL pattern
+    int64_t n_chunks = 0, ell_blocks = 0;
     svx::DevBuf<int64_t> inc_ptr;
     svx::DevBuf<int32_t> inc_elem;
     svx::DevBuf<int8_t> inc_slot;
@@ -66,7 +68,9 @@ struct svx_system {
     svx::DevBuf<int64_t> d_cons_rows, d_cons_off;
     svx::DevBuf<double> d_cons_vals;  // original values of the constrained rows
     svx::DevBuf<double> dinv;
-    svx::DevBuf<float> dinv32, bval;
+    svx::DevBuf<float> dinv32;
+    svx::DevBuf<double> ell64;  // K in the ELL layout (FP64 PCG, FP32-IR fallback)
+    svx::DevBuf<float> ell32;   // K in the ELL layout, FP32 (FP32-IR)
     svx::DevBuf<double> u, r, p, z, q;
     svx::DevBuf<float> x32, r32, p32, z32, q32;
     svx::DevBuf<svx::InstScal> scal;

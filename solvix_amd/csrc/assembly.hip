@@ -445,6 +445,40 @@ __global__ void fill_bsr_kernel(int64_t n, int bs, const int32_t* __restrict__ r
     }
 }
 
+// Node-row sliced ELL copy of a node-block CSR matrix: thread (chunk c, lane
+// l) packs node row 64c + l. Value (block k, entry a*bs+b) of that row goes
+// to out[(ell_off[c] + k*64) * bs*bs + (a*bs + b)*64 + l], i.e. per chunk
+// [block][entry][lane], so that a wavefront reading entry (k, a, b) of its 64
+// rows reads 64 consecutive words. Padding blocks and lanes past the last
+// node get zeros; a lone node (its rows hold only their diagonal) becomes one
+// diagonal block.
+template <int BS, typename V>
+__global__ void ell_pack_kernel(int64_t n_nodes, int64_t n_chunks, const int32_t* __restrict__ ell_off,
+                                const int32_t* __restrict__ ell_nb, const int32_t* __restrict__ row_ptr,
+                                const int32_t* __restrict__ brow_ptr, const double* __restrict__ vals,
+                                V* __restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = g / 64;
+    const int l = (int)(g % 64);
+    if (c >= n_chunks) return;
+    const int64_t r = c * 64 + l;
+    int32_t nb = 0, r0 = 0;
+    bool lone = false;
+    if (r < n_nodes) {
+        nb = brow_ptr[r + 1] - brow_ptr[r];
+        r0 = row_ptr[r * BS];
+        lone = row_ptr[r * BS + 1] - r0 != BS * nb;
+    }
+    V* o = out + (int64_t)ell_off[c] * (BS * BS) + l;
+    for (int32_t k = 0; k < ell_nb[c]; ++k)
+        for (int a = 0; a < BS; ++a)
+            for (int b = 0; b < BS; ++b) {
+                double v = 0.0;
+                if (k < nb) v = lone ? (a == b ? vals[row_ptr[r * BS + a]] : 0.0) : vals[r0 + a * BS * nb + BS * k + b];
+                o[((int64_t)k * BS * BS + a * BS + b) * 64] = (V)v;
+            }
+}
+
 // (K_orig u)[row] with the row's original values (reactions, stress.rs:103-118).
 __global__ void reaction_rows_kernel(int64_t n_sel, const int64_t* __restrict__ rows,
                                      const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
@@ -518,6 +552,23 @@ void launch_fill_bsr(int64_t n_rows, int bs, const int32_t* row_ptr, const int32
                      const int32_t* brow_ptr, float* bval, hipStream_t s) {
     if (n_rows <= 0) return;
     fill_bsr_kernel<<<grid_for(n_rows, kBlock), kBlock, 0, s>>>(n_rows, bs, row_ptr, col, vals, brow_ptr, bval);
+    SVX_HIP(hipGetLastError());
+}
+
+void launch_ell_pack(int bs, int64_t n_nodes, const int32_t* ell_off, const int32_t* ell_nb, const int32_t* row_ptr,
+                     const int32_t* brow_ptr, const double* vals, double* out64, float* out32, hipStream_t s) {
+    if (n_nodes <= 0) return;
+    const int64_t n_chunks = (n_nodes + 63) / 64;
+    const unsigned grid = grid_for(n_chunks * 64, kBlock);
+    if (bs == 2) {
+        if (out64) ell_pack_kernel<2, double><<<grid, kBlock, 0, s>>>(n_nodes, n_chunks, ell_off, ell_nb, row_ptr, brow_ptr, vals, out64);
+        if (out32) ell_pack_kernel<2, float><<<grid, kBlock, 0, s>>>(n_nodes, n_chunks, ell_off, ell_nb, row_ptr, brow_ptr, vals, out32);
+    } else if (bs == 3) {
+        if (out64) ell_pack_kernel<3, double><<<grid, kBlock, 0, s>>>(n_nodes, n_chunks, ell_off, ell_nb, row_ptr, brow_ptr, vals, out64);
+        if (out32) ell_pack_kernel<3, float><<<grid, kBlock, 0, s>>>(n_nodes, n_chunks, ell_off, ell_nb, row_ptr, brow_ptr, vals, out32);
+    } else {
+        throw std::invalid_argument("ELL layout needs node blocks of 2 or 3 dofs");
+    }
     SVX_HIP(hipGetLastError());
 }
 

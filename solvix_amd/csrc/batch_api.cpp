@@ -36,8 +36,8 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
     EngineShape sh;
     for (int bs = 1; bs <= 3; ++bs) {
         sh.rows_per_tile[bs] = tp.rows_per_tile > 0 ? tp.rows_per_tile : default_rows_per_tile(bs, precision);
-        sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : lanes_for_bs(bs);
-        sh.blanes[bs] = tp.blanes > 0 ? tp.blanes : blanes_for_bs(bs);
+        sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : 16;  // CSR rows of the FP64 residual and warm start
+        sh.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
     }
     return sh;
 }
@@ -116,7 +116,7 @@ std::vector<InstDesc> batch_setup(svx_batch* b, int precision) {
 
 extern "C" {
 
-int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* lanes, int32_t* rows_per_tile,
+int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* unroll, int32_t* rows_per_tile,
                        double* spmv_ms) {
     if (!b || reps <= 0 || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
     return guarded_b(nullptr, [&]() -> int {
@@ -129,18 +129,16 @@ int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* lanes, in
         EngineShape best = batch_shape(b->sys, b->ctx->tune, precision);
         b->engine->set_instances(descs, all, best);
         double best_ms = b->engine->time_spmv(precision, reps);
-        static const int kLanes[] = {2, 4, 8, 16, 32};
-        static const int kRows[] = {64, 128, 192, 256, 384, 512};
-        for (int bs = 1; bs <= 3; ++bs) {
+        // blocks in flight per lane x ELL chunks per tile (one per wavefront,
+        // or two / four in turn per wavefront)
+        static const int kChunksPerTile[] = {kBlock / 64, 2 * kBlock / 64, 4 * kBlock / 64};
+        for (int bs = 2; bs <= 3; ++bs) {
             if (!present[bs]) continue;
-            for (int l : kLanes)
-                for (int r : kRows) {
+            for (int u = 1; u <= kMaxUnroll; ++u)
+                for (int c : kChunksPerTile) {
                     EngineShape sh = best;
-                    if (precision == SVX_FP64)
-                        sh.lanes[bs] = l;
-                    else
-                        sh.blanes[bs] = l;
-                    sh.rows_per_tile[bs] = r;
+                    sh.unroll[bs] = u;
+                    sh.rows_per_tile[bs] = c * kEllRows * bs;
                     b->engine->set_instances(descs, all, sh);
                     const double ms = b->engine->time_spmv(precision, reps);
                     if (ms < best_ms) best_ms = ms, best = sh;
@@ -149,7 +147,7 @@ int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* lanes, in
         b->tuned_shape = best;
         b->tuned_precision = precision;
         for (int bs = 0; bs < 4; ++bs) {
-            if (lanes) lanes[bs] = precision == SVX_FP64 ? best.lanes[bs] : best.blanes[bs];
+            if (unroll) unroll[bs] = best.unroll[bs];
             if (rows_per_tile) rows_per_tile[bs] = best.rows_per_tile[bs];
         }
         if (spmv_ms) *spmv_ms = best_ms;

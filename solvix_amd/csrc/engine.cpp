@@ -10,16 +10,16 @@ int default_lanes(double avg) {
     return l;
 }
 
-// Launch shapes measured on the 265,923-DOF C3D8 block (FP64: 16 lanes per
-// node row, 288 rows per tile; FP32: 16 lanes per 3x3 block row, 288 rows
-// per tile) and on 2e5-DOF plane meshes (4 lanes per node row, 256 rows).
+// Node-block instances: one 64-node-row ELL chunk per wavefront, four per
+// workgroup tile. Blocks in flight per lane, measured on the 265,923-DOF C3D8
+// block and on 2e5 / 4e6-DOF plane meshes: one for the FP64 3x3 blocks (nine
+// FP64 loads per block already), three otherwise.
 int default_rows_per_tile(int bs, int precision) {
     (void)precision;
-    return bs == 3 ? 288 : 256;
+    return bs >= 2 ? (kBlock / 64) * kEllRows * bs : 256;
 }
 
-int lanes_for_bs(int bs) { return bs == 3 ? 16 : 4; }
-int blanes_for_bs(int bs) { return bs == 3 ? 16 : 4; }
+int unroll_for(int bs, int precision) { return (bs == 3 && precision == SVX_FP64) ? 1 : 3; }
 
 Engine::~Engine() { release_graph(); }
 
@@ -57,7 +57,10 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     for (int bs = 1; bs <= 3; ++bs) {
         first[bs] = (int)tiles.size();
         int rpt = shape_.rows_per_tile[bs] > 0 ? shape_.rows_per_tile[bs] : default_rows_per_tile(bs, SVX_FP64);
-        rpt = std::max(bs, (rpt / bs) * bs);
+        if (bs >= 2) {  // whole ELL chunks
+            const int cr = kEllRows * bs;
+            rpt = std::max(cr, (rpt + cr - 1) / cr * cr);
+        }
         for (size_t i = 0; i < h_descs_.size(); ++i) {
             InstDesc& d = h_descs_[i];
             if (d.bs != bs || !running[i]) continue;
@@ -107,7 +110,7 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         k.n_inst = (int32_t)h_descs_.size();
         k.partials = partials_.p + kPartialSlots * (size_t)first[bs];
         k.lanes = shape_.lanes[bs] > 0 ? shape_.lanes[bs] : 16;
-        k.blanes = shape_.blanes[bs] > 0 ? shape_.blanes[bs] : 8;
+        k.unroll = shape_.unroll[bs] > 0 ? shape_.unroll[bs] : 2;
         k.bs = bs;
     }
     SVX_HIP(hipStreamSynchronize(stream_));

@@ -22,9 +22,9 @@ struct EngineOptions {
 };
 
 struct EngineShape {
-    int lanes[4] = {0, 0, 0, 0};   // CSR lanes per row, per block size
-    int blanes[4] = {0, 0, 0, 0};  // BSR lanes per block row, per block size
-    int rows_per_tile[4] = {0, 0, 0, 0};
+    int lanes[4] = {0, 0, 0, 0};   // lanes per row (raw CSR, block size 1)
+    int unroll[4] = {0, 0, 0, 0};  // blocks in flight per lane (ELL, block sizes 2, 3)
+    int rows_per_tile[4] = {0, 0, 0, 0};  // block sizes 2, 3: rounded to whole 64-node-row chunks
 };
 
 class Engine {
@@ -84,11 +84,10 @@ class Engine {
 };
 
 int default_lanes(double avg_row_len);
+// Default tile and ELL unroll for node-block size bs (2, 3) and a precision:
+// functions of (bs, precision) alone, so an instance gets the same launch
+// shape, and the same bits, alone and inside any batch.
 int default_rows_per_tile(int bs, int precision);
-// Lanes per row (CSR) and per block row (BSR) for node-block size bs (2, 3);
-// a function of bs alone, so an instance gets the same launch shape — and the
-// same bits — alone and inside any batch.
-int lanes_for_bs(int bs);
-int blanes_for_bs(int bs);
+int unroll_for(int bs, int precision);
 
 }  // namespace svx

@@ -63,8 +63,9 @@ __device__ __forceinline__ gptr<T> global(T* p) {
 struct Dev {
     int32_t n;
     gptr<const int32_t> row_ptr, col, brow_ptr, bcol;
-    gptr<const double> val, dinv, f;
-    gptr<const float> bval, dinv32;
+    gptr<const int32_t> ell_off, ell_nb, ell_col;
+    gptr<const double> val, ell_val, dinv, f;
+    gptr<const float> bval, ell_val32, dinv32;
     gptr<double> u, r, p, z, q;
     gptr<float> x32, r32, p32, z32, q32;
 };
@@ -76,6 +77,11 @@ __device__ __forceinline__ Dev view(const InstDesc& d) {
     v.col = global(d.col);
     v.brow_ptr = global(d.brow_ptr);
     v.bcol = global(d.bcol);
+    v.ell_off = global(d.ell_off);
+    v.ell_nb = global(d.ell_nb);
+    v.ell_col = global(d.ell_col);
+    v.ell_val = global(d.ell_val);
+    v.ell_val32 = global(d.ell_val32);
     v.val = global(d.val);
     v.dinv = global(d.dinv);
     v.f = global(d.f);
@@ -173,77 +179,114 @@ __device__ __forceinline__ double csr_row(gptr<const int32_t> rp, gptr<const int
     return acc;
 }
 
-// One node row (BS consecutive rows of one node) of the FP64 CSR matrix,
-// read as node blocks: the rows of a node share their column structure (every
-// neighbour node contributes BS consecutive columns to each row), so the
-// block column list bcol replaces the per-entry column indices and row a of
-// the node holds block k's entries at val[rp[br*BS] + a*BS*nb + BS*k + b].
-// A node whose rows hold only their diagonal (no element touches it) is
-// multiplied entry by entry.
-template <int BS, int LANES>
-__device__ __forceinline__ void node_row64(gptr<const int32_t> rp, gptr<const int32_t> brp, gptr<const int32_t> bcol,
-                                           gptr<const double> val, gptr<const double> x, int32_t br, int lane,
-                                           double (&acc)[BS]) {
+// One row of the FP32 copy of a raw CSR matrix times x; FP64 accumulation.
+template <int LANES>
+__device__ __forceinline__ double csr32_row(gptr<const int32_t> rp, gptr<const int32_t> col, gptr<const float> val,
+                                            gptr<const float> x, int32_t row, int lane) {
+    double acc = 0.0;
+    const int32_t kb = rp[row], ke = rp[row + 1];
+#pragma unroll 4
+    for (int32_t k = kb + lane; k < ke; k += LANES) acc += (double)val[k] * (double)x[col[k]];
 #pragma unroll
-    for (int a = 0; a < BS; ++a) acc[a] = 0.0;
-    const int32_t kb = brp[br], nb = brp[br + 1] - kb;
-    const int32_t r0 = rp[br * BS];
-    if (rp[br * BS + BS] - r0 != BS * BS * nb) {  // isolated node: diagonal entries only
-        if (lane == 0)
+    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    return acc;
+}
+
+// One node row of the node-row sliced ELL layout times x: cl / vl point at
+// the row's first block column / value (chunk offset plus lane), nb is the
+// chunk's padded block count. The loads of UNR blocks are issued together;
+// the sums run in block order whatever UNR, so UNR changes timing only.
+// ABS: also accumulate |K_ij x_j| (the FP64 evaluation floor of f - K x).
+template <int BS, int UNR, typename V, bool ABS>
+__device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr<const V> vl, gptr<const V> x,
+                                        double (&acc)[BS], double (&aacc)[BS]) {
+    constexpr int BB = BS * BS;
 #pragma unroll
-            for (int a = 0; a < BS; ++a) acc[a] = val[rp[br * BS + a]] * x[br * BS + a];
-    } else {
-        const gptr<const double> v0 = val + r0;
-#pragma unroll 1
-        for (int32_t k = lane; k < nb; k += LANES) {
-            const int32_t j = bcol[kb + k];
+    for (int a = 0; a < BS; ++a) acc[a] = 0.0, aacc[a] = 0.0;
+    int32_t k = 0;
+    for (; k + UNR <= nb; k += UNR) {
+        int32_t j[UNR];
+        V w[UNR][BB];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            j[u] = cl[(k + u) * kEllRows];
+#pragma unroll
+            for (int m = 0; m < BB; ++m) w[u][m] = vl[((k + u) * BB + m) * kEllRows];
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
             double xj[BS];
 #pragma unroll
-            for (int b = 0; b < BS; ++b) xj[b] = x[j * BS + b];
+            for (int b = 0; b < BS; ++b) xj[b] = (double)x[j[u] * BS + b];
 #pragma unroll
             for (int a = 0; a < BS; ++a)
 #pragma unroll
-                for (int b = 0; b < BS; ++b) acc[a] += v0[a * BS * nb + BS * k + b] * xj[b];
+                for (int b = 0; b < BS; ++b) {
+                    const double t = (double)w[u][a * BS + b] * xj[b];
+                    acc[a] += t;
+                    if constexpr (ABS) aacc[a] += fabs(t);
+                }
         }
     }
+    for (; k < nb; ++k) {
+        const int32_t jj = cl[k * kEllRows];
+        V w[BB];
 #pragma unroll
-    for (int a = 0; a < BS; ++a)
-#pragma unroll
-        for (int off = LANES / 2; off > 0; off >>= 1) acc[a] += __shfl_xor(acc[a], off, LANES);
-}
-
-// One block row of the FP32 node-block matrix (values stored component-major:
-// bval[(a*BS+b)*nblk + k]) times x; FP64 accumulation.
-template <int BS, int LANES>
-__device__ __forceinline__ void bsr_row(gptr<const int32_t> brp, gptr<const int32_t> bcol, gptr<const float> bval,
-                                        int32_t nblk, gptr<const float> x, int32_t br, int lane, double (&acc)[BS]) {
-#pragma unroll
-    for (int a = 0; a < BS; ++a) acc[a] = 0.0;
-    const int32_t kb = brp[br], ke = brp[br + 1];
-#pragma unroll 1
-    for (int32_t k = kb + lane; k < ke; k += LANES) {
-        int32_t j = bcol[k];
+        for (int m = 0; m < BB; ++m) w[m] = vl[(k * BB + m) * kEllRows];
         double xj[BS];
 #pragma unroll
-        for (int b = 0; b < BS; ++b) xj[b] = (double)x[j * BS + b];
+        for (int b = 0; b < BS; ++b) xj[b] = (double)x[jj * BS + b];
 #pragma unroll
         for (int a = 0; a < BS; ++a)
 #pragma unroll
-            for (int b = 0; b < BS; ++b) acc[a] += (double)bval[(int64_t)(a * BS + b) * nblk + k] * xj[b];
+            for (int b = 0; b < BS; ++b) {
+                const double t = (double)w[a * BS + b] * xj[b];
+                acc[a] += t;
+                if constexpr (ABS) aacc[a] += fabs(t);
+            }
     }
-#pragma unroll
-    for (int a = 0; a < BS; ++a)
-#pragma unroll
-        for (int off = LANES / 2; off > 0; off >>= 1) acc[a] += __shfl_xor(acc[a], off, LANES);
 }
 
-// acc[a] for a < BS, as selects: an indexed read would move acc to LDS.
+// q = K p over the node-row sliced ELL chunks [c0, c1) of an instance with
+// n_nodes node rows. Wavefront w of the workgroup takes chunks c0 + w,
+// c0 + w + kWaves, ...; lane l owns node row 64c + l, accumulating in FP64.
+// Every load of a wavefront covers 64 consecutive words (values, columns) or
+// the rows' own x entries; the column gathers of x hit the caches. Returns
+// this thread's share of p.q.
+template <int BS, int UNR, typename V>
+__device__ __forceinline__ double ell_rows(int32_t c0, int32_t c1, int32_t n_nodes,
+                                           gptr<const int32_t> __restrict__ off, gptr<const int32_t> __restrict__ nbs,
+                                           gptr<const int32_t> __restrict__ col, gptr<const V> __restrict__ val,
+                                           gptr<const V> __restrict__ p, gptr<V> __restrict__ q) {
+    const int lane = threadIdx.x % kWave;
+    double dot = 0.0;
+    for (int32_t c = c0 + (int32_t)(threadIdx.x / kWave); c < c1; c += kWaves) {
+        const int64_t o = off[c];
+        const int32_t r = c * kEllRows + lane;
+        const bool own = r < n_nodes;
+        V pr[BS];
+#pragma unroll
+        for (int a = 0; a < BS; ++a) pr[a] = own ? p[r * BS + a] : V(0);
+        double acc[BS], unused[BS];
+        ell_row<BS, UNR, V, false>(nbs[c], col + o + lane, val + o * (BS * BS) + lane, p, acc, unused);
+        if (own) {
+#pragma unroll
+            for (int a = 0; a < BS; ++a) {
+                const V qa = (V)acc[a];
+                q[r * BS + a] = qa;
+                dot += (double)pr[a] * (double)qa;
+            }
+        }
+    }
+    return dot;
+}
+
+// The ELL chunks of a tile (tiles of node-block instances are whole chunks).
 template <int BS>
-__device__ __forceinline__ double pick(const double (&acc)[BS], int a) {
-    static_assert(BS >= 1 && BS <= 3, "node blocks of 1, 2 or 3 dofs");
-    if constexpr (BS == 1) return acc[0];
-    else if constexpr (BS == 2) return a == 0 ? acc[0] : acc[1];
-    else return a == 0 ? acc[0] : (a == 1 ? acc[1] : acc[2]);
+__device__ __forceinline__ void tile_chunks(const Tile& t, int32_t& c0, int32_t& c1) {
+    constexpr int32_t cr = kEllRows * BS;
+    c0 = t.row_begin / cr;
+    c1 = (t.row_end + cr - 1) / cr;
 }
 
 __device__ __forceinline__ bool is_leader(const Tile& t) { return t.row_begin == 0 && threadIdx.x == 0; }
@@ -272,23 +315,6 @@ __global__ __launch_bounds__(kBlock) void spmv_csr64_kernel(int64_t n, const int
     if (lane == 0) y[row] = acc;
 }
 
-template <int BS, int LANES>
-__global__ __launch_bounds__(kBlock) void spmv_bsr32_kernel(int64_t nb, const int32_t* __restrict__ brp,
-                                                            const int32_t* __restrict__ bcol,
-                                                            const float* __restrict__ bval,
-                                                            const float* __restrict__ x, float* __restrict__ y) {
-    const int lane = threadIdx.x % LANES;
-    int64_t br = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / LANES;
-    if (br >= nb) return;
-    double acc[BS];
-    bsr_row<BS, LANES>(global(brp), global(bcol), global(bval), brp[nb], global(x), (int32_t)br, lane, acc);
-    if constexpr (LANES >= BS) {
-        if (lane < BS) y[br * BS + lane] = (float)pick<BS>(acc, lane);
-    } else {
-        for (int a = lane; a < BS; a += LANES) y[br * BS + a] = (float)pick<BS>(acc, a);
-    }
-}
-
 // ------------------------------------------------------------ initialisation
 // Cold start (u = 0), FP64 vectors.
 __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restrict__ tiles,
@@ -313,8 +339,9 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
-// Warm start: FP64 recurrence from the current u, r = f - K u.
-template <int LANES>
+// Warm start: FP64 recurrence from the current u, r = f - K u. BS = 1: CSR
+// rows with W lanes each; BS = 2, 3: ELL node rows, W blocks in flight.
+template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __restrict__ tiles,
                                                                 const InstDesc* __restrict__ descs,
                                                                 double* __restrict__ partials, int pstride) {
@@ -322,19 +349,46 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
     const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != (FL_FRESH | FL_WARM)) return;
-    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
-    constexpr int G = kBlock / LANES;
     double v[2] = {0.0, 0.0};
-    for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
-        double ku = csr_row<LANES>(D.row_ptr, D.col, D.val, D.u, row, lane);
-        if (lane == 0) {
-            double ri = D.f[row] - ku;
-            double zi = ri * D.dinv[row];
-            D.r[row] = ri;
-            D.z[row] = zi;
-            D.p[row] = zi;
-            v[0] += ri * ri;
-            v[1] += ri * zi;
+    if constexpr (BS == 1) {
+        const int lane = threadIdx.x % W, grp = threadIdx.x / W;
+        constexpr int G = kBlock / W;
+        for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
+            double ku = csr_row<W>(D.row_ptr, D.col, D.val, D.u, row, lane);
+            if (lane == 0) {
+                double ri = D.f[row] - ku;
+                double zi = ri * D.dinv[row];
+                D.r[row] = ri;
+                D.z[row] = zi;
+                D.p[row] = zi;
+                v[0] += ri * ri;
+                v[1] += ri * zi;
+            }
+        }
+    } else {
+        int32_t c0, c1;
+        tile_chunks<BS>(t, c0, c1);
+        const int32_t n_nodes = D.n / BS;
+        const int lane = threadIdx.x % kWave;
+        for (int32_t c = c0 + (int32_t)(threadIdx.x / kWave); c < c1; c += kWaves) {
+            const int64_t o = D.ell_off[c];
+            const int32_t r = c * kEllRows + lane;
+            double ku[BS], unused[BS];
+            ell_row<BS, W, double, false>(D.ell_nb[c], D.ell_col + o + lane, D.ell_val + o * (BS * BS) + lane, D.u,
+                                          ku, unused);
+            if (r < n_nodes) {
+#pragma unroll
+                for (int a = 0; a < BS; ++a) {
+                    const int32_t i = r * BS + a;
+                    const double ri = D.f[i] - ku[a];
+                    const double zi = ri * D.dinv[i];
+                    D.r[i] = ri;
+                    D.z[i] = zi;
+                    D.p[i] = zi;
+                    v[0] += ri * ri;
+                    v[1] += ri * zi;
+                }
+            }
         }
     }
     const int slot[2] = {P_IFF, P_IRZ};
@@ -411,69 +465,59 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
 // q, r, u, z and p are distinct arrays, and saying so lets the compiler issue
 // every load of a row before its stores and keep stores out of the waits.
 
-// q = K p over node rows [b0, b1) (BS = 1: CSR rows); returns this thread's
-// share of p.q.
-template <int BS, int LANES>
-__device__ __forceinline__ double spmv64_rows(int32_t b0, int32_t b1, gptr<const int32_t> __restrict__ rp,
-                                              gptr<const int32_t> __restrict__ brp,
-                                              gptr<const int32_t> __restrict__ bcol,
-                                              gptr<const double> __restrict__ val, gptr<const double> __restrict__ p,
-                                              gptr<double> __restrict__ q) {
+// q = K p over CSR rows [b0, b1) of a raw CSR instance; returns this
+// thread's share of p.q.
+template <int LANES>
+__device__ __forceinline__ double csr64_rows(int32_t b0, int32_t b1, gptr<const int32_t> __restrict__ rp,
+                                             gptr<const int32_t> __restrict__ col, gptr<const double> __restrict__ val,
+                                             gptr<const double> __restrict__ p, gptr<double> __restrict__ q) {
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
     double dot = 0.0;
-    if constexpr (BS == 1) {
-        for (int32_t row = b0 + grp; row < b1; row += G) {
-            const double pr = p[row];  // dot operand, issued with the row's loads
-            const double acc = csr_row<LANES>(rp, bcol, val, p, row, lane);
-            if (lane == 0) {
-                q[row] = acc;
-                dot += pr * acc;
-            }
-        }
-    } else {
-        for (int32_t br = b0 + grp; br < b1; br += G) {
-            double acc[BS];
-            if constexpr (LANES >= BS) {
-                const int32_t i = br * BS + (lane < BS ? lane : 0);
-                const double pi = p[i];
-                node_row64<BS, LANES>(rp, brp, bcol, val, p, br, lane, acc);
-                if (lane < BS) {
-                    const double qi = pick<BS>(acc, lane);
-                    q[i] = qi;
-                    dot += pi * qi;
-                }
-            } else {
-                node_row64<BS, LANES>(rp, brp, bcol, val, p, br, lane, acc);
-                for (int a = lane; a < BS; a += LANES) {
-                    const int32_t i = br * BS + a;
-                    const double qi = pick<BS>(acc, a);
-                    const double pi = p[i];
-                    q[i] = qi;
-                    dot += pi * qi;
-                }
-            }
+    for (int32_t row = b0 + grp; row < b1; row += G) {
+        const double pr = p[row];  // dot operand, issued with the row's loads
+        const double acc = csr_row<LANES>(rp, col, val, p, row, lane);
+        if (lane == 0) {
+            q[row] = acc;
+            dot += pr * acc;
         }
     }
     return dot;
 }
 
-// q = K p and the partial p.q. BS = 1: one CSR row per LANES lanes;
-// BS = 2, 3: one node row (BS rows) per LANES lanes, read as node blocks.
-template <int BS, int LANES>
+// Rows b + threadIdx.x + j * kBlock (j < kPre) of a tile are loaded before
+// the tile reduces its instance's partials, so that the vector loads and the
+// partial loads are in flight together; rows past b + kPre * kBlock (tiles
+// longer than the default shapes) go through the row loops afterwards.
+constexpr int kPre = 3;
+
+template <typename T>
+__device__ __forceinline__ void preload(gptr<const T> a, int32_t b, int32_t e, T (&v)[kPre]) {
+#pragma unroll
+    for (int j = 0; j < kPre; ++j) {
+        const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
+        v[j] = i < e ? a[i] : T(0);
+    }
+}
+
+// q = K p and the partial p.q. BS = 1 (raw CSR): one row per W lanes;
+// BS = 2, 3: node-row sliced ELL with W blocks in flight per lane.
+template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const Dev D = view(descs[t.inst]);
     double v[1];
-    if constexpr (BS == 1)
-        v[0] = spmv64_rows<1, LANES>(t.row_begin, t.row_end, D.row_ptr, D.brow_ptr, D.col, D.val, D.p, D.q);
-    else
-        v[0] = spmv64_rows<BS, LANES>(t.row_begin / BS, t.row_end / BS, D.row_ptr, D.brow_ptr, D.bcol, D.val, D.p,
-                                      D.q);
+    if constexpr (BS == 1) {
+        v[0] = csr64_rows<W>(t.row_begin, t.row_end, D.row_ptr, D.col, D.val, D.p, D.q);
+    } else {
+        int32_t c0, c1;
+        tile_chunks<BS>(t, c0, c1);
+        v[0] = ell_rows<BS, W, double>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, D.ell_val, D.p, D.q);
+    }
     const int slot[1] = {P_PQ};
     write_partials<1>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -508,12 +552,20 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
                                                              const InstDesc* __restrict__ descs,
                                                              double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const Dev D = view(descs[t.inst]);
+    const int32_t b = t.row_begin, e = t.row_end;
+    double p0[kPre], q0[kPre], d0[kPre], u0[kPre], r0[kPre];
+    preload<double>(D.p, b, e, p0);
+    preload<double>(D.q, b, e, q0);
+    preload<double>(D.dinv, b, e, d0);
+    preload<double>(D.u, b, e, u0);
+    preload<double>(D.r, b, e, r0);
+    const double rz = S->rz;  // written by the previous phase's leader only
     const int sp[1] = {P_PQ};
     double pq[1];
     reduce_tiles<1>(partials, pstride, t.tile_begin, t.tile_count, sp, pq);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     if (pq[0] <= 0.0 || pq[0] != pq[0]) {  // solver.rs:118-123
         if (is_leader(t)) {
             S->pap = pq[0];
@@ -523,15 +575,28 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
         }
         return;
     }
-    const double alpha = S->rz / pq[0];
+    const double alpha = rz / pq[0];
     if (is_leader(t)) {
         S->pap = pq[0];
         S->alpha = alpha;
-        S->snap_rz = S->rz;
+        S->snap_rz = rz;
         S->snap_it = S->it;
     }
     double v[2] = {0.0, 0.0};
-    cg64_update_rows(t.row_begin, t.row_end, alpha, D.p, D.q, D.dinv, D.u, D.r, D.z, v);
+#pragma unroll
+    for (int j = 0; j < kPre; ++j) {
+        const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
+        if (i < e) {
+            D.u[i] = u0[j] + alpha * p0[j];
+            const double ri = r0[j] - alpha * q0[j];
+            D.r[i] = ri;
+            const double zi = ri * d0[j];
+            D.z[i] = zi;
+            v[0] += ri * ri;
+            v[1] += ri * zi;
+        }
+    }
+    cg64_update_rows(b + kPre * kBlock, e, alpha, D.p, D.q, D.dinv, D.u, D.r, D.z, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -540,17 +605,23 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
                                                               const InstDesc* __restrict__ descs,
                                                               const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const Dev D = view(descs[t.inst]);
+    const int32_t b = t.row_begin, e = t.row_end;
+    double z0[kPre], p0[kPre];
+    preload<double>(D.z, b, e, z0);
+    preload<double>(D.p, b, e, p0);
+    const double snap_rz = S->snap_rz;
+    const int64_t snap_it = S->snap_it;
     const int sl[2] = {P_RR, P_RZ};
     double v[2];
     reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const double res = sqrt(v[0]) / S->bnorm;
     const bool conv = res <= S->tol;
-    const double beta = v[1] / S->snap_rz;
+    const double beta = v[1] / snap_rz;
     if (is_leader(t)) {
-        const int64_t it = S->snap_it + 1;
+        const int64_t it = snap_it + 1;
         S->it = it;
         S->rr = v[0];
         S->res = res;
@@ -563,55 +634,51 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
         }
     }
     if (conv) return;
-    direction_rows(t.row_begin, t.row_end, beta, D.z, D.p);
+#pragma unroll
+    for (int j = 0; j < kPre; ++j) {
+        const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
+        if (i < e) D.p[i] = z0[j] + beta * p0[j];
+    }
+    direction_rows(b + kPre * kBlock, e, beta, D.z, D.p);
 }
 
 // ------------------------------------------------------------ FP32-IR
-// q32 = K32 p32 over block rows [b0, b1); this thread's share of p.q.
-template <int BS, int LANES>
-__device__ __forceinline__ double spmv32_rows(int32_t b0, int32_t b1, gptr<const int32_t> __restrict__ brp,
-                                              gptr<const int32_t> __restrict__ bcol,
-                                              gptr<const float> __restrict__ bval, int32_t nblk,
-                                              gptr<const float> __restrict__ p, gptr<float> __restrict__ q) {
+// q32 = K32 p32 over CSR rows [b0, b1) of a raw CSR instance (FP32 copy of the
+// values, FP64 accumulation); this thread's share of p.q.
+template <int LANES>
+__device__ __forceinline__ double csr32_rows(int32_t b0, int32_t b1, gptr<const int32_t> __restrict__ rp,
+                                             gptr<const int32_t> __restrict__ col, gptr<const float> __restrict__ val,
+                                             gptr<const float> __restrict__ p, gptr<float> __restrict__ q) {
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
     double dot = 0.0;
-    for (int32_t br = b0 + grp; br < b1; br += G) {
-        double acc[BS];
-        if constexpr (LANES >= BS) {
-            const int32_t i = br * BS + (lane < BS ? lane : 0);
-            const float pi = p[i];
-            bsr_row<BS, LANES>(brp, bcol, bval, nblk, p, br, lane, acc);
-            if (lane < BS) {
-                const float qv = (float)pick<BS>(acc, lane);
-                q[i] = qv;
-                dot += (double)pi * (double)qv;
-            }
-        } else {
-            bsr_row<BS, LANES>(brp, bcol, bval, nblk, p, br, lane, acc);
-            for (int c = lane; c < BS; c += LANES) {
-                const int32_t i = br * BS + c;
-                const float qv = (float)pick<BS>(acc, c);
-                const float pi = p[i];
-                q[i] = qv;
-                dot += (double)pi * (double)qv;
-            }
+    for (int32_t row = b0 + grp; row < b1; row += G) {
+        const float pr = p[row];
+        const float qv = (float)csr32_row<LANES>(rp, col, val, p, row, lane);
+        if (lane == 0) {
+            q[row] = qv;
+            dot += (double)pr * (double)qv;
         }
     }
     return dot;
 }
 
-template <int BS, int LANES>
+template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
-    const int32_t nblk = D.brow_ptr[D.n / BS];
+    const Dev D = view(descs[t.inst]);
     double v[1];
-    v[0] = spmv32_rows<BS, LANES>(t.row_begin / BS, t.row_end / BS, D.brow_ptr, D.bcol, D.bval, nblk, D.p32, D.q32);
+    if constexpr (BS == 1) {
+        v[0] = csr32_rows<W>(t.row_begin, t.row_end, D.brow_ptr, D.bcol, D.bval, D.p32, D.q32);
+    } else {
+        int32_t c0, c1;
+        tile_chunks<BS>(t, c0, c1);
+        v[0] = ell_rows<BS, W, float>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, D.ell_val32, D.p32, D.q32);
+    }
     const int slot[1] = {P_PQ};
     write_partials<1>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -655,12 +722,20 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
                                                              const InstDesc* __restrict__ descs,
                                                              double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
-    const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
+    const Dev D = view(descs[t.inst]);
+    const int32_t b = t.row_begin, e = t.row_end;
+    float p0[kPre], q0[kPre], d0[kPre], x0[kPre], r0[kPre];
+    preload<float>(D.p32, b, e, p0);
+    preload<float>(D.q32, b, e, q0);
+    preload<float>(D.dinv32, b, e, d0);
+    preload<float>(D.x32, b, e, x0);
+    preload<float>(D.r32, b, e, r0);
+    const double rz = S->rz;  // written by the previous phases' leaders only
     const int sp[1] = {P_PQ};
     double pq[1];
     reduce_tiles<1>(partials, pstride, t.tile_begin, t.tile_count, sp, pq);
-    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     if (pq[0] <= 0.0 || pq[0] != pq[0]) {
         if (is_leader(t)) {
             S->pap = pq[0];
@@ -670,18 +745,31 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
         }
         return;
     }
-    const double alpha_d = S->rz / pq[0];
+    const double alpha_d = rz / pq[0];
     if (is_leader(t)) {
         S->pap = pq[0];
         S->alpha = alpha_d;
-        S->snap_rz = S->rz;
+        S->snap_rz = rz;
         S->snap_rmax = S->rmax;
         S->snap_it = S->it;
         S->snap_flags = S->flags;
     }
     const float alpha = (float)alpha_d;
     double v[2] = {0.0, 0.0};
-    ir32_update_rows(t.row_begin, t.row_end, alpha, D.p32, D.q32, D.dinv32, D.x32, D.r32, D.z32, v);
+#pragma unroll
+    for (int j = 0; j < kPre; ++j) {
+        const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
+        if (i < e) {
+            D.x32[i] = x0[j] + alpha * p0[j];
+            const float ri = r0[j] - alpha * q0[j];
+            D.r32[i] = ri;
+            const float zi = ri * d0[j];
+            D.z32[i] = zi;
+            v[0] += (double)ri * (double)ri;
+            v[1] += (double)ri * (double)zi;
+        }
+    }
+    ir32_update_rows(b + kPre * kBlock, e, alpha, D.p32, D.q32, D.dinv32, D.x32, D.r32, D.z32, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -694,18 +782,25 @@ __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __rest
                                                               const double* __restrict__ partials, int pstride,
                                                               int period_end) {
     const Tile t = tiles[blockIdx.x];
-    const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
+    const Dev D = view(descs[t.inst]);
+    const int32_t b = t.row_begin, e = t.row_end;
+    float z0[kPre], p0[kPre];
+    preload<float>(D.z32, b, e, z0);
+    preload<float>(D.p32, b, e, p0);
+    const int64_t snap_it = S->snap_it;
+    const double snap_rmax = S->snap_rmax, snap_rz = S->snap_rz;
+    const int32_t snap_flags = S->snap_flags;
     const int sl[2] = {P_RR, P_RZ};
     double v[2];
     reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
-    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
-    const int64_t it = S->snap_it + 1;
+    const int64_t it = snap_it + 1;
     const double est = sqrt(v[0]);
-    const double rmax = fmax(S->snap_rmax, est);
-    const double beta = v[1] / S->snap_rz;
+    const double rmax = fmax(snap_rmax, est);
+    const double beta = v[1] / snap_rz;
     const bool hold = it >= S->max_iter;
-    const bool refresh = hold || (S->snap_flags & FL_REFRESH) || est / S->bnorm <= S->tol ||
+    const bool refresh = hold || (snap_flags & FL_REFRESH) || est / S->bnorm <= S->tol ||
                          est <= S->ru_delta * rmax || it - S->it_refresh >= kForcedRefresh;
     if (is_leader(t)) {
         S->it = it;
@@ -713,10 +808,16 @@ __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __rest
         S->rz = v[1];
         S->beta = beta;
         S->rmax = rmax;
-        S->flags = S->snap_flags | (refresh ? FL_REFRESH : 0) | (hold ? FL_HOLD : 0);
+        S->flags = snap_flags | (refresh ? FL_REFRESH : 0) | (hold ? FL_HOLD : 0);
     }
     if (hold || (period_end && refresh)) return;
-    direction32_rows(t.row_begin, t.row_end, (float)beta, D.z32, D.p32);
+    const float bf = (float)beta;
+#pragma unroll
+    for (int j = 0; j < kPre; ++j) {
+        const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
+        if (i < e) D.p32[i] = z0[j] + bf * p0[j];
+    }
+    direction32_rows(b + kPre * kBlock, e, bf, D.z32, D.p32);
 }
 
 // Refresh step 1: fold the FP32 correction into the FP64 solution.
@@ -772,7 +873,48 @@ __device__ __forceinline__ void residual_rows(int32_t b, int32_t e, gptr<const i
     }
 }
 
-template <int LANES>
+// The same over ELL node rows [c0, c1) of a node-block instance.
+template <int BS, int UNR>
+__device__ __forceinline__ void ell_residual_rows(int32_t c0, int32_t c1, int32_t n_nodes,
+                                                  gptr<const int32_t> __restrict__ off,
+                                                  gptr<const int32_t> __restrict__ nbs,
+                                                  gptr<const int32_t> __restrict__ col,
+                                                  gptr<const double> __restrict__ val,
+                                                  gptr<const double> __restrict__ u, gptr<const double> __restrict__ f,
+                                                  gptr<const float> __restrict__ dinv32, gptr<float> __restrict__ r32,
+                                                  gptr<float> __restrict__ z32, double (&v)[3]) {
+    const int lane = threadIdx.x % kWave;
+    for (int32_t c = c0 + (int32_t)(threadIdx.x / kWave); c < c1; c += kWaves) {
+        const int64_t o = off[c];
+        const int32_t r = c * kEllRows + lane;
+        const bool own = r < n_nodes;
+        double fi[BS];
+        float di[BS];
+#pragma unroll
+        for (int a = 0; a < BS; ++a) {
+            fi[a] = own ? f[r * BS + a] : 0.0;
+            di[a] = own ? dinv32[r * BS + a] : 0.0f;
+        }
+        double ku[BS], aku[BS];
+        ell_row<BS, UNR, double, true>(nbs[c], col + o + lane, val + o * (BS * BS) + lane, u, ku, aku);
+        if (own) {
+#pragma unroll
+            for (int a = 0; a < BS; ++a) {
+                const double ri = fi[a] - ku[a];
+                const float rs = (float)ri;
+                const float zs = rs * di[a];
+                r32[r * BS + a] = rs;
+                z32[r * BS + a] = zs;
+                v[0] += ri * ri;
+                v[1] += (double)rs * (double)zs;
+                const double aa = aku[a] + fabs(fi[a]);
+                v[2] += aa * aa;
+            }
+        }
+    }
+}
+
+template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __restrict__ tiles,
                                                                const InstDesc* __restrict__ descs,
                                                                double* __restrict__ partials, int pstride) {
@@ -782,7 +924,14 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
     if (is_leader(t)) S->evals += 1;
     double v[3] = {0.0, 0.0, 0.0};
-    residual_rows<LANES>(t.row_begin, t.row_end, D.row_ptr, D.col, D.val, D.u, D.f, D.dinv32, D.r32, D.z32, v);
+    if constexpr (BS == 1) {
+        residual_rows<W>(t.row_begin, t.row_end, D.row_ptr, D.col, D.val, D.u, D.f, D.dinv32, D.r32, D.z32, v);
+    } else {
+        int32_t c0, c1;
+        tile_chunks<BS>(t, c0, c1);
+        ell_residual_rows<BS, W>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, D.ell_val, D.u, D.f, D.dinv32,
+                                 D.r32, D.z32, v);
+    }
     const int slot[3] = {P_TRR, P_TRZ, P_TFL};
     write_partials<3>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -900,17 +1049,41 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
         default: MACRO(64); break;       \
     }
 
+#define SVX_UNROLL_DISPATCH(unroll, MACRO) \
+    switch (unroll) {                      \
+        case 1: MACRO(1); break;           \
+        case 2: MACRO(2); break;           \
+        case 3: MACRO(3); break;           \
+        default: MACRO(4); break;          \
+    }
+
 void launch_cg64_spmv(const KrylovSet& k, hipStream_t s) {
     if (k.bs == 3) {
-#define L(LN) cg64_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.lanes, L)
+#define L(U) cg64_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(LN) cg64_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.lanes, L)
+#define L(U) cg64_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
 #define L(LN) cg64_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_LANES_DISPATCH(k.lanes, L)
+#undef L
+    }
+}
+
+void launch_ir32_spmv(const KrylovSet& k, hipStream_t s) {
+    if (k.bs == 3) {
+#define L(U) ir32_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_UNROLL_DISPATCH(k.unroll, L)
+#undef L
+    } else if (k.bs == 2) {
+#define L(U) ir32_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_UNROLL_DISPATCH(k.unroll, L)
+#undef L
+    } else {
+#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }
@@ -934,27 +1107,22 @@ void launch_spmv_csr64(int64_t n, const int32_t* row_ptr, const int32_t* col, co
     SVX_HIP(hipGetLastError());
 }
 
-void launch_spmv_bsr32(int64_t nb, int bs, const int32_t* brow_ptr, const int32_t* bcol, const float* bval,
-                       const float* x, float* y, int lanes, hipStream_t s) {
-    if (nb <= 0) return;
-    if (bs == 3) {
-#define L(LN) spmv_bsr32_kernel<3, LN><<<grid_for(nb * LN, kBlock), kBlock, 0, s>>>(nb, brow_ptr, bcol, bval, x, y)
-        SVX_LANES_DISPATCH(lanes, L)
-#undef L
-    } else {
-#define L(LN) spmv_bsr32_kernel<2, LN><<<grid_for(nb * LN, kBlock), kBlock, 0, s>>>(nb, brow_ptr, bcol, bval, x, y)
-        SVX_LANES_DISPATCH(lanes, L)
-#undef L
-    }
-    SVX_HIP(hipGetLastError());
-}
-
 void cg64_init(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     cg64_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
-#define L(LN) cg64_init_warm_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-    SVX_LANES_DISPATCH(k.lanes, L)
+    if (k.bs == 3) {
+#define L(U) cg64_init_warm_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
+    } else if (k.bs == 2) {
+#define L(U) cg64_init_warm_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_UNROLL_DISPATCH(k.unroll, L)
+#undef L
+    } else {
+#define L(LN) cg64_init_warm_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_LANES_DISPATCH(k.lanes, L)
+#undef L
+    }
     init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
@@ -976,19 +1144,7 @@ void ir32_init(const KrylovSet& k, hipStream_t s) {
 
 void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end) {
     if (k.n_tiles <= 0) return;
-    if (k.bs == 3) {
-#define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.blanes, L)
-#undef L
-    } else if (k.bs == 2) {
-#define L(LN) ir32_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.blanes, L)
-#undef L
-    } else {
-#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.blanes, L)
-#undef L
-    }
+    launch_ir32_spmv(k, s);
     ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, period_end ? 1 : 0);
     SVX_HIP(hipGetLastError());
@@ -997,9 +1153,19 @@ void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end) {
 void ir32_refresh(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     ir32_fold_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
-#define L(LN) ir32_residual_kernel<LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-    SVX_LANES_DISPATCH(k.lanes, L)
+    if (k.bs == 3) {
+#define L(U) ir32_residual_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
+    } else if (k.bs == 2) {
+#define L(U) ir32_residual_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_UNROLL_DISPATCH(k.unroll, L)
+#undef L
+    } else {
+#define L(LN) ir32_residual_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+        SVX_LANES_DISPATCH(k.lanes, L)
+#undef L
+    }
     ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
@@ -1040,21 +1206,10 @@ void pupdate_phase(const KrylovSet& k, int precision, hipStream_t s) {
 
 void spmv_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-    if (precision == SVX_FP64) {
+    if (precision == SVX_FP64)
         launch_cg64_spmv(k, s);
-    } else if (k.bs == 3) {
-#define L(LN) ir32_spmv_kernel<3, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.blanes, L)
-#undef L
-    } else if (k.bs == 2) {
-#define L(LN) ir32_spmv_kernel<2, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.blanes, L)
-#undef L
-    } else {
-#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
-        SVX_LANES_DISPATCH(k.blanes, L)
-#undef L
-    }
+    else
+        launch_ir32_spmv(k, s);
     SVX_HIP(hipGetLastError());
 }
 

@@ -4,7 +4,8 @@
 //    node*ndof + a holds every column j*ndof + b for j in the node's
 //    element-neighbourhood (sorted), or only its diagonal when the node
 //    belongs to no element (assembly.rs:300-302);
-//  - the node-block (BSR) pattern with the same neighbourhoods.
+//  - the node-block (BSR) pattern with the same neighbourhoods, and its
+//    node-row sliced ELL form for the Krylov SpMV.
 #include "pattern.hpp"
 
 #include <algorithm>
@@ -106,6 +107,31 @@ void build_pattern(int ndof, int64_t n_nodes, int64_t n_elem, const int64_t* con
                     for (int bd = 0; bd < ndof; ++bd) *out++ = L[q] * ndof + bd;
             }
         }
+    });
+    // sliced ELL of the block pattern
+    const int64_t nch = (n_nodes + kEllRows - 1) / kEllRows;
+    P.ell_off.assign(nch + 1, 0);
+    P.ell_nb.assign(nch, 0);
+    int64_t ne = 0;
+    for (int64_t c = 0; c < nch; ++c) {
+        int32_t m = 0;
+        for (int64_t r = c * kEllRows; r < std::min(n_nodes, (c + 1) * kEllRows); ++r) m = std::max(m, deg[r]);
+        P.ell_nb[c] = m;
+        ne += (int64_t)m * kEllRows;
+        if (ne > INT32_MAX / (ndof * ndof)) throw std::runtime_error("ELL pattern exceeds 2^31 entries");
+        P.ell_off[c + 1] = (int32_t)ne;
+    }
+    P.ell_col.resize(ne);
+    parallel_for(nch, [&](int64_t a, int64_t b) {
+        for (int64_t c = a; c < b; ++c)
+            for (int l = 0; l < kEllRows; ++l) {
+                const int64_t r = c * kEllRows + l;
+                for (int32_t k = 0; k < P.ell_nb[c]; ++k) {
+                    int32_t v = 0;
+                    if (r < n_nodes) v = k < deg[r] ? P.bcol[P.brow_ptr[r] + k] : (int32_t)r;
+                    P.ell_col[P.ell_off[c] + (int64_t)k * kEllRows + l] = v;
+                }
+            }
     });
 }
 

@@ -124,15 +124,17 @@ struct Request {
 };
 
 // Device bytes a request holds while admitted: mesh arrays, CSR values and
-// vectors, the FP32 copy for FP32-IR, and the larger of the transient element
-// stiffness and nodal-stress buffers.
+// vectors, the ELL copies of K (FP64, and FP32 for FP32-IR), and the larger
+// of the transient element stiffness and nodal-stress buffers.
 int64_t footprint(const svx_mesh& m, int precision) {
     const int64_t n = m.n, nn = m.n_nodes, ne = m.n_elem, nnz = m.nnz;
+    const int64_t ell = (int64_t)m.hp.ell_col.size(), ell_vals = ell * m.ndof * m.ndof;
     int64_t b = 24 * nn + 12 * ne + 4 * (int64_t)m.h_conn.size() + 8 * (ne + 1);  // coords, kinds/offsets, conn
     b += 4 * (n + 1) + 4 * nnz + 4 * (nn + 1) + 4 * m.nblk;                        // CSR + node-block patterns
+    b += 4 * ell + 8 * ((int64_t)m.hp.ell_nb.size() + 1);                          // ELL pattern
     b += 8 * (nn + 1) + 5 * (int64_t)m.hp.inc_elem.size() + 16 * ne;              // incidence, element offsets
-    b += 8 * nnz + 8 * n * 9 + 4 * n;                                              // values, f, dinv, u r p z q, dinv32
-    if (precision == SVX_FP32_IR) b += 4 * nnz + 4 * n * 5;                        // FP32 blocks + vectors
+    b += 8 * nnz + 8 * ell_vals + 8 * n * 9 + 4 * n;                              // CSR + ELL values, vectors, dinv32
+    if (precision == SVX_FP32_IR) b += 4 * ell_vals + 4 * n * 5;                   // FP32 ELL values + vectors
     b += 8 * std::max(m.ke_total, m.ns_total + nn * 6);                            // transient assembly / recovery
     return b + (1 << 16);
 }

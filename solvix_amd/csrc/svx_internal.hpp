@@ -112,19 +112,27 @@ struct alignas(16) InstScal {
     int64_t max_iter;
 };
 
-// Device pointers of one instance. FP64 CSR always; FP32 block-sparse
-// (node-block BSR, bs x bs blocks) for the FP32-IR inner iteration.
+// Device pointers of one instance. FP64 CSR always (FP32-IR residuals,
+// warm starts). Node-block instances (bs = 2, 3) multiply in the node-row
+// sliced ELL layout, FP64 values for FP64 PCG and FP32 values for the FP32-IR
+// inner iteration; raw CSR instances (bs = 1) multiply the CSR directly, and
+// FP32-IR reads an FP32 copy of the CSR values (bval).
 struct InstDesc {
     int32_t n;           // rows
     int32_t tile_begin;  // first tile of this instance in the tile table
     int32_t tile_count;
-    int32_t bs;          // block size of the BSR arrays (ndof), 0 = none
+    int32_t bs;          // node-block size (ndof); 1 = raw CSR
     const int32_t* row_ptr;
     const int32_t* col;
     const double* val;
-    const int32_t* brow_ptr;  // [n/bs + 1]
-    const int32_t* bcol;      // block columns (node indices)
-    const float* bval;        // bs*bs per block, row-major
+    const int32_t* brow_ptr;  // bs = 1: row_ptr
+    const int32_t* bcol;      // bs = 1: col
+    const float* bval;        // bs = 1: FP32 CSR values
+    const int32_t* ell_off;   // [n/bs/64 + 1] chunk offsets in blocks
+    const int32_t* ell_nb;    // padded blocks per row, per chunk
+    const int32_t* ell_col;   // block columns, [k][lane] per chunk
+    const double* ell_val;    // [block][a*bs+b][lane] per chunk
+    const float* ell_val32;
     const double* dinv;
     const float* dinv32;
     const double* f;
@@ -156,6 +164,8 @@ struct alignas(16) Tile {
 };
 
 constexpr int kBlock = 256;
+constexpr int kMaxUnroll = 4;  // ELL SpMV: blocks in flight per lane, 1..kMaxUnroll
+constexpr int kEllRows = 64;   // node rows per ELL chunk (one wavefront)
 // Per-tile partial sums, one slot per reduced quantity.
 enum : int { P_PQ = 0, P_RR = 1, P_RZ = 2, P_TRR = 3, P_TRZ = 4, P_TFL = 5, P_IFF = 6, P_IRZ = 7, kPartialSlots = 8 };
 
@@ -185,6 +195,10 @@ void launch_diag_inv(int64_t n_rows, const int32_t* row_ptr, const int32_t* col,
                      hipStream_t s);
 void launch_fill_bsr(int64_t n_rows, int bs, const int32_t* row_ptr, const int32_t* col, const double* vals,
                      const int32_t* brow_ptr, float* bval, hipStream_t s);
+// CSR values of a node-block matrix (bs = 2, 3) into the node-row sliced ELL
+// layout of the mesh (ell_off / ell_nb over n_nodes node rows), FP64 or FP32.
+void launch_ell_pack(int bs, int64_t n_nodes, const int32_t* ell_off, const int32_t* ell_nb, const int32_t* row_ptr,
+                     const int32_t* brow_ptr, const double* vals, double* out64, float* out32, hipStream_t s);
 void launch_reaction_rows(int64_t n_sel, const int64_t* rows, const int32_t* row_ptr, const int32_t* col,
                           const int64_t* off, const double* orig_vals, const double* u, double* out,
                           hipStream_t s);
@@ -203,8 +217,6 @@ void launch_spmv_ordered(int64_t n, const int32_t* row_ptr, const int32_t* col, 
                          const double* x, double* y, hipStream_t s);
 void launch_spmv_csr64(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
                        const double* x, double* y, int lanes, hipStream_t s);
-void launch_spmv_bsr32(int64_t nb, int bs, const int32_t* brow_ptr, const int32_t* bcol, const float* bval,
-                       const float* x, float* y, int lanes, hipStream_t s);
 
 // One CG iteration phase over a set of instances. `tiles` / `n_tiles`
 // describe the work; `descs` the instances; `partials` holds kPartialSlots
@@ -215,9 +227,9 @@ struct KrylovSet {
     const InstDesc* descs = nullptr;
     int32_t n_inst = 0;
     double* partials = nullptr;  // [kPartialSlots * n_tiles]
-    int lanes = 16;              // CSR lanes per row (FP64 SpMV)
-    int blanes = 8;              // BSR lanes per block row (FP32 SpMV)
-    int bs = 0;                  // common BSR block size of the set (2 or 3), 0 = mixed/none
+    int lanes = 16;              // lanes per row (raw CSR instances, bs = 1)
+    int unroll = 2;              // blocks in flight per lane (ELL instances, bs = 2, 3)
+    int bs = 0;                  // node-block size shared by the set's instances (1, 2, 3)
 };
 void cg64_init(const KrylovSet& k, hipStream_t s);
 void cg64_iteration(const KrylovSet& k, hipStream_t s);
