@@ -274,7 +274,7 @@ void system_prepare(svx_system* sys, int precision) {
     if (!sys->u.p) sys->u.alloc(n);
     if (!sys->scal.p) sys->scal.alloc(1);
     if (!sys->r.p) {  // FP64 vectors: the FP64 arm, and the FP32-IR fallback
-        sys->r.alloc(n), sys->p.alloc(n), sys->z.alloc(n), sys->q.alloc(n);
+        sys->r.alloc(n), sys->p.alloc(n), sys->z.alloc(n), sys->q.alloc(n), sys->p_alt.alloc(n);
     }
     const size_t ell_vals = (size_t)std::max<int64_t>(1, m->ell_blocks) * m->ndof * m->ndof;
     if (!sys->ell64.p) {  // FP64 PCG, and the FP64 fallback of FP32-IR
@@ -290,6 +290,7 @@ void system_prepare(svx_system* sys, int precision) {
         }
         if (!sys->x32.p) {
             sys->x32.alloc(n), sys->r32.alloc(n), sys->p32.alloc(n), sys->z32.alloc(n), sys->q32.alloc(n);
+            sys->p32_alt.alloc(n);
         }
     }
     SVX_HIP(hipStreamSynchronize(s));
@@ -323,17 +324,20 @@ InstDesc system_desc(svx_system* sys) {
     d.p32 = sys->p32.p;
     d.z32 = sys->z32.p;
     d.q32 = sys->q32.p;
+    d.p_alt = sys->p_alt.p;
+    d.p32_alt = sys->p32_alt.p;
     d.s = sys->scal.p;
     return d;
 }
 
-// Algorithmic HBM bytes of one solver SpMV launch on this mesh's matrix: the
-// matrix values (nnz, without the ELL padding), one block column per block,
-// the chunk table, one read of x and one write of y.
+// Algorithmic HBM bytes of one solver SpMV launch on this mesh's matrix (the
+// direction step fused in): the matrix values (nnz, without the ELL padding),
+// one block column per block, the chunk table, one read of z and of the old
+// direction, one write of the new direction and of q.
 double spmv_bytes(const svx_mesh* m, int precision) {
     const double n = (double)m->n, nnz = (double)m->nnz, nblk = (double)m->nblk, ch = (double)m->n_chunks;
-    if (precision == SVX_FP64) return 8.0 * nnz + 4.0 * nblk + 8.0 * ch + 16.0 * n;
-    return 4.0 * nnz + 4.0 * nblk + 8.0 * ch + 8.0 * n;
+    if (precision == SVX_FP64) return 8.0 * nnz + 4.0 * nblk + 8.0 * ch + 32.0 * n;
+    return 4.0 * nnz + 4.0 * nblk + 8.0 * ch + 16.0 * n;
 }
 
 InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len) {
@@ -698,8 +702,8 @@ int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int
         hipStream_t s = ctx->stream;
         const int64_t n = csr->n;
         int64_t mi = max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * n);
-        DevBuf<double> df(n), dinv(n), du(n), dr(n), dp(n), dz(n), dq(n);
-        DevBuf<float> dinv32(n), bval, x32, r32, p32, z32, q32;
+        DevBuf<double> df(n), dinv(n), du(n), dr(n), dp(n), dz(n), dq(n), dp_alt(n);
+        DevBuf<float> dinv32(n), bval, x32, r32, p32, z32, q32, p32_alt;
         df.upload(f, n, s);
         launch_diag_inv(n, csr->row_ptr.p, csr->col.p, csr->vals.p, dinv.p, dinv32.p, nullptr, nullptr, s);
         InstDesc d{};
@@ -712,15 +716,15 @@ int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int
         d.dinv32 = dinv32.p;
         d.f = df.p;
         d.u = du.p;
-        d.r = dr.p, d.p = dp.p, d.z = dz.p, d.q = dq.p;
+        d.r = dr.p, d.p = dp.p, d.z = dz.p, d.q = dq.p, d.p_alt = dp_alt.p;
         if (precision == SVX_FP32_IR) {
             bval.alloc(std::max<int64_t>(1, csr->nnz));
             launch_fill_bsr(n, 1, csr->row_ptr.p, csr->col.p, csr->vals.p, csr->row_ptr.p, bval.p, s);
-            x32.alloc(n), r32.alloc(n), p32.alloc(n), z32.alloc(n), q32.alloc(n);
+            x32.alloc(n), r32.alloc(n), p32.alloc(n), z32.alloc(n), q32.alloc(n), p32_alt.alloc(n);
             d.brow_ptr = csr->row_ptr.p;
             d.bcol = csr->col.p;
             d.bval = bval.p;
-            d.x32 = x32.p, d.r32 = r32.p, d.p32 = p32.p, d.z32 = z32.p, d.q32 = q32.p;
+            d.x32 = x32.p, d.r32 = r32.p, d.p32 = p32.p, d.z32 = z32.p, d.q32 = q32.p, d.p32_alt = p32_alt.p;
         }
         DevBuf<InstScal> scal(1);
         InstScal sc = initial_scal(tol, mi, ctx->tune.ru_delta, csr->max_row_len);
@@ -778,16 +782,18 @@ int svx_system_time_phases(svx_system* sys, int precision, int reps, double* avg
         sys->engine->time_phases(precision, reps, avg_ms);
         if (bytes_per_launch) {
             const double n = (double)m->n, nnz = (double)m->nnz;
+            const double nblk = (double)m->nblk, ch = (double)m->n_chunks;
             bytes_per_launch[0] = spmv_bytes(m, precision);
             if (precision == SVX_FP64) {
                 bytes_per_launch[1] = 64.0 * n;  // p q r u dinv read, u r z written
-                bytes_per_launch[2] = 24.0 * n;  // z p read, p written
+                bytes_per_launch[2] = 0.0;
                 bytes_per_launch[3] = 0.0;
             } else {
                 bytes_per_launch[1] = 32.0 * n;  // p q r x dinv read, x r z written (FP32)
-                bytes_per_launch[2] = 12.0 * n;
-                // fold (u, x32 read+write) + FP64 residual (CSR, u, f, dinv32; r32 z32 written) + repoint
-                bytes_per_launch[3] = 24.0 * n + (12.0 * nnz + 4.0 * (n + 1) + 28.0 * n) + 12.0 * n;
+                bytes_per_launch[2] = 12.0 * n;  // period end: z p read, p written
+                // fold (u, x32 read+write) + FP64 residual (ELL values and columns, u, f, dinv32; r32 z32
+                // written) + repoint (z32 p32 read, p32 written)
+                bytes_per_launch[3] = 24.0 * n + (8.0 * nnz + 4.0 * nblk + 8.0 * ch + 28.0 * n) + 12.0 * n;
             }
         }
         return SVX_OK;

@@ -71,8 +71,8 @@ struct svx_system {
     svx::DevBuf<float> dinv32;
     svx::DevBuf<double> ell64;  // K in the ELL layout (FP64 PCG, FP32-IR fallback)
     svx::DevBuf<float> ell32;   // K in the ELL layout, FP32 (FP32-IR)
-    svx::DevBuf<double> u, r, p, z, q;
-    svx::DevBuf<float> x32, r32, p32, z32, q32;
+    svx::DevBuf<double> u, r, p, z, q, p_alt;
+    svx::DevBuf<float> x32, r32, p32, z32, q32, p32_alt;
     svx::DevBuf<svx::InstScal> scal;
     std::unique_ptr<svx::Engine> engine;
     int engine_precision = -1;  // precision the engine's descriptors were built for

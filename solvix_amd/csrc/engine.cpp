@@ -125,14 +125,19 @@ void Engine::enqueue_init(const EngineOptions& opt) {
     }
 }
 
+// `iters` is even (and a multiple of the even FP32-IR period): the chunk
+// starts and ends with the current, complete direction in P[0] and no
+// pending partial sums, so the tile tables can change between chunks.
 void Engine::enqueue_chunk(const EngineOptions& opt, int iters) {
     if (opt.precision == SVX_FP64) {
         for (int i = 0; i < iters; ++i)
-            for (int bs = 1; bs <= 3; ++bs) cg64_iteration(groups_[bs], stream_);
+            for (int bs = 1; bs <= 3; ++bs) cg64_iteration(groups_[bs], stream_, i & 1);
+        for (int bs = 1; bs <= 3; ++bs) cg64_chunk_end(groups_[bs], stream_);
     } else {
         for (int i = 0; i < iters; i += opt.period) {
             for (int j = 0; j < opt.period; ++j)
-                for (int bs = 1; bs <= 3; ++bs) ir32_iteration(groups_[bs], stream_, j == opt.period - 1);
+                for (int bs = 1; bs <= 3; ++bs) ir32_iteration(groups_[bs], stream_, j & 1);
+            for (int bs = 1; bs <= 3; ++bs) ir32_period_end(groups_[bs], stream_);
             for (int bs = 1; bs <= 3; ++bs) ir32_refresh(groups_[bs], stream_);
         }
     }
@@ -156,8 +161,10 @@ int Engine::run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int 
     return run_phase(o64, h_flag, d_flag, -1);
 }
 
-int Engine::run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
-    int chunk = opt.chunk;
+int Engine::run_phase(const EngineOptions& opt_in, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
+    EngineOptions opt = opt_in;
+    opt.period = std::max(2, (opt.period + 1) / 2 * 2);  // even: the period ends with the direction in P[0]
+    int chunk = std::max(2, (opt.chunk + 1) / 2 * 2);
     if (opt.precision != SVX_FP64) chunk = std::max(opt.period, (chunk / opt.period) * opt.period);
     int done_chunks = 0;
     for (;;) {
@@ -234,14 +241,14 @@ void Engine::time_phases(int precision, int reps, double ms[4]) {
     const int n_inst = (int32_t)h_descs_.size();
     for (int ph = 0; ph < 4; ++ph) {
         ms[ph] = 0.0;
-        if (ph == 3 && precision == SVX_FP64) continue;
+        if (ph >= 2 && precision == SVX_FP64) continue;
         for (int r = 0; r <= reps; ++r) {  // r == 0 warms up
             launch_set_running(descs_.p, n_inst, stream_, ph == 3 ? FL_REFRESH : 0);
             SVX_HIP(hipEventRecord(a, stream_));
             for (int bs = 1; bs <= 3; ++bs) {
                 if (ph == 0) spmv_phase(groups_[bs], precision, stream_);
                 if (ph == 1) update_phase(groups_[bs], precision, stream_);
-                if (ph == 2) pupdate_phase(groups_[bs], precision, stream_);
+                if (ph == 2) period_end_phase(groups_[bs], precision, stream_);
                 if (ph == 3) ir32_refresh(groups_[bs], stream_);
             }
             SVX_HIP(hipEventRecord(b, stream_));

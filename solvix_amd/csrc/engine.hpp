@@ -45,9 +45,10 @@ class Engine {
     // them), e.g. before timing the whole set.
     void tile_all() { build_tiles(std::vector<char>(h_descs_.size(), 1)); }
     // Average device time of each phase of an iteration, launched alone on
-    // the current set with every instance forced to run: [0] SpMV, [1] update,
-    // [2] direction update, [3] FP32-IR refresh (fold, FP64 residual,
-    // repoint; 0 for FP64). Leaves the instances' solver state undefined.
+    // the current set with every instance forced to run: [0] direction step
+    // and SpMV, [1] update, [2] FP32-IR period end (0 for FP64), [3] FP32-IR
+    // refresh (fold, FP64 residual, repoint; 0 for FP64). Leaves the
+    // instances' solver state undefined.
     void time_phases(int precision, int reps, double ms[4]);
     int64_t launches() const { return launches_; }
     int n_instances() const { return (int)h_descs_.size(); }

@@ -66,8 +66,8 @@ struct Dev {
     gptr<const int32_t> ell_off, ell_nb, ell_col;
     gptr<const double> val, ell_val, dinv, f;
     gptr<const float> bval, ell_val32, dinv32;
-    gptr<double> u, r, p, z, q;
-    gptr<float> x32, r32, p32, z32, q32;
+    gptr<double> u, r, p, z, q, p_alt;
+    gptr<float> x32, r32, p32, z32, q32, p32_alt;
 };
 
 __device__ __forceinline__ Dev view(const InstDesc& d) {
@@ -97,6 +97,8 @@ __device__ __forceinline__ Dev view(const InstDesc& d) {
     v.p32 = global(d.p32);
     v.z32 = global(d.z32);
     v.q32 = global(d.q32);
+    v.p_alt = global(d.p_alt);
+    v.p32_alt = global(d.p32_alt);
     return v;
 }
 
@@ -167,26 +169,45 @@ __device__ __forceinline__ void reduce_tiles(const double* __restrict__ partials
     block_allreduce<NV>(out);
 }
 
-template <int LANES>
+// x_j as an SpMV reads it: a stored vector, or a direction step p' = z + beta p
+// evaluated on the fly with one fused multiply-add in the vector precision
+// (the owner of row j stores exactly this value).
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <typename V>
+struct Plain {
+    gptr<const V> x;
+    __device__ __forceinline__ V operator()(int32_t i) const { return x[i]; }
+};
+
+template <typename V>
+struct Step {
+    gptr<const V> z, p;
+    V beta;
+    __device__ __forceinline__ V operator()(int32_t i) const { return fmadd(beta, p[i], z[i]); }
+};
+
+template <int LANES, typename X>
 __device__ __forceinline__ double csr_row(gptr<const int32_t> rp, gptr<const int32_t> col, gptr<const double> val,
-                                          gptr<const double> x, int32_t row, int lane) {
+                                          const X& x, int32_t row, int lane) {
     double acc = 0.0;
     const int32_t kb = rp[row], ke = rp[row + 1];
 #pragma unroll 4
-    for (int32_t k = kb + lane; k < ke; k += LANES) acc += val[k] * x[col[k]];
+    for (int32_t k = kb + lane; k < ke; k += LANES) acc += val[k] * (double)x(col[k]);
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
     return acc;
 }
 
 // One row of the FP32 copy of a raw CSR matrix times x; FP64 accumulation.
-template <int LANES>
+template <int LANES, typename X>
 __device__ __forceinline__ double csr32_row(gptr<const int32_t> rp, gptr<const int32_t> col, gptr<const float> val,
-                                            gptr<const float> x, int32_t row, int lane) {
+                                            const X& x, int32_t row, int lane) {
     double acc = 0.0;
     const int32_t kb = rp[row], ke = rp[row + 1];
 #pragma unroll 4
-    for (int32_t k = kb + lane; k < ke; k += LANES) acc += (double)val[k] * (double)x[col[k]];
+    for (int32_t k = kb + lane; k < ke; k += LANES) acc += (double)val[k] * (double)x(col[k]);
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
     return acc;
@@ -197,8 +218,8 @@ __device__ __forceinline__ double csr32_row(gptr<const int32_t> rp, gptr<const i
 // chunk's padded block count. The loads of UNR blocks are issued together;
 // the sums run in block order whatever UNR, so UNR changes timing only.
 // ABS: also accumulate |K_ij x_j| (the FP64 evaluation floor of f - K x).
-template <int BS, int UNR, typename V, bool ABS>
-__device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr<const V> vl, gptr<const V> x,
+template <int BS, int UNR, typename V, bool ABS, typename X>
+__device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr<const V> vl, const X& x,
                                         double (&acc)[BS], double (&aacc)[BS]) {
     constexpr int BB = BS * BS;
 #pragma unroll
@@ -217,7 +238,7 @@ __device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr
         for (int u = 0; u < UNR; ++u) {
             double xj[BS];
 #pragma unroll
-            for (int b = 0; b < BS; ++b) xj[b] = (double)x[j[u] * BS + b];
+            for (int b = 0; b < BS; ++b) xj[b] = (double)x(j[u] * BS + b);
 #pragma unroll
             for (int a = 0; a < BS; ++a)
 #pragma unroll
@@ -235,7 +256,7 @@ __device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr
         for (int m = 0; m < BB; ++m) w[m] = vl[(k * BB + m) * kEllRows];
         double xj[BS];
 #pragma unroll
-        for (int b = 0; b < BS; ++b) xj[b] = (double)x[jj * BS + b];
+        for (int b = 0; b < BS; ++b) xj[b] = (double)x(jj * BS + b);
 #pragma unroll
         for (int a = 0; a < BS; ++a)
 #pragma unroll
@@ -247,17 +268,17 @@ __device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr
     }
 }
 
-// q = K p over the node-row sliced ELL chunks [c0, c1) of an instance with
-// n_nodes node rows. Wavefront w of the workgroup takes chunks c0 + w,
-// c0 + w + kWaves, ...; lane l owns node row 64c + l, accumulating in FP64.
-// Every load of a wavefront covers 64 consecutive words (values, columns) or
-// the rows' own x entries; the column gathers of x hit the caches. Returns
-// this thread's share of p.q.
-template <int BS, int UNR, typename V>
-__device__ __forceinline__ double ell_rows(int32_t c0, int32_t c1, int32_t n_nodes,
-                                           gptr<const int32_t> __restrict__ off, gptr<const int32_t> __restrict__ nbs,
-                                           gptr<const int32_t> __restrict__ col, gptr<const V> __restrict__ val,
-                                           gptr<const V> __restrict__ p, gptr<V> __restrict__ q) {
+// q = K p' over the node-row sliced ELL chunks [c0, c1) of an instance with
+// n_nodes node rows, where p' is the direction x(.) evaluates (the stored p,
+// or z + beta p); p' of the own rows is also stored into nxt. Wavefront w of
+// the workgroup takes chunks c0 + w, c0 + w + kWaves, ...; lane l owns node
+// row 64c + l, accumulating in FP64. Every load of a wavefront covers 64
+// consecutive words (values, columns) or the rows' own vector entries; the
+// column gathers hit the caches. Returns this thread's share of p'.q.
+template <int BS, int UNR, typename V, typename X>
+__device__ __forceinline__ double ell_rows(int32_t c0, int32_t c1, int32_t n_nodes, gptr<const int32_t> off,
+                                           gptr<const int32_t> nbs, gptr<const int32_t> col, gptr<const V> val,
+                                           const X& x, gptr<V> nxt, gptr<V> q) {
     const int lane = threadIdx.x % kWave;
     double dot = 0.0;
     for (int32_t c = c0 + (int32_t)(threadIdx.x / kWave); c < c1; c += kWaves) {
@@ -266,13 +287,14 @@ __device__ __forceinline__ double ell_rows(int32_t c0, int32_t c1, int32_t n_nod
         const bool own = r < n_nodes;
         V pr[BS];
 #pragma unroll
-        for (int a = 0; a < BS; ++a) pr[a] = own ? p[r * BS + a] : V(0);
+        for (int a = 0; a < BS; ++a) pr[a] = own ? x(r * BS + a) : V(0);
         double acc[BS], unused[BS];
-        ell_row<BS, UNR, V, false>(nbs[c], col + o + lane, val + o * (BS * BS) + lane, p, acc, unused);
+        ell_row<BS, UNR, V, false>(nbs[c], col + o + lane, val + o * (BS * BS) + lane, x, acc, unused);
         if (own) {
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
                 const V qa = (V)acc[a];
+                nxt[r * BS + a] = pr[a];
                 q[r * BS + a] = qa;
                 dot += (double)pr[a] * (double)qa;
             }
@@ -311,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void spmv_csr64_kernel(int64_t n, const int
     const int lane = threadIdx.x % LANES;
     int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / LANES;
     if (row >= n) return;
-    double acc = csr_row<LANES>(global(rp), global(col), global(val), global(x), (int32_t)row, lane);
+    double acc = csr_row<LANES>(global(rp), global(col), global(val), Plain<double>{global(x)}, (int32_t)row, lane);
     if (lane == 0) y[row] = acc;
 }
 
@@ -354,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
         const int lane = threadIdx.x % W, grp = threadIdx.x / W;
         constexpr int G = kBlock / W;
         for (int32_t row = t.row_begin + grp; row < t.row_end; row += G) {
-            double ku = csr_row<W>(D.row_ptr, D.col, D.val, D.u, row, lane);
+            double ku = csr_row<W>(D.row_ptr, D.col, D.val, Plain<double>{D.u}, row, lane);
             if (lane == 0) {
                 double ri = D.f[row] - ku;
                 double zi = ri * D.dinv[row];
@@ -374,8 +396,8 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
             const int64_t o = D.ell_off[c];
             const int32_t r = c * kEllRows + lane;
             double ku[BS], unused[BS];
-            ell_row<BS, W, double, false>(D.ell_nb[c], D.ell_col + o + lane, D.ell_val + o * (BS * BS) + lane, D.u,
-                                          ku, unused);
+            ell_row<BS, W, double, false>(D.ell_nb[c], D.ell_col + o + lane, D.ell_val + o * (BS * BS) + lane,
+                                          Plain<double>{D.u}, ku, unused);
             if (r < n_nodes) {
 #pragma unroll
                 for (int a = 0; a < BS; ++a) {
@@ -438,6 +460,7 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
         S->rz = v[1];
         S->res = sqrt(v[0]) / S->bnorm;
         S->flags = 0;
+        S->dir = 1;  // p = z is a complete direction
         if (S->res <= S->tol) S->status = ST_CONVERGED;
         return;
     }
@@ -454,6 +477,7 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
     S->evals = 0;
     S->floor = 0.0;
     S->flags = 0;
+    S->dir = 1;  // p = z is a complete direction
     if (S->bnorm == 0.0) {  // solver.rs:93-95
         S->res = 0.0;
         S->status = ST_CONVERGED;
@@ -465,24 +489,42 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
 // q, r, u, z and p are distinct arrays, and saying so lets the compiler issue
 // every load of a row before its stores and keep stores out of the waits.
 
-// q = K p over CSR rows [b0, b1) of a raw CSR instance; returns this
-// thread's share of p.q.
-template <int LANES>
-__device__ __forceinline__ double csr64_rows(int32_t b0, int32_t b1, gptr<const int32_t> __restrict__ rp,
-                                             gptr<const int32_t> __restrict__ col, gptr<const double> __restrict__ val,
-                                             gptr<const double> __restrict__ p, gptr<double> __restrict__ q) {
+// q = K p' over CSR rows [b0, b1) of a raw CSR instance, p' as x(.) evaluates
+// it, stored into nxt for the own rows; this thread's share of p'.q.
+template <int LANES, typename V, typename X>
+__device__ __forceinline__ double csr_rows(int32_t b0, int32_t b1, gptr<const int32_t> rp, gptr<const int32_t> col,
+                                           gptr<const V> val, const X& x, gptr<V> nxt, gptr<V> q) {
     const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
     constexpr int G = kBlock / LANES;
     double dot = 0.0;
     for (int32_t row = b0 + grp; row < b1; row += G) {
-        const double pr = p[row];  // dot operand, issued with the row's loads
-        const double acc = csr_row<LANES>(rp, col, val, p, row, lane);
+        const V pr = x(row);  // dot operand, issued with the row's loads
+        V qv;
+        if constexpr (sizeof(V) == 8)
+            qv = (V)csr_row<LANES>(rp, col, (gptr<const double>)val, x, row, lane);
+        else
+            qv = (V)csr32_row<LANES>(rp, col, (gptr<const float>)val, x, row, lane);
         if (lane == 0) {
-            q[row] = acc;
-            dot += pr * acc;
+            nxt[row] = pr;
+            q[row] = qv;
+            dot += (double)pr * (double)qv;
         }
     }
     return dot;
+}
+
+// q = K p' over a tile's rows with the direction x(.): CSR rows (BS = 1, W
+// lanes each) or ELL node rows (BS = 2, 3, W blocks in flight).
+template <int BS, int W, typename V, typename X>
+__device__ __forceinline__ double tile_spmv(const Tile& t, const Dev& D, gptr<const V> ell_val, gptr<const V> csr_val,
+                                           const X& x, gptr<V> nxt, gptr<V> q) {
+    if constexpr (BS == 1) {
+        return csr_rows<W, V>(t.row_begin, t.row_end, D.row_ptr, D.col, csr_val, x, nxt, q);
+    } else {
+        int32_t c0, c1;
+        tile_chunks<BS>(t, c0, c1);
+        return ell_rows<BS, W, V>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, ell_val, x, nxt, q);
+    }
 }
 
 // Rows b + threadIdx.x + j * kBlock (j < kPre) of a tile are loaded before
@@ -500,23 +542,57 @@ __device__ __forceinline__ void preload(gptr<const T> a, int32_t b, int32_t e, T
     }
 }
 
-// q = K p and the partial p.q. BS = 1 (raw CSR): one row per W lanes;
-// BS = 2, 3: node-row sliced ELL with W blocks in flight per lane.
+// Direction step and q = K p of one FP64 iteration (solver.rs:100-148 in the
+// order direction, product). The current direction is in P[par] (p, or
+// p_alt for par = 1); the new one goes to P[1 - par], so the gathers of every
+// tile read an array no tile writes in this phase.
+// Pending step: the tile re-reduces r.r and r.z of the last update, tests
+// convergence and the budget, and takes p' = z + beta p, evaluated with the
+// same fused multiply-add at its own rows and at every gathered column.
+// dir set (after an initialisation): the direction is complete, p' = p.
+// force_dir (timing only): beta = 1, no reduction, no scalar writes.
 template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
-                                                           double* __restrict__ partials, int pstride) {
+                                                           double* __restrict__ partials, int pstride, int par,
+                                                           int force_dir) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const Dev D = view(descs[t.inst]);
+    const gptr<const double> cur = par ? D.p_alt : D.p;
+    const gptr<double> nxt = par ? D.p : D.p_alt;
     double v[1];
-    if constexpr (BS == 1) {
-        v[0] = csr64_rows<W>(t.row_begin, t.row_end, D.row_ptr, D.col, D.val, D.p, D.q);
+    if (!force_dir && S->dir) {
+        v[0] = tile_spmv<BS, W, double>(t, D, D.ell_val, D.val, Plain<double>{cur}, nxt, D.q);
     } else {
-        int32_t c0, c1;
-        tile_chunks<BS>(t, c0, c1);
-        v[0] = ell_rows<BS, W, double>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, D.ell_val, D.p, D.q);
+        double beta = 1.0;
+        if (!force_dir) {
+            const double snap_rz = S->snap_rz;
+            const int64_t snap_it = S->snap_it;
+            const int sl[2] = {P_RR, P_RZ};
+            double rr[2];
+            reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, rr);
+            const double res = sqrt(rr[0]) / S->bnorm;
+            const bool conv = res <= S->tol;
+            const int64_t it = snap_it + 1;
+            const bool budget = it >= S->max_iter;
+            beta = rr[1] / snap_rz;
+            if (is_leader(t)) {
+                S->it = it;
+                S->rr = rr[0];
+                S->res = res;
+                if (conv) {
+                    S->status = ST_CONVERGED;
+                } else {
+                    S->beta = beta;
+                    S->rz = rr[1];
+                    if (budget) S->status = ST_MAXITER;
+                }
+            }
+            if (conv || budget) return;
+        }
+        v[0] = tile_spmv<BS, W, double>(t, D, D.ell_val, D.val, Step<double>{D.z, cur, beta}, nxt, D.q);
     }
     const int slot[1] = {P_PQ};
     write_partials<1>(v, slot, partials, pstride, blockIdx.x);
@@ -540,29 +616,22 @@ __device__ __forceinline__ void cg64_update_rows(int32_t b, int32_t e, double al
     }
 }
 
-__device__ __forceinline__ void direction_rows(int32_t b, int32_t e, double beta, gptr<const double> __restrict__ z,
-                                               gptr<double> __restrict__ p) {
-    for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
-        const double zi = z[i], pi = p[i];
-        p[i] = zi + beta * pi;
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
-                                                             double* __restrict__ partials, int pstride) {
+                                                             double* __restrict__ partials, int pstride, int par) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const Dev D = view(descs[t.inst]);
+    const gptr<double> pd = par ? D.p : D.p_alt;
     const int32_t b = t.row_begin, e = t.row_end;
     double p0[kPre], q0[kPre], d0[kPre], u0[kPre], r0[kPre];
-    preload<double>(D.p, b, e, p0);
+    preload<double>(pd, b, e, p0);
     preload<double>(D.q, b, e, q0);
     preload<double>(D.dinv, b, e, d0);
     preload<double>(D.u, b, e, u0);
     preload<double>(D.r, b, e, r0);
-    const double rz = S->rz;  // written by the previous phase's leader only
+    const double rz = S->rz;  // written by the previous phases' leaders only
     const int sp[1] = {P_PQ};
     double pq[1];
     reduce_tiles<1>(partials, pstride, t.tile_begin, t.tile_count, sp, pq);
@@ -581,6 +650,7 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
         S->alpha = alpha;
         S->snap_rz = rz;
         S->snap_it = S->it;
+        S->dir = 0;  // the next SpMV takes the direction step
     }
     double v[2] = {0.0, 0.0};
 #pragma unroll
@@ -596,17 +666,22 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
             v[1] += ri * zi;
         }
     }
-    cg64_update_rows(b + kPre * kBlock, e, alpha, D.p, D.q, D.dinv, D.u, D.r, D.z, v);
+    cg64_update_rows(b + kPre * kBlock, e, alpha, pd, D.q, D.dinv, D.u, D.r, D.z, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
-__global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __restrict__ tiles,
-                                                              const InstDesc* __restrict__ descs,
-                                                              const double* __restrict__ partials, int pstride) {
+// End of a chunk (an even number of iterations, so the current direction is
+// in P[0] = p): the pending direction step of the SpMV (convergence, budget,
+// p = z + beta p) done in place, so that no partial sum and no direction
+// step is pending at a chunk boundary: the tile tables may then be rebuilt
+// (finished instances dropped, new ones added) before the next chunk.
+__global__ __launch_bounds__(kBlock) void cg64_chunk_end_kernel(const Tile* __restrict__ tiles,
+                                                                const InstDesc* __restrict__ descs,
+                                                                const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH) || S->dir) return;
     const Dev D = view(descs[t.inst]);
     const int32_t b = t.row_begin, e = t.row_end;
     double z0[kPre], p0[kPre];
@@ -619,9 +694,10 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
     reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, v);
     const double res = sqrt(v[0]) / S->bnorm;
     const bool conv = res <= S->tol;
+    const int64_t it = snap_it + 1;
+    const bool budget = it >= S->max_iter;
     const double beta = v[1] / snap_rz;
     if (is_leader(t)) {
-        const int64_t it = snap_it + 1;
         S->it = it;
         S->rr = v[0];
         S->res = res;
@@ -630,54 +706,70 @@ __global__ __launch_bounds__(kBlock) void cg64_pupdate_kernel(const Tile* __rest
         } else {
             S->beta = beta;
             S->rz = v[1];
-            if (it >= S->max_iter) S->status = ST_MAXITER;
+            S->dir = 1;
+            if (budget) S->status = ST_MAXITER;
         }
     }
-    if (conv) return;
+    if (conv || budget) return;
 #pragma unroll
     for (int j = 0; j < kPre; ++j) {
         const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
-        if (i < e) D.p[i] = z0[j] + beta * p0[j];
+        if (i < e) D.p[i] = fmadd(beta, p0[j], z0[j]);
     }
-    direction_rows(b + kPre * kBlock, e, beta, D.z, D.p);
+    for (int32_t i = b + kPre * kBlock + (int32_t)threadIdx.x; i < e; i += kBlock) D.p[i] = fmadd(beta, D.p[i], D.z[i]);
 }
 
 // ------------------------------------------------------------ FP32-IR
-// q32 = K32 p32 over CSR rows [b0, b1) of a raw CSR instance (FP32 copy of the
-// values, FP64 accumulation); this thread's share of p.q.
-template <int LANES>
-__device__ __forceinline__ double csr32_rows(int32_t b0, int32_t b1, gptr<const int32_t> __restrict__ rp,
-                                             gptr<const int32_t> __restrict__ col, gptr<const float> __restrict__ val,
-                                             gptr<const float> __restrict__ p, gptr<float> __restrict__ q) {
-    const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
-    constexpr int G = kBlock / LANES;
-    double dot = 0.0;
-    for (int32_t row = b0 + grp; row < b1; row += G) {
-        const float pr = p[row];
-        const float qv = (float)csr32_row<LANES>(rp, col, val, p, row, lane);
-        if (lane == 0) {
-            q[row] = qv;
-            dot += (double)pr * (double)qv;
-        }
-    }
-    return dot;
-}
-
+// Direction step and q32 = K32 p32 of one FP32-IR inner iteration: the FP64
+// PCG scheme above on the FP32 vectors and the FP32 copy of K (ELL, or the
+// FP32 CSR values of a raw CSR instance), FP64 accumulation. The pending step
+// also takes the refresh decision of the reliable-update scheme: refresh at
+// the end of the period when the recursive residual fell below ru_delta
+// times its maximum since the last refresh, reached tol, or went
+// kForcedRefresh iterations unchecked; at the budget the instance holds
+// (FL_HOLD) for the final FP64 check.
 template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
-                                                           double* __restrict__ partials, int pstride) {
+                                                           double* __restrict__ partials, int pstride, int par,
+                                                           int force_dir) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const Dev D = view(descs[t.inst]);
+    const gptr<const float> cur = par ? D.p32_alt : D.p32;
+    const gptr<float> nxt = par ? D.p32 : D.p32_alt;
     double v[1];
-    if constexpr (BS == 1) {
-        v[0] = csr32_rows<W>(t.row_begin, t.row_end, D.brow_ptr, D.bcol, D.bval, D.p32, D.q32);
+    if (!force_dir && S->dir) {
+        v[0] = tile_spmv<BS, W, float>(t, D, D.ell_val32, D.bval, Plain<float>{cur}, nxt, D.q32);
     } else {
-        int32_t c0, c1;
-        tile_chunks<BS>(t, c0, c1);
-        v[0] = ell_rows<BS, W, float>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, D.ell_val32, D.p32, D.q32);
+        float beta = 1.0f;
+        if (!force_dir) {
+            const int64_t snap_it = S->snap_it;
+            const double snap_rmax = S->snap_rmax, snap_rz = S->snap_rz;
+            const int32_t snap_flags = S->snap_flags;
+            const int sl[2] = {P_RR, P_RZ};
+            double rr[2];
+            reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, rr);
+            const int64_t it = snap_it + 1;
+            const double est = sqrt(rr[0]);
+            const double rmax = fmax(snap_rmax, est);
+            const double beta_d = rr[1] / snap_rz;
+            const bool hold = it >= S->max_iter;
+            const bool refresh = hold || (snap_flags & FL_REFRESH) || est / S->bnorm <= S->tol ||
+                                 est <= S->ru_delta * rmax || it - S->it_refresh >= kForcedRefresh;
+            if (is_leader(t)) {
+                S->it = it;
+                S->rr = rr[0];
+                S->rz = rr[1];
+                S->beta = beta_d;
+                S->rmax = rmax;
+                S->flags = snap_flags | (refresh ? FL_REFRESH : 0) | (hold ? FL_HOLD : 0);
+            }
+            if (hold) return;
+            beta = (float)beta_d;
+        }
+        v[0] = tile_spmv<BS, W, float>(t, D, D.ell_val32, D.bval, Step<float>{D.z32, cur, beta}, nxt, D.q32);
     }
     const int slot[1] = {P_PQ};
     write_partials<1>(v, slot, partials, pstride, blockIdx.x);
@@ -704,7 +796,7 @@ __device__ __forceinline__ void direction32_rows(int32_t b, int32_t e, float bet
                                                  gptr<float> __restrict__ p) {
     for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
         const float zi = z[i], pi = p[i];
-        p[i] = zi + beta * pi;
+        p[i] = fmadd(beta, pi, zi);
     }
 }
 
@@ -718,16 +810,19 @@ __device__ __forceinline__ void fold_rows(int32_t b, int32_t e, gptr<float> __re
     }
 }
 
+// x32 += alpha p32, r32 -= alpha q32, z32 = D^-1 r32 and the partials, with
+// p32 the direction the SpMV of this iteration stored (P[1 - par]).
 __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
-                                                             double* __restrict__ partials, int pstride) {
+                                                             double* __restrict__ partials, int pstride, int par) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
     const Dev D = view(descs[t.inst]);
+    const gptr<float> pd = par ? D.p32 : D.p32_alt;
     const int32_t b = t.row_begin, e = t.row_end;
     float p0[kPre], q0[kPre], d0[kPre], x0[kPre], r0[kPre];
-    preload<float>(D.p32, b, e, p0);
+    preload<float>(pd, b, e, p0);
     preload<float>(D.q32, b, e, q0);
     preload<float>(D.dinv32, b, e, d0);
     preload<float>(D.x32, b, e, x0);
@@ -753,6 +848,7 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
         S->snap_rmax = S->rmax;
         S->snap_it = S->it;
         S->snap_flags = S->flags;
+        S->dir = 0;  // the next SpMV (or the period end) takes the direction step
     }
     const float alpha = (float)alpha_d;
     double v[2] = {0.0, 0.0};
@@ -769,18 +865,19 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
             v[1] += (double)ri * (double)zi;
         }
     }
-    ir32_update_rows(b + kPre * kBlock, e, alpha, D.p32, D.q32, D.dinv32, D.x32, D.r32, D.z32, v);
+    ir32_update_rows(b + kPre * kBlock, e, alpha, pd, D.q32, D.dinv32, D.x32, D.r32, D.z32, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
-// p = z + beta p and the refresh decision. At the last iteration of a period
-// an instance flagged for refresh skips the update: its direction is rebuilt
-// from the replaced residual.
-__global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __restrict__ tiles,
-                                                              const InstDesc* __restrict__ descs,
-                                                              const double* __restrict__ partials, int pstride,
-                                                              int period_end) {
+// End of a period (an even number of iterations, so the current direction is
+// in P[0] = p32): the pending step's bookkeeping and refresh decision as in
+// the SpMV; an instance flagged for refresh keeps its direction (it is
+// rebuilt from the replaced residual), any other takes p32 = z32 + beta p32
+// in place and marks it complete.
+__global__ __launch_bounds__(kBlock) void ir32_period_end_kernel(const Tile* __restrict__ tiles,
+                                                                 const InstDesc* __restrict__ descs,
+                                                                 const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
@@ -809,13 +906,14 @@ __global__ __launch_bounds__(kBlock) void ir32_pupdate_kernel(const Tile* __rest
         S->beta = beta;
         S->rmax = rmax;
         S->flags = snap_flags | (refresh ? FL_REFRESH : 0) | (hold ? FL_HOLD : 0);
+        if (!refresh) S->dir = 1;
     }
-    if (hold || (period_end && refresh)) return;
+    if (refresh) return;
     const float bf = (float)beta;
 #pragma unroll
     for (int j = 0; j < kPre; ++j) {
         const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
-        if (i < e) D.p32[i] = z0[j] + bf * p0[j];
+        if (i < e) D.p32[i] = fmadd(bf, p0[j], z0[j]);
     }
     direction32_rows(b + kPre * kBlock, e, bf, D.z32, D.p32);
 }
@@ -896,7 +994,7 @@ __device__ __forceinline__ void ell_residual_rows(int32_t c0, int32_t c1, int32_
             di[a] = own ? dinv32[r * BS + a] : 0.0f;
         }
         double ku[BS], aku[BS];
-        ell_row<BS, UNR, double, true>(nbs[c], col + o + lane, val + o * (BS * BS) + lane, u, ku, aku);
+        ell_row<BS, UNR, double, true>(nbs[c], col + o + lane, val + o * (BS * BS) + lane, Plain<double>{u}, ku, aku);
         if (own) {
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
@@ -979,6 +1077,7 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
             S->rz = v[1];
             S->beta = beta;
             S->flags = 0;
+            S->dir = 1;  // the direction below is complete
         }
     }
     if (!cont) return;
@@ -1057,33 +1156,39 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
         default: MACRO(4); break;          \
     }
 
-void launch_cg64_spmv(const KrylovSet& k, hipStream_t s) {
+void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {
     if (k.bs == 3) {
-#define L(U) cg64_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(U) \
+    cg64_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(U) cg64_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(U) \
+    cg64_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
-#define L(LN) cg64_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(LN) \
+    cg64_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }
 }
 
-void launch_ir32_spmv(const KrylovSet& k, hipStream_t s) {
+void launch_ir32_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {
     if (k.bs == 3) {
-#define L(U) ir32_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(U) \
+    ir32_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(U) ir32_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(U) \
+    ir32_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
-#define L(LN) ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(LN) \
+    ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }
@@ -1127,11 +1232,16 @@ void cg64_init(const KrylovSet& k, hipStream_t s) {
     SVX_HIP(hipGetLastError());
 }
 
-void cg64_iteration(const KrylovSet& k, hipStream_t s) {
+void cg64_iteration(const KrylovSet& k, hipStream_t s, int par) {
     if (k.n_tiles <= 0) return;
-    launch_cg64_spmv(k, s);
-    cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
-    cg64_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    launch_cg64_spmv(k, s, par, 0);
+    cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par);
+    SVX_HIP(hipGetLastError());
+}
+
+void cg64_chunk_end(const KrylovSet& k, hipStream_t s) {
+    if (k.n_tiles <= 0) return;
+    cg64_chunk_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
 
@@ -1142,11 +1252,16 @@ void ir32_init(const KrylovSet& k, hipStream_t s) {
     SVX_HIP(hipGetLastError());
 }
 
-void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end) {
+void ir32_iteration(const KrylovSet& k, hipStream_t s, int par) {
     if (k.n_tiles <= 0) return;
-    launch_ir32_spmv(k, s);
-    ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
-    ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, period_end ? 1 : 0);
+    launch_ir32_spmv(k, s, par, 0);
+    ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par);
+    SVX_HIP(hipGetLastError());
+}
+
+void ir32_period_end(const KrylovSet& k, hipStream_t s) {
+    if (k.n_tiles <= 0) return;
+    ir32_period_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
 
@@ -1189,27 +1304,24 @@ void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s, in
 void update_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     if (precision == SVX_FP64)
-        cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+        cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, 0);
     else
-        ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+        ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, 0);
     SVX_HIP(hipGetLastError());
 }
 
-void pupdate_phase(const KrylovSet& k, int precision, hipStream_t s) {
-    if (k.n_tiles <= 0) return;
-    if (precision == SVX_FP64)
-        cg64_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
-    else
-        ir32_pupdate_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, 0);
+void period_end_phase(const KrylovSet& k, int precision, hipStream_t s) {
+    if (k.n_tiles <= 0 || precision == SVX_FP64) return;
+    ir32_period_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
     SVX_HIP(hipGetLastError());
 }
 
 void spmv_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     if (precision == SVX_FP64)
-        launch_cg64_spmv(k, s);
+        launch_cg64_spmv(k, s, 0, 1);
     else
-        launch_ir32_spmv(k, s);
+        launch_ir32_spmv(k, s, 0, 1);
     SVX_HIP(hipGetLastError());
 }
 

@@ -133,8 +133,8 @@ int64_t footprint(const svx_mesh& m, int precision) {
     b += 4 * (n + 1) + 4 * nnz + 4 * (nn + 1) + 4 * m.nblk;                        // CSR + node-block patterns
     b += 4 * ell + 8 * ((int64_t)m.hp.ell_nb.size() + 1);                          // ELL pattern
     b += 8 * (nn + 1) + 5 * (int64_t)m.hp.inc_elem.size() + 16 * ne;              // incidence, element offsets
-    b += 8 * nnz + 8 * ell_vals + 8 * n * 9 + 4 * n;                              // CSR + ELL values, vectors, dinv32
-    if (precision == SVX_FP32_IR) b += 4 * ell_vals + 4 * n * 5;                   // FP32 ELL values + vectors
+    b += 8 * nnz + 8 * ell_vals + 8 * n * 10 + 4 * n;                             // CSR + ELL values, vectors, dinv32
+    if (precision == SVX_FP32_IR) b += 4 * ell_vals + 4 * n * 6;                   // FP32 ELL values + vectors
     b += 8 * std::max(m.ke_total, m.ns_total + nn * 6);                            // transient assembly / recovery
     return b + (1 << 16);
 }

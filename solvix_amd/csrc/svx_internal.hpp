@@ -97,7 +97,7 @@ struct alignas(16) InstScal {
     int32_t status;     // ST_*
     int32_t flags;      // FL_*
     int32_t stall;      // FP32-IR refreshes without halving res_best
-    int32_t pad0;
+    int32_t dir;        // 1: the direction in P[0] is complete (after init / refresh); 0: the next SpMV takes the step
     // snapshots
     double snap_rz;
     double snap_rmax;
@@ -146,6 +146,8 @@ struct InstDesc {
     float* p32;
     float* z32;
     float* q32;
+    double* p_alt;    // second direction buffer (the SpMV writes the new direction there)
+    float* p32_alt;
     InstScal* s;
 };
 
@@ -231,10 +233,23 @@ struct KrylovSet {
     int unroll = 2;              // blocks in flight per lane (ELL instances, bs = 2, 3)
     int bs = 0;                  // node-block size shared by the set's instances (1, 2, 3)
 };
+// An iteration is two launches: the direction step fused into the SpMV, then
+// the update. The direction is double-buffered: iteration parity par reads
+// the current direction from P[par] and writes the new one to P[1 - par]
+// (P[0] = p / p32, P[1] = p_alt / p32_alt); a chunk starts with par = 0 and
+// holds an even number of iterations, so the current direction is in P[0]
+// at every chunk and period boundary.
 void cg64_init(const KrylovSet& k, hipStream_t s);
-void cg64_iteration(const KrylovSet& k, hipStream_t s);
+void cg64_iteration(const KrylovSet& k, hipStream_t s, int par);
+// After the last iteration of an FP64 chunk: the pending direction step in
+// place, so that nothing is pending across a chunk boundary.
+void cg64_chunk_end(const KrylovSet& k, hipStream_t s);
 void ir32_init(const KrylovSet& k, hipStream_t s);
-void ir32_iteration(const KrylovSet& k, hipStream_t s, bool period_end);
+void ir32_iteration(const KrylovSet& k, hipStream_t s, int par);
+// After the last iteration of an FP32-IR period: refresh decision, and the
+// direction step of instances that do not refresh (a chunk is whole periods,
+// so nothing is pending across a chunk boundary either).
+void ir32_period_end(const KrylovSet& k, hipStream_t s);
 void ir32_refresh(const KrylovSet& k, hipStream_t s);
 // out[0] = running instances, out[1] = their tiles in the current tables.
 void launch_count_running(const InstDesc* descs, int32_t n_inst, int32_t* out /* [2] */, hipStream_t s);
@@ -242,13 +257,15 @@ void launch_count_running(const InstDesc* descs, int32_t n_inst, int32_t* out /*
 // new tile g takes old tile src[g] (src[g] < 0: left untouched).
 void launch_carry_partials(const double* old_p, int old_stride, double* new_p, int new_stride, const int32_t* src,
                            int n_new, hipStream_t s);
-// The SpMV phase of one iteration alone (cg64 / ir32), for timing.
+// The SpMV phase of one iteration alone (cg64 / ir32, direction step with
+// beta = 1, no reduction, no scalar writes), for timing.
 void spmv_phase(const KrylovSet& k, int precision, hipStream_t s);
 // Set every instance of the set back to ST_RUNNING (timing helper).
 void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s, int32_t flags = 0);
-// The update and direction phases of one iteration alone, for timing.
+// The update phase of one iteration, and the FP32-IR period end, alone, for
+// timing.
 void update_phase(const KrylovSet& k, int precision, hipStream_t s);
-void pupdate_phase(const KrylovSet& k, int precision, hipStream_t s);
+void period_end_phase(const KrylovSet& k, int precision, hipStream_t s);
 // ST_FALLBACK instances -> ST_RUNNING with FL_FRESH | FL_WARM.
 void launch_promote_fallback(const InstDesc* descs, int32_t n_inst, hipStream_t s);
 // Copy every instance's scalars into a contiguous array.
