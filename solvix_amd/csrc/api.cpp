@@ -368,7 +368,7 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     if (!sys->engine) sys->engine = std::make_unique<Engine>(s);
     EngineShape shape;
     const int bs = m->ndof;
-    shape.rows_per_tile[bs] = tp.rows_per_tile > 0 ? tp.rows_per_tile : default_rows_per_tile(bs, precision);
+    shape.rows_per_tile[bs] = tp.rows_per_tile;  // 0: the instance's default
     shape.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;

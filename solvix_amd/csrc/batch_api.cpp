@@ -35,7 +35,7 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
     (void)systems;
     EngineShape sh;
     for (int bs = 1; bs <= 3; ++bs) {
-        sh.rows_per_tile[bs] = tp.rows_per_tile > 0 ? tp.rows_per_tile : default_rows_per_tile(bs, precision);
+        sh.rows_per_tile[bs] = tp.rows_per_tile;  // 0: each instance's default
         sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : 16;  // CSR rows of the FP64 residual and warm start
         sh.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
     }
@@ -93,6 +93,17 @@ int64_t svx_batch_size(const svx_batch* b) { return b ? (int64_t)b->sys.size() :
 }  // extern "C"
 
 namespace {
+// Device time of one iteration of the whole set: the SpMV (with its
+// direction step) and the update, plus the FP32-IR period end spread over
+// the default period of 8 iterations. The tile size moves work between the
+// SpMV and the partial-sum reductions of every phase, so the tuner times all
+// of them.
+double iteration_ms(Engine& e, int precision, int reps) {
+    double ms[4];
+    e.time_phases(precision, reps, ms);
+    return ms[0] + ms[1] + (precision == SVX_FP64 ? 0.0 : ms[2] / 8.0);
+}
+
 // Prepare every system of the batch for `precision`, reset the instance
 // state and return the instance descriptors.
 std::vector<InstDesc> batch_setup(svx_batch* b, int precision) {
@@ -128,10 +139,10 @@ int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* unroll, i
         for (const InstDesc& d : descs) present[d.bs] = true;
         EngineShape best = batch_shape(b->sys, b->ctx->tune, precision);
         b->engine->set_instances(descs, all, best);
-        double best_ms = b->engine->time_spmv(precision, reps);
-        // blocks in flight per lane x ELL chunks per tile (one per wavefront,
-        // or two / four in turn per wavefront)
-        static const int kChunksPerTile[] = {kBlock / 64, 2 * kBlock / 64, 4 * kBlock / 64};
+        double best_ms = iteration_ms(*b->engine, precision, reps);
+        // blocks in flight per lane x ELL chunks per tile (one, split over the
+        // wavefronts; one per wavefront; two or four in turn per wavefront)
+        static const int kChunksPerTile[] = {1, kBlock / 64, 2 * kBlock / 64, 4 * kBlock / 64};
         for (int bs = 2; bs <= 3; ++bs) {
             if (!present[bs]) continue;
             for (int u = 1; u <= kMaxUnroll; ++u)
@@ -140,7 +151,7 @@ int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* unroll, i
                     sh.unroll[bs] = u;
                     sh.rows_per_tile[bs] = c * kEllRows * bs;
                     b->engine->set_instances(descs, all, sh);
-                    const double ms = b->engine->time_spmv(precision, reps);
+                    const double ms = iteration_ms(*b->engine, precision, reps);
                     if (ms < best_ms) best_ms = ms, best = sh;
                 }
         }

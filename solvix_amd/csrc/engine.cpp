@@ -10,13 +10,19 @@ int default_lanes(double avg) {
     return l;
 }
 
-// Node-block instances: one 64-node-row ELL chunk per wavefront, four per
-// workgroup tile. Blocks in flight per lane, measured on the 265,923-DOF C3D8
-// block and on 2e5 / 4e6-DOF plane meshes: one for the FP64 3x3 blocks (nine
-// FP64 loads per block already), three otherwise.
-int default_rows_per_tile(int bs, int precision) {
-    (void)precision;
-    return bs >= 2 ? (kBlock / 64) * kEllRows * bs : 256;
+// Node-block instances: four 64-node-row ELL chunks per workgroup tile, one
+// per wavefront; instances of fewer than kSmallChunks chunks (65,536 node
+// rows) one chunk per tile, its blocks split over the four wavefronts, so
+// that they still put four wavefronts per chunk on the device. Blocks in
+// flight per lane, measured on the 265,923-DOF C3D8 block and on 2e5 /
+// 4e6-DOF plane meshes: one for the FP64 3x3 blocks (nine FP64 loads per
+// block already), three otherwise.
+constexpr int64_t kSmallChunks = 1024;
+
+int default_rows_per_tile(int bs, int64_t n_rows) {
+    if (bs < 2) return 256;
+    const int64_t chunks = (n_rows / bs + kEllRows - 1) / kEllRows;
+    return (chunks >= kSmallChunks ? kBlock / 64 : 1) * kEllRows * bs;
 }
 
 int unroll_for(int bs, int precision) { return (bs == 3 && precision == SVX_FP64) ? 1 : 3; }
@@ -56,14 +62,14 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     tiled_.assign(h_descs_.size(), 0);
     for (int bs = 1; bs <= 3; ++bs) {
         first[bs] = (int)tiles.size();
-        int rpt = shape_.rows_per_tile[bs] > 0 ? shape_.rows_per_tile[bs] : default_rows_per_tile(bs, SVX_FP64);
-        if (bs >= 2) {  // whole ELL chunks
-            const int cr = kEllRows * bs;
-            rpt = std::max(cr, (rpt + cr - 1) / cr * cr);
-        }
         for (size_t i = 0; i < h_descs_.size(); ++i) {
             InstDesc& d = h_descs_[i];
             if (d.bs != bs || !running[i]) continue;
+            int rpt = shape_.rows_per_tile[bs] > 0 ? shape_.rows_per_tile[bs] : default_rows_per_tile(bs, d.n);
+            if (bs >= 2) {  // whole ELL chunks
+                const int cr = kEllRows * bs;
+                rpt = std::max(cr, (rpt + cr - 1) / cr * cr);
+            }
             d.tile_begin = (int32_t)tiles.size() - first[bs];
             d.tile_count = (d.n + rpt - 1) / rpt;
             for (int32_t r = 0; r < d.n; r += rpt)

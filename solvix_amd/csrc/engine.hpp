@@ -24,7 +24,7 @@ struct EngineOptions {
 struct EngineShape {
     int lanes[4] = {0, 0, 0, 0};   // lanes per row (raw CSR, block size 1)
     int unroll[4] = {0, 0, 0, 0};  // blocks in flight per lane (ELL, block sizes 2, 3)
-    int rows_per_tile[4] = {0, 0, 0, 0};  // block sizes 2, 3: rounded to whole 64-node-row chunks
+    int rows_per_tile[4] = {0, 0, 0, 0};  // 0: per-instance default; block sizes 2, 3: whole 64-node-row chunks
 };
 
 class Engine {
@@ -85,10 +85,11 @@ class Engine {
 };
 
 int default_lanes(double avg_row_len);
-// Default tile and ELL unroll for node-block size bs (2, 3) and a precision:
-// functions of (bs, precision) alone, so an instance gets the same launch
-// shape, and the same bits, alone and inside any batch.
-int default_rows_per_tile(int bs, int precision);
+// Default tile of an instance (node-block size bs, n_rows rows) and ELL
+// unroll of a node-block size and precision: functions of the instance alone,
+// so an instance gets the same launch shape, and the same bits, alone and
+// inside any batch.
+int default_rows_per_tile(int bs, int64_t n_rows);
 int unroll_for(int bs, int precision);
 
 }  // namespace svx

@@ -213,26 +213,28 @@ __device__ __forceinline__ double csr32_row(gptr<const int32_t> rp, gptr<const i
     return acc;
 }
 
-// One node row of the node-row sliced ELL layout times x: cl / vl point at
-// the row's first block column / value (chunk offset plus lane), nb is the
-// chunk's padded block count. The loads of UNR blocks are issued together;
-// the sums run in block order whatever UNR, so UNR changes timing only.
+// One node row of the node-row sliced ELL layout times x over the blocks
+// k = k0, k0 + ks, k0 + 2 ks, ... < nb: cl / vl point at the row's first
+// block column / value (chunk offset plus lane), nb is the chunk's padded
+// block count. The loads of UNR blocks are issued together; the sums run in
+// block order whatever UNR, so UNR changes timing only.
 // ABS: also accumulate |K_ij x_j| (the FP64 evaluation floor of f - K x).
 template <int BS, int UNR, typename V, bool ABS, typename X>
-__device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr<const V> vl, const X& x,
-                                        double (&acc)[BS], double (&aacc)[BS]) {
+__device__ __forceinline__ void ell_row(int32_t nb, int32_t k0, int32_t ks, gptr<const int32_t> cl,
+                                        gptr<const V> vl, const X& x, double (&acc)[BS], double (&aacc)[BS]) {
     constexpr int BB = BS * BS;
 #pragma unroll
     for (int a = 0; a < BS; ++a) acc[a] = 0.0, aacc[a] = 0.0;
-    int32_t k = 0;
-    for (; k + UNR <= nb; k += UNR) {
+    int32_t k = k0;
+    for (; k + (UNR - 1) * ks < nb; k += UNR * ks) {
         int32_t j[UNR];
         V w[UNR][BB];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            j[u] = cl[(k + u) * kEllRows];
+            const int32_t ku = k + u * ks;
+            j[u] = cl[ku * kEllRows];
 #pragma unroll
-            for (int m = 0; m < BB; ++m) w[u][m] = vl[((k + u) * BB + m) * kEllRows];
+            for (int m = 0; m < BB; ++m) w[u][m] = vl[(ku * BB + m) * kEllRows];
         }
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
@@ -249,7 +251,7 @@ __device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr
                 }
         }
     }
-    for (; k < nb; ++k) {
+    for (; k < nb; k += ks) {
         const int32_t jj = cl[k * kEllRows];
         V w[BB];
 #pragma unroll
@@ -268,38 +270,76 @@ __device__ __forceinline__ void ell_row(int32_t nb, gptr<const int32_t> cl, gptr
     }
 }
 
-// q = K p' over the node-row sliced ELL chunks [c0, c1) of an instance with
-// n_nodes node rows, where p' is the direction x(.) evaluates (the stored p,
-// or z + beta p); p' of the own rows is also stored into nxt. Wavefront w of
-// the workgroup takes chunks c0 + w, c0 + w + kWaves, ...; lane l owns node
-// row 64c + l, accumulating in FP64. Every load of a wavefront covers 64
-// consecutive words (values, columns) or the rows' own vector entries; the
-// column gathers hit the caches. Returns this thread's share of p'.q.
+// (K x)[r] (and (|K||x|)[r] with ABS) for every node row r of the ELL chunks
+// [c0, c1) of an instance with n_nodes node rows, handed to epi(r, acc, aacc)
+// (lanes past the last node row are skipped). Lane l of a wavefront owns node
+// row 64c + l; every load of a wavefront covers 64 consecutive words (values,
+// columns) and the column gathers hit the caches. Four or more chunks: one
+// chunk per wavefront at a time. One or two chunks (the tiles of a small
+// instance): the S = 4 or 2 wavefronts of a chunk take every S-th block each
+// and the owner adds their sums in LDS in wavefront order, so a small
+// instance still spreads over all four wavefronts. The split is a function of
+// the tile, hence of the instance alone.
+template <int BS, int UNR, typename V, bool ABS, typename X, typename E>
+__device__ __forceinline__ void ell_tile(int32_t c0, int32_t c1, int32_t n_nodes, gptr<const int32_t> off,
+                                         gptr<const int32_t> nbs, gptr<const int32_t> col, gptr<const V> val,
+                                         const X& x, E&& epi) {
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int32_t nc = c1 - c0;
+    double acc[BS], aacc[BS];
+    if (nc > 2) {
+        for (int32_t c = c0 + w; c < c1; c += kWaves) {
+            const int64_t o = off[c];
+            ell_row<BS, UNR, V, ABS>(nbs[c], 0, 1, col + o + lane, val + o * (BS * BS) + lane, x, acc, aacc);
+            const int32_t r = c * kEllRows + lane;
+            if (r < n_nodes) epi(r, acc, aacc);
+        }
+        return;
+    }
+    constexpr int NS = ABS ? 2 * BS : BS;
+    __shared__ double red[kWaves][NS][kWave];
+    const int S = nc == 1 ? kWaves : kWaves / 2;
+    const int32_t c = c0 + w / S;
+    const int32_t sw = w % S;
+    const int64_t o = off[c];
+    ell_row<BS, UNR, V, ABS>(nbs[c], sw, S, col + o + lane, val + o * (BS * BS) + lane, x, acc, aacc);
+#pragma unroll
+    for (int a = 0; a < BS; ++a) {
+        red[w][a][lane] = acc[a];
+        if constexpr (ABS) red[w][BS + a][lane] = aacc[a];
+    }
+    __syncthreads();
+    if (sw == 0) {
+        for (int t = 1; t < S; ++t)
+#pragma unroll
+            for (int a = 0; a < BS; ++a) {
+                acc[a] += red[w + t][a][lane];
+                if constexpr (ABS) aacc[a] += red[w + t][BS + a][lane];
+            }
+        const int32_t r = c * kEllRows + lane;
+        if (r < n_nodes) epi(r, acc, aacc);
+    }
+}
+
+// q = K p' over the ELL chunks [c0, c1), where p' is the direction x(.)
+// evaluates (the stored p, or z + beta p); p' of the own rows is stored into
+// nxt. Returns this thread's share of p'.q.
 template <int BS, int UNR, typename V, typename X>
 __device__ __forceinline__ double ell_rows(int32_t c0, int32_t c1, int32_t n_nodes, gptr<const int32_t> off,
                                            gptr<const int32_t> nbs, gptr<const int32_t> col, gptr<const V> val,
                                            const X& x, gptr<V> nxt, gptr<V> q) {
-    const int lane = threadIdx.x % kWave;
     double dot = 0.0;
-    for (int32_t c = c0 + (int32_t)(threadIdx.x / kWave); c < c1; c += kWaves) {
-        const int64_t o = off[c];
-        const int32_t r = c * kEllRows + lane;
-        const bool own = r < n_nodes;
-        V pr[BS];
+    ell_tile<BS, UNR, V, false>(c0, c1, n_nodes, off, nbs, col, val, x,
+                                [&](int32_t r, const double (&acc)[BS], const double (&)[BS]) {
 #pragma unroll
-        for (int a = 0; a < BS; ++a) pr[a] = own ? x(r * BS + a) : V(0);
-        double acc[BS], unused[BS];
-        ell_row<BS, UNR, V, false>(nbs[c], col + o + lane, val + o * (BS * BS) + lane, x, acc, unused);
-        if (own) {
-#pragma unroll
-            for (int a = 0; a < BS; ++a) {
-                const V qa = (V)acc[a];
-                nxt[r * BS + a] = pr[a];
-                q[r * BS + a] = qa;
-                dot += (double)pr[a] * (double)qa;
-            }
-        }
-    }
+                                    for (int a = 0; a < BS; ++a) {
+                                        const V pa = x(r * BS + a);
+                                        const V qa = (V)acc[a];
+                                        nxt[r * BS + a] = pa;
+                                        q[r * BS + a] = qa;
+                                        dot += (double)pa * (double)qa;
+                                    }
+                                });
     return dot;
 }
 
@@ -390,28 +430,21 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
     } else {
         int32_t c0, c1;
         tile_chunks<BS>(t, c0, c1);
-        const int32_t n_nodes = D.n / BS;
-        const int lane = threadIdx.x % kWave;
-        for (int32_t c = c0 + (int32_t)(threadIdx.x / kWave); c < c1; c += kWaves) {
-            const int64_t o = D.ell_off[c];
-            const int32_t r = c * kEllRows + lane;
-            double ku[BS], unused[BS];
-            ell_row<BS, W, double, false>(D.ell_nb[c], D.ell_col + o + lane, D.ell_val + o * (BS * BS) + lane,
-                                          Plain<double>{D.u}, ku, unused);
-            if (r < n_nodes) {
+        ell_tile<BS, W, double, false>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, D.ell_val,
+                                       Plain<double>{D.u},
+                                       [&](int32_t r, const double (&ku)[BS], const double (&)[BS]) {
 #pragma unroll
-                for (int a = 0; a < BS; ++a) {
-                    const int32_t i = r * BS + a;
-                    const double ri = D.f[i] - ku[a];
-                    const double zi = ri * D.dinv[i];
-                    D.r[i] = ri;
-                    D.z[i] = zi;
-                    D.p[i] = zi;
-                    v[0] += ri * ri;
-                    v[1] += ri * zi;
-                }
-            }
-        }
+                                           for (int a = 0; a < BS; ++a) {
+                                               const int32_t i = r * BS + a;
+                                               const double ri = D.f[i] - ku[a];
+                                               const double zi = ri * D.dinv[i];
+                                               D.r[i] = ri;
+                                               D.z[i] = zi;
+                                               D.p[i] = zi;
+                                               v[0] += ri * ri;
+                                               v[1] += ri * zi;
+                                           }
+                                       });
     }
     const int slot[2] = {P_IFF, P_IRZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
@@ -681,7 +714,10 @@ __global__ __launch_bounds__(kBlock) void cg64_chunk_end_kernel(const Tile* __re
                                                                 const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH) || S->dir) return;
+    // A running instance always has its direction step pending here (every
+    // chunk ends with an update). S->dir is not read: this kernel's leader
+    // sets it, and a tile that read the new value would skip its rows.
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
     const Dev D = view(descs[t.inst]);
     const int32_t b = t.row_begin, e = t.row_end;
     double z0[kPre], p0[kPre];
@@ -971,45 +1007,26 @@ __device__ __forceinline__ void residual_rows(int32_t b, int32_t e, gptr<const i
     }
 }
 
-// The same over ELL node rows [c0, c1) of a node-block instance.
+// The same over the ELL chunks [c0, c1) of a node-block instance.
 template <int BS, int UNR>
-__device__ __forceinline__ void ell_residual_rows(int32_t c0, int32_t c1, int32_t n_nodes,
-                                                  gptr<const int32_t> __restrict__ off,
-                                                  gptr<const int32_t> __restrict__ nbs,
-                                                  gptr<const int32_t> __restrict__ col,
-                                                  gptr<const double> __restrict__ val,
-                                                  gptr<const double> __restrict__ u, gptr<const double> __restrict__ f,
-                                                  gptr<const float> __restrict__ dinv32, gptr<float> __restrict__ r32,
-                                                  gptr<float> __restrict__ z32, double (&v)[3]) {
-    const int lane = threadIdx.x % kWave;
-    for (int32_t c = c0 + (int32_t)(threadIdx.x / kWave); c < c1; c += kWaves) {
-        const int64_t o = off[c];
-        const int32_t r = c * kEllRows + lane;
-        const bool own = r < n_nodes;
-        double fi[BS];
-        float di[BS];
+__device__ __forceinline__ void ell_residual_rows(int32_t c0, int32_t c1, const Dev& D, double (&v)[3]) {
+    ell_tile<BS, UNR, double, true>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, D.ell_val, Plain<double>{D.u},
+                                    [&](int32_t r, const double (&ku)[BS], const double (&aku)[BS]) {
 #pragma unroll
-        for (int a = 0; a < BS; ++a) {
-            fi[a] = own ? f[r * BS + a] : 0.0;
-            di[a] = own ? dinv32[r * BS + a] : 0.0f;
-        }
-        double ku[BS], aku[BS];
-        ell_row<BS, UNR, double, true>(nbs[c], col + o + lane, val + o * (BS * BS) + lane, Plain<double>{u}, ku, aku);
-        if (own) {
-#pragma unroll
-            for (int a = 0; a < BS; ++a) {
-                const double ri = fi[a] - ku[a];
-                const float rs = (float)ri;
-                const float zs = rs * di[a];
-                r32[r * BS + a] = rs;
-                z32[r * BS + a] = zs;
-                v[0] += ri * ri;
-                v[1] += (double)rs * (double)zs;
-                const double aa = aku[a] + fabs(fi[a]);
-                v[2] += aa * aa;
-            }
-        }
-    }
+                                        for (int a = 0; a < BS; ++a) {
+                                            const int32_t i = r * BS + a;
+                                            const double fi = D.f[i];
+                                            const double ri = fi - ku[a];
+                                            const float rs = (float)ri;
+                                            const float zs = rs * D.dinv32[i];
+                                            D.r32[i] = rs;
+                                            D.z32[i] = zs;
+                                            v[0] += ri * ri;
+                                            v[1] += (double)rs * (double)zs;
+                                            const double aa = aku[a] + fabs(fi);
+                                            v[2] += aa * aa;
+                                        }
+                                    });
 }
 
 template <int BS, int W>
@@ -1027,8 +1044,7 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
     } else {
         int32_t c0, c1;
         tile_chunks<BS>(t, c0, c1);
-        ell_residual_rows<BS, W>(c0, c1, D.n / BS, D.ell_off, D.ell_nb, D.ell_col, D.ell_val, D.u, D.f, D.dinv32,
-                                 D.r32, D.z32, v);
+        ell_residual_rows<BS, W>(c0, c1, D, v);
     }
     const int slot[3] = {P_TRR, P_TRZ, P_TFL};
     write_partials<3>(v, slot, partials, pstride, blockIdx.x);

@@ -87,3 +87,34 @@ def test_autotuned_batch_matches_oracle(precision, ctx):
         e = rel_err(s.solution(), ref.u)
         print(f"{m.name} [{precision}] its {its} (oracle {ref.iterations}) err u {e:.1e}")
         assert ok and e <= 1e-8
+
+
+@pytest.mark.parametrize("rows_per_tile", [128, 1024, 2048], ids=["one_chunk", "8_chunks", "16_chunks"])
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+def test_many_tiles_equal_solo(precision, rows_per_tile, ctx):
+    """A batch with far more workgroup tiles than the device holds at once
+    (so tiles of one instance start after others of that instance have
+    finished) and tiles longer than a workgroup's preloaded rows: every
+    result equals the instance's solo solve with the same shape bit for bit.
+    Regression test for a kernel whose tiles read a scalar its own leader
+    tile writes in the same launch."""
+    models = [decks.plane_instance(decks.SplitMix64(0x7113 ^ i), 2e4, 6e4) for i in range(24)]
+    try:
+        ctx.set_tuning(rows_per_tile=rows_per_tile)
+        solo = []
+        for m in models:
+            mesh, s = _system(m, ctx)
+            solo.append(s.solve(1e-10, 0, precision))
+        keep = []
+        b = Batch(ctx)
+        for m in models:
+            mesh, s = _system(m, ctx)
+            keep.append((mesh, s))
+            b.add(s)
+        results = b.solve(precision)
+    finally:
+        ctx.set_tuning()
+    for (mesh, s), (u0, its0, res0), (its, res, ok), m in zip(keep, solo, results, models):
+        print(f"{m.name} [{precision}] rows/tile {rows_per_tile}: its {its} (solo {its0}) res {res:.2e}")
+        assert ok and its == its0 and res == res0, m.name
+        assert np.array_equal(s.solution().view(np.uint64), u0.view(np.uint64)), m.name
