@@ -298,8 +298,8 @@ int svx_batch_result(const svx_batch* batch, int64_t slot,
                      int32_t* converged);
 /* SpMV autotuner: times one solver iteration (SpMV with its direction step,
  * update, and the FP32-IR period end per iteration) over the whole batch for
- * blocks in flight per lane in {1, 2, 3, 4} and 1, 4, 8 or 16 ELL chunks
- * (64 node rows each) per workgroup tile, one node-block size at a time, and
+ * blocks in flight per lane in {1, 2, 3, 4} and 1 or 4 ELL chunks (64 node
+ * rows each) per workgroup tile, one node-block size at a time, and
  * keeps the fastest shape for later svx_batch_solve calls with this precision
  * (until svx_batch_add). unroll[bs] / rows_per_tile[bs] (arrays of 4,
  * indexed by block size 1..3, or NULL; rows_per_tile 0 = each instance's
