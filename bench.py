@@ -17,6 +17,7 @@ Prints one JSON line.
 """
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -147,6 +148,7 @@ def bench_batch(args):
         return us, np.array(its), np.array(ok)
 
     tuned = {}
+    stats = []  # per-instance solve statistics of the last batched pass
 
     def batched(prec):
         t0 = time.perf_counter()
@@ -170,10 +172,13 @@ def bench_batch(args):
         us = [sy.solution() for sy in systems]
         its = np.array([r[0] for r in res])
         ok = np.array([r[2] for r in res])
+        stats.clear()
+        stats.extend(sy.stats() for sy in systems)
         return b, systems, us, its, ok
 
     # autotune once per precision (untimed; the tuned shape depends on the batch only)
-    for prec in (Precision.FP32_IR, Precision.FP64):
+    precs = (Precision.FP64,) if args.batch_arms == "fp64_batched" else (Precision.FP32_IR, Precision.FP64)
+    for prec in precs:
         systems = [assemble(i) for i in range(len(prep))]
         b = Batch(ctx)
         for sy in systems:
@@ -185,7 +190,10 @@ def bench_batch(args):
             sy.close()
 
     out = {}
-    for label in ("fp64_sequential", "fp64_batched", "fp32_ir_batched"):
+    labels = ("fp64_sequential", "fp64_batched", "fp32_ir_batched")
+    if args.batch_arms == "fp64_batched":  # profiling runs: the batched FP64 engine alone
+        labels = ("fp64_batched",)
+    for label in labels:
         for k in range(args.warmup + args.steps):
             phase.clear()
             t0 = time.perf_counter()
@@ -202,12 +210,13 @@ def bench_batch(args):
                   f"{int(ok.sum())}/{ok.size}, iterations max {int(its.max())} mean {its.mean():.0f}",
                   file=sys.stderr, flush=True)
             if timed:
-                rec = out.setdefault(label, {"dt": 0.0})
+                rec = out.setdefault(label, {"dt": 0.0, "solve_s": 0.0})
                 rec["dt"] += dt
+                rec["solve_s"] += phase.get("solve", 0.0)
                 rec.update(us=us, its=its, ok=ok)
                 if b is not None:
                     ms, by = b.time_spmv(20)
-                    rec.update(spmv_ms=ms, spmv_bytes=by)
+                    rec.update(spmv_ms=ms, spmv_bytes=by, stats=list(stats))
             if b is not None:
                 b.close()
                 for sy in systems:
@@ -216,12 +225,47 @@ def bench_batch(args):
     def dps(rec):
         return float(dofs[rec["ok"]].sum()) * args.steps / rec["dt"]
 
+    if "fp64_sequential" not in out:
+        b64 = out["fp64_batched"]
+        print(json.dumps({"arm": "fp64_batched", "value": round(dps(b64), 1), "unit": "solved DOF/s",
+                          "ms_per_step": round(b64["dt"] / args.steps * 1e3, 3),
+                          "converged": int(b64["ok"].sum())}), flush=True)
+        return
+
     seq = out["fp64_sequential"]
     mix = out["fp32_ir_batched"]
     b64 = out["fp64_batched"]
 
     def err(rec):
         return float(max(np.abs(a - c).max() / max(np.abs(c).max(), 1e-300) for a, c in zip(rec["us"], seq["us"])))
+
+    # Algorithmic bytes of a whole batched solve: every instance's iterations
+    # times the bytes of one iteration (SpMV with its direction step, update,
+    # the per-chunk / per-period direction step), FP32-IR refreshes and FP64
+    # fallback iterations counted as such; over the measured solve time this is
+    # the bandwidth the whole solve sustains, tails and launches included.
+    def solve_bytes(rec, prec):
+        total = 0.0
+        for i, st in enumerate(rec["stats"]):
+            mesh = prep[i][0]
+            bs = models[i].ndof()
+            n, nnz = float(mesh.n), float(mesh.nnz)
+            nblk, ch = nnz / (bs * bs), math.ceil(mesh.n / bs / 64)
+            it64 = 8 * nnz + 4 * nblk + 8 * ch + 32 * n + 64 * n + 24 * n / 32
+            if prec == Precision.FP64:
+                total += st["iterations"] * it64
+                continue
+            it32 = 4 * nnz + 4 * nblk + 8 * ch + 16 * n + 32 * n + 12 * n / 8
+            refresh = 24 * n + (8 * nnz + 4 * nblk + 8 * ch + 28 * n) + 12 * n
+            fp64_its = st["fp64_iterations"]
+            total += (st["iterations"] - fp64_its) * it32 + st["refreshes"] * refresh + fp64_its * it64
+        return total
+
+    def whole(rec, prec):
+        by = solve_bytes(rec, prec)
+        t = rec["solve_s"] / args.steps
+        return {"algorithmic_bytes": by, "solve_s": round(t, 3), "GBps": round(by / t / 1e9, 1),
+                "frac": round(by / t / 1e9 / HBM_PEAK_GBS, 4)}
 
     def roof(rec, name):
         ach = rec["spmv_bytes"] / (rec["spmv_ms"] * 1e-3) / 1e9
@@ -248,12 +292,14 @@ def bench_batch(args):
         "iterations_max": int(mix["its"].max()),
         "max_rel_err_u_vs_fp64": err(mix),
         "roofline": roof(mix, "ir32_spmv_kernel<2,...> (whole batch)"),
+        "whole_solve": whole(mix, Precision.FP32_IR),
         "autotune": {"fp32_ir": tuned[Precision.FP32_IR], "fp64": tuned[Precision.FP64]},
         "fp64_sequential": {"value": round(dps(seq), 1), "ms_per_step": round(seq["dt"] / args.steps * 1e3, 3),
                             "converged": int(seq["ok"].sum()), "iterations_max": int(seq["its"].max())},
         "fp64_batched": {"value": round(dps(b64), 1), "ms_per_step": round(b64["dt"] / args.steps * 1e3, 3),
                          "converged": int(b64["ok"].sum()), "max_rel_err_u_vs_fp64": err(b64),
-                         "roofline": roof(b64, "cg64_spmv_kernel<2,...> (whole batch)")},
+                         "roofline": roof(b64, "cg64_spmv_kernel<2,...> (whole batch)"),
+                         "whole_solve": whole(b64, Precision.FP64)},
         "speedup_vs_fp64": round(dps(mix) / dps(seq), 3),
         "speedup_fp64_batched_vs_fp64": round(dps(b64) / dps(seq), 3),
     }
@@ -427,6 +473,8 @@ def main():
     ap.add_argument("--arms", default="both", choices=["both", "fp64", "fp32_ir"],
                     help="run one precision only (profiling); the JSON line needs both")
     ap.add_argument("--spmv-reps", type=int, default=100, help="launches of the timed SpMV kernel")
+    ap.add_argument("--batch-arms", default="all", choices=["all", "fp64_batched"],
+                    help="batched workload: every arm, or the batched FP64 engine alone (profiling)")
     ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block"],
                     help="multi-RHS workload: run one arm only (profiling)")
     ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
