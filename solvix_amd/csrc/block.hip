@@ -148,34 +148,37 @@ __global__ __launch_bounds__(kBlock) void blk_spmv_kernel(BlockArgs A) {
 // as node blocks of the CSR storage (row a of the node holds block k's
 // entries at val[row_ptr[node*BS] + a*BS*nb + BS*k + b]). Each row still sums
 // its entries in column order, so the results equal the row form's.
+// The node is uniform over the wavefront and the arrays reach the loop as
+// restrict-qualified parameters (the matrix is not written in this kernel),
+// so the block columns and matrix values (the same for all 64 load cases)
+// are scalar loads and the vector memory pipeline carries only the gathers
+// of P and the stores of Q.
 template <int BS>
-__global__ __launch_bounds__(kBlock) void blk_spmv_node_kernel(BlockArgs A) {
-    if (!A.S->any_running) return;
-    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int tile = blockIdx.x;
-    const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
-    const bool on = lane < A.R;
-    const int R = A.R;
+__device__ __forceinline__ double blk_node_rows(int32_t n0, int32_t n1, int w, int lane, int R, bool on,
+                                                const int32_t* __restrict__ row_ptr,
+                                                const int32_t* __restrict__ brow_ptr,
+                                                const int32_t* __restrict__ bcol, const double* __restrict__ val,
+                                                const double* __restrict__ P, double* __restrict__ Q) {
     double pq = 0.0;
-    for (int32_t node = r0 / BS + w; node < r1 / BS; node += kWavesPerBlock) {
+    for (int32_t node = n0 + w; node < n1; node += kWavesPerBlock) {
         double acc[BS];
 #pragma unroll
         for (int a = 0; a < BS; ++a) acc[a] = 0.0;
-        const int32_t kb = A.brow_ptr[node], nb = A.brow_ptr[node + 1] - kb;
-        const int32_t rs = A.row_ptr[node * BS];
+        const int32_t kb = brow_ptr[node], nb = brow_ptr[node + 1] - kb;
+        const int32_t rs = row_ptr[node * BS];
         if (on) {
-            if (A.row_ptr[node * BS + BS] - rs != BS * BS * nb) {  // isolated node: diagonal entries only
+            if (row_ptr[node * BS + BS] - rs != BS * BS * nb) {  // isolated node: diagonal entries only
 #pragma unroll
                 for (int a = 0; a < BS; ++a)
-                    acc[a] = A.val[A.row_ptr[node * BS + a]] * A.P[(size_t)(node * BS + a) * R + lane];
+                    acc[a] = val[row_ptr[node * BS + a]] * P[(size_t)(node * BS + a) * R + lane];
             } else {
-                const double* v0 = A.val + rs;
+                const double* v0 = val + rs;
 #pragma unroll 2
                 for (int32_t k = 0; k < nb; ++k) {
-                    const int32_t j = A.bcol[kb + k];
+                    const int32_t j = bcol[kb + k];
                     double p[BS];
 #pragma unroll
-                    for (int b = 0; b < BS; ++b) p[b] = A.P[(size_t)(j * BS + b) * R + lane];
+                    for (int b = 0; b < BS; ++b) p[b] = P[(size_t)(j * BS + b) * R + lane];
 #pragma unroll
                     for (int a = 0; a < BS; ++a)
 #pragma unroll
@@ -185,11 +188,22 @@ __global__ __launch_bounds__(kBlock) void blk_spmv_node_kernel(BlockArgs A) {
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
                 const size_t o = (size_t)(node * BS + a) * R + lane;
-                A.Q[o] = acc[a];
-                pq += A.P[o] * acc[a];
+                Q[o] = acc[a];
+                pq += P[o] * acc[a];
             }
         }
     }
+    return pq;
+}
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void blk_spmv_node_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
+    const double pq = blk_node_rows<BS>(r0 / BS, r1 / BS, w, lane, A.R, lane < A.R, A.row_ptr, A.brow_ptr, A.bcol,
+                                        A.val, A.P, A.Q);
     write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
 }
 
