@@ -72,6 +72,20 @@ enum { SVX_METHOD_AUTO = 0, SVX_METHOD_PCG = 1, SVX_METHOD_LDLT = 2 };
  *              in FP64 on the FP64 matrix. */
 enum { SVX_FP64 = 0, SVX_FP32_IR = 1 };
 
+/* Operator of the FP64 Krylov iterations (svx_set_operator).
+ * SVX_OP_ASSEMBLED:   the assembled K (Csr::mat_vec, assembly.rs:38-72); the
+ *                     drop-in, whose products sum in the reference's order.
+ * SVX_OP_MATRIX_FREE: systems on plane meshes (ndof 2) of one element kind
+ *                     (CPS3 or CPS4, at most 8 / 4 elements per node) and one
+ *                     material multiply by K without its values: K p is
+ *                     evaluated element by element from the coordinates with
+ *                     the formulas of element.rs:91-211, the same product up
+ *                     to rounding, so iterates differ from the assembled ones
+ *                     in the last bits. Other systems, the FP32 inner
+ *                     iterations of SVX_FP32_IR and residual evaluations keep
+ *                     the assembled K. */
+enum { SVX_OP_ASSEMBLED = 0, SVX_OP_MATRIX_FREE = 1 };
+
 typedef struct svx_error {
     int32_t code;       /* SVX_* status */
     int32_t reason;     /* SVX_ELEM_* for element errors, else 0 */
@@ -180,6 +194,26 @@ const char* svx_last_error(void);
  * counterpart: the autotuner drives it. */
 int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile,
                    int period, int chunk, int use_graph, double ru_delta);
+/* Operator of this context's subsequent FP64 solves (svx_solve,
+ * svx_batch_solve): SVX_OP_ASSEMBLED (default) or SVX_OP_MATRIX_FREE. No
+ * reference counterpart: the reference multiplies its assembled CSR only. */
+int svx_set_operator(svx_ctx* ctx, int op);
+/* SVX_OP_MATRIX_FREE once a matrix-free solve has prepared the system's
+ * operator (and until its constraints change), else SVX_OP_ASSEMBLED; -1 for
+ * NULL. No reference counterpart. */
+int svx_system_operator(const svx_system* sys);
+/* Host-only check of the matrix-free operator's node patches for a mesh (no
+ * device use): plans the patches, checks their limits and that every node
+ * has exactly one owner, and evaluates K x patch by patch as the kernel
+ * does against the element-by-element product (seeded x, unit material).
+ * stats[5] = {kind (4 CPS4, 3 CPS3, 0 none), patches, largest patch nodes,
+ * largest patch element slots, element evaluations per product};
+ * *max_rel_diff = max |difference| / max |K x| (unset when kind is 0, the
+ * reason in svx_last_error). SVX_E_INVALID_MODEL if a check fails. No
+ * reference counterpart: a test and diagnostics entry. */
+int svx_mf_plan_check(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem,
+                      const uint8_t* kinds, const int64_t* conn_ptr, const int32_t* conn_idx,
+                      int64_t* stats, double* max_rel_diff);
 
 /* ---------------------------------------------------------------------------
  * Whole pipeline — replaces solve_with_method(&Model, Method) (lib.rs:586),

@@ -26,6 +26,9 @@ SIGNATURES = [
     ("svx_device_name", C.c_int, [_vp, C.c_char_p, C.c_size_t]),
     ("svx_last_error", C.c_char_p, []),
     ("svx_set_tuning", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
+    ("svx_set_operator", C.c_int, [_vp, C.c_int]),
+    ("svx_system_operator", C.c_int, [_vp]),
+    ("svx_mf_plan_check", C.c_int, [C.c_int, _i64, _dp, _i64, C.POINTER(C.c_uint8), _i64p, _i32p, _i64p, _dp]),
     ("svx_model_reaction_count", _i64, [C.POINTER(_abi.SvxModelDesc)]),
     ("svx_solve_model", C.c_int, [_vp, C.POINTER(_abi.SvxModelDesc), C.c_int, C.c_int,
                                   C.POINTER(_abi.SvxSolution), _errp]),

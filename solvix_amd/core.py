@@ -62,6 +62,12 @@ class Context:
     def set_tuning(self, lanes=0, unroll=0, rows_per_tile=0, period=0, chunk=0, use_graph=True, ru_delta=0.0):
         _raise(lib().svx_set_tuning(self._h, lanes, unroll, rows_per_tile, period, chunk, int(use_graph), ru_delta))
 
+    def set_operator(self, op: str):
+        """'assembled' (default) or 'matrix_free': the operator of subsequent
+        FP64 solves on plane meshes of one element kind and one material."""
+        code = {"assembled": _abi.SVX_OP_ASSEMBLED, "matrix_free": _abi.SVX_OP_MATRIX_FREE}[op]
+        _raise(lib().svx_set_operator(self._h, code))
+
     def close(self):
         if self._h:
             lib().svx_ctx_destroy(self._h)
@@ -170,6 +176,26 @@ def element_materials(model: Model) -> Tuple[np.ndarray, np.ndarray]:
 
 
 # --------------------------------------------------------------- the seam
+def mf_plan_check(ndof: int, coords, kinds, conn_ptr, conn_idx):
+    """Host-only check of the matrix-free operator's node patches (see
+    svx_mf_plan_check): returns (stats dict, max relative difference of the
+    patch-wise product against the element-by-element one, or None when the
+    mesh has no matrix-free operator)."""
+    coords = np.ascontiguousarray(coords, np.float64)
+    kinds = np.ascontiguousarray(kinds, np.uint8)
+    conn_ptr = np.ascontiguousarray(conn_ptr, np.int64)
+    conn_idx = np.ascontiguousarray(conn_idx, np.int32)
+    stats = np.zeros(5, np.int64)
+    diff = C.c_double(-1.0)
+    rc = lib().svx_mf_plan_check(ndof, coords.shape[0], _d(coords), kinds.size,
+                                 kinds.ctypes.data_as(C.POINTER(C.c_uint8)), _abi.ptr(conn_ptr, C.c_int64),
+                                 _abi.ptr(conn_idx, C.c_int32), _abi.ptr(stats, C.c_int64), C.byref(diff))
+    _raise(rc)
+    keys = ("kind", "patches", "cap_nodes", "cap_elem", "elem_slots")
+    st = dict(zip(keys, (int(v) for v in stats)))
+    return st, (diff.value if st["kind"] else None)
+
+
 class Mesh:
     """Device-resident mesh with its CSR pattern (built once per mesh)."""
 
@@ -263,6 +289,11 @@ class System:
         if rc != _abi.SVX_OK and (raise_on_failure or rc != _abi.SVX_E_NOT_CONVERGED):
             _raise(rc, err)
         return U, its, res, ok.astype(bool), ms.value
+
+    @property
+    def operator(self) -> str:
+        """'matrix_free' once a matrix-free solve prepared this system's operator, else 'assembled'."""
+        return "matrix_free" if lib().svx_system_operator(self._h) == _abi.SVX_OP_MATRIX_FREE else "assembled"
 
     def solution(self) -> np.ndarray:
         """The current solution u (after solve() or a batched solve)."""

@@ -115,6 +115,7 @@ void mesh_upload(svx_mesh* m, const double* coords) {
     const uint8_t* kinds = m->h_kinds.data();
     m->coords.alloc(n_nodes * 3);
     m->coords.upload(coords, n_nodes * 3, s);
+    m->h_coords.assign(coords, coords + n_nodes * 3);
     m->conn_ptr.alloc(n_elem + 1);
     m->conn_ptr.upload(conn_ptr, n_elem + 1, s);
     m->conn.alloc(m->h_conn.size());
@@ -177,6 +178,11 @@ void system_assemble(svx_system* sys, const double* E, const double* nu, double 
     sys->nu.alloc(m->n_elem);
     sys->nu.upload(nu, m->n_elem, s);
     sys->thickness = thickness;
+    sys->mat_uniform = m->n_elem > 0;
+    for (int64_t e = 1; e < m->n_elem && sys->mat_uniform; ++e) sys->mat_uniform = E[e] == E[0] && nu[e] == nu[0];
+    sys->E0 = m->n_elem > 0 ? E[0] : 0.0;
+    sys->nu0 = m->n_elem > 0 ? nu[0] : 0.0;
+    sys->mf_ready = false;
     DevBuf<double> ke(std::max<int64_t>(1, m->ke_total));
     DevBuf<uint8_t> reason(std::max<int64_t>(1, m->n_elem));
     DevBuf<unsigned long long> first_bad(1);
@@ -248,14 +254,56 @@ void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {
     sys->dinv32.release();
     sys->ell64.release();
     sys->ell32.release();
+    sys->mf_mask.release();
+    sys->mf_ready = false;
     SVX_HIP(hipStreamSynchronize(s));
 }
 
+bool mesh_mf_prepare(svx_mesh* m) {
+    if (m->mf_state != 0) return m->mf_state > 0;
+    MfPlan plan;
+    build_mf_plan(m->ndof, m->n_nodes, m->h_coords.data(), m->n_elem, m->h_kinds.data(), m->h_conn_ptr.data(),
+                  m->h_conn.data(), m->hp, plan);
+    if (plan.kind == 0) {
+        m->mf_state = -1;
+        return false;
+    }
+    hipStream_t s = m->ctx->stream;
+    m->mf_patch.alloc(plan.patches.size());
+    m->mf_patch.upload(plan.patches.data(), plan.patches.size(), s);
+    m->mf_nodes.alloc(plan.nodes.size());
+    m->mf_nodes.upload(plan.nodes.data(), plan.nodes.size(), s);
+    m->mf_econn.alloc(plan.econn.size());
+    m->mf_econn.upload(plan.econn.data(), plan.econn.size(), s);
+    m->mf_adj.alloc(std::max<size_t>(1, plan.adj.size()));
+    m->mf_adj.upload(plan.adj.data(), plan.adj.size(), s);
+    std::vector<double> xy((size_t)m->n_nodes * 2);
+    for (int64_t i = 0; i < m->n_nodes; ++i) xy[2 * i] = m->h_coords[3 * i], xy[2 * i + 1] = m->h_coords[3 * i + 1];
+    m->mf_xy.alloc(xy.size());
+    m->mf_xy.upload(xy.data(), xy.size(), s);
+    SVX_HIP(hipStreamSynchronize(s));
+    m->mf_kind = plan.kind;
+    m->mf_patches = (int32_t)plan.patches.size();
+    m->mf_cap_nodes = plan.cap_nodes;
+    m->mf_cap_elem = plan.cap_elem;
+    m->mf_elem_slots = plan.elem_slots;
+    m->mf_state = 1;
+    return true;
+}
+
 // Per-system preparation of the Krylov data for one precision.
-void system_prepare(svx_system* sys, int precision) {
+void system_prepare(svx_system* sys, int precision, bool matrix_free) {
     svx_mesh* m = sys->mesh;
     hipStream_t s = m->ctx->stream;
     const int64_t n = m->n;
+    if (matrix_free && precision == SVX_FP64 && !sys->mf_ready && sys->mat_uniform && sys->dirichlet &&
+        mesh_mf_prepare(m)) {
+        std::vector<uint8_t> mask(m->n_nodes, 0);
+        for (int64_t r : sys->cons_rows) mask[r / 2] |= (uint8_t)(1u << (r % 2));
+        sys->mf_mask.alloc(m->n_nodes);
+        sys->mf_mask.upload(mask.data(), m->n_nodes, s);
+        sys->mf_ready = true;
+    }
     if (!sys->dinv.p) {
         sys->dinv.alloc(n);
         sys->dinv32.alloc(n);
@@ -327,6 +375,23 @@ InstDesc system_desc(svx_system* sys) {
     d.p_alt = sys->p_alt.p;
     d.p32_alt = sys->p32_alt.p;
     d.s = sys->scal.p;
+    if (sys->mf_ready) {
+        d.mf_kind = m->mf_kind;
+        d.mf_patch = m->mf_patch.p;
+        d.mf_nodes = m->mf_nodes.p;
+        d.mf_econn = m->mf_econn.p;
+        d.mf_adj = m->mf_adj.p;
+        d.mf_xy = m->mf_xy.p;
+        d.mf_mask = sys->mf_mask.p;
+        const double E = sys->E0, nu = sys->nu0, c = E / (1.0 - nu * nu);  // element.rs:10-17
+        d.mf_c11 = c;
+        d.mf_c12 = nu * c;
+        d.mf_c33 = c * (1.0 - nu) / 2.0;
+        d.mf_t = sys->thickness;
+        d.mf_patches = m->mf_patches;
+        d.mf_cap_nodes = m->mf_cap_nodes;
+        d.mf_cap_elem = m->mf_cap_elem;
+    }
     return d;
 }
 
@@ -338,6 +403,13 @@ double spmv_bytes(const svx_mesh* m, int precision) {
     const double n = (double)m->n, nnz = (double)m->nnz, nblk = (double)m->nblk, ch = (double)m->n_chunks;
     if (precision == SVX_FP64) return 8.0 * nnz + 4.0 * nblk + 8.0 * ch + 32.0 * n;
     return 4.0 * nnz + 4.0 * nblk + 8.0 * ch + 16.0 * n;
+}
+
+// Algorithmic HBM bytes of one matrix-free product: coordinates, z and the
+// old direction of every node read, the new direction and q written,
+// connectivity (four bytes per element node) and the constrained-dof mask.
+double mf_spmv_bytes(const svx_mesh* m) {
+    return 81.0 * (double)m->n_nodes + 4.0 * (double)m->mf_kind * (double)m->n_elem;
 }
 
 InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len) {
@@ -361,7 +433,7 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     svx_mesh* m = sys->mesh;
     svx_ctx* ctx = m->ctx;
     hipStream_t s = ctx->stream;
-    system_prepare(sys, precision);
+    system_prepare(sys, precision, tp.matrix_free);
     InstScal sc = initial_scal(tol, max_iter, tp.ru_delta, m->max_row_len);
     sys->scal.upload(&sc, 1, s);
     std::vector<InstDesc> descs{system_desc(sys)};
@@ -370,6 +442,7 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     const int bs = m->ndof;
     shape.rows_per_tile[bs] = tp.rows_per_tile;  // 0: the instance's default
     shape.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
+    shape.matrix_free = tp.matrix_free && precision == SVX_FP64;
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;
     EngineOptions opt;
@@ -816,6 +889,44 @@ int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile, int p
     t.use_graph = use_graph != 0;
     if (ru_delta > 0) t.ru_delta = ru_delta;
     return SVX_OK;
+}
+
+int svx_set_operator(svx_ctx* ctx, int op) {
+    if (!ctx || (op != SVX_OP_ASSEMBLED && op != SVX_OP_MATRIX_FREE)) return SVX_E_ARGUMENT;
+    ctx->tune.matrix_free = op == SVX_OP_MATRIX_FREE;
+    return SVX_OK;
+}
+
+int svx_mf_plan_check(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const uint8_t* kinds,
+                      const int64_t* conn_ptr, const int32_t* conn_idx, int64_t* stats, double* max_rel_diff) {
+    if (ndof < 1 || n_nodes <= 0 || n_elem < 0 || !coords || !kinds || !conn_ptr || !conn_idx || !stats)
+        return SVX_E_ARGUMENT;
+    try {
+        HostPattern hp;
+        build_pattern(ndof, n_nodes, n_elem, conn_ptr, conn_idx, hp);
+        MfPlan plan;
+        build_mf_plan(ndof, n_nodes, coords, n_elem, kinds, conn_ptr, conn_idx, hp, plan);
+        stats[0] = plan.kind;
+        stats[1] = (int64_t)plan.patches.size();
+        stats[2] = plan.cap_nodes;
+        stats[3] = plan.cap_elem;
+        stats[4] = plan.elem_slots;
+        if (plan.kind == 0) {
+            g_last_error = plan.why;
+            return SVX_OK;
+        }
+        const double d = verify_mf_plan(plan, n_nodes, coords, n_elem, conn_ptr, conn_idx);
+        if (max_rel_diff) *max_rel_diff = d;
+        return SVX_OK;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SVX_E_INVALID_MODEL;
+    }
+}
+
+int svx_system_operator(const svx_system* sys) {
+    if (!sys) return -1;
+    return sys->mf_ready ? SVX_OP_MATRIX_FREE : SVX_OP_ASSEMBLED;
 }
 
 // ------------------------------------------------------------ whole model

@@ -19,6 +19,7 @@ struct TuneParams {
     int chunk = 32;         // iterations per captured graph
     bool use_graph = true;
     double ru_delta = 0.1;  // FP32-IR: refresh when ||r|| < ru_delta * ||r_last||
+    bool matrix_free = false;  // FP64 iterations on the matrix-free operator where a system has one
 };
 
 }  // namespace svx
@@ -54,12 +55,26 @@ struct svx_mesh {
     int64_t nsel[3] = {0, 0, 0};
     svx::DevBuf<int64_t> ke_off, ns_off;
     int64_t ke_total = 0, ns_total = 0;
+    std::vector<double> h_coords;  // [n_nodes][3]
+    // matrix-free operator, planned on first use: 0 not yet, 1 ready, -1 none
+    int mf_state = 0;
+    int mf_kind = 0;
+    int32_t mf_patches = 0, mf_cap_nodes = 0, mf_cap_elem = 0;
+    int64_t mf_elem_slots = 0;
+    svx::DevBuf<svx::MfPatch> mf_patch;
+    svx::DevBuf<int32_t> mf_nodes;
+    svx::DevBuf<uint16_t> mf_econn, mf_adj;
+    svx::DevBuf<double> mf_xy;
 };
 
 struct svx_system {
     svx_mesh* mesh = nullptr;
     svx::DevBuf<double> E, nu;
     double thickness = 1.0;
+    bool mat_uniform = false;  // one (E, nu) for every element
+    double E0 = 0.0, nu0 = 0.0;
+    svx::DevBuf<uint8_t> mf_mask;  // constrained dofs per node, for the matrix-free operator
+    bool mf_ready = false;
     svx::DevBuf<double> vals, f;
     std::vector<double> h_f;  // current load vector (host copy)
     bool dirichlet = false;
@@ -124,9 +139,15 @@ void system_assemble(svx_system* sys, const double* E, const double* nu, double 
                      int* bad_reason);
 void system_set_f(svx_system* sys, const double* f);
 void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows);
-void system_prepare(svx_system* sys, int precision);
+// matrix_free: also prepare the matrix-free operator (FP64, plane meshes of
+// one element kind and one material); system_desc then describes it.
+void system_prepare(svx_system* sys, int precision, bool matrix_free = false);
 InstDesc system_desc(svx_system* sys);
+// Plan and upload the mesh's matrix-free patches once; false if it has none.
+bool mesh_mf_prepare(svx_mesh* m);
 double spmv_bytes(const svx_mesh* m, int precision);
+// Algorithmic bytes of one matrix-free product (direction step fused in).
+double mf_spmv_bytes(const svx_mesh* m);
 InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len);
 void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,
                   const TuneParams& tp);

@@ -39,6 +39,7 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
         sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : 16;  // CSR rows of the FP64 residual and warm start
         sh.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
     }
+    sh.matrix_free = tp.matrix_free && precision == SVX_FP64;
     return sh;
 }
 
@@ -109,7 +110,7 @@ double iteration_ms(Engine& e, int precision, int reps) {
 std::vector<InstDesc> batch_setup(svx_batch* b, int precision) {
     hipStream_t s = b->ctx->stream;
     const size_t n = b->sys.size();
-    for (svx_system* sy : b->sys) system_prepare(sy, precision);
+    for (svx_system* sy : b->sys) system_prepare(sy, precision, b->ctx->tune.matrix_free);
     b->scal.alloc(n);
     std::vector<InstScal> init(n);
     std::vector<InstDesc> descs(n);
@@ -224,7 +225,9 @@ int svx_batch_time_spmv(svx_batch* b, int reps, double* avg_ms, double* bytes_pe
         b->engine->tile_all();
         double ms = b->engine->time_spmv(b->precision, reps);
         double bytes = 0.0;
-        for (svx_system* sy : b->sys) bytes += spmv_bytes(sy->mesh, b->precision);
+        const bool mf = b->ctx->tune.matrix_free && b->precision == SVX_FP64;
+        for (svx_system* sy : b->sys)
+            bytes += mf && sy->mf_ready ? mf_spmv_bytes(sy->mesh) : spmv_bytes(sy->mesh, b->precision);
         if (avg_ms) *avg_ms = ms;
         if (bytes_per_launch) *bytes_per_launch = bytes;
         return SVX_OK;

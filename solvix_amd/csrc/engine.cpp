@@ -42,44 +42,89 @@ void Engine::set_instances(const std::vector<InstDesc>& descs, const std::vector
     build_tiles(running);
 }
 
+bool Engine::uses_mf(const InstDesc& d) const { return shape_.matrix_free && d.bs == 2 && d.mf_kind != 0; }
+
 void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     release_graph();
     // previous layout, for carrying partials
     std::vector<int32_t> old_begin(h_descs_.size(), -1);
-    int old_first[4] = {0, 0, 0, 0}, old_count[4] = {0, 0, 0, 0};
+    int old_first[4] = {0, 0, 0, 0}, old_count[4] = {0, 0, 0, 0}, old_stride[4] = {0, 0, 0, 0};
+    size_t old_off[4] = {0, 0, 0, 0};
     for (int bs = 1; bs <= 3; ++bs) {
         old_first[bs] = groups_[bs].tiles ? (int)(groups_[bs].tiles - tiles_.p) : 0;
         old_count[bs] = groups_[bs].n_tiles;
+        old_stride[bs] = groups_[bs].pstride;
+        old_off[bs] = groups_[bs].partials ? (size_t)(groups_[bs].partials - partials_.p) : 0;
     }
     carry = carry && tiled_.size() == h_descs_.size() && partials_.p;
     if (carry)
         for (size_t i = 0; i < h_descs_.size(); ++i)
             if (tiled_[i]) old_begin[i] = h_descs_[i].tile_begin;
 
+    // Matrix-free patch tiles (node-block size 2 only): CPS4 instances, then
+    // CPS3 ones. mf_begin[i]: the instance's first patch tile.
+    std::vector<Tile> mf_tiles;
+    std::vector<int32_t> mf_begin(h_descs_.size(), -1);
+    int32_t n_mf[2] = {0, 0}, cap_n[2] = {0, 0}, cap_e[2] = {0, 0};
+    for (int kk = 0; kk < 2; ++kk) {
+        const int kind = kk == 0 ? 4 : 3;
+        for (size_t i = 0; i < h_descs_.size(); ++i) {
+            const InstDesc& d = h_descs_[i];
+            if (!running[i] || !uses_mf(d) || d.mf_kind != kind) continue;
+            mf_begin[i] = (int32_t)mf_tiles.size();
+            for (int32_t p = 0; p < d.mf_patches; ++p)
+                mf_tiles.push_back(Tile{(int32_t)i, p, p + 1, 0, 0, P_MPQ, mf_begin[i], d.mf_patches, d.s});
+            cap_n[kk] = std::max(cap_n[kk], d.mf_cap_nodes);
+            cap_e[kk] = std::max(cap_e[kk], d.mf_cap_elem);
+        }
+        n_mf[kk] = (int32_t)mf_tiles.size() - (kk ? n_mf[0] : 0);
+    }
+
     std::vector<Tile> tiles;
-    int first[4] = {0, 0, 0, 0}, count[4] = {0, 0, 0, 0};
+    int first[4] = {0, 0, 0, 0}, count[4] = {0, 0, 0, 0}, n_ell[4] = {0, 0, 0, 0};
     tiled_instances_ = 0;
     tiled_.assign(h_descs_.size(), 0);
     for (int bs = 1; bs <= 3; ++bs) {
         first[bs] = (int)tiles.size();
-        for (size_t i = 0; i < h_descs_.size(); ++i) {
-            InstDesc& d = h_descs_[i];
-            if (d.bs != bs || !running[i]) continue;
-            int rpt = shape_.rows_per_tile[bs] > 0 ? shape_.rows_per_tile[bs] : default_rows_per_tile(bs, d.n);
-            if (bs >= 2) {  // whole ELL chunks
-                const int cr = kEllRows * bs;
-                rpt = std::max(cr, (rpt + cr - 1) / cr * cr);
+        // assembled-matrix instances first: the FP64 SpMV launch covers them
+        // only, the matrix-free tiles the rest
+        for (int mf = 0; mf <= 1; ++mf) {
+            for (size_t i = 0; i < h_descs_.size(); ++i) {
+                InstDesc& d = h_descs_[i];
+                if (d.bs != bs || !running[i] || (int)uses_mf(d) != mf) continue;
+                int rpt = shape_.rows_per_tile[bs] > 0 ? shape_.rows_per_tile[bs] : default_rows_per_tile(bs, d.n);
+                if (bs >= 2) {  // whole ELL chunks
+                    const int cr = kEllRows * bs;
+                    rpt = std::max(cr, (rpt + cr - 1) / cr * cr);
+                }
+                d.tile_begin = (int32_t)tiles.size() - first[bs];
+                d.tile_count = (d.n + rpt - 1) / rpt;
+                for (int32_t r = 0; r < d.n; r += rpt) {
+                    Tile t{(int32_t)i, r, std::min(d.n, r + rpt), d.tile_begin, d.tile_count, P_PQ, d.tile_begin,
+                           d.tile_count, d.s};
+                    if (mf) t.pq_slot = P_MPQ, t.pq_begin = mf_begin[i], t.pq_count = d.mf_patches;
+                    tiles.push_back(t);
+                }
+                tiled_instances_++;
+                tiled_[i] = 1;
             }
-            d.tile_begin = (int32_t)tiles.size() - first[bs];
-            d.tile_count = (d.n + rpt - 1) / rpt;
-            for (int32_t r = 0; r < d.n; r += rpt)
-                tiles.push_back(Tile{(int32_t)i, r, std::min(d.n, r + rpt), d.tile_begin, d.tile_count, 0, d.s});
-            tiled_instances_++;
-            tiled_[i] = 1;
+            if (mf == 0) n_ell[bs] = (int)tiles.size() - first[bs];
         }
         count[bs] = (int)tiles.size() - first[bs];
     }
+    // patch tiles reduce their instance's r.r / r.z over its row tiles
+    for (Tile& t : mf_tiles) {
+        t.tile_begin = h_descs_[t.inst].tile_begin;
+        t.tile_count = h_descs_[t.inst].tile_count;
+    }
     tiled_tiles_ = (int)tiles.size();
+    int stride[4] = {0, 0, 0, 0};
+    size_t off[4] = {0, 0, 0, 0}, total = 0;
+    for (int bs = 1; bs <= 3; ++bs) {
+        stride[bs] = std::max(count[bs], bs == 2 ? n_mf[0] + n_mf[1] : 0);
+        off[bs] = total;
+        total += (size_t)kPartialSlots * stride[bs];
+    }
     // old local tile index feeding each new tile (-1: nothing to carry)
     std::vector<int32_t> src(tiles.size(), -1);
     bool any_src = false;
@@ -96,28 +141,38 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     if (any_src) old_partials = std::move(partials_);
     tiles_.alloc(std::max<size_t>(1, tiles.size()));
     tiles_.upload(tiles.data(), tiles.size(), stream_);
+    mf_tiles_.alloc(std::max<size_t>(1, mf_tiles.size()));
+    mf_tiles_.upload(mf_tiles.data(), mf_tiles.size(), stream_);
     descs_.alloc(std::max<size_t>(1, h_descs_.size()));
     descs_.upload(h_descs_.data(), h_descs_.size(), stream_);
-    partials_.alloc(std::max<size_t>(kPartialSlots, kPartialSlots * tiles.size()));
+    partials_.alloc(std::max<size_t>(kPartialSlots, total));
     if (any_src) {
         DevBuf<int32_t> d_src(src.size());
         d_src.upload(src.data(), src.size(), stream_);
         for (int bs = 1; bs <= 3; ++bs)
-            launch_carry_partials(old_partials.p + (size_t)kPartialSlots * old_first[bs], old_count[bs],
-                                  partials_.p + (size_t)kPartialSlots * first[bs], count[bs], d_src.p + first[bs],
-                                  count[bs], stream_);
+            launch_carry_partials(old_partials.p + old_off[bs], old_stride[bs], partials_.p + off[bs], stride[bs],
+                                  d_src.p + first[bs], count[bs], stream_);
         SVX_HIP(hipStreamSynchronize(stream_));
     }
     for (int bs = 1; bs <= 3; ++bs) {
         KrylovSet& k = groups_[bs];
         k.tiles = tiles_.p + first[bs];
         k.n_tiles = count[bs];
+        k.n_ell_tiles = n_ell[bs];
         k.descs = descs_.p;
         k.n_inst = (int32_t)h_descs_.size();
-        k.partials = partials_.p + kPartialSlots * (size_t)first[bs];
+        k.partials = partials_.p + off[bs];
+        k.pstride = stride[bs];
         k.lanes = shape_.lanes[bs] > 0 ? shape_.lanes[bs] : 16;
         k.unroll = shape_.unroll[bs] > 0 ? shape_.unroll[bs] : 2;
         k.bs = bs;
+        k.mf_tiles = bs == 2 ? mf_tiles_.p : nullptr;
+        k.n_mf4 = bs == 2 ? n_mf[0] : 0;
+        k.n_mf3 = bs == 2 ? n_mf[1] : 0;
+        for (int kk = 0; kk < 2; ++kk) {
+            k.mf_cap_nodes[kk] = bs == 2 ? cap_n[kk] : 0;
+            k.mf_cap_elem[kk] = bs == 2 ? cap_e[kk] : 0;
+        }
     }
     SVX_HIP(hipStreamSynchronize(stream_));
 }

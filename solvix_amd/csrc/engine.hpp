@@ -25,6 +25,9 @@ struct EngineShape {
     int lanes[4] = {0, 0, 0, 0};   // lanes per row (raw CSR, block size 1)
     int unroll[4] = {0, 0, 0, 0};  // blocks in flight per lane (ELL, block sizes 2, 3)
     int rows_per_tile[4] = {0, 0, 0, 0};  // 0: per-instance default; block sizes 2, 3: whole 64-node-row chunks
+    // FP64 SpMV of instances that carry a matrix-free operator (InstDesc::mf_kind)
+    // on that operator instead of the assembled matrix.
+    bool matrix_free = false;
 };
 
 class Engine {
@@ -71,7 +74,9 @@ class Engine {
     hipStream_t stream_;
     std::vector<InstDesc> h_descs_;
     EngineShape shape_;
+    bool uses_mf(const InstDesc& d) const;
     DevBuf<Tile> tiles_;
+    DevBuf<Tile> mf_tiles_;
     DevBuf<InstDesc> descs_;
     DevBuf<double> partials_;
     mutable DevBuf<InstScal> scal_tmp_;  // read_scalars staging

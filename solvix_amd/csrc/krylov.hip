@@ -34,6 +34,7 @@
 // the residual). An instance whose FP64 residual stops improving above that
 // level (four replacements without halving) falls back to FP64 PCG from its
 // current u.
+#include "mf_element.hpp"
 #include "svx_internal.hpp"
 
 namespace svx {
@@ -631,6 +632,198 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
     write_partials<1>(v, slot, partials, pstride, blockIdx.x);
 }
 
+// ------------------------------------------------ matrix-free plane stress
+// q = K p' for CPS4 / CPS3 meshes of one material without the assembled
+// matrix: K p' = sum_e K_e p'_e with K_e p'_e evaluated from the element
+// formulas of element.rs:91-211 (the B matrices of cps3_b_matrix and quad_b,
+// D of d_matrix, 2x2 Gauss points in the order of cps4_stiffness), i.e. the
+// product the assembled values give, up to rounding. One workgroup per node
+// patch (up to kBlock owned nodes, a run of the mesh's Morton order):
+//  1. the patch's nodes (owned, then halo): coordinates and p' = z + beta p
+//     into LDS (the owned nodes' p' is the value the tile stores);
+//  2. one thread per incident element: its nodal forces f_e into LDS;
+//  3. one thread per owned node: q = the sum of its elements' f_e in element
+//     order (a property of the mesh, so independent of the tiling and of the
+//     other instances in the launch); constrained dofs take q = p', the
+//     identity rows of the constrained matrix (solver.rs:43-62).
+// Element work per patch is one (CPS4) or two (CPS3) element slots per
+// thread.
+struct MfDev {
+    gptr<const MfPatch> patch;
+    gptr<const int32_t> nodes;
+    gptr<const uint64_t> econn;  // four 16-bit local node positions per element slot
+    gptr<const uint16_t> adj;
+    gptr<const double> xy;
+    gptr<const uint8_t> mask;
+    MfMaterial mat;
+};
+
+__device__ __forceinline__ MfDev mf_view(const InstDesc& d) {
+    MfDev m;
+    m.patch = global(d.mf_patch);
+    m.nodes = global(d.mf_nodes);
+    m.econn = global((const uint64_t*)d.mf_econn);
+    m.adj = global(d.mf_adj);
+    m.xy = global(d.mf_xy);
+    m.mask = global(d.mf_mask);
+    m.mat = MfMaterial{d.mf_c11, d.mf_c12, d.mf_c33, d.mf_t};
+    return m;
+}
+
+// LDS of one patch: coordinates and p' of cap_nodes nodes, then the nodal
+// forces of cap_elem element slots plus a zero slot, component-major.
+inline size_t mf_lds_bytes(int nen, int cap_nodes, int cap_elem) {
+    return sizeof(double) * (4 * (size_t)cap_nodes + 2 * (size_t)nen * (cap_elem + 1));
+}
+
+// One patch of q = K p' (the pending direction step evaluated by x); stores
+// p' and q of the owned nodes; returns this thread's p'.q.
+template <int NEN, typename X>
+__device__ __forceinline__ double mf_patch_spmv(const MfDev& M, int32_t patch, const X& x, gptr<double> nxt,
+                                                gptr<double> q, int cap_nodes, int cap_elem) {
+    constexpr int MAXA = NEN == 4 ? 4 : 8;
+    constexpr int ROUNDS = NEN == 4 ? 1 : 2;
+    extern __shared__ double mf_lds[];
+    MfPatch P;
+    {
+        const gptr<const MfPatch> pp = M.patch + patch;
+        P.node_off = pp->node_off, P.n_nodes = pp->n_nodes, P.n_owned = pp->n_owned, P.elem_off = pp->elem_off;
+        P.n_elem = pp->n_elem, P.adj_off = pp->adj_off, P.n_adj = pp->n_adj;
+    }
+    double* sx = mf_lds;
+    double* sy = sx + cap_nodes;
+    double* v0 = sy + cap_nodes;
+    double* v1 = v0 + cap_nodes;
+    double* fe = v1 + cap_nodes;
+    const int fs = cap_elem + 1;
+    const int tid = threadIdx.x;
+    const bool owner = tid < P.n_owned;
+    uint32_t code[MAXA];
+#pragma unroll
+    for (int k = 0; k < MAXA; ++k)
+        code[k] = (k < P.n_adj && owner) ? (uint32_t)M.adj[P.adj_off + k * kBlock + tid] : (uint32_t)(P.n_elem * 4);
+    uint64_t ec[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int sl = tid + r * kBlock;
+        ec[r] = sl < P.n_elem ? M.econn[P.elem_off + sl] : 0ull;
+    }
+    double own0 = 0.0, own1 = 0.0;
+    int32_t gown = 0;
+    uint32_t mk = 0;
+#pragma unroll
+    for (int r = 0; r < kMfMaxNodes / kBlock; ++r) {
+        const int j = tid + r * kBlock;
+        if (j < P.n_nodes) {
+            const int32_t g = M.nodes[P.node_off + j];
+            const double cx = M.xy[2 * g], cy = M.xy[2 * g + 1];
+            const double a = x(2 * g), b = x(2 * g + 1);
+            sx[j] = cx;
+            sy[j] = cy;
+            v0[j] = a;
+            v1[j] = b;
+            if (r == 0) own0 = a, own1 = b, gown = g;
+        }
+    }
+    if (owner) mk = M.mask[gown];
+    if (tid < 2 * NEN) fe[tid * fs + P.n_elem] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int sl = tid + r * kBlock;
+        if (sl < P.n_elem) {
+            const int l[4] = {(int)(ec[r] & 0xffffu), (int)((ec[r] >> 16) & 0xffffu), (int)((ec[r] >> 32) & 0xffffu),
+                              (int)(ec[r] >> 48)};
+            double xe[NEN][2], ue[NEN][2], f[NEN][2];
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                xe[a][0] = sx[l[a]];
+                xe[a][1] = sy[l[a]];
+                ue[a][0] = v0[l[a]];
+                ue[a][1] = v1[l[a]];
+            }
+            mf_forces<NEN>(xe, ue, M.mat, f);
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                fe[(2 * a) * fs + sl] = f[a][0];
+                fe[(2 * a + 1) * fs + sl] = f[a][1];
+            }
+        }
+    }
+    __syncthreads();
+    double dot = 0.0;
+    if (owner) {
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXA; ++k) {
+            const uint32_t sl = code[k] >> 2, cr = code[k] & 3u;  // the sentinel reads the zero slot
+            a0 += fe[(2 * cr) * fs + sl];
+            a1 += fe[(2 * cr + 1) * fs + sl];
+        }
+        const double q0 = (mk & 1u) ? own0 : a0;
+        const double q1 = (mk & 2u) ? own1 : a1;
+        nxt[2 * gown] = own0;
+        nxt[2 * gown + 1] = own1;
+        q[2 * gown] = q0;
+        q[2 * gown + 1] = q1;
+        dot = own0 * q0 + own1 * q1;
+    }
+    return dot;
+}
+
+// Direction step and q = K p of one FP64 iteration on the matrix-free
+// operator: cg64_spmv_kernel with patches for row tiles. Tile t is
+// mf_tiles[base + blockIdx.x]; its p.q partial goes to slot P_MPQ, column
+// base + blockIdx.x.
+template <int NEN>
+__global__ __launch_bounds__(kBlock) void cg64_mf_spmv_kernel(const Tile* __restrict__ mf_tiles, int base,
+                                                              const InstDesc* __restrict__ descs,
+                                                              double* __restrict__ partials, int pstride, int par,
+                                                              int force_dir, int cap_nodes, int cap_elem) {
+    const Tile t = mf_tiles[base + blockIdx.x];
+    const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const InstDesc& dd = descs[t.inst];
+    const Dev D = view(dd);
+    const MfDev M = mf_view(dd);
+    const gptr<const double> cur = par ? D.p_alt : D.p;
+    const gptr<double> nxt = par ? D.p : D.p_alt;
+    double v[1];
+    if (!force_dir && S->dir) {
+        v[0] = mf_patch_spmv<NEN>(M, t.row_begin, Plain<double>{cur}, nxt, D.q, cap_nodes, cap_elem);
+    } else {
+        double beta = 1.0;
+        if (!force_dir) {
+            const double snap_rz = S->snap_rz;
+            const int64_t snap_it = S->snap_it;
+            const int sl[2] = {P_RR, P_RZ};
+            double rr[2];
+            reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, rr);
+            const double res = sqrt(rr[0]) / S->bnorm;
+            const bool conv = res <= S->tol;
+            const int64_t it = snap_it + 1;
+            const bool budget = it >= S->max_iter;
+            beta = rr[1] / snap_rz;
+            if (is_leader(t)) {
+                S->it = it;
+                S->rr = rr[0];
+                S->res = res;
+                if (conv) {
+                    S->status = ST_CONVERGED;
+                } else {
+                    S->beta = beta;
+                    S->rz = rr[1];
+                    if (budget) S->status = ST_MAXITER;
+                }
+            }
+            if (conv || budget) return;
+        }
+        v[0] = mf_patch_spmv<NEN>(M, t.row_begin, Step<double>{D.z, cur, beta}, nxt, D.q, cap_nodes, cap_elem);
+    }
+    const int slot[1] = {P_MPQ};
+    write_partials<1>(v, slot, partials, pstride, base + blockIdx.x);
+}
+
 // u += alpha p, r -= alpha q, z = D^-1 r over [b, e); this thread's r.r, r.z.
 __device__ __forceinline__ void cg64_update_rows(int32_t b, int32_t e, double alpha, gptr<const double> __restrict__ p,
                                                  gptr<const double> __restrict__ q,
@@ -665,9 +858,9 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
     preload<double>(D.u, b, e, u0);
     preload<double>(D.r, b, e, r0);
     const double rz = S->rz;  // written by the previous phases' leaders only
-    const int sp[1] = {P_PQ};
+    const int sp[1] = {t.pq_slot};  // P_PQ over the row tiles, or P_MPQ over the matrix-free tiles
     double pq[1];
-    reduce_tiles<1>(partials, pstride, t.tile_begin, t.tile_count, sp, pq);
+    reduce_tiles<1>(partials, pstride, t.pq_begin, t.pq_count, sp, pq);
     if (pq[0] <= 0.0 || pq[0] != pq[0]) {  // solver.rs:118-123
         if (is_leader(t)) {
             S->pap = pq[0];
@@ -1173,38 +1366,44 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
     }
 
 void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {
-    if (k.bs == 3) {
-#define L(U) \
-    cg64_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
-        SVX_UNROLL_DISPATCH(k.unroll, L)
+    const int g = k.n_ell_tiles;
+    if (g > 0) {
+        if (k.bs == 3) {
+#define L(U) cg64_spmv_kernel<3, U><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+            SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
-    } else if (k.bs == 2) {
-#define L(U) \
-    cg64_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
-        SVX_UNROLL_DISPATCH(k.unroll, L)
+        } else if (k.bs == 2) {
+#define L(U) cg64_spmv_kernel<2, U><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+            SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
-    } else {
-#define L(LN) \
-    cg64_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
-        SVX_LANES_DISPATCH(k.lanes, L)
+        } else {
+#define L(LN) cg64_spmv_kernel<1, LN><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+            SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
+        }
     }
+    if (k.n_mf4 > 0)
+        cg64_mf_spmv_kernel<4><<<k.n_mf4, kBlock, mf_lds_bytes(4, k.mf_cap_nodes[0], k.mf_cap_elem[0]), s>>>(
+            k.mf_tiles, 0, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[0], k.mf_cap_elem[0]);
+    if (k.n_mf3 > 0)
+        cg64_mf_spmv_kernel<3><<<k.n_mf3, kBlock, mf_lds_bytes(3, k.mf_cap_nodes[1], k.mf_cap_elem[1]), s>>>(
+            k.mf_tiles, k.n_mf4, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[1], k.mf_cap_elem[1]);
 }
 
 void launch_ir32_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {
     if (k.bs == 3) {
 #define L(U) \
-    ir32_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
+    ir32_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
 #define L(U) \
-    ir32_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
+    ir32_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
 #define L(LN) \
-    ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par, force_dir)
+    ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }
@@ -1230,54 +1429,54 @@ void launch_spmv_csr64(int64_t n, const int32_t* row_ptr, const int32_t* col, co
 
 void cg64_init(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-    cg64_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    cg64_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
     if (k.bs == 3) {
-#define L(U) cg64_init_warm_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(U) cg64_init_warm_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(U) cg64_init_warm_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(U) cg64_init_warm_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
-#define L(LN) cg64_init_warm_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(LN) cg64_init_warm_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }
-    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
     SVX_HIP(hipGetLastError());
 }
 
 void cg64_iteration(const KrylovSet& k, hipStream_t s, int par) {
     if (k.n_tiles <= 0) return;
     launch_cg64_spmv(k, s, par, 0);
-    cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par);
+    cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
     SVX_HIP(hipGetLastError());
 }
 
 void cg64_chunk_end(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-    cg64_chunk_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    cg64_chunk_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
     SVX_HIP(hipGetLastError());
 }
 
 void ir32_init(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-    ir32_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
-    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    ir32_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
+    init_finish_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
     SVX_HIP(hipGetLastError());
 }
 
 void ir32_iteration(const KrylovSet& k, hipStream_t s, int par) {
     if (k.n_tiles <= 0) return;
     launch_ir32_spmv(k, s, par, 0);
-    ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, par);
+    ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
     SVX_HIP(hipGetLastError());
 }
 
 void ir32_period_end(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-    ir32_period_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    ir32_period_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
     SVX_HIP(hipGetLastError());
 }
 
@@ -1285,19 +1484,19 @@ void ir32_refresh(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     ir32_fold_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
     if (k.bs == 3) {
-#define L(U) ir32_residual_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(U) ir32_residual_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(U) ir32_residual_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(U) ir32_residual_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
-#define L(LN) ir32_residual_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles)
+#define L(LN) ir32_residual_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }
-    ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    ir32_repoint_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
     SVX_HIP(hipGetLastError());
 }
 
@@ -1320,15 +1519,15 @@ void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s, in
 void update_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     if (precision == SVX_FP64)
-        cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, 0);
+        cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, 0);
     else
-        ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles, 0);
+        ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, 0);
     SVX_HIP(hipGetLastError());
 }
 
 void period_end_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0 || precision == SVX_FP64) return;
-    ir32_period_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.n_tiles);
+    ir32_period_end_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
     SVX_HIP(hipGetLastError());
 }
 

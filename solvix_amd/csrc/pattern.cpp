@@ -5,12 +5,18 @@
 //    element-neighbourhood (sorted), or only its diagonal when the node
 //    belongs to no element (assembly.rs:300-302);
 //  - the node-block (BSR) pattern with the same neighbourhoods, and its
-//    node-row sliced ELL form for the Krylov SpMV.
+//    node-row sliced ELL form for the Krylov SpMV;
+//  - on request, the node patches of the matrix-free plane-stress operator.
 #include "pattern.hpp"
 
+#include <cmath>
+
 #include <algorithm>
+#include <numeric>
 #include <stdexcept>
 #include <thread>
+
+#include "mf_element.hpp"
 
 namespace svx {
 
@@ -133,6 +139,233 @@ void build_pattern(int ndof, int64_t n_nodes, int64_t n_elem, const int64_t* con
                 }
             }
     });
+}
+
+namespace {
+uint64_t spread_bits(uint64_t v) {  // bit i -> bit 2i, 32 bits
+    v &= 0xffffffffull;
+    v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+    v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+    v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+}
+}  // namespace
+
+void build_mf_plan(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const uint8_t* kinds,
+                   const int64_t* conn_ptr, const int32_t* conn, const HostPattern& hp, MfPlan& M) {
+    M = MfPlan{};
+    if (ndof != 2 || n_elem == 0 || n_nodes == 0) {
+        M.why = "not a plane mesh";
+        return;
+    }
+    const int kind = kinds[0] == SVX_CPS4 ? 4 : kinds[0] == SVX_CPS3 ? 3 : 0;
+    if (kind == 0) {
+        M.why = "element kind";
+        return;
+    }
+    for (int64_t e = 0; e < n_elem; ++e)
+        if (kinds[e] != kinds[0] || conn_ptr[e + 1] - conn_ptr[e] != kind) {
+            M.why = "mixed element kinds";
+            return;
+        }
+    const int max_adj = mf_max_adj(kind), cap_e = mf_max_elem(kind);
+    for (int64_t i = 0; i < n_nodes; ++i)
+        if (hp.inc_ptr[i + 1] - hp.inc_ptr[i] > max_adj) {
+            M.why = "too many elements at a node";
+            return;
+        }
+    // Morton order of (x, y), 22 bits per axis over the bounding square
+    double lo[2] = {coords[0], coords[1]}, hi[2] = {coords[0], coords[1]};
+    for (int64_t i = 0; i < n_nodes; ++i)
+        for (int a = 0; a < 2; ++a) lo[a] = std::min(lo[a], coords[3 * i + a]), hi[a] = std::max(hi[a], coords[3 * i + a]);
+    double span = std::max(hi[0] - lo[0], hi[1] - lo[1]);
+    if (!(span > 0.0)) span = 1.0;
+    const double scale = 4194303.0 / span;
+    std::vector<uint64_t> key(n_nodes);
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        const uint64_t qx = (uint64_t)std::min(4194303.0, std::max(0.0, (coords[3 * i] - lo[0]) * scale));
+        const uint64_t qy = (uint64_t)std::min(4194303.0, std::max(0.0, (coords[3 * i + 1] - lo[1]) * scale));
+        key[i] = spread_bits(qx) | (spread_bits(qy) << 1);
+    }
+    std::vector<int32_t> order(n_nodes);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    std::vector<int64_t> estamp(n_elem, -1), nstamp(n_nodes, -1);
+    std::vector<int32_t> loc(n_nodes, 0), elems, owned, halo;
+    std::vector<uint16_t> lists;
+    std::vector<int32_t> lcount;
+    int64_t pos = 0, p = 0;
+    while (pos < n_nodes) {
+        // grow the patch along the Morton order while its elements and nodes fit
+        elems.clear(), owned.clear();
+        int32_t n_nodes_p = 0;
+        while (pos < n_nodes && (int)owned.size() < kBlock) {
+            const int32_t g = order[pos];
+            int add_e = 0, add_n = nstamp[g] != p ? 1 : 0;
+            for (int64_t q = hp.inc_ptr[g]; q < hp.inc_ptr[g + 1]; ++q) {
+                const int32_t e = hp.inc_elem[q];
+                if (estamp[e] == p) continue;
+                add_e++;
+                for (int64_t k = conn_ptr[e]; k < conn_ptr[e + 1]; ++k) add_n += nstamp[conn[k]] != p && conn[k] != g;
+            }
+            // add_n over-counts nodes shared by two new elements: an upper bound
+            if (!owned.empty() && ((int)elems.size() + add_e > cap_e || n_nodes_p + add_n > kMfMaxNodes)) break;
+            if (nstamp[g] != p) nstamp[g] = p, n_nodes_p++;
+            for (int64_t q = hp.inc_ptr[g]; q < hp.inc_ptr[g + 1]; ++q) {
+                const int32_t e = hp.inc_elem[q];
+                if (estamp[e] == p) continue;
+                estamp[e] = p;
+                elems.push_back(e);
+                for (int64_t k = conn_ptr[e]; k < conn_ptr[e + 1]; ++k)
+                    if (nstamp[conn[k]] != p) nstamp[conn[k]] = p, n_nodes_p++;
+            }
+            owned.push_back(g);
+            ++pos;
+        }
+        MfPatch pt{};
+        pt.n_owned = (int32_t)owned.size();
+        std::sort(elems.begin(), elems.end());
+        // node list: owned nodes in patch order, then the halo ascending
+        const int64_t tag = -2 - p;  // second stamp of this patch: node placed
+        pt.node_off = (int32_t)M.nodes.size();
+        for (int32_t k = 0; k < pt.n_owned; ++k) {
+            nstamp[owned[k]] = tag;
+            loc[owned[k]] = k;
+            M.nodes.push_back(owned[k]);
+        }
+        halo.clear();
+        for (int32_t e : elems)
+            for (int64_t k = conn_ptr[e]; k < conn_ptr[e + 1]; ++k)
+                if (nstamp[conn[k]] != tag) nstamp[conn[k]] = tag, halo.push_back(conn[k]);
+        std::sort(halo.begin(), halo.end());
+        for (size_t h = 0; h < halo.size(); ++h) {
+            loc[halo[h]] = pt.n_owned + (int32_t)h;
+            M.nodes.push_back(halo[h]);
+        }
+        pt.n_nodes = pt.n_owned + (int32_t)halo.size();
+        if (pt.n_nodes > kMfMaxNodes || (int)elems.size() > cap_e) throw std::logic_error("matrix-free patch overflow");
+        pt.elem_off = (int32_t)(M.econn.size() / 4);
+        pt.n_elem = (int32_t)elems.size();
+        for (int32_t e : elems)
+            for (int a = 0; a < 4; ++a) M.econn.push_back(a < kind ? (uint16_t)loc[conn[conn_ptr[e] + a]] : 0);
+        // incidences of the owned nodes, in element order
+        lcount.assign(pt.n_owned, 0);
+        lists.assign((size_t)pt.n_owned * max_adj, 0);
+        for (size_t sl = 0; sl < elems.size(); ++sl)
+            for (int a = 0; a < kind; ++a) {
+                const int32_t l = loc[conn[conn_ptr[elems[sl]] + a]];
+                if (l < pt.n_owned) lists[(size_t)l * max_adj + lcount[l]++] = (uint16_t)(sl * 4 + a);
+            }
+        int32_t A = 0;
+        for (int32_t l = 0; l < pt.n_owned; ++l) A = std::max(A, lcount[l]);
+        pt.n_adj = A;
+        pt.adj_off = (int32_t)M.adj.size();
+        M.adj.resize(M.adj.size() + (size_t)A * kBlock, (uint16_t)(pt.n_elem * 4));  // padding: the zero slot
+        for (int32_t l = 0; l < pt.n_owned; ++l)
+            for (int32_t k = 0; k < lcount[l]; ++k) M.adj[pt.adj_off + (size_t)k * kBlock + l] = lists[(size_t)l * max_adj + k];
+        M.cap_nodes = std::max(M.cap_nodes, pt.n_nodes);
+        M.cap_elem = std::max(M.cap_elem, pt.n_elem);
+        M.elem_slots += pt.n_elem;
+        M.patches.push_back(pt);
+        ++p;
+        if (M.nodes.size() > (size_t)INT32_MAX || M.adj.size() > (size_t)INT32_MAX || M.econn.size() / 4 > (size_t)INT32_MAX)
+            throw std::runtime_error("matrix-free plan exceeds 2^31 entries");
+    }
+    M.kind = kind;
+}
+
+namespace {
+template <int NEN>
+void mf_element(const double* coords, const int64_t* conn_ptr, const int32_t* conn, int64_t e,
+                const std::vector<double>& x, const MfMaterial& mat, double (&f)[NEN][2]) {
+    double xe[NEN][2], ue[NEN][2];
+    for (int a = 0; a < NEN; ++a) {
+        const int32_t g = conn[conn_ptr[e] + a];
+        xe[a][0] = coords[3 * g], xe[a][1] = coords[3 * g + 1];
+        ue[a][0] = x[2 * g], ue[a][1] = x[2 * g + 1];
+    }
+    mf_forces<NEN>(xe, ue, mat, f);
+}
+
+template <int NEN>
+double verify_plan(const MfPlan& M, int64_t n_nodes, const double* coords, int64_t n_elem, const int64_t* conn_ptr,
+                   const int32_t* conn) {
+    auto fail = [](const char* what) { throw std::logic_error(std::string("matrix-free plan: ") + what); };
+    const int max_adj = mf_max_adj(NEN), max_elem = mf_max_elem(NEN);
+    const double c = 1.0 / (1.0 - 0.3 * 0.3);
+    const MfMaterial mat{c, 0.3 * c, c * (1.0 - 0.3) / 2.0, 1.0};
+    std::vector<double> x(2 * n_nodes);
+    uint64_t st = 0x5317000000000007ull;
+    for (auto& v : x) {  // SplitMix64, (x >> 11) 2^-53 - 0.5
+        uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        v = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    // element by element, in element order
+    std::vector<double> ref(2 * n_nodes, 0.0);
+    for (int64_t e = 0; e < n_elem; ++e) {
+        double f[NEN][2];
+        mf_element<NEN>(coords, conn_ptr, conn, e, x, mat, f);
+        for (int a = 0; a < NEN; ++a) {
+            const int32_t g = conn[conn_ptr[e] + a];
+            ref[2 * g] += f[a][0], ref[2 * g + 1] += f[a][1];
+        }
+    }
+    // patch by patch
+    std::vector<double> q(2 * n_nodes, 0.0);
+    std::vector<int> owned_by(n_nodes, 0);
+    std::vector<double> fe;
+    for (const MfPatch& P : M.patches) {
+        if (P.n_owned < 1 || P.n_owned > 256 || P.n_nodes < P.n_owned || P.n_nodes > kMfMaxNodes) fail("node counts");
+        if (P.n_elem < 0 || P.n_elem > max_elem || P.n_adj < 0 || P.n_adj > max_adj) fail("element counts");
+        if ((size_t)P.node_off + P.n_nodes > M.nodes.size() || (size_t)(P.elem_off + P.n_elem) * 4 > M.econn.size() ||
+            (size_t)P.adj_off + (size_t)P.n_adj * 256 > M.adj.size())
+            fail("offsets");
+        const int32_t* nl = M.nodes.data() + P.node_off;
+        for (int j = 0; j < P.n_nodes; ++j)
+            if (nl[j] < 0 || nl[j] >= n_nodes) fail("node index");
+        for (int j = 0; j < P.n_owned; ++j) owned_by[nl[j]]++;
+        fe.assign((size_t)2 * NEN * (P.n_elem + 1), 0.0);  // the last slot stays zero
+        for (int sl = 0; sl < P.n_elem; ++sl) {
+            const uint16_t* ec = M.econn.data() + (size_t)(P.elem_off + sl) * 4;
+            double xe[NEN][2], ue[NEN][2], f[NEN][2];
+            for (int a = 0; a < NEN; ++a) {
+                if (ec[a] >= P.n_nodes) fail("local node");
+                const int32_t g = nl[ec[a]];
+                xe[a][0] = coords[3 * g], xe[a][1] = coords[3 * g + 1];
+                ue[a][0] = x[2 * g], ue[a][1] = x[2 * g + 1];
+            }
+            mf_forces<NEN>(xe, ue, mat, f);
+            for (int a = 0; a < NEN; ++a) fe[(2 * a) * (P.n_elem + 1) + sl] = f[a][0], fe[(2 * a + 1) * (P.n_elem + 1) + sl] = f[a][1];
+        }
+        for (int l = 0; l < P.n_owned; ++l) {
+            double a0 = 0.0, a1 = 0.0;
+            for (int k = 0; k < P.n_adj; ++k) {
+                const uint32_t code = M.adj[P.adj_off + (size_t)k * 256 + l], sl = code >> 2, cr = code & 3u;
+                if ((int)sl > P.n_elem || (int)cr >= NEN) fail("incidence code");
+                a0 += fe[(2 * cr) * (P.n_elem + 1) + sl];
+                a1 += fe[(2 * cr + 1) * (P.n_elem + 1) + sl];
+            }
+            q[2 * nl[l]] = a0, q[2 * nl[l] + 1] = a1;
+        }
+    }
+    for (int64_t i = 0; i < n_nodes; ++i)
+        if (owned_by[i] != 1) fail("a node is not owned by exactly one patch");
+    double num = 0.0, den = 0.0;
+    for (int64_t i = 0; i < 2 * n_nodes; ++i) num = std::max(num, std::fabs(q[i] - ref[i])), den = std::max(den, std::fabs(ref[i]));
+    return den > 0.0 ? num / den : num;
+}
+}  // namespace
+
+double verify_mf_plan(const MfPlan& M, int64_t n_nodes, const double* coords, int64_t n_elem, const int64_t* conn_ptr,
+                      const int32_t* conn) {
+    if (M.kind == 4) return verify_plan<4>(M, n_nodes, coords, n_elem, conn_ptr, conn);
+    if (M.kind == 3) return verify_plan<3>(M, n_nodes, coords, n_elem, conn_ptr, conn);
+    throw std::logic_error("matrix-free plan: no plan");
 }
 
 }  // namespace svx

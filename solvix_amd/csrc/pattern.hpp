@@ -26,8 +26,33 @@ struct HostPattern {
     std::vector<int32_t> ell_col;  // [ell_off[n_chunks]] node-block columns
 };
 
-
 void build_pattern(int ndof, int64_t n_nodes, int64_t n_elem, const int64_t* conn_ptr, const int32_t* conn,
                    HostPattern& out);
+
+// Node patches of the matrix-free plane-stress operator (krylov.hip): the
+// nodes in Morton order of their (x, y) coordinates, cut greedily into runs
+// of at most kBlock owned nodes whose incident elements fit one (CPS4) or two
+// (CPS3) element slots per thread and whose nodes fit kMfMaxNodes.
+struct MfPlan {
+    int kind = 0;  // 4 (CPS4) or 3 (CPS3); 0: the mesh has no matrix-free operator (see `why`)
+    std::vector<MfPatch> patches;
+    std::vector<int32_t> nodes;   // per patch: owned nodes (Morton order), then halo nodes (ascending)
+    std::vector<uint16_t> econn;  // [element slot][4] patch-local node positions (CPS3: fourth unused)
+    std::vector<uint16_t> adj;    // per patch [k][kBlock]: (slot << 2 | corner) of owned node lane's k-th incidence
+    int32_t cap_nodes = 0, cap_elem = 0;
+    int64_t elem_slots = 0;  // element evaluations per product (patch overlap included)
+    const char* why = "";
+};
+// Plane meshes (ndof 2) of one element kind, CPS3 or CPS4, with at most 8
+// (CPS3) or 4 (CPS4) elements per node; `hp` supplies the incidence lists.
+void build_mf_plan(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const uint8_t* kinds,
+                   const int64_t* conn_ptr, const int32_t* conn, const HostPattern& hp, MfPlan& out);
+// Host check of a plan: its structural limits, every node owned by exactly
+// one patch, and the product q = K x evaluated patch by patch the way the
+// kernel does (LDS staging replaced by arrays) against the element-by-element
+// product, for a seeded x and unit material. Returns max |difference| /
+// max |q|; throws std::logic_error naming a violated limit.
+double verify_mf_plan(const MfPlan& plan, int64_t n_nodes, const double* coords, int64_t n_elem,
+                      const int64_t* conn_ptr, const int32_t* conn);
 
 }  // namespace svx

@@ -112,6 +112,13 @@ struct alignas(16) InstScal {
     int64_t max_iter;
 };
 
+// One node patch of the matrix-free operator: up to kBlock owned nodes (a
+// run of the mesh's Morton order), the elements incident to them and the
+// nodes of those elements.
+struct MfPatch {
+    int32_t node_off, n_nodes, n_owned, elem_off, n_elem, adj_off, n_adj, pad;
+};
+
 // Device pointers of one instance. FP64 CSR always (FP32-IR residuals,
 // warm starts). Node-block instances (bs = 2, 3) multiply in the node-row
 // sliced ELL layout, FP64 values for FP64 PCG and FP32 values for the FP32-IR
@@ -149,7 +156,22 @@ struct InstDesc {
     double* p_alt;    // second direction buffer (the SpMV writes the new direction there)
     float* p32_alt;
     InstScal* s;
+    // Matrix-free plane-stress operator (mf_kind 3 = CPS3, 4 = CPS4, 0 =
+    // none): q = K p from the element formulas over the mesh's node patches,
+    // instead of the assembled values (FP64 iterations only).
+    const MfPatch* mf_patch;
+    const int32_t* mf_nodes;    // patch node lists: owned nodes, then halo nodes
+    const uint16_t* mf_econn;   // [element slot][4] patch-local node positions
+    const uint16_t* mf_adj;     // [patch][k][kBlock] (slot << 2 | corner) of owned nodes
+    const double* mf_xy;        // [n_nodes][2] coordinates
+    const uint8_t* mf_mask;     // [n_nodes] constrained dofs (bit d: dof d + 1)
+    double mf_c11, mf_c12, mf_c33, mf_t;  // plane-stress D (element.rs:10-17) and thickness
+    int32_t mf_kind;
+    int32_t mf_patches;
+    int32_t mf_cap_nodes;  // largest patch (nodes, elements)
+    int32_t mf_cap_elem;
 };
+
 
 // A tile is a contiguous row range of one instance handled by one workgroup.
 // It carries its instance's scalar-state pointer and tile range, so a kernel
@@ -157,11 +179,13 @@ struct InstDesc {
 // descriptor together, one dependent step after the tile itself.
 struct alignas(16) Tile {
     int32_t inst;
-    int32_t row_begin;
+    int32_t row_begin;   // matrix-free SpMV tiles: the patch
     int32_t row_end;
-    int32_t tile_begin;  // the instance's first tile (in its group)
-    int32_t tile_count;  // the instance's tiles
-    int32_t pad;
+    int32_t tile_begin;  // the instance's first row tile (in its group)
+    int32_t tile_count;  // the instance's row tiles
+    int32_t pq_slot;     // partial slot and tile range of the instance's p.q sums
+    int32_t pq_begin;
+    int32_t pq_count;
     InstScal* s;
 };
 
@@ -169,7 +193,14 @@ constexpr int kBlock = 256;
 constexpr int kMaxUnroll = 4;  // ELL SpMV: blocks in flight per lane, 1..kMaxUnroll
 constexpr int kEllRows = 64;   // node rows per ELL chunk (one wavefront)
 // Per-tile partial sums, one slot per reduced quantity.
-enum : int { P_PQ = 0, P_RR = 1, P_RZ = 2, P_TRR = 3, P_TRZ = 4, P_TFL = 5, P_IFF = 6, P_IRZ = 7, kPartialSlots = 8 };
+// P_MPQ: p.q of the matrix-free SpMV, indexed by its patch tiles.
+enum : int { P_PQ = 0, P_RR = 1, P_RZ = 2, P_TRR = 3, P_TRZ = 4, P_TFL = 5, P_IFF = 6, P_IRZ = 7, P_MPQ = 8, kPartialSlots = 9 };
+// Matrix-free operator limits: nodes per patch (two per thread), element
+// slots per patch (one per thread for CPS4, two for CPS3) and incident
+// elements per owned node (CPS4 4, CPS3 8).
+constexpr int kMfMaxNodes = 2 * 256;
+inline int mf_max_elem(int kind) { return kind == 3 ? 2 * 256 : 256; }
+inline int mf_max_adj(int kind) { return kind == 3 ? 8 : 4; }
 
 // SpMV launch shape chosen per instance set: lanes per row (CSR) or per
 // block row (BSR) and rows per tile.
@@ -228,10 +259,19 @@ struct KrylovSet {
     int32_t n_tiles = 0;
     const InstDesc* descs = nullptr;
     int32_t n_inst = 0;
-    double* partials = nullptr;  // [kPartialSlots * n_tiles]
+    double* partials = nullptr;  // [kPartialSlots * pstride]
+    int32_t pstride = 0;         // partial slot stride, max(n_tiles, n_mf_tiles)
     int lanes = 16;              // lanes per row (raw CSR instances, bs = 1)
     int unroll = 2;              // blocks in flight per lane (ELL instances, bs = 2, 3)
     int bs = 0;                  // node-block size shared by the set's instances (1, 2, 3)
+    // FP64 SpMV: row tiles [0, n_ell_tiles) multiply the assembled matrix;
+    // instances with matrix-free tiles (mf_tiles, CPS4 first, then CPS3) own
+    // the row tiles after them. FP32 SpMV launches cover every row tile.
+    int32_t n_ell_tiles = 0;
+    const Tile* mf_tiles = nullptr;
+    int32_t n_mf4 = 0, n_mf3 = 0;
+    int32_t mf_cap_nodes[2] = {0, 0};  // CPS4, CPS3: largest patch of the set
+    int32_t mf_cap_elem[2] = {0, 0};
 };
 // An iteration is two launches: the direction step fused into the SpMV, then
 // the update. The direction is double-buffered: iteration parity par reads
