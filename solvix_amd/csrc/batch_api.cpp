@@ -25,6 +25,7 @@ struct svx_batch {
     int precision = -1;
     EngineShape tuned_shape;   // from svx_batch_autotune
     int tuned_precision = -1;  // precision the tuned shape belongs to, -1 = none
+    bool matrix_free = false;  // the last solve's FP64 products used the matrix-free operator
 };
 
 namespace svx {
@@ -181,9 +182,11 @@ int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
         const size_t n = b->sys.size();
         if (n == 0) return SVX_OK;
         std::vector<InstDesc> descs = batch_setup(b, precision);
-        const EngineShape shape =
+        EngineShape shape =
             b->tuned_precision == precision ? b->tuned_shape : batch_shape(b->sys, b->ctx->tune, precision);
+        shape.matrix_free = b->ctx->tune.matrix_free && precision == SVX_FP64;  // the operator in force now
         b->engine->set_instances(descs, std::vector<char>(n, 1), shape);
+        b->matrix_free = shape.matrix_free;
         EngineOptions opt;
         opt.precision = precision;
         opt.period = b->ctx->tune.period;
@@ -225,7 +228,7 @@ int svx_batch_time_spmv(svx_batch* b, int reps, double* avg_ms, double* bytes_pe
         b->engine->tile_all();
         double ms = b->engine->time_spmv(b->precision, reps);
         double bytes = 0.0;
-        const bool mf = b->ctx->tune.matrix_free && b->precision == SVX_FP64;
+        const bool mf = b->matrix_free;
         for (svx_system* sy : b->sys)
             bytes += mf && sy->mf_ready ? mf_spmv_bytes(sy->mesh) : spmv_bytes(sy->mesh, b->precision);
         if (avg_ms) *avg_ms = ms;
