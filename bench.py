@@ -240,7 +240,10 @@ def bench_batch(args):
         rec = out[lab]
         print(json.dumps({"arm": lab, "value": round(dps(rec), 1), "unit": "solved DOF/s",
                           "ms_per_step": round(rec["dt"] / args.steps * 1e3, 3),
-                          "converged": int(rec["ok"].sum())}), flush=True)
+                          "solve_s": round(rec["solve_s"] / args.steps, 3), "converged": int(rec["ok"].sum()),
+                          "iterations_mean": round(float(rec["its"].mean()), 1),
+                          "spmv_us": round(rec.get("spmv_ms", 0.0) * 1e3, 2),
+                          "spmv_bytes": rec.get("spmv_bytes")}), flush=True)
         return
 
     seq = out["fp64_sequential"]
