@@ -43,9 +43,11 @@ CONFIGS = {
 }
 
 
-def build_model(name):
+def build_model(name, plane_size=None):
     from solvix_amd import decks
-    return decks.cfg2() if name == "cfg2" else decks.cfg1()
+    if name == "cfg2":
+        return decks.cfg2()
+    return decks.cfg1(nx=plane_size[0], ny=plane_size[1]) if plane_size else decks.cfg1()
 
 
 def cpu_baseline(model, full_iterations, cap):
@@ -508,6 +510,10 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=100, help="launches of the timed SpMV kernel")
     ap.add_argument("--batch-arms", default="all", choices=["all", "fp64_batched", "fp64_batched_mf", "fp32_ir_batched"],
                     help="batched workload: every arm, or one batched engine alone (profiling)")
+    ap.add_argument("--operator", default="assembled", choices=["assembled", "matrix_free"],
+                    help="single solves: operator of the FP64 arm (matrix_free: plane meshes)")
+    ap.add_argument("--plane-size", default=None, metavar="NXxNY",
+                    help="cfg1 only: the cantilever on an NX x NY mesh (profiling at other sizes)")
     ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block"],
                     help="multi-RHS workload: run one arm only (profiling)")
     ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
@@ -539,9 +545,11 @@ def main():
     from solvix_amd.core import Context, Mesh, constrained_dofs, element_materials
     from solvix_amd.model import Precision
 
-    model = build_model(args.config)
+    size = tuple(int(v) for v in args.plane_size.split("x")) if args.plane_size else None
+    model = build_model(args.config, size)
     n_dofs = model.n_dofs()
     ctx = Context(0)
+    ctx.set_operator(args.operator)
     mesh = Mesh.from_model(model, ctx)
     E, nu = element_materials(model)
     ndof = model.ndof()
@@ -583,11 +591,12 @@ def main():
     def roof(arm, name, label):
         ach = arm["spmv_bytes"] / (arm["spmv_ms"] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES.get((args.config, label)),
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None if (args.plane_size or (label == "fp64" and mf64)) else TRAFFIC_BYTES.get((args.config, label)),
                 "kernel": name,
                 "bytes_per_launch": arm["spmv_bytes"], "avg_launch_us": round(arm["spmv_ms"] * 1e3, 2)}
 
     spmv32 = "ir32_spmv_kernel<%d,...>" % ndof
+    mf64 = args.operator == "matrix_free" and ndof == 2
     out = {
         "metric": "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 PCG",
         "value": round(a["dofs_per_s"], 1),
@@ -604,7 +613,8 @@ def main():
         "config": {"workload": CONFIGS[args.config][0], "n_dof": n_dofs, "nnz": int(mesh.nnz),
                    "tolerance": 1e-10, "precision": "FP32 Krylov, FP64 refinement"},
         "fp64": {"value": round(b["dofs_per_s"], 1), "ms_per_step": round(b["dt"] / args.steps * 1e3, 3),
-                 "iterations": int(b["its"]), "residual": b["res"], "roofline": roof(b, "cg64_spmv_kernel<%d,...>" % ndof, "fp64")},
+                 "iterations": int(b["its"]), "residual": b["res"], "operator": args.operator,
+                 "roofline": roof(b, "cg64_mf_spmv_kernel<4>" if mf64 else "cg64_spmv_kernel<%d,...>" % ndof, "fp64")},
         "speedup_vs_fp64": round(a["dofs_per_s"] / b["dofs_per_s"], 3),
         "iterations": int(a["its"]),
         "residual": a["res"],

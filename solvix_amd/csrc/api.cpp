@@ -445,6 +445,7 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     shape.matrix_free = tp.matrix_free && precision == SVX_FP64;
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;
+    sys->engine_mf = shape.matrix_free && sys->mf_ready;
     EngineOptions opt;
     opt.precision = precision;
     opt.period = tp.period;
@@ -841,7 +842,7 @@ int svx_system_time_spmv(svx_system* sys, int precision, int reps, double* avg_m
         svx_mesh* m = sys->mesh;
         double ms = sys->engine->time_spmv(precision, reps);
         if (avg_ms) *avg_ms = ms;
-        if (bytes_per_launch) *bytes_per_launch = spmv_bytes(m, precision);
+        if (bytes_per_launch) *bytes_per_launch = sys->engine_mf ? mf_spmv_bytes(m) : spmv_bytes(m, precision);
         return SVX_OK;
     });
 }
@@ -856,7 +857,7 @@ int svx_system_time_phases(svx_system* sys, int precision, int reps, double* avg
         if (bytes_per_launch) {
             const double n = (double)m->n, nnz = (double)m->nnz;
             const double nblk = (double)m->nblk, ch = (double)m->n_chunks;
-            bytes_per_launch[0] = spmv_bytes(m, precision);
+            bytes_per_launch[0] = sys->engine_mf ? mf_spmv_bytes(m) : spmv_bytes(m, precision);
             if (precision == SVX_FP64) {
                 bytes_per_launch[1] = 64.0 * n;  // p q r u dinv read, u r z written
                 bytes_per_launch[2] = 0.0;

@@ -91,6 +91,7 @@ struct svx_system {
     svx::DevBuf<svx::InstScal> scal;
     std::unique_ptr<svx::Engine> engine;
     int engine_precision = -1;  // precision the engine's descriptors were built for
+    bool engine_mf = false;     // the engine's FP64 products use the matrix-free operator
     double cond = 0.0;
     bool solved = false, has_u = false;
     svx::InstScal last{};    // scalars of the last solve

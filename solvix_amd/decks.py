@@ -168,14 +168,13 @@ def readme_cantilever(nx: int, ny: int) -> Model:
 
 
 # ------------------------------------------------------------ workloads
-def cfg1(seed: int = SEED_BASE + 1) -> Model:
+def cfg1(seed: int = SEED_BASE + 1, nx: int = 400, ny: int = 40) -> Model:
     """2D CPS4 cantilever: 10 m x 1 m on 400x40 (16,000 elements, 16,441
     nodes, n = 32,882), interior jitter +/-0.15 h, E = 210e9, nu = 0.3,
     t = 0.01, left edge fixed in dofs 1-2, -1e4 N in dof 2 at the tip
-    mid-height node."""
-    nx, ny = 400, 40
+    mid-height node. Other nx x ny: the same cantilever on a finer mesh."""
     rng = SplitMix64(seed)
-    m = quad_model(nx, ny, 10.0, 1.0, 210e9, 0.3, 0.01, jitter=0.15, rng=rng, name="cfg1_cps4_400x40")
+    m = quad_model(nx, ny, 10.0, 1.0, 210e9, 0.3, 0.01, jitter=0.15, rng=rng, name=f"cfg1_cps4_{nx}x{ny}")
     m.bcs = [(left_edge_ids(nx, ny), np.array([1, 2], np.int32))]
     tip = (ny // 2) * (nx + 1) + nx + 1
     m.cload_node, m.cload_dof, m.cload_value = (np.array([tip], np.uint32), np.array([2], np.int32),
