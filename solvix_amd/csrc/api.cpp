@@ -183,38 +183,39 @@ void system_assemble(svx_system* sys, const double* E, const double* nu, double 
     sys->E0 = m->n_elem > 0 ? E[0] : 0.0;
     sys->nu0 = m->n_elem > 0 ? nu[0] : 0.0;
     sys->mf_ready = false;
-    DevBuf<double> ke(std::max<int64_t>(1, m->ke_total));
-    DevBuf<uint8_t> reason(std::max<int64_t>(1, m->n_elem));
-    DevBuf<unsigned long long> first_bad(1);
+    void* tmp[3];
+    m->ctx->scratch.carve(s, {sizeof(double) * std::max<int64_t>(1, m->ke_total), (size_t)std::max<int64_t>(1, m->n_elem),
+                              sizeof(unsigned long long)}, tmp);
+    double* ke = (double*)tmp[0];
+    uint8_t* reason = (uint8_t*)tmp[1];
+    unsigned long long* first_bad = (unsigned long long*)tmp[2];
     unsigned long long init = ~0ull;
-    first_bad.upload(&init, 1, s);
+    SVX_HIP(hipMemcpyAsync(first_bad, &init, sizeof init, hipMemcpyHostToDevice, s));
     for (int k = 0; k < 3; ++k)
         launch_element_ke(k, m->nsel[k], m->sel[k].p, m->conn_ptr.p, m->conn.p, m->coords.p, sys->E.p, sys->nu.p,
-                          thickness, m->ndof, ke.p, m->ke_off.p, reason.p, first_bad.p, s);
+                          thickness, m->ndof, ke, m->ke_off.p, reason, first_bad, s);
     unsigned long long fb = 0;
-    first_bad.download(&fb, 1, s);
+    SVX_HIP(hipMemcpyAsync(&fb, first_bad, sizeof fb, hipMemcpyDeviceToHost, s));
     SVX_HIP(hipStreamSynchronize(s));
     *bad_index = -1;
     *bad_reason = 0;
     if (fb != ~0ull) {
         uint8_t r = 0;
-        SVX_HIP(hipMemcpy(&r, reason.p + fb, 1, hipMemcpyDeviceToHost));
+        SVX_HIP(hipMemcpy(&r, reason + fb, 1, hipMemcpyDeviceToHost));
         *bad_index = (int64_t)fb;
         *bad_reason = r;
         return;
     }
     sys->vals.alloc(m->nnz);
     launch_gather_assembly(m->n, m->ndof, m->row_ptr.p, m->col.p, m->inc_ptr.p, m->inc_elem.p, m->inc_slot.p,
-                           m->conn_ptr.p, m->conn.p, m->kinds.p, ke.p, m->ke_off.p, sys->vals.p, s);
-    SVX_HIP(hipStreamSynchronize(s));
+                           m->conn_ptr.p, m->conn.p, m->kinds.p, ke, m->ke_off.p, sys->vals.p, s);
 }
 
 void system_set_f(svx_system* sys, const double* f) {
     svx_mesh* m = sys->mesh;
     sys->h_f.assign(f, f + m->n);
     sys->f.alloc(m->n);
-    sys->f.upload(f, m->n, m->ctx->stream);
-    SVX_HIP(hipStreamSynchronize(m->ctx->stream));
+    sys->f.upload(sys->h_f.data(), m->n, m->ctx->stream);  // h_f outlives the copy
 }
 
 void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {
@@ -244,9 +245,11 @@ void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {
     sys->d_cons_vals.alloc(std::max<int64_t>(1, tot));
     launch_gather_rows((int64_t)sys->cons_rows.size(), sys->d_cons_rows.p, m->row_ptr.p, sys->vals.p,
                        sys->d_cons_off.p, sys->d_cons_vals.p, s);
-    DevBuf<uint8_t> dmask(m->n);
-    dmask.upload(mask.data(), m->n, s);
-    launch_dirichlet(m->n, m->row_ptr.p, m->col.p, dmask.p, sys->vals.p, sys->f.p, s);
+    void* tmp[1];
+    m->ctx->scratch.carve(s, {(size_t)m->n}, tmp);
+    uint8_t* dmask = (uint8_t*)tmp[0];
+    SVX_HIP(hipMemcpyAsync(dmask, mask.data(), m->n, hipMemcpyHostToDevice, s));
+    launch_dirichlet(m->n, m->row_ptr.p, m->col.p, dmask, sys->vals.p, sys->f.p, s);
     for (int64_t r : sys->cons_rows) sys->h_f[r] = 0.0;
     sys->dirichlet = true;
     // K changed: the preconditioner and the ELL copies are rebuilt on the next solve
@@ -465,31 +468,32 @@ void system_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n
     svx_mesh* m = sys->mesh;
     hipStream_t s = m->ctx->stream;
     if (!sys->has_u) throw std::invalid_argument("no solution to recover from");
+    if (n_rows > 0 && reactions && !sys->dirichlet) throw std::invalid_argument("reactions need svx_apply_dirichlet first");
+    const size_t nc = sys->cons_rows.size();
+    void* tmp[4];
+    m->ctx->scratch.carve(s, {sizeof(double) * std::max<int64_t>(1, m->n_elem), sizeof(double) * std::max<int64_t>(1, m->ns_total),
+                              sizeof(double) * std::max<int64_t>(1, m->n_nodes), sizeof(double) * std::max<size_t>(1, nc)},
+                          tmp);
+    double *d_vm = (double*)tmp[0], *d_ns = (double*)tmp[1], *d_vn = (double*)tmp[2], *d_ku = (double*)tmp[3];
     if (vm_elem || vm_node) {
-        DevBuf<double> d_vm(std::max<int64_t>(1, m->n_elem));
-        DevBuf<double> d_ns(std::max<int64_t>(1, m->ns_total));
         for (int k = 0; k < 3; ++k)
             launch_element_stress(k, m->nsel[k], m->sel[k].p, m->conn_ptr.p, m->conn.p, m->coords.p, sys->E.p,
-                                  sys->nu.p, m->ndof, sys->u.p, d_vm.p, d_ns.p, m->ns_off.p, s);
-        if (vm_elem) d_vm.download(vm_elem, m->n_elem, s);
+                                  sys->nu.p, m->ndof, sys->u.p, d_vm, d_ns, m->ns_off.p, s);
+        if (vm_elem) SVX_HIP(hipMemcpyAsync(vm_elem, d_vm, sizeof(double) * m->n_elem, hipMemcpyDeviceToHost, s));
         if (vm_node) {
-            DevBuf<double> d_vn(std::max<int64_t>(1, m->n_nodes));
-            launch_nodal_average(m->n_nodes, m->inc_ptr.p, m->inc_elem.p, m->inc_slot.p, m->ns_off.p, d_ns.p,
-                                 d_vn.p, s);
-            d_vn.download(vm_node, m->n_nodes, s);
-            SVX_HIP(hipStreamSynchronize(s));
+            launch_nodal_average(m->n_nodes, m->inc_ptr.p, m->inc_elem.p, m->inc_slot.p, m->ns_off.p, d_ns, d_vn, s);
+            SVX_HIP(hipMemcpyAsync(vm_node, d_vn, sizeof(double) * m->n_nodes, hipMemcpyDeviceToHost, s));
         }
-        SVX_HIP(hipStreamSynchronize(s));
     }
+    std::vector<double> ku;
     if (n_rows > 0 && reactions) {
-        if (!sys->dirichlet) throw std::invalid_argument("reactions need svx_apply_dirichlet first");
-        const size_t nc = sys->cons_rows.size();
-        DevBuf<double> d_ku(std::max<size_t>(1, nc));
         launch_reaction_rows((int64_t)nc, sys->d_cons_rows.p, m->row_ptr.p, m->col.p, sys->d_cons_off.p,
-                             sys->d_cons_vals.p, sys->u.p, d_ku.p, s);
-        std::vector<double> ku(nc);
-        d_ku.download(ku.data(), nc, s);
-        SVX_HIP(hipStreamSynchronize(s));
+                             sys->d_cons_vals.p, sys->u.p, d_ku, s);
+        ku.resize(nc);
+        if (nc) SVX_HIP(hipMemcpyAsync(ku.data(), d_ku, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    }
+    SVX_HIP(hipStreamSynchronize(s));
+    if (n_rows > 0 && reactions) {
         for (int64_t i = 0; i < n_rows; ++i) {
             auto it = std::lower_bound(sys->cons_rows.begin(), sys->cons_rows.end(), rows[i]);
             if (it == sys->cons_rows.end() || *it != rows[i])

@@ -1,5 +1,6 @@
 // Handle types behind the C ABI and the host-side model preparation.
 #pragma once
+#include <initializer_list>
 #include <memory>
 #include <string>
 #include <vector>
@@ -22,6 +23,29 @@ struct TuneParams {
     bool matrix_free = false;  // FP64 iterations on the matrix-free operator where a system has one
 };
 
+// Grow-only device scratch of one context for the temporaries of a call
+// (element matrices, masks, recovery buffers), so that assembling or
+// recovering a system allocates and frees no device memory. Every use is on
+// the context's stream: a later call's temporaries are ordered after the
+// kernels of an earlier call that used the same bytes.
+struct Scratch {
+    DevBuf<char> buf;
+    // Room for `sizes` (each 256-byte aligned) carved in order; pointers out[i].
+    void carve(hipStream_t s, std::initializer_list<size_t> sizes, void** out) {
+        size_t total = 0;
+        for (size_t b : sizes) total += (b + 255) / 256 * 256;
+        if (total > buf.n) {
+            SVX_HIP(hipStreamSynchronize(s));  // the old bytes may still be in use
+            buf.alloc(total + total / 2);
+        }
+        size_t off = 0, i = 0;
+        for (size_t b : sizes) {
+            out[i++] = buf.p + off;
+            off += (b + 255) / 256 * 256;
+        }
+    }
+};
+
 }  // namespace svx
 
 struct svx_ctx {
@@ -31,6 +55,7 @@ struct svx_ctx {
     int32_t* h_flag = nullptr;  // pinned
     int32_t* d_flag = nullptr;
     svx::TuneParams tune;
+    svx::Scratch scratch;
 };
 
 struct svx_mesh {
