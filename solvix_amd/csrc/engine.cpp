@@ -117,6 +117,9 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         t.tile_begin = h_descs_[t.inst].tile_begin;
         t.tile_count = h_descs_[t.inst].tile_count;
     }
+    std::vector<Tile> mf_leaders;
+    for (const Tile& t : mf_tiles)
+        if (t.row_begin == 0) mf_leaders.push_back(t);
     tiled_tiles_ = (int)tiles.size();
     int stride[4] = {0, 0, 0, 0};
     size_t off[4] = {0, 0, 0, 0}, total = 0;
@@ -143,6 +146,8 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     tiles_.upload(tiles.data(), tiles.size(), stream_);
     mf_tiles_.alloc(std::max<size_t>(1, mf_tiles.size()));
     mf_tiles_.upload(mf_tiles.data(), mf_tiles.size(), stream_);
+    mf_leaders_.alloc(std::max<size_t>(1, mf_leaders.size()));
+    mf_leaders_.upload(mf_leaders.data(), mf_leaders.size(), stream_);
     descs_.alloc(std::max<size_t>(1, h_descs_.size()));
     descs_.upload(h_descs_.data(), h_descs_.size(), stream_);
     partials_.alloc(std::max<size_t>(kPartialSlots, total));
@@ -167,6 +172,8 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         k.unroll = shape_.unroll[bs] > 0 ? shape_.unroll[bs] : 2;
         k.bs = bs;
         k.mf_tiles = bs == 2 ? mf_tiles_.p : nullptr;
+        k.mf_leaders = bs == 2 ? mf_leaders_.p : nullptr;
+        k.n_mf_inst = bs == 2 ? (int32_t)mf_leaders.size() : 0;
         k.n_mf4 = bs == 2 ? n_mf[0] : 0;
         k.n_mf3 = bs == 2 ? n_mf[1] : 0;
         for (int kk = 0; kk < 2; ++kk) {

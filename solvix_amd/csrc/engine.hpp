@@ -77,6 +77,7 @@ class Engine {
     bool uses_mf(const InstDesc& d) const;
     DevBuf<Tile> tiles_;
     DevBuf<Tile> mf_tiles_;
+    DevBuf<Tile> mf_leaders_;
     DevBuf<InstDesc> descs_;
     DevBuf<double> partials_;
     mutable DevBuf<InstScal> scal_tmp_;  // read_scalars staging

@@ -771,8 +771,40 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, int32_t patch, c
     return dot;
 }
 
+// The pending direction step's scalars for the matrix-free instances, one
+// workgroup per instance (its leader tile): r.r and r.z of the last update
+// over the instance's row tiles, convergence and budget (solver.rs:126-131,
+// 144-148), beta. The product kernel then reads beta and the status instead
+// of reducing the partials in every patch tile: the reduction's latency is
+// paid once per instance rather than at the start of every patch.
+__global__ __launch_bounds__(kBlock) void cg64_mf_step_kernel(const Tile* __restrict__ leaders,
+                                                              const double* __restrict__ partials, int pstride) {
+    const Tile t = leaders[blockIdx.x];
+    const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH) || S->dir) return;
+    const double snap_rz = S->snap_rz;
+    const int64_t snap_it = S->snap_it;
+    const int sl[2] = {P_RR, P_RZ};
+    double rr[2];
+    reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, rr);
+    if (threadIdx.x != 0) return;
+    const double res = sqrt(rr[0]) / S->bnorm;
+    const int64_t it = snap_it + 1;
+    S->it = it;
+    S->rr = rr[0];
+    S->res = res;
+    if (res <= S->tol) {
+        S->status = ST_CONVERGED;
+    } else {
+        S->beta = rr[1] / snap_rz;
+        S->rz = rr[1];
+        if (it >= S->max_iter) S->status = ST_MAXITER;
+    }
+}
+
 // Direction step and q = K p of one FP64 iteration on the matrix-free
-// operator: cg64_spmv_kernel with patches for row tiles. Tile t is
+// operator (cg64_spmv_kernel with node patches for row tiles), after
+// cg64_mf_step_kernel has tested convergence and set beta. Tile t is
 // mf_tiles[base + blockIdx.x]; its p.q partial goes to slot P_MPQ, column
 // base + blockIdx.x.
 template <int NEN>
@@ -792,32 +824,7 @@ __global__ __launch_bounds__(kBlock) void cg64_mf_spmv_kernel(const Tile* __rest
     if (!force_dir && S->dir) {
         v[0] = mf_patch_spmv<NEN>(M, t.row_begin, Plain<double>{cur}, nxt, D.q, cap_nodes, cap_elem);
     } else {
-        double beta = 1.0;
-        if (!force_dir) {
-            const double snap_rz = S->snap_rz;
-            const int64_t snap_it = S->snap_it;
-            const int sl[2] = {P_RR, P_RZ};
-            double rr[2];
-            reduce_tiles<2>(partials, pstride, t.tile_begin, t.tile_count, sl, rr);
-            const double res = sqrt(rr[0]) / S->bnorm;
-            const bool conv = res <= S->tol;
-            const int64_t it = snap_it + 1;
-            const bool budget = it >= S->max_iter;
-            beta = rr[1] / snap_rz;
-            if (is_leader(t)) {
-                S->it = it;
-                S->rr = rr[0];
-                S->res = res;
-                if (conv) {
-                    S->status = ST_CONVERGED;
-                } else {
-                    S->beta = beta;
-                    S->rz = rr[1];
-                    if (budget) S->status = ST_MAXITER;
-                }
-            }
-            if (conv || budget) return;
-        }
+        const double beta = force_dir ? 1.0 : S->beta;
         v[0] = mf_patch_spmv<NEN>(M, t.row_begin, Step<double>{D.z, cur, beta}, nxt, D.q, cap_nodes, cap_elem);
     }
     const int slot[1] = {P_MPQ};
@@ -1382,6 +1389,8 @@ void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir)
 #undef L
         }
     }
+    if (k.n_mf_inst > 0 && !force_dir)
+        cg64_mf_step_kernel<<<k.n_mf_inst, kBlock, 0, s>>>(k.mf_leaders, k.partials, k.pstride);
     if (k.n_mf4 > 0)
         cg64_mf_spmv_kernel<4><<<k.n_mf4, kBlock, mf_lds_bytes(4, k.mf_cap_nodes[0], k.mf_cap_elem[0]), s>>>(
             k.mf_tiles, 0, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[0], k.mf_cap_elem[0]);

@@ -270,6 +270,8 @@ struct KrylovSet {
     int32_t n_ell_tiles = 0;
     const Tile* mf_tiles = nullptr;
     int32_t n_mf4 = 0, n_mf3 = 0;
+    const Tile* mf_leaders = nullptr;  // one per matrix-free instance (its first patch tile)
+    int32_t n_mf_inst = 0;
     int32_t mf_cap_nodes[2] = {0, 0};  // CPS4, CPS3: largest patch of the set
     int32_t mf_cap_elem[2] = {0, 0};
 };
