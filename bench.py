@@ -597,29 +597,41 @@ def main():
 
     spmv32 = "ir32_spmv_kernel<%d,...>" % ndof
     mf64 = args.operator == "matrix_free" and ndof == 2
+    arm_out = {
+        "fp64": {"value": round(b["dofs_per_s"], 1), "ms_per_step": round(b["dt"] / args.steps * 1e3, 3),
+                 "dtype": "f64", "iterations": int(b["its"]), "residual": b["res"], "operator": args.operator,
+                 "max_rel_err_u_vs_fp64": 0.0,
+                 "roofline": roof(b, "cg64_mf_spmv_kernel<4>" if mf64 else "cg64_spmv_kernel<%d,...>" % ndof, "fp64")},
+        "fp32_ir": {"value": round(a["dofs_per_s"], 1), "ms_per_step": round(a["dt"] / args.steps * 1e3, 3),
+                    "dtype": "f32 Krylov + f64 refinement", "iterations": int(a["its"]), "residual": a["res"],
+                    "max_rel_err_u_vs_fp64": err, "roofline": roof(a, spmv32, "fp32_ir")},
+    }
+    # the value is the faster arm (both solve to 1e-10; the FP32-IR arm within 1e-8 of the FP64 one)
+    best = "fp32_ir" if a["dofs_per_s"] > b["dofs_per_s"] and err <= 1e-8 else "fp64"
+    top = arm_out[best]
     out = {
         "metric": "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 PCG",
-        "value": round(a["dofs_per_s"], 1),
+        "value": top["value"],
         "unit": "solved DOF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(a["dt"] / args.steps * 1e3, 3),
+        "ms_per_step": top["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 Krylov + f64 refinement",
+        "dtype": top["dtype"],
         "data": "synthetic (seeded SplitMix64 deck generator)",
         "config": {"workload": CONFIGS[args.config][0], "n_dof": n_dofs, "nnz": int(mesh.nnz),
-                   "tolerance": 1e-10, "precision": "FP32 Krylov, FP64 refinement"},
-        "fp64": {"value": round(b["dofs_per_s"], 1), "ms_per_step": round(b["dt"] / args.steps * 1e3, 3),
-                 "iterations": int(b["its"]), "residual": b["res"], "operator": args.operator,
-                 "roofline": roof(b, "cg64_mf_spmv_kernel<4>" if mf64 else "cg64_spmv_kernel<%d,...>" % ndof, "fp64")},
-        "speedup_vs_fp64": round(a["dofs_per_s"] / b["dofs_per_s"], 3),
-        "iterations": int(a["its"]),
-        "residual": a["res"],
-        "max_rel_err_u_vs_fp64": err,
-        "roofline": roof(a, spmv32, "fp32_ir"),
+                   "tolerance": 1e-10},
+        "arm": best,
+        "iterations": top["iterations"],
+        "residual": top["residual"],
+        "roofline": top["roofline"],
+        "fp64": arm_out["fp64"],
+        "fp32_ir": arm_out["fp32_ir"],
+        "speedup_fp32_ir_vs_fp64": round(a["dofs_per_s"] / b["dofs_per_s"], 3),
+        "max_rel_err_u_fp32_ir_vs_fp64": err,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model, int(b["its"]), args.cpu_cap)

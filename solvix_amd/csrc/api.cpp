@@ -215,7 +215,8 @@ void system_set_f(svx_system* sys, const double* f) {
     svx_mesh* m = sys->mesh;
     sys->h_f.assign(f, f + m->n);
     sys->f.alloc(m->n);
-    sys->f.upload(sys->h_f.data(), m->n, m->ctx->stream);  // h_f outlives the copy
+    sys->f.upload(sys->h_f.data(), m->n, m->ctx->stream);
+    SVX_HIP(hipStreamSynchronize(m->ctx->stream));  // a later call may rewrite h_f
 }
 
 void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {

@@ -130,3 +130,23 @@ def test_matrix_free_larger_instances_and_compaction(mfctx):
             e = rel_err(s.solution(), ref.u)
             print(f"{m.name}: its {its} (oracle {ref.iterations}) err u {e:.1e}")
             assert e <= 1e-8, m.name
+
+
+def test_matrix_free_budget_and_zero_load(mfctx, ctx):
+    """The iteration budget and a zero load behave as on the assembled matrix
+    (solver.rs:93-95, 144-148): NotConverged after exactly max_iter updates;
+    u = 0 with no iteration."""
+    from solvix_amd.model import SolveError
+    m = _plane_models()[1]
+    got = []
+    for c in (ctx, mfctx):
+        mesh, s = _system(m, c)
+        with pytest.raises(SolveError) as e:
+            s.solve(1e-10, 40, Precision.FP64)
+        assert e.value.kind == "NotConverged" and e.value.iterations == 40
+        got.append(e.value.residual)
+        s.set_rhs(np.zeros(mesh.n))
+        u, its, res = s.solve(1e-10, 0, Precision.FP64)
+        assert its == 0 and not np.any(u)
+    # after 40 updates both operators report nearly the same residual
+    assert abs(got[0] - got[1]) <= 1e-6 * got[0]
