@@ -75,15 +75,16 @@ enum { SVX_FP64 = 0, SVX_FP32_IR = 1 };
 /* Operator of the FP64 Krylov iterations (svx_set_operator).
  * SVX_OP_ASSEMBLED:   the assembled K (Csr::mat_vec, assembly.rs:38-72); the
  *                     drop-in, whose products sum in the reference's order.
- * SVX_OP_MATRIX_FREE: systems on plane meshes (ndof 2) of one element kind
- *                     (CPS3 or CPS4, at most 8 / 4 elements per node) and one
- *                     material multiply by K without its values: K p is
- *                     evaluated element by element from the coordinates with
- *                     the formulas of element.rs:91-211, the same product up
- *                     to rounding, so iterates differ from the assembled ones
- *                     in the last bits. Other systems, the FP32 inner
- *                     iterations of SVX_FP32_IR and residual evaluations keep
- *                     the assembled K. */
+ * SVX_OP_MATRIX_FREE: systems of one material on meshes of one element kind,
+ *                     plane (CPS3 or CPS4, at most 8 / 4 elements per node)
+ *                     or solid (C3D8), multiply by K without its values: K p
+ *                     is evaluated element by element from the coordinates
+ *                     with the formulas of element.rs (91-211 plane, 427-561
+ *                     solid, the reference's J^-1 sign pattern included), the
+ *                     same product up to rounding, so iterates differ from the
+ *                     assembled ones in the last bits. Other systems, the FP32
+ *                     inner iterations of SVX_FP32_IR and residual
+ *                     evaluations keep the assembled K. */
 enum { SVX_OP_ASSEMBLED = 0, SVX_OP_MATRIX_FREE = 1 };
 
 typedef struct svx_error {
@@ -202,8 +203,8 @@ int svx_set_operator(svx_ctx* ctx, int op);
  * operator (and until its constraints change), else SVX_OP_ASSEMBLED; -1 for
  * NULL. No reference counterpart. */
 int svx_system_operator(const svx_system* sys);
-/* Host-only check of the matrix-free operator's node patches for a mesh (no
- * device use): plans the patches, checks their limits and that every node
+/* Host-only check of the plane matrix-free operator's node patches for a
+ * mesh (no device use; solid meshes report kind 0): plans the patches, checks their limits and that every node
  * has exactly one owner, and evaluates K x patch by patch as the kernel
  * does against the element-by-element product (seeded x, unit material).
  * stats[5] = {kind (4 CPS4, 3 CPS3, 0 none), patches, largest patch nodes,

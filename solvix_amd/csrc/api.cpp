@@ -265,6 +265,27 @@ void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {
 
 bool mesh_mf_prepare(svx_mesh* m) {
     if (m->mf_state != 0) return m->mf_state > 0;
+    if (m->ndof == 3) {  // C3D8 meshes: ranks of the elements at their nodes
+        bool hex = m->n_elem > 0;
+        for (int64_t e = 0; e < m->n_elem && hex; ++e)
+            hex = m->h_kinds[e] == SVX_C3D8 && m->h_conn_ptr[e] == 8 * e;
+        int32_t width = 0;
+        for (int64_t i = 0; i < m->n_nodes && hex; ++i)
+            width = std::max<int32_t>(width, (int32_t)(m->hp.inc_ptr[i + 1] - m->hp.inc_ptr[i]));
+        if (width > 255 || m->n_nodes * width > (int64_t)INT32_MAX / 3) hex = false;
+        m->mf_state = hex ? 1 : -1;
+        m->mf_kind = hex ? 8 : 0;
+        if (!hex) return false;
+        std::vector<uint8_t> rank((size_t)m->n_elem * 8);
+        for (int64_t i = 0; i < m->n_nodes; ++i)  // incidences are in (element, corner) order
+            for (int64_t q = m->hp.inc_ptr[i]; q < m->hp.inc_ptr[i + 1]; ++q)
+                rank[(size_t)m->hp.inc_elem[q] * 8 + m->hp.inc_slot[q]] = (uint8_t)(q - m->hp.inc_ptr[i]);
+        m->mf_rank.alloc(rank.size());
+        m->mf_rank.upload(rank.data(), rank.size(), m->ctx->stream);
+        SVX_HIP(hipStreamSynchronize(m->ctx->stream));
+        m->mf_width = width;
+        return true;
+    }
     MfPlan plan;
     build_mf_plan(m->ndof, m->n_nodes, m->h_coords.data(), m->n_elem, m->h_kinds.data(), m->h_conn_ptr.data(),
                   m->h_conn.data(), m->hp, plan);
@@ -303,9 +324,14 @@ void system_prepare(svx_system* sys, int precision, bool matrix_free) {
     if (matrix_free && precision == SVX_FP64 && !sys->mf_ready && sys->mat_uniform && sys->dirichlet &&
         mesh_mf_prepare(m)) {
         std::vector<uint8_t> mask(m->n_nodes, 0);
-        for (int64_t r : sys->cons_rows) mask[r / 2] |= (uint8_t)(1u << (r % 2));
+        const int nd = m->ndof;
+        for (int64_t r : sys->cons_rows) mask[r / nd] |= (uint8_t)(1u << (r % nd));
         sys->mf_mask.alloc(m->n_nodes);
         sys->mf_mask.upload(mask.data(), m->n_nodes, s);
+        if (m->mf_kind == 8) {  // unused slots must read as zero
+            sys->mf_fe.alloc((size_t)m->n_nodes * m->mf_width * 3);
+            SVX_HIP(hipMemsetAsync(sys->mf_fe.p, 0, sys->mf_fe.bytes(), s));
+        }
         sys->mf_ready = true;
     }
     if (!sys->dinv.p) {
@@ -379,7 +405,22 @@ InstDesc system_desc(svx_system* sys) {
     d.p_alt = sys->p_alt.p;
     d.p32_alt = sys->p32_alt.p;
     d.s = sys->scal.p;
-    if (sys->mf_ready) {
+    if (sys->mf_ready && m->mf_kind == 8) {
+        d.mf_kind = 8;
+        d.mf_xy = m->coords.p;
+        d.mf_conn = m->conn.p;
+        d.mf_rank = m->mf_rank.p;
+        d.mf_width = m->mf_width;
+        d.mf_fe = sys->mf_fe.p;
+        d.mf_mask = sys->mf_mask.p;
+        d.mf_n_elem = (int32_t)m->n_elem;
+        // d3_matrix (element.rs:33-47)
+        const double E = sys->E0, nu = sys->nu0;
+        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+        d.mf_c11 = lam + 2.0 * mu;
+        d.mf_c12 = lam;
+        d.mf_c33 = mu;
+    } else if (sys->mf_ready) {
         d.mf_kind = m->mf_kind;
         d.mf_patch = m->mf_patch.p;
         d.mf_nodes = m->mf_nodes.p;
@@ -413,6 +454,9 @@ double spmv_bytes(const svx_mesh* m, int precision) {
 // old direction of every node read, the new direction and q written,
 // connectivity (four bytes per element node) and the constrained-dof mask.
 double mf_spmv_bytes(const svx_mesh* m) {
+    if (m->mf_kind == 8)  // two passes: forces per (element, corner) written, read back per node
+        return 24.0 * m->n_nodes + 48.0 * m->n_nodes + 40.0 * m->n_elem + 192.0 * m->n_elem  // element pass
+               + 192.0 * m->n_elem + 1.0 * m->n_nodes + 96.0 * m->n_nodes;                  // node pass
     return 81.0 * (double)m->n_nodes + 4.0 * (double)m->mf_kind * (double)m->n_elem;
 }
 

@@ -90,6 +90,8 @@ struct svx_mesh {
     svx::DevBuf<int32_t> mf_nodes;
     svx::DevBuf<uint16_t> mf_econn, mf_adj;
     svx::DevBuf<double> mf_xy;
+    svx::DevBuf<uint8_t> mf_rank;  // C3D8: rank of each (element, corner) among the corner node's elements
+    int32_t mf_width = 0;          // C3D8: most elements at one node
 };
 
 struct svx_system {
@@ -99,6 +101,7 @@ struct svx_system {
     bool mat_uniform = false;  // one (E, nu) for every element
     double E0 = 0.0, nu0 = 0.0;
     svx::DevBuf<uint8_t> mf_mask;  // constrained dofs per node, for the matrix-free operator
+    svx::DevBuf<double> mf_fe;  // C3D8 matrix-free product: forces per (node, incident element)
     bool mf_ready = false;
     svx::DevBuf<double> vals, f;
     std::vector<double> h_f;  // current load vector (host copy)

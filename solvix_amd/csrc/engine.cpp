@@ -42,7 +42,9 @@ void Engine::set_instances(const std::vector<InstDesc>& descs, const std::vector
     build_tiles(running);
 }
 
-bool Engine::uses_mf(const InstDesc& d) const { return shape_.matrix_free && d.bs == 2 && d.mf_kind != 0; }
+bool Engine::uses_mf(const InstDesc& d) const {
+    return shape_.matrix_free && ((d.bs == 2 && (d.mf_kind == 3 || d.mf_kind == 4)) || (d.bs == 3 && d.mf_kind == 8));
+}
 
 void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     release_graph();
@@ -102,7 +104,7 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
                 for (int32_t r = 0; r < d.n; r += rpt) {
                     Tile t{(int32_t)i, r, std::min(d.n, r + rpt), d.tile_begin, d.tile_count, P_PQ, d.tile_begin,
                            d.tile_count, d.s};
-                    if (mf) t.pq_slot = P_MPQ, t.pq_begin = mf_begin[i], t.pq_count = d.mf_patches;
+                    if (mf && bs == 2) t.pq_slot = P_MPQ, t.pq_begin = mf_begin[i], t.pq_count = d.mf_patches;
                     tiles.push_back(t);
                 }
                 tiled_instances_++;
@@ -117,9 +119,20 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         t.tile_begin = h_descs_[t.inst].tile_begin;
         t.tile_count = h_descs_[t.inst].tile_count;
     }
-    std::vector<Tile> mf_leaders;
+    // one leader per matrix-free instance for the step kernel: plane
+    // instances (group 2) first, then C3D8 ones (group 3); C3D8 element tiles
+    std::vector<Tile> mf_leaders, mf3_tiles;
     for (const Tile& t : mf_tiles)
         if (t.row_begin == 0) mf_leaders.push_back(t);
+    const int32_t n_lead2 = (int32_t)mf_leaders.size();
+    for (size_t i = 0; i < h_descs_.size(); ++i) {
+        const InstDesc& d = h_descs_[i];
+        if (!running[i] || d.bs != 3 || !uses_mf(d)) continue;
+        mf_leaders.push_back(Tile{(int32_t)i, 0, 0, d.tile_begin, d.tile_count, P_PQ, d.tile_begin, d.tile_count, d.s});
+        for (int32_t e = 0; e < d.mf_n_elem; e += kMf3Tile)
+            mf3_tiles.push_back(Tile{(int32_t)i, e, std::min(d.mf_n_elem, e + kMf3Tile), d.tile_begin, d.tile_count,
+                                     P_PQ, d.tile_begin, d.tile_count, d.s});
+    }
     tiled_tiles_ = (int)tiles.size();
     int stride[4] = {0, 0, 0, 0};
     size_t off[4] = {0, 0, 0, 0}, total = 0;
@@ -148,6 +161,8 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     mf_tiles_.upload(mf_tiles.data(), mf_tiles.size(), stream_);
     mf_leaders_.alloc(std::max<size_t>(1, mf_leaders.size()));
     mf_leaders_.upload(mf_leaders.data(), mf_leaders.size(), stream_);
+    mf3_tiles_.alloc(std::max<size_t>(1, mf3_tiles.size()));
+    mf3_tiles_.upload(mf3_tiles.data(), mf3_tiles.size(), stream_);
     descs_.alloc(std::max<size_t>(1, h_descs_.size()));
     descs_.upload(h_descs_.data(), h_descs_.size(), stream_);
     partials_.alloc(std::max<size_t>(kPartialSlots, total));
@@ -172,8 +187,10 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         k.unroll = shape_.unroll[bs] > 0 ? shape_.unroll[bs] : 2;
         k.bs = bs;
         k.mf_tiles = bs == 2 ? mf_tiles_.p : nullptr;
-        k.mf_leaders = bs == 2 ? mf_leaders_.p : nullptr;
-        k.n_mf_inst = bs == 2 ? (int32_t)mf_leaders.size() : 0;
+        k.mf_leaders = bs == 2 ? mf_leaders_.p : bs == 3 ? mf_leaders_.p + n_lead2 : nullptr;
+        k.n_mf_inst = bs == 2 ? n_lead2 : bs == 3 ? (int32_t)mf_leaders.size() - n_lead2 : 0;
+        k.mf3_tiles = bs == 3 ? mf3_tiles_.p : nullptr;
+        k.n_mf3_tiles = bs == 3 ? (int32_t)mf3_tiles.size() : 0;
         k.n_mf4 = bs == 2 ? n_mf[0] : 0;
         k.n_mf3 = bs == 2 ? n_mf[1] : 0;
         for (int kk = 0; kk < 2; ++kk) {

@@ -831,6 +831,148 @@ __global__ __launch_bounds__(kBlock) void cg64_mf_spmv_kernel(const Tile* __rest
     write_partials<1>(v, slot, partials, pstride, base + blockIdx.x);
 }
 
+// ------------------------------------------------ matrix-free C3D8
+// q = K p' for meshes of C3D8 elements of one material, in two passes:
+//  1. cg64_mf3_elem_kernel: batches of kMf3Elems elements per workgroup
+//     (kMf3Tile per tile), eight threads per element, one per Gauss point;
+//     the element's node coordinates and
+//     p' = z + beta p staged in LDS; each thread evaluates its point's
+//     detJ B^T D B p'_e (mf_hex_gp_forces), the element's nodal forces are
+//     the sums over the points (a fixed cross-lane tree), stored node-major
+//     in `fe` (node, rank of the element among the node's elements);
+//  2. cg64_mf3_node_kernel, over the instance's row tiles: each node sums the
+//     forces of its incident elements in element order (the element pass
+//     stores each force in the node's slot for that element, so this pass
+//     reads one contiguous run per node), constrained dofs take q = p', and
+//     the tile stores p' and q and its p'.q partial in the slot the update
+//     reads.
+// Every element is evaluated once; the sums are in fixed orders, so an
+// instance gives the same bits alone and in any batch.
+__global__ __launch_bounds__(kMf3Elems * 8) void cg64_mf3_elem_kernel(const Tile* __restrict__ etiles,
+                                                                      const InstDesc* __restrict__ descs, int par,
+                                                                      int force_dir) {
+    __shared__ double sx[kMf3Elems][8][3];
+    __shared__ double su[kMf3Elems][8][3];
+    const Tile t = etiles[blockIdx.x];
+    const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const InstDesc& dd = descs[t.inst];
+    const bool step = force_dir || !S->dir;
+    const double beta = force_dir ? 1.0 : (step ? S->beta : 0.0);
+    const gptr<const double> cur = global(par ? dd.p_alt : dd.p), z = global(dd.z), X = global(dd.mf_xy);
+    const gptr<const int32_t> conn = global(dd.mf_conn);
+    const gptr<const uint8_t> rk = global(dd.mf_rank);
+    const int el = threadIdx.x >> 3, k = threadIdx.x & 7;
+    // The tile's elements in batches of kMf3Elems: the node loads of the next
+    // batch are issued before the current batch's arithmetic.
+    int32_t g = 0;
+    uint32_t rank = 0;
+    double xv[3] = {0.0, 0.0, 0.0}, pv[3] = {0.0, 0.0, 0.0}, zv[3] = {0.0, 0.0, 0.0};
+    auto fetch = [&](int32_t e) {
+        if (e < t.row_end) {
+            g = conn[8 * (int64_t)e + k];
+            rank = rk[8 * (int64_t)e + k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int32_t i = 3 * g + c;
+                xv[c] = X[i];
+                pv[c] = cur[i];
+                zv[c] = z[i];
+            }
+        }
+    };
+    fetch(t.row_begin + el);
+    for (int32_t e0 = t.row_begin; e0 < t.row_end; e0 += kMf3Elems) {
+        const int32_t e = e0 + el;
+        const bool valid = e < t.row_end;
+        if (e0 != t.row_begin) __syncthreads();  // the previous batch has read the LDS
+        if (valid) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                sx[el][k][c] = xv[c];
+                su[el][k][c] = step ? fmadd(beta, pv[c], zv[c]) : pv[c];
+            }
+        }
+        const int32_t g_cur = g;
+        const uint32_t rank_cur = rank;
+        __syncthreads();
+        fetch(e + kMf3Elems);
+        double f[8][3];
+        if (valid) {
+            mf_hex_gp_forces(k, sx[el], su[el], dd.mf_c11, dd.mf_c12, dd.mf_c33, f);
+        } else {
+#pragma unroll
+            for (int n = 0; n < 8; ++n) f[n][0] = f[n][1] = f[n][2] = 0.0;
+        }
+        // Sum over the element's eight point threads, reduce-scatter across
+        // the eight lanes (partners k ^ 4, k ^ 2, k ^ 1): lane k ends with
+        // node k's forces, summed in a fixed tree that depends on the node
+        // only.
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double mine = (k & 4) ? f[n ^ 4][c] : f[n][c], other = (k & 4) ? f[n][c] : f[n ^ 4][c];
+                f[n][c] = mine + __shfl_xor(other, 4, 8);
+            }
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double mine = (k & 2) ? f[n ^ 2][c] : f[n][c], other = (k & 2) ? f[n][c] : f[n ^ 2][c];
+                f[n][c] = mine + __shfl_xor(other, 2, 8);
+            }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double mine = (k & 1) ? f[1][c] : f[0][c], other = (k & 1) ? f[0][c] : f[1][c];
+            f[0][c] = mine + __shfl_xor(other, 1, 8);
+        }
+        if (valid) {  // node g's slot for this element
+            const gptr<double> fe = global(dd.mf_fe) + ((int64_t)g_cur * dd.mf_width + rank_cur) * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) fe[c] = f[0][c];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void cg64_mf3_node_kernel(const Tile* __restrict__ tiles, int first,
+                                                               const InstDesc* __restrict__ descs,
+                                                               double* __restrict__ partials, int pstride, int par,
+                                                               int force_dir) {
+    const Tile t = tiles[first + blockIdx.x];
+    const gptr<InstScal> S = global(t.s);
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const InstDesc& dd = descs[t.inst];
+    const bool step = force_dir || !S->dir;
+    const double beta = force_dir ? 1.0 : (step ? S->beta : 0.0);
+    const gptr<const double> cur = global(par ? dd.p_alt : dd.p), z = global(dd.z), fe = global(dd.mf_fe);
+    const gptr<double> nxt = global(par ? dd.p : dd.p_alt), q = global(dd.q);
+    const gptr<const uint8_t> mask = global(dd.mf_mask);
+    const int W = dd.mf_width;
+    double v[1] = {0.0};
+    for (int32_t i = t.row_begin / 3 + (int32_t)threadIdx.x; i < t.row_end / 3; i += kBlock) {
+        double a[3] = {0.0, 0.0, 0.0};
+        const gptr<const double> fi = fe + (int64_t)i * W * 3;
+        for (int w = 0; w < W; ++w) {  // element order; unused slots hold zeros
+            a[0] += fi[3 * w];
+            a[1] += fi[3 * w + 1];
+            a[2] += fi[3 * w + 2];
+        }
+        const uint32_t mk = mask[i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int32_t r = 3 * i + c;
+            const double pv = step ? fmadd(beta, cur[r], z[r]) : cur[r];
+            const double qv = (mk >> c) & 1u ? pv : a[c];  // constrained dofs: identity rows
+            nxt[r] = pv;
+            q[r] = qv;
+            v[0] += pv * qv;
+        }
+    }
+    const int slot[1] = {P_PQ};
+    write_partials<1>(v, slot, partials, pstride, first + blockIdx.x);
+}
+
 // u += alpha p, r -= alpha q, z = D^-1 r over [b, e); this thread's r.r, r.z.
 __device__ __forceinline__ void cg64_update_rows(int32_t b, int32_t e, double alpha, gptr<const double> __restrict__ p,
                                                  gptr<const double> __restrict__ q,
@@ -1391,6 +1533,11 @@ void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir)
     }
     if (k.n_mf_inst > 0 && !force_dir)
         cg64_mf_step_kernel<<<k.n_mf_inst, kBlock, 0, s>>>(k.mf_leaders, k.partials, k.pstride);
+    if (k.n_mf3_tiles > 0) {
+        cg64_mf3_elem_kernel<<<k.n_mf3_tiles, kMf3Elems * 8, 0, s>>>(k.mf3_tiles, k.descs, par, force_dir);
+        cg64_mf3_node_kernel<<<k.n_tiles - k.n_ell_tiles, kBlock, 0, s>>>(k.tiles, k.n_ell_tiles, k.descs,
+                                                                            k.partials, k.pstride, par, force_dir);
+    }
     if (k.n_mf4 > 0)
         cg64_mf_spmv_kernel<4><<<k.n_mf4, kBlock, mf_lds_bytes(4, k.mf_cap_nodes[0], k.mf_cap_elem[0]), s>>>(
             k.mf_tiles, 0, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[0], k.mf_cap_elem[0]);

@@ -102,3 +102,91 @@ __host__ __device__ inline void mf_forces(const double (&x)[NEN][2], const doubl
 }
 
 }  // namespace svx
+
+namespace svx {
+
+// One Gauss point of a C3D8 element (c3d8_stiffness, element.rs:535-561:
+// 2x2x2 points, xi outer, zeta inner, unit weights): adds
+// detJ B^T D (B u) for the point gp (bits 2, 1, 0: the signs of xi, eta,
+// zeta) to f, where B is hex_b's (element.rs:466-521) with the reference's
+// J^-1 (inv3, element.rs:447-462, whose odd-parity entries carry an extra
+// sign) and D is d3_matrix (element.rs:33-47: diag = lambda + 2 mu, off =
+// lambda, shear = mu; Voigt order xx, yy, zz, xy, yz, xz). x and u are the
+// element's nodal coordinates and values, [node][3]; f receives [node][3].
+template <typename In, typename Out>
+__host__ __device__ inline void mf_hex_gp_forces(int gp, const In& x, const In& u, double diag, double lam, double mu,
+                                                 Out& f) {
+    const double g = 1.0 / __builtin_sqrt(3.0);
+    const double xi = (gp & 4) ? g : -g, eta = (gp & 2) ? g : -g, zeta = (gp & 1) ? g : -g;
+    const double ca[8] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0};
+    const double cb[8] = {-1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0};
+    const double cc[8] = {-1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0};
+    // dN_i/d(xi, eta, zeta) = 1/8 (a (1 + eta b)(1 + zeta c), ...): with the
+    // corner signs a, b, c = +-1 each derivative takes one of four magnitudes
+    const double x8[2] = {0.125 * (1.0 - xi), 0.125 * (1.0 + xi)};
+    const double e1[2] = {1.0 - eta, 1.0 + eta}, z1[2] = {1.0 - zeta, 1.0 + zeta};
+    const double e8[2] = {0.125 * e1[0], 0.125 * e1[1]};
+    const double dxi[2][2] = {{e8[0] * z1[0], e8[0] * z1[1]}, {e8[1] * z1[0], e8[1] * z1[1]}};
+    const double deta[2][2] = {{x8[0] * z1[0], x8[0] * z1[1]}, {x8[1] * z1[0], x8[1] * z1[1]}};
+    const double dzeta[2][2] = {{x8[0] * e1[0], x8[0] * e1[1]}, {x8[1] * e1[0], x8[1] * e1[1]}};
+    auto dn = [&](int i, double (&d)[3]) {
+        const int ia = ca[i] > 0.0, ib = cb[i] > 0.0, ic = cc[i] > 0.0;
+        d[0] = ca[i] > 0.0 ? dxi[ib][ic] : -dxi[ib][ic];
+        d[1] = cb[i] > 0.0 ? deta[ia][ic] : -deta[ia][ic];
+        d[2] = cc[i] > 0.0 ? dzeta[ia][ib] : -dzeta[ia][ib];
+    };
+    double m[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        double d[3];
+        dn(i, d);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) m[a][b] += d[a] * x[i][b];
+    }
+    const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                       m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    const double id = 1.0 / det;
+    double inv[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int r = j, c = i;
+            const int ra = (r + 1) % 3, rb = (r + 2) % 3, cd = (c + 1) % 3, ce = (c + 2) % 3;
+            const double sign = ((r + c) % 2 == 0) ? 1.0 : -1.0;
+            inv[i][j] = sign * (m[ra][cd] * m[rb][ce] - m[ra][ce] * m[rb][cd]) * id;
+        }
+    // strain B u (engineering shear: xy, yz, xz); B's node columns kept for the forces
+    double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double gx[8][3];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        double d[3];
+        dn(i, d);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) gx[i][b] = inv[b][0] * d[0] + inv[b][1] * d[1] + inv[b][2] * d[2];
+        const double ux = u[i][0], uy = u[i][1], uz = u[i][2];
+        e[0] += gx[i][0] * ux;
+        e[1] += gx[i][1] * uy;
+        e[2] += gx[i][2] * uz;
+        e[3] += gx[i][1] * ux + gx[i][0] * uy;
+        e[4] += gx[i][2] * uy + gx[i][1] * uz;
+        e[5] += gx[i][2] * ux + gx[i][0] * uz;
+    }
+    const double s0 = det * (diag * e[0] + lam * (e[1] + e[2]));
+    const double s1 = det * (diag * e[1] + lam * (e[0] + e[2]));
+    const double s2 = det * (diag * e[2] + lam * (e[0] + e[1]));
+    const double s3 = det * (mu * e[3]), s4 = det * (mu * e[4]), s5 = det * (mu * e[5]);
+    // f_i += B_i^T sigma
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        f[i][0] = gx[i][0] * s0 + gx[i][1] * s3 + gx[i][2] * s5;
+        f[i][1] = gx[i][1] * s1 + gx[i][0] * s3 + gx[i][2] * s4;
+        f[i][2] = gx[i][2] * s2 + gx[i][1] * s4 + gx[i][0] * s5;
+    }
+}
+
+}  // namespace svx

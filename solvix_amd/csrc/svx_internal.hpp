@@ -165,7 +165,19 @@ struct InstDesc {
     const uint16_t* mf_adj;     // [patch][k][kBlock] (slot << 2 | corner) of owned nodes
     const double* mf_xy;        // [n_nodes][2] coordinates
     const uint8_t* mf_mask;     // [n_nodes] constrained dofs (bit d: dof d + 1)
-    double mf_c11, mf_c12, mf_c33, mf_t;  // plane-stress D (element.rs:10-17) and thickness
+    double mf_c11, mf_c12, mf_c33, mf_t;  // plane-stress D (element.rs:10-17) and thickness;
+                                          // C3D8: lambda + 2 mu, lambda, mu (element.rs:33-47)
+    // C3D8 (mf_kind 8): connectivity [n_elem][8]; for each (element, corner)
+    // the element's rank among the corner node's incident elements; the
+    // force buffer of the two-pass product, node-major [n_nodes][mf_width][3]
+    // (slot k of node i: the force of its k-th incident element, in element
+    // order; unused slots stay zero); mf_xy is the mesh coordinates
+    // [n_nodes][3]
+    const int32_t* mf_conn;
+    const uint8_t* mf_rank;
+    double* mf_fe;
+    int32_t mf_n_elem;
+    int32_t mf_width;
     int32_t mf_kind;
     int32_t mf_patches;
     int32_t mf_cap_nodes;  // largest patch (nodes, elements)
@@ -199,6 +211,8 @@ enum : int { P_PQ = 0, P_RR = 1, P_RZ = 2, P_TRR = 3, P_TRZ = 4, P_TFL = 5, P_IF
 // slots per patch (one per thread for CPS4, two for CPS3) and incident
 // elements per owned node (CPS4 4, CPS3 8).
 constexpr int kMfMaxNodes = 2 * 256;
+constexpr int kMf3Elems = 16;  // C3D8 matrix-free product: elements per batch (eight threads each)
+constexpr int kMf3Tile = 4 * kMf3Elems;  // elements per workgroup tile (batches pipelined)
 inline int mf_max_elem(int kind) { return kind == 3 ? 2 * 256 : 256; }
 inline int mf_max_adj(int kind) { return kind == 3 ? 8 : 4; }
 
@@ -272,6 +286,8 @@ struct KrylovSet {
     int32_t n_mf4 = 0, n_mf3 = 0;
     const Tile* mf_leaders = nullptr;  // one per matrix-free instance (its first patch tile)
     int32_t n_mf_inst = 0;
+    const Tile* mf3_tiles = nullptr;  // C3D8 matrix-free element tiles (row_begin / row_end: elements)
+    int32_t n_mf3_tiles = 0;
     int32_t mf_cap_nodes[2] = {0, 0};  // CPS4, CPS3: largest patch of the set
     int32_t mf_cap_elem[2] = {0, 0};
 };

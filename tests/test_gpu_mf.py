@@ -1,9 +1,10 @@
-"""The matrix-free plane-stress operator on the device: FP64 solves whose
-products are evaluated from the element formulas instead of the assembled
-values agree with the oracle within the north-star tolerance, batched
-results equal solo matrix-free results bit for bit, and systems without a
-matrix-free operator (solid meshes, several materials) keep the assembled
-matrix in the same batch, bit for bit as their assembled solo solves."""
+"""The matrix-free operators on the device (plane-stress patches, two-pass
+C3D8): FP64 solves whose products are evaluated from the element formulas
+instead of the assembled values agree with the oracle within the north-star
+tolerance, batched results equal solo matrix-free results bit for bit, and
+systems without a matrix-free operator (several materials) keep the
+assembled matrix in the same batch, bit for bit as their assembled solo
+solves."""
 import numpy as np
 import pytest
 
@@ -86,12 +87,11 @@ def test_matrix_free_batch_equals_solo_bitwise(mfctx, ctx):
         mesh, s = _system(m, mfctx)
         u, its, res = s.solve(1e-10, 0, Precision.FP64)
         solo.append((u, its, res, s.operator))
-    assert [op for *_, op in solo] == ["matrix_free"] * len(plane) + ["assembled", "assembled"]
-    # the assembled solo solves of the two systems without the operator
-    for m, k in ((mixed, -2), (solid, -1)):
-        mesh, s = _system(m, ctx)
-        u, its, res = s.solve(1e-10, 0, Precision.FP64)
-        assert np.array_equal(u.view(np.uint64), solo[k][0].view(np.uint64)), m.name
+    assert [op for *_, op in solo] == ["matrix_free"] * len(plane) + ["assembled", "matrix_free"]
+    # the assembled solo solve of the system without the operator
+    mesh, s = _system(mixed, ctx)
+    u, its, res = s.solve(1e-10, 0, Precision.FP64)
+    assert np.array_equal(u.view(np.uint64), solo[-2][0].view(np.uint64)), mixed.name
     keep = []
     b = Batch(mfctx)
     for m in models:
@@ -150,3 +150,25 @@ def test_matrix_free_budget_and_zero_load(mfctx, ctx):
         assert its == 0 and not np.any(u)
     # after 40 updates both operators report nearly the same residual
     assert abs(got[0] - got[1]) <= 1e-6 * got[0]
+
+
+def _solid_models():
+    out = [decks.cfg2(nx=12, ny=3, nz=3), decks.solid_instance(decks.SplitMix64(5), 3e3, 5e3, name="blk")]
+    m = decks.cfg2(nx=24, ny=4, nz=4)
+    out.append(m)
+    return out
+
+
+def test_matrix_free_solid_matches_oracle(mfctx):
+    """C3D8 meshes (jittered, so the reference's J^-1 sign pattern matters)."""
+    for m in _solid_models():
+        mesh, s = _system(m, mfctx)
+        u, its, res = s.solve(1e-10, 0, Precision.FP64)
+        assert s.operator == "matrix_free", m.name
+        ref = oracle.solve(m)
+        vm, vn, _, _ = s.recover([])
+        eu, ev = rel_err(u, ref.u), rel_err(vm, ref.von_mises)
+        print(f"{m.name}: its {its} (oracle {ref.iterations}) res {res:.2e} err u {eu:.1e} vm {ev:.1e}")
+        assert res <= 1e-10
+        assert eu <= 1e-8 and ev <= 1e-8, m.name
+        assert its <= 1.5 * ref.iterations + 10, m.name
