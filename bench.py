@@ -432,7 +432,8 @@ def bench_serve(args):
     print(f"[bench] {len(stream)} requests generated ({args.rate}/s over {args.lead + args.duration} s)",
           file=sys.stderr, flush=True)
     ctx = Context(0)
-    srv = Server(ctx, Precision.FP32_IR, hbm_budget=200 << 30, max_active=1024)
+    prec = Precision.FP64 if args.serve_precision == "fp64" else Precision.FP32_IR
+    srv = Server(ctx, prec, hbm_budget=200 << 30, max_active=1024)
     subs = [None] * len(stream)
     t0 = time.perf_counter()
 
@@ -473,6 +474,17 @@ def bench_serve(args):
         ref = solve_with_method(req.model, Method.AUTO, Precision.FP64, ctx)
         e = np.abs(d[5].displacements - ref.displacements).max() / np.abs(ref.displacements).max()
         errs.append(float(e))
+    # plain FP64 PCG, one request after another, on a sample of the measured
+    # requests (every k-th, about 60): the same mix solved without the server
+    k = max(1, len(meas) // 60)
+    plain = meas[::k]
+    t1 = time.perf_counter()
+    for d in plain:
+        req = stream[[r.arrival_s for r in stream].index(d[0])]
+        solve_with_method(req.model, Method.AUTO, Precision.FP64, ctx)
+    plain_dps = float(sum(d[4] for d in plain)) / (time.perf_counter() - t1)
+    print(f"[bench] plain FP64 one by one on {len(plain)} sampled requests: {plain_dps / 1e6:.2f} M DOF/s",
+          file=sys.stderr, flush=True)
     line = {
         "metric": "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 PCG",
         "value": round(dof / span, 1),
@@ -484,7 +496,7 @@ def bench_serve(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 Krylov + f64 refinement",
+        "dtype": "f64" if prec == Precision.FP64 else "f32 Krylov + f64 refinement",
         "data": "synthetic (seeded SplitMix64 deck generator)",
         "config": {"workload": CONFIGS["cfg5"][0], "rate_per_s": args.rate, "measured_s": args.duration,
                    "lead_s": args.lead, "requests": len(stream), "measured_requests": len(meas),
@@ -493,6 +505,8 @@ def bench_serve(args):
                        "max": float(lat.max())},
         "server": st,
         "max_rel_err_u_vs_fp64_sample": max(errs) if errs else None,
+        "fp64_sequential_sample": {"value": round(plain_dps, 1), "requests": len(plain)},
+        "speedup_vs_fp64": round(dof / span / plain_dps, 3),
     }
     print(json.dumps(line), flush=True)
 
@@ -517,6 +531,8 @@ def main():
     ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block"],
                     help="multi-RHS workload: run one arm only (profiling)")
     ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
+    ap.add_argument("--serve-precision", default="fp32_ir", choices=["fp32_ir", "fp64"],
+                    help="served stream: precision of the server's solves")
     ap.add_argument("--duration", type=float, default=60.0, help="served stream: measured seconds")
     ap.add_argument("--lead", type=float, default=10.0, help="served stream: unmeasured seconds before")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

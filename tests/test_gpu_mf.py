@@ -172,3 +172,16 @@ def test_matrix_free_solid_matches_oracle(mfctx):
         assert res <= 1e-10
         assert eu <= 1e-8 and ev <= 1e-8, m.name
         assert its <= 1.5 * ref.iterations + 10, m.name
+
+
+def test_fp32_ir_ignores_the_operator(mfctx, ctx):
+    """FP32-IR inner iterations and residuals use the assembled K whatever the
+    context's operator: the same bits as on an assembled-operator context."""
+    for m in (_plane_models()[2], _solid_models()[0]):
+        out = []
+        for c in (ctx, mfctx):
+            mesh, s = _system(m, c)
+            u, its, res = s.solve(1e-10, 0, Precision.FP32_IR)
+            out.append((u, its, res))
+            assert s.operator == "assembled", m.name
+        assert out[0][1] == out[1][1] and np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64)), m.name
