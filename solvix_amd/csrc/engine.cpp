@@ -129,8 +129,8 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         const InstDesc& d = h_descs_[i];
         if (!running[i] || d.bs != 3 || !uses_mf(d)) continue;
         mf_leaders.push_back(Tile{(int32_t)i, 0, 0, d.tile_begin, d.tile_count, P_PQ, d.tile_begin, d.tile_count, d.s});
-        for (int32_t e = 0; e < d.mf_n_elem; e += kMf3Tile)
-            mf3_tiles.push_back(Tile{(int32_t)i, e, std::min(d.mf_n_elem, e + kMf3Tile), d.tile_begin, d.tile_count,
+        for (int32_t e = 0; e < d.mf_n_elem; e += kMf3Elems)
+            mf3_tiles.push_back(Tile{(int32_t)i, e, std::min(d.mf_n_elem, e + kMf3Elems), d.tile_begin, d.tile_count,
                                      P_PQ, d.tile_begin, d.tile_count, d.s});
     }
     tiled_tiles_ = (int)tiles.size();

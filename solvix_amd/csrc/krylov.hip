@@ -833,9 +833,8 @@ __global__ __launch_bounds__(kBlock) void cg64_mf_spmv_kernel(const Tile* __rest
 
 // ------------------------------------------------ matrix-free C3D8
 // q = K p' for meshes of C3D8 elements of one material, in two passes:
-//  1. cg64_mf3_elem_kernel: batches of kMf3Elems elements per workgroup
-//     (kMf3Tile per tile), eight threads per element, one per Gauss point;
-//     the element's node coordinates and
+//  1. cg64_mf3_elem_kernel: kMf3Elems elements per workgroup, eight threads
+//     per element, one per Gauss point; the element's node coordinates and
 //     p' = z + beta p staged in LDS; each thread evaluates its point's
 //     detJ B^T D B p'_e (mf_hex_gp_forces), the element's nodal forces are
 //     the sums over the points (a fixed cross-lane tree), stored node-major
@@ -860,78 +859,62 @@ __global__ __launch_bounds__(kMf3Elems * 8) void cg64_mf3_elem_kernel(const Tile
     const bool step = force_dir || !S->dir;
     const double beta = force_dir ? 1.0 : (step ? S->beta : 0.0);
     const gptr<const double> cur = global(par ? dd.p_alt : dd.p), z = global(dd.z), X = global(dd.mf_xy);
-    const gptr<const int32_t> conn = global(dd.mf_conn);
-    const gptr<const uint8_t> rk = global(dd.mf_rank);
     const int el = threadIdx.x >> 3, k = threadIdx.x & 7;
-    // The tile's elements in batches of kMf3Elems: the node loads of the next
-    // batch are issued before the current batch's arithmetic.
+    const int32_t e = t.row_begin + el;
+    const bool valid = e < t.row_end;
     int32_t g = 0;
     uint32_t rank = 0;
-    double xv[3] = {0.0, 0.0, 0.0}, pv[3] = {0.0, 0.0, 0.0}, zv[3] = {0.0, 0.0, 0.0};
-    auto fetch = [&](int32_t e) {
-        if (e < t.row_end) {
-            g = conn[8 * (int64_t)e + k];
-            rank = rk[8 * (int64_t)e + k];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const int32_t i = 3 * g + c;
-                xv[c] = X[i];
-                pv[c] = cur[i];
-                zv[c] = z[i];
-            }
-        }
-    };
-    fetch(t.row_begin + el);
-    for (int32_t e0 = t.row_begin; e0 < t.row_end; e0 += kMf3Elems) {
-        const int32_t e = e0 + el;
-        const bool valid = e < t.row_end;
-        if (e0 != t.row_begin) __syncthreads();  // the previous batch has read the LDS
-        if (valid) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                sx[el][k][c] = xv[c];
-                su[el][k][c] = step ? fmadd(beta, pv[c], zv[c]) : pv[c];
-            }
-        }
-        const int32_t g_cur = g;
-        const uint32_t rank_cur = rank;
-        __syncthreads();
-        fetch(e + kMf3Elems);
-        double f[8][3];
-        if (valid) {
-            mf_hex_gp_forces(k, sx[el], su[el], dd.mf_c11, dd.mf_c12, dd.mf_c33, f);
-        } else {
-#pragma unroll
-            for (int n = 0; n < 8; ++n) f[n][0] = f[n][1] = f[n][2] = 0.0;
-        }
-        // Sum over the element's eight point threads, reduce-scatter across
-        // the eight lanes (partners k ^ 4, k ^ 2, k ^ 1): lane k ends with
-        // node k's forces, summed in a fixed tree that depends on the node
-        // only.
-#pragma unroll
-        for (int n = 0; n < 4; ++n)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double mine = (k & 4) ? f[n ^ 4][c] : f[n][c], other = (k & 4) ? f[n][c] : f[n ^ 4][c];
-                f[n][c] = mine + __shfl_xor(other, 4, 8);
-            }
-#pragma unroll
-        for (int n = 0; n < 2; ++n)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double mine = (k & 2) ? f[n ^ 2][c] : f[n][c], other = (k & 2) ? f[n][c] : f[n ^ 2][c];
-                f[n][c] = mine + __shfl_xor(other, 2, 8);
-            }
+    if (valid) {  // every load of the node issued before the first LDS store
+        g = global(dd.mf_conn)[8 * (int64_t)e + k];
+        rank = global(dd.mf_rank)[8 * (int64_t)e + k];
+        double xv[3], pv[3], zv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double mine = (k & 1) ? f[1][c] : f[0][c], other = (k & 1) ? f[0][c] : f[1][c];
-            f[0][c] = mine + __shfl_xor(other, 1, 8);
+            const int32_t i = 3 * g + c;
+            xv[c] = X[i];
+            pv[c] = cur[i];
+            zv[c] = z[i];
         }
-        if (valid) {  // node g's slot for this element
-            const gptr<double> fe = global(dd.mf_fe) + ((int64_t)g_cur * dd.mf_width + rank_cur) * 3;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) fe[c] = f[0][c];
+        for (int c = 0; c < 3; ++c) {
+            sx[el][k][c] = xv[c];
+            su[el][k][c] = step ? fmadd(beta, pv[c], zv[c]) : pv[c];
         }
+    }
+    __syncthreads();
+    double f[8][3];
+    if (valid) {
+        mf_hex_gp_forces(k, sx[el], su[el], dd.mf_c11, dd.mf_c12, dd.mf_c33, f);
+    } else {
+#pragma unroll
+        for (int n = 0; n < 8; ++n) f[n][0] = f[n][1] = f[n][2] = 0.0;
+    }
+    // Sum over the element's eight point threads, reduce-scatter across the
+    // eight lanes (partners k ^ 4, k ^ 2, k ^ 1): lane k ends with node k's
+    // forces, summed in a fixed tree that depends on the node only.
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double mine = (k & 4) ? f[n ^ 4][c] : f[n][c], other = (k & 4) ? f[n][c] : f[n ^ 4][c];
+            f[n][c] = mine + __shfl_xor(other, 4, 8);
+        }
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double mine = (k & 2) ? f[n ^ 2][c] : f[n][c], other = (k & 2) ? f[n][c] : f[n ^ 2][c];
+            f[n][c] = mine + __shfl_xor(other, 2, 8);
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double mine = (k & 1) ? f[1][c] : f[0][c], other = (k & 1) ? f[0][c] : f[1][c];
+        f[0][c] = mine + __shfl_xor(other, 1, 8);
+    }
+    if (valid) {  // node g's slot for this element
+        const gptr<double> fe = global(dd.mf_fe) + ((int64_t)g * dd.mf_width + rank) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fe[c] = f[0][c];
     }
 }
 

@@ -159,33 +159,43 @@ __host__ __device__ inline void mf_hex_gp_forces(int gp, const In& x, const In& 
             const double sign = ((r + c) % 2 == 0) ? 1.0 : -1.0;
             inv[i][j] = sign * (m[ra][cd] * m[rb][ce] - m[ra][ce] * m[rb][cd]) * id;
         }
-    // strain B u (engineering shear: xy, yz, xz); B's node columns kept for the forces
-    double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double gx[8][3];
+    // B's node columns are g_i = inv dN_i, so with G = sum_i dN_i u_i^T the
+    // displacement gradient is H = inv G (H[b][c] = sum_i g_i[b] u_i[c]) and
+    // the strain (engineering shear xy, yz, xz) comes from H
+    double G[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         double d[3];
         dn(i, d);
 #pragma unroll
-        for (int b = 0; b < 3; ++b) gx[i][b] = inv[b][0] * d[0] + inv[b][1] * d[1] + inv[b][2] * d[2];
-        const double ux = u[i][0], uy = u[i][1], uz = u[i][2];
-        e[0] += gx[i][0] * ux;
-        e[1] += gx[i][1] * uy;
-        e[2] += gx[i][2] * uz;
-        e[3] += gx[i][1] * ux + gx[i][0] * uy;
-        e[4] += gx[i][2] * uy + gx[i][1] * uz;
-        e[5] += gx[i][2] * ux + gx[i][0] * uz;
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) G[a][c] += d[a] * u[i][c];
     }
-    const double s0 = det * (diag * e[0] + lam * (e[1] + e[2]));
-    const double s1 = det * (diag * e[1] + lam * (e[0] + e[2]));
-    const double s2 = det * (diag * e[2] + lam * (e[0] + e[1]));
-    const double s3 = det * (mu * e[3]), s4 = det * (mu * e[4]), s5 = det * (mu * e[5]);
-    // f_i += B_i^T sigma
+    double H[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) H[b][c] = inv[b][0] * G[0][c] + inv[b][1] * G[1][c] + inv[b][2] * G[2][c];
+    const double exx = H[0][0], eyy = H[1][1], ezz = H[2][2];
+    const double gxy = H[1][0] + H[0][1], gyz = H[2][1] + H[1][2], gxz = H[2][0] + H[0][2];
+    const double s0 = det * (diag * exx + lam * (eyy + ezz));
+    const double s1 = det * (diag * eyy + lam * (exx + ezz));
+    const double s2 = det * (diag * ezz + lam * (exx + eyy));
+    const double s3 = det * (mu * gxy), s4 = det * (mu * gyz), s5 = det * (mu * gxz);
+    // f_i = B_i^T sigma = sum_b g_i[b] S[b][.] = dN_i^T T with T = inv^T S
+    const double S[3][3] = {{s0, s3, s5}, {s3, s1, s4}, {s5, s4, s2}};
+    double T[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) T[a][c] = inv[0][a] * S[0][c] + inv[1][a] * S[1][c] + inv[2][a] * S[2][c];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        f[i][0] = gx[i][0] * s0 + gx[i][1] * s3 + gx[i][2] * s5;
-        f[i][1] = gx[i][1] * s1 + gx[i][0] * s3 + gx[i][2] * s4;
-        f[i][2] = gx[i][2] * s2 + gx[i][1] * s4 + gx[i][0] * s5;
+        double d[3];
+        dn(i, d);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[i][c] = d[0] * T[0][c] + d[1] * T[1][c] + d[2] * T[2][c];
     }
 }
 

@@ -211,8 +211,7 @@ enum : int { P_PQ = 0, P_RR = 1, P_RZ = 2, P_TRR = 3, P_TRZ = 4, P_TFL = 5, P_IF
 // slots per patch (one per thread for CPS4, two for CPS3) and incident
 // elements per owned node (CPS4 4, CPS3 8).
 constexpr int kMfMaxNodes = 2 * 256;
-constexpr int kMf3Elems = 16;  // C3D8 matrix-free product: elements per batch (eight threads each)
-constexpr int kMf3Tile = 4 * kMf3Elems;  // elements per workgroup tile (batches pipelined)
+constexpr int kMf3Elems = 32;  // C3D8 matrix-free product: elements per workgroup (eight threads each)
 inline int mf_max_elem(int kind) { return kind == 3 ? 2 * 256 : 256; }
 inline int mf_max_adj(int kind) { return kind == 3 ? 8 : 4; }
 
