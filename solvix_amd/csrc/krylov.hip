@@ -354,6 +354,21 @@ __device__ __forceinline__ void tile_chunks(const Tile& t, int32_t& c0, int32_t&
 
 __device__ __forceinline__ bool is_leader(const Tile& t) { return t.row_begin == 0 && threadIdx.x == 0; }
 
+// An empty asm statement taking uniform values as operands: their loads
+// complete before it, so loads written ahead of an instance's status test
+// are issued together with the test's own loads instead of being sunk past
+// the early return.
+template <typename T>
+__device__ __forceinline__ int issue_one(const T& v) {
+    asm volatile("" ::"s"(v));
+    return 0;
+}
+template <typename... T>
+__device__ __forceinline__ void issue_before_test(const T&... v) {
+    const int done[] = {issue_one(v)...};
+    (void)done;
+}
+
 // ------------------------------------------------------------ plain SpMV
 __global__ void spmv_ordered_kernel(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                     const double* __restrict__ val, const double* __restrict__ x,
@@ -482,7 +497,6 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
                                                              const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     if (t.row_begin != 0) return;
-    const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || !(S->flags & FL_FRESH)) return;
     const int slot[2] = {P_IFF, P_IRZ};
@@ -592,12 +606,14 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
                                                            int force_dir) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
-    const Dev D = view(descs[t.inst]);
+    const int32_t status = S->status, flags = S->flags, dir = S->dir;
+    const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
+    issue_before_test(D.p, D.p_alt, D.z, D.q, D.ell_off, D.ell_nb, D.ell_col, D.ell_val, par, force_dir);
+    if (status != ST_RUNNING || (flags & FL_FRESH)) return;
     const gptr<const double> cur = par ? D.p_alt : D.p;
     const gptr<double> nxt = par ? D.p : D.p_alt;
     double v[1];
-    if (!force_dir && S->dir) {
+    if (!force_dir && dir) {
         v[0] = tile_spmv<BS, W, double>(t, D, D.ell_val, D.val, Plain<double>{cur}, nxt, D.q);
     } else {
         double beta = 1.0;
@@ -679,17 +695,11 @@ inline size_t mf_lds_bytes(int nen, int cap_nodes, int cap_elem) {
 // One patch of q = K p' (the pending direction step evaluated by x); stores
 // p' and q of the owned nodes; returns this thread's p'.q.
 template <int NEN, typename X>
-__device__ __forceinline__ double mf_patch_spmv(const MfDev& M, int32_t patch, const X& x, gptr<double> nxt,
+__device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P, const X& x, gptr<double> nxt,
                                                 gptr<double> q, int cap_nodes, int cap_elem) {
     constexpr int MAXA = NEN == 4 ? 4 : 8;
     constexpr int ROUNDS = NEN == 4 ? 1 : 2;
     extern __shared__ double mf_lds[];
-    MfPatch P;
-    {
-        const gptr<const MfPatch> pp = M.patch + patch;
-        P.node_off = pp->node_off, P.n_nodes = pp->n_nodes, P.n_owned = pp->n_owned, P.elem_off = pp->elem_off;
-        P.n_elem = pp->n_elem, P.adj_off = pp->adj_off, P.n_adj = pp->n_adj;
-    }
     double* sx = mf_lds;
     double* sy = sx + cap_nodes;
     double* v0 = sy + cap_nodes;
@@ -708,24 +718,27 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, int32_t patch, c
         const int sl = tid + r * kBlock;
         ec[r] = sl < P.n_elem ? M.econn[P.elem_off + sl] : 0ull;
     }
-    double own0 = 0.0, own1 = 0.0;
-    int32_t gown = 0;
-    uint32_t mk = 0;
+    // every global load of the staging phase is issued before its LDS
+    // stores: the node list (clamped to the patch, so no branch), then the
+    // coordinates and p' of both rounds
+    constexpr int NR = kMfMaxNodes / kBlock;
+    int32_t g[NR];
 #pragma unroll
-    for (int r = 0; r < kMfMaxNodes / kBlock; ++r) {
-        const int j = tid + r * kBlock;
-        if (j < P.n_nodes) {
-            const int32_t g = M.nodes[P.node_off + j];
-            const double cx = M.xy[2 * g], cy = M.xy[2 * g + 1];
-            const double a = x(2 * g), b = x(2 * g + 1);
-            sx[j] = cx;
-            sy[j] = cy;
-            v0[j] = a;
-            v1[j] = b;
-            if (r == 0) own0 = a, own1 = b, gown = g;
-        }
+    for (int r = 0; r < NR; ++r) g[r] = M.nodes[P.node_off + min(tid + r * kBlock, P.n_nodes - 1)];
+    double cx[NR], cy[NR], va[NR], vb[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        cx[r] = M.xy[2 * g[r]], cy[r] = M.xy[2 * g[r] + 1];
+        va[r] = x(2 * g[r]), vb[r] = x(2 * g[r] + 1);
     }
-    if (owner) mk = M.mask[gown];
+    const uint32_t mk = owner ? M.mask[g[0]] : 0u;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int j = tid + r * kBlock;
+        if (j < P.n_nodes) sx[j] = cx[r], sy[j] = cy[r], v0[j] = va[r], v1[j] = vb[r];
+    }
+    const double own0 = va[0], own1 = vb[0];
+    const int32_t gown = g[0];
     if (tid < 2 * NEN) fe[tid * fs + P.n_elem] = 0.0;
     __syncthreads();
 #pragma unroll
@@ -813,19 +826,32 @@ __global__ __launch_bounds__(kBlock) void cg64_mf_spmv_kernel(const Tile* __rest
                                                               double* __restrict__ partials, int pstride, int par,
                                                               int force_dir, int cap_nodes, int cap_elem) {
     const Tile t = mf_tiles[base + blockIdx.x];
+    // the instance's state, its descriptor and the patch record are read
+    // together, before the status test, so that their latencies overlap
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const int32_t status = S->status, flags = S->flags, dir = S->dir;
+    const double sbeta = S->beta;
     const InstDesc& dd = descs[t.inst];
     const Dev D = view(dd);
     const MfDev M = mf_view(dd);
+    MfPatch P;
+    {
+        const gptr<const MfPatch> pp = M.patch + t.row_begin;
+        P.node_off = pp->node_off, P.n_nodes = pp->n_nodes, P.n_owned = pp->n_owned, P.elem_off = pp->elem_off;
+        P.n_elem = pp->n_elem, P.adj_off = pp->adj_off, P.n_adj = pp->n_adj;
+    }
+    issue_before_test(D.p, D.p_alt, D.z, D.q, M.nodes, M.econn, M.adj, M.xy, M.mask, P.node_off, P.n_nodes, P.n_owned,
+                      P.elem_off, P.n_elem, P.adj_off, P.n_adj, par, force_dir, dir, sbeta, cap_nodes,
+                      cap_elem, M.mat.c11, M.mat.c12, M.mat.c33, M.mat.t);
+    if (status != ST_RUNNING || (flags & FL_FRESH)) return;
     const gptr<const double> cur = par ? D.p_alt : D.p;
     const gptr<double> nxt = par ? D.p : D.p_alt;
     double v[1];
-    if (!force_dir && S->dir) {
-        v[0] = mf_patch_spmv<NEN>(M, t.row_begin, Plain<double>{cur}, nxt, D.q, cap_nodes, cap_elem);
+    if (!force_dir && dir) {
+        v[0] = mf_patch_spmv<NEN>(M, P, Plain<double>{cur}, nxt, D.q, cap_nodes, cap_elem);
     } else {
-        const double beta = force_dir ? 1.0 : S->beta;
-        v[0] = mf_patch_spmv<NEN>(M, t.row_begin, Step<double>{D.z, cur, beta}, nxt, D.q, cap_nodes, cap_elem);
+        const double beta = force_dir ? 1.0 : sbeta;
+        v[0] = mf_patch_spmv<NEN>(M, P, Step<double>{D.z, cur, beta}, nxt, D.q, cap_nodes, cap_elem);
     }
     const int slot[1] = {P_MPQ};
     write_partials<1>(v, slot, partials, pstride, base + blockIdx.x);
@@ -854,11 +880,13 @@ __global__ __launch_bounds__(kMf3Elems * 8) void cg64_mf3_elem_kernel(const Tile
     __shared__ double su[kMf3Elems][8][3];
     const Tile t = etiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const int32_t status = S->status, flags = S->flags, dir = S->dir;
+    const double sbeta = S->beta;
     const InstDesc& dd = descs[t.inst];
-    const bool step = force_dir || !S->dir;
-    const double beta = force_dir ? 1.0 : (step ? S->beta : 0.0);
     const gptr<const double> cur = global(par ? dd.p_alt : dd.p), z = global(dd.z), X = global(dd.mf_xy);
+    if (status != ST_RUNNING || (flags & FL_FRESH)) return;
+    const bool step = force_dir || !dir;
+    const double beta = force_dir ? 1.0 : (step ? sbeta : 0.0);
     const int el = threadIdx.x >> 3, k = threadIdx.x & 7;
     const int32_t e = t.row_begin + el;
     const bool valid = e < t.row_end;
@@ -924,11 +952,13 @@ __global__ __launch_bounds__(kBlock) void cg64_mf3_node_kernel(const Tile* __res
                                                                int force_dir) {
     const Tile t = tiles[first + blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const int32_t status = S->status, flags = S->flags, dir = S->dir;
+    const double sbeta = S->beta;
     const InstDesc& dd = descs[t.inst];
-    const bool step = force_dir || !S->dir;
-    const double beta = force_dir ? 1.0 : (step ? S->beta : 0.0);
     const gptr<const double> cur = global(par ? dd.p_alt : dd.p), z = global(dd.z), fe = global(dd.mf_fe);
+    if (status != ST_RUNNING || (flags & FL_FRESH)) return;
+    const bool step = force_dir || !dir;
+    const double beta = force_dir ? 1.0 : (step ? sbeta : 0.0);
     const gptr<double> nxt = global(par ? dd.p : dd.p_alt), q = global(dd.q);
     const gptr<const uint8_t> mask = global(dd.mf_mask);
     const int W = dd.mf_width;
@@ -979,8 +1009,10 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
                                                              double* __restrict__ partials, int pstride, int par) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
-    const Dev D = view(descs[t.inst]);
+    const int32_t status = S->status, flags = S->flags;
+    const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
+    issue_before_test(D.p, D.p_alt, D.q, D.dinv, D.u, D.r, D.z, par);
+    if (status != ST_RUNNING || (flags & FL_FRESH)) return;
     const gptr<double> pd = par ? D.p : D.p_alt;
     const int32_t b = t.row_begin, e = t.row_end;
     double p0[kPre], q0[kPre], d0[kPre], u0[kPre], r0[kPre];
@@ -1042,8 +1074,9 @@ __global__ __launch_bounds__(kBlock) void cg64_chunk_end_kernel(const Tile* __re
     // A running instance always has its direction step pending here (every
     // chunk ends with an update). S->dir is not read: this kernel's leader
     // sets it, and a tile that read the new value would skip its rows.
-    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
-    const Dev D = view(descs[t.inst]);
+    const int32_t status = S->status, flags = S->flags;
+    const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
+    if (status != ST_RUNNING || (flags & FL_FRESH)) return;
     const int32_t b = t.row_begin, e = t.row_end;
     double z0[kPre], p0[kPre];
     preload<double>(D.z, b, e, z0);
@@ -1096,8 +1129,9 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
                                                            int force_dir) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
-    const Dev D = view(descs[t.inst]);
+    const int32_t status = S->status, flags = S->flags;
+    const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
+    if (status != ST_RUNNING || (flags & (FL_HOLD | FL_FRESH))) return;
     const gptr<const float> cur = par ? D.p32_alt : D.p32;
     const gptr<float> nxt = par ? D.p32 : D.p32_alt;
     double v[1];
@@ -1178,8 +1212,9 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
                                                              double* __restrict__ partials, int pstride, int par) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
-    const Dev D = view(descs[t.inst]);
+    const int32_t status = S->status, flags = S->flags;
+    const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
+    if (status != ST_RUNNING || (flags & (FL_HOLD | FL_FRESH))) return;
     const gptr<float> pd = par ? D.p32 : D.p32_alt;
     const int32_t b = t.row_begin, e = t.row_end;
     float p0[kPre], q0[kPre], d0[kPre], x0[kPre], r0[kPre];
@@ -1241,8 +1276,9 @@ __global__ __launch_bounds__(kBlock) void ir32_period_end_kernel(const Tile* __r
                                                                  const double* __restrict__ partials, int pstride) {
     const Tile t = tiles[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    if (S->status != ST_RUNNING || (S->flags & (FL_HOLD | FL_FRESH))) return;
-    const Dev D = view(descs[t.inst]);
+    const int32_t status = S->status, flags = S->flags;
+    const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
+    if (status != ST_RUNNING || (flags & (FL_HOLD | FL_FRESH))) return;
     const int32_t b = t.row_begin, e = t.row_end;
     float z0[kPre], p0[kPre];
     preload<float>(D.z32, b, e, z0);
