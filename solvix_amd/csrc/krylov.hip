@@ -34,6 +34,8 @@
 // the residual). An instance whose FP64 residual stops improving above that
 // level (four replacements without halving) falls back to FP64 PCG from its
 // current u.
+#include <algorithm>
+
 #include "mf_element.hpp"
 #include "svx_internal.hpp"
 
@@ -657,7 +659,9 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
 // patch (up to kBlock owned nodes, a run of the mesh's Morton order):
 //  1. the patch's nodes (owned, then halo): coordinates and p' = z + beta p
 //     into LDS (the owned nodes' p' is the value the tile stores);
-//  2. one thread per incident element: its nodal forces f_e into LDS;
+//  2. one thread per incident element: its nodal forces f_e, kept in
+//     registers until every thread has read the node arrays, then stored
+//     over them in LDS;
 //  3. one thread per owned node: q = the sum of its elements' f_e in element
 //     order (a property of the mesh, so independent of the tiling and of the
 //     other instances in the launch); constrained dofs take q = p', the
@@ -686,10 +690,12 @@ __device__ __forceinline__ MfDev mf_view(const InstDesc& d) {
     return m;
 }
 
-// LDS of one patch: coordinates and p' of cap_nodes nodes, then the nodal
-// forces of cap_elem element slots plus a zero slot, component-major.
+// LDS of one patch: coordinates and p' of cap_nodes nodes; once every
+// element has read them, the same bytes hold the nodal forces of cap_elem
+// element slots plus a zero slot, component-major (the smaller footprint
+// lets six CPS3 patches share a compute unit instead of four).
 inline size_t mf_lds_bytes(int nen, int cap_nodes, int cap_elem) {
-    return sizeof(double) * (4 * (size_t)cap_nodes + 2 * (size_t)nen * (cap_elem + 1));
+    return sizeof(double) * std::max(4 * (size_t)cap_nodes, 2 * (size_t)nen * (cap_elem + 1));
 }
 
 // One patch of q = K p' (the pending direction step evaluated by x); stores
@@ -704,7 +710,7 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P
     double* sy = sx + cap_nodes;
     double* v0 = sy + cap_nodes;
     double* v1 = v0 + cap_nodes;
-    double* fe = v1 + cap_nodes;
+    double* fe = mf_lds;  // written once the node arrays are no longer read
     const int fs = cap_elem + 1;
     const int tid = threadIdx.x;
     const bool owner = tid < P.n_owned;
@@ -739,15 +745,15 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P
     }
     const double own0 = va[0], own1 = vb[0];
     const int32_t gown = g[0];
-    if (tid < 2 * NEN) fe[tid * fs + P.n_elem] = 0.0;
     __syncthreads();
+    double f[ROUNDS][NEN][2];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const int sl = tid + r * kBlock;
         if (sl < P.n_elem) {
             const int l[4] = {(int)(ec[r] & 0xffffu), (int)((ec[r] >> 16) & 0xffffu), (int)((ec[r] >> 32) & 0xffffu),
                               (int)(ec[r] >> 48)};
-            double xe[NEN][2], ue[NEN][2], f[NEN][2];
+            double xe[NEN][2], ue[NEN][2];
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
                 xe[a][0] = sx[l[a]];
@@ -755,14 +761,22 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P
                 ue[a][0] = v0[l[a]];
                 ue[a][1] = v1[l[a]];
             }
-            mf_forces<NEN>(xe, ue, M.mat, f);
+            mf_forces<NEN>(xe, ue, M.mat, f[r]);
+        }
+    }
+    __syncthreads();  // the node arrays are dead: the forces take their place
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int sl = tid + r * kBlock;
+        if (sl < P.n_elem) {
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
-                fe[(2 * a) * fs + sl] = f[a][0];
-                fe[(2 * a + 1) * fs + sl] = f[a][1];
+                fe[(2 * a) * fs + sl] = f[r][a][0];
+                fe[(2 * a + 1) * fs + sl] = f[r][a][1];
             }
         }
     }
+    if (tid < 2 * NEN) fe[tid * fs + P.n_elem] = 0.0;
     __syncthreads();
     double dot = 0.0;
     if (owner) {
