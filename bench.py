@@ -37,7 +37,7 @@ CONFIGS = {
     "cfg4": ("64 load cases on one C3D8 120x12x12 cantilever (61,347 DOF), 1-8 point loads each; "
              "one block solve vs 64 single solves", "cfg4"),
     "cfg5": ("served Poisson stream: 20% plane instances (DOF 2e3..2e5), 80% C3D8 blocks (DOF 1e4..3e5), "
-             "FP32 Krylov + FP64 refinement, 200 GB admission budget", "cfg5"),
+             "200 GB admission budget", "cfg5"),
     "cfg3": ("512 uneven plane-stress instances, DOF log-uniform 2e3..2e5, CPS4 p=0.6 else CPS3, "
              "left edge clamped, tip load; one batched launch vs one-by-one FP64", "cfg3"),
 }
@@ -433,7 +433,9 @@ def bench_serve(args):
           file=sys.stderr, flush=True)
     ctx = Context(0)
     prec = Precision.FP64 if args.serve_precision == "fp64" else Precision.FP32_IR
+    ctx.set_operator(args.serve_operator)  # the server keeps the operator in force at its creation
     srv = Server(ctx, prec, hbm_budget=200 << 30, max_active=1024)
+    ctx.set_operator("assembled")  # the FP64 references below multiply the assembled K
     subs = [None] * len(stream)
     t0 = time.perf_counter()
 
@@ -498,12 +500,15 @@ def bench_serve(args):
         "vs_baseline": None,
         "dtype": "f64" if prec == Precision.FP64 else "f32 Krylov + f64 refinement",
         "data": "synthetic (seeded SplitMix64 deck generator)",
-        "config": {"workload": CONFIGS["cfg5"][0], "rate_per_s": args.rate, "measured_s": args.duration,
+        "config": {"workload": CONFIGS["cfg5"][0], "precision": args.serve_precision,
+                   "operator": args.serve_operator if prec == Precision.FP64 else "assembled",
+                   "rate_per_s": args.rate, "measured_s": args.duration,
                    "lead_s": args.lead, "requests": len(stream), "measured_requests": len(meas),
                    "offered_dof_per_s": round(sum(r.model.n_dofs() for r in stream) / (args.lead + args.duration), 1)},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
         "server": st,
+        "operator": args.serve_operator if prec == Precision.FP64 else "assembled",
         "max_rel_err_u_vs_fp64_sample": max(errs) if errs else None,
         "fp64_sequential_sample": {"value": round(plain_dps, 1), "requests": len(plain)},
         "speedup_vs_fp64": round(dof / span / plain_dps, 3),
@@ -531,7 +536,9 @@ def main():
     ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block"],
                     help="multi-RHS workload: run one arm only (profiling)")
     ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
-    ap.add_argument("--serve-precision", default="fp32_ir", choices=["fp32_ir", "fp64"],
+    ap.add_argument("--serve-operator", default="matrix_free", choices=["assembled", "matrix_free"],
+                    help="served stream: operator of the server's FP64 iterations")
+    ap.add_argument("--serve-precision", default="fp64", choices=["fp32_ir", "fp64"],
                     help="served stream: precision of the server's solves")
     ap.add_argument("--duration", type=float, default=60.0, help="served stream: measured seconds")
     ap.add_argument("--lead", type=float, default=10.0, help="served stream: unmeasured seconds before")

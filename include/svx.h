@@ -196,7 +196,8 @@ const char* svx_last_error(void);
 int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile,
                    int period, int chunk, int use_graph, double ru_delta);
 /* Operator of this context's subsequent FP64 solves (svx_solve,
- * svx_batch_solve): SVX_OP_ASSEMBLED (default) or SVX_OP_MATRIX_FREE. No
+ * svx_batch_solve, and the FP64 engine of a server created on the context
+ * afterwards): SVX_OP_ASSEMBLED (default) or SVX_OP_MATRIX_FREE. No
  * reference counterpart: the reference multiplies its assembled CSR only. */
 int svx_set_operator(svx_ctx* ctx, int op);
 /* SVX_OP_MATRIX_FREE once a matrix-free solve has prepared the system's

@@ -144,6 +144,7 @@ int64_t footprint(const svx_mesh& m, int precision) {
 struct svx_server {
     svx_ctx* ctx = nullptr;
     svx_server_config cfg{};
+    svx::TuneParams tune;  // the context's tuning and operator when the server was created
     std::mutex mu;
     std::condition_variable cv_prep, cv_dev, cv_done;
     std::deque<Request*> prep_q, ready_q;
@@ -244,10 +245,11 @@ void svx_server::dev_loop() {
         std::vector<Request*> in32, in64;  // instance order of each engine
         bool dirty32 = false, dirty64 = false;
         const int main_prec = cfg.precision;
+        const bool mf = tune.matrix_free;  // FP64 products on the matrix-free operator where a system has one
         EngineOptions opt;
-        opt.period = ctx->tune.period;
-        opt.chunk = cfg.chunk > 0 ? cfg.chunk : ctx->tune.chunk;
-        opt.use_graph = ctx->tune.use_graph;
+        opt.period = tune.period;
+        opt.chunk = cfg.chunk > 0 ? cfg.chunk : tune.chunk;
+        opt.use_graph = tune.use_graph;
         for (;;) {
             // admit
             std::vector<Request*> admit;
@@ -293,9 +295,9 @@ void svx_server::dev_loop() {
                 }
                 system_set_f(r->sys.get(), r->pm.f.data());
                 system_dirichlet(r->sys.get(), (int64_t)r->pm.cons_rows.size(), r->pm.cons_rows.data());
-                system_prepare(r->sys.get(), main_prec);
+                system_prepare(r->sys.get(), main_prec, mf);
                 const int64_t n = r->mesh->n;
-                InstScal sc = initial_scal(cfg.tol, std::max<int64_t>(1000, 10 * n), ctx->tune.ru_delta,
+                InstScal sc = initial_scal(cfg.tol, std::max<int64_t>(1000, 10 * n), tune.ru_delta,
                                            r->mesh->max_row_len);
                 r->sys->scal.upload(&sc, 1, s);
                 SVX_HIP(hipStreamSynchronize(s));
@@ -313,7 +315,7 @@ void svx_server::dev_loop() {
                         descs.push_back(system_desc(r->sys.get()));
                         systems.push_back(r->sys.get());
                     }
-                    eng.set_instances(descs, std::vector<char>(in.size(), 1), batch_shape(systems, ctx->tune, prec));
+                    eng.set_instances(descs, std::vector<char>(in.size(), 1), batch_shape(systems, tune, prec));
                     dirty = false;
                 }
                 EngineOptions o = opt;
@@ -421,6 +423,7 @@ int svx_server_create(svx_ctx* ctx, const svx_server_config* cfg, svx_server** o
         auto s = std::make_unique<svx_server>();
         s->ctx = ctx;
         s->cfg = *cfg;
+        s->tune = ctx->tune;
         svx_server* p = s.get();
         const int workers = std::max(1, std::min(4, (int)std::thread::hardware_concurrency() / 4));
         for (int i = 0; i < workers; ++i) p->prep_threads.emplace_back([p] { p->prep_loop(); });

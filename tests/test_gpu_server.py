@@ -28,12 +28,28 @@ def _decks():
     return ms
 
 
-@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
-def test_results_alone_equal_results_in_the_mix(precision, ctx):
+@pytest.mark.parametrize("precision,operator", [(Precision.FP64, "assembled"), (Precision.FP32_IR, "assembled"),
+                                                (Precision.FP64, "matrix_free")],
+                         ids=["fp64", "fp32ir", "fp64_matrix_free"])
+def test_results_alone_equal_results_in_the_mix(precision, operator):
+    """matrix_free: a server created on a context with that operator runs its
+    FP64 instances (plane meshes and C3D8 blocks of one material) on it, as
+    the same context's solve does alone."""
     models = _decks()
-    alone = [solve_with_method(m, Method.AUTO, precision, ctx) for m in models]
-    sctx = Context(0)
-    srv = Server(sctx, precision, hbm_budget=1 << 30, max_active=4)
+    actx, sctx = Context(0), Context(0)
+    actx.set_operator(operator)
+    sctx.set_operator(operator)
+    try:
+        alone = [solve_with_method(m, Method.AUTO, precision, actx) for m in models]
+        srv = Server(sctx, precision, hbm_budget=1 << 30, max_active=4)
+        sctx.set_operator("assembled")  # the server keeps the operator it was created with
+        _check_mix(models, alone, srv, precision)
+    finally:
+        actx.close()
+        sctx.close()
+
+
+def _check_mix(models, alone, srv, precision):
     try:
         tickets = [srv.submit(m) for m in models[:4]]
         time.sleep(0.05)  # the rest arrives while the first ones are on the device
@@ -59,7 +75,6 @@ def test_results_alone_equal_results_in_the_mix(precision, ctx):
         assert st["p99_ms"] >= st["p50_ms"] > 0
     finally:
         srv.close()
-        sctx.close()
 
 
 def test_admission_respects_the_budget_and_reports_errors(ctx):
