@@ -263,7 +263,7 @@ void system_dirichlet(svx_system* sys, int64_t n_rows, const int64_t* rows) {
     SVX_HIP(hipStreamSynchronize(s));
 }
 
-bool mesh_mf_prepare(svx_mesh* m) {
+bool mesh_mf_plan(svx_mesh* m, const double* coords) {
     if (m->mf_state != 0) return m->mf_state > 0;
     if (m->ndof == 3) {  // C3D8 meshes: ranks of the elements at their nodes
         bool hex = m->n_elem > 0;
@@ -273,27 +273,46 @@ bool mesh_mf_prepare(svx_mesh* m) {
         for (int64_t i = 0; i < m->n_nodes && hex; ++i)
             width = std::max<int32_t>(width, (int32_t)(m->hp.inc_ptr[i + 1] - m->hp.inc_ptr[i]));
         if (width > 255 || m->n_nodes * width > (int64_t)INT32_MAX / 3) hex = false;
-        m->mf_state = hex ? 1 : -1;
+        m->mf_state = hex ? 2 : -1;
         m->mf_kind = hex ? 8 : 0;
         if (!hex) return false;
-        std::vector<uint8_t> rank((size_t)m->n_elem * 8);
+        m->mf_rank_host.assign((size_t)m->n_elem * 8, 0);
         for (int64_t i = 0; i < m->n_nodes; ++i)  // incidences are in (element, corner) order
             for (int64_t q = m->hp.inc_ptr[i]; q < m->hp.inc_ptr[i + 1]; ++q)
-                rank[(size_t)m->hp.inc_elem[q] * 8 + m->hp.inc_slot[q]] = (uint8_t)(q - m->hp.inc_ptr[i]);
-        m->mf_rank.alloc(rank.size());
-        m->mf_rank.upload(rank.data(), rank.size(), m->ctx->stream);
-        SVX_HIP(hipStreamSynchronize(m->ctx->stream));
+                m->mf_rank_host[(size_t)m->hp.inc_elem[q] * 8 + m->hp.inc_slot[q]] = (uint8_t)(q - m->hp.inc_ptr[i]);
         m->mf_width = width;
         return true;
     }
-    MfPlan plan;
-    build_mf_plan(m->ndof, m->n_nodes, m->h_coords.data(), m->n_elem, m->h_kinds.data(), m->h_conn_ptr.data(),
-                  m->h_conn.data(), m->hp, plan);
-    if (plan.kind == 0) {
+    auto plan = std::make_unique<MfPlan>();
+    build_mf_plan(m->ndof, m->n_nodes, coords, m->n_elem, m->h_kinds.data(), m->h_conn_ptr.data(), m->h_conn.data(),
+                  m->hp, *plan);
+    if (plan->kind == 0) {
         m->mf_state = -1;
         return false;
     }
+    m->mf_kind = plan->kind;
+    m->mf_patches = (int32_t)plan->patches.size();
+    m->mf_cap_nodes = plan->cap_nodes;
+    m->mf_cap_elem = plan->cap_elem;
+    m->mf_elem_slots = plan->elem_slots;
+    m->mf_plan = std::move(plan);
+    m->mf_state = 2;
+    return true;
+}
+
+bool mesh_mf_prepare(svx_mesh* m) {
+    if (m->mf_state == 0) mesh_mf_plan(m, m->h_coords.data());
+    if (m->mf_state != 2) return m->mf_state > 0;
     hipStream_t s = m->ctx->stream;
+    if (m->mf_kind == 8) {
+        m->mf_rank.alloc(m->mf_rank_host.size());
+        m->mf_rank.upload(m->mf_rank_host.data(), m->mf_rank_host.size(), s);
+        SVX_HIP(hipStreamSynchronize(s));
+        std::vector<uint8_t>().swap(m->mf_rank_host);
+        m->mf_state = 1;
+        return true;
+    }
+    const MfPlan& plan = *m->mf_plan;
     m->mf_patch.alloc(plan.patches.size());
     m->mf_patch.upload(plan.patches.data(), plan.patches.size(), s);
     m->mf_nodes.alloc(plan.nodes.size());
@@ -307,11 +326,7 @@ bool mesh_mf_prepare(svx_mesh* m) {
     m->mf_xy.alloc(xy.size());
     m->mf_xy.upload(xy.data(), xy.size(), s);
     SVX_HIP(hipStreamSynchronize(s));
-    m->mf_kind = plan.kind;
-    m->mf_patches = (int32_t)plan.patches.size();
-    m->mf_cap_nodes = plan.cap_nodes;
-    m->mf_cap_elem = plan.cap_elem;
-    m->mf_elem_slots = plan.elem_slots;
+    m->mf_plan.reset();
     m->mf_state = 1;
     return true;
 }

@@ -81,8 +81,11 @@ struct svx_mesh {
     svx::DevBuf<int64_t> ke_off, ns_off;
     int64_t ke_total = 0, ns_total = 0;
     std::vector<double> h_coords;  // [n_nodes][3]
-    // matrix-free operator, planned on first use: 0 not yet, 1 ready, -1 none
+    // matrix-free operator, planned on first use: 0 not yet, 2 planned on the
+    // host (not uploaded), 1 ready, -1 none
     int mf_state = 0;
+    std::unique_ptr<svx::MfPlan> mf_plan;  // plane meshes, while mf_state == 2
+    std::vector<uint8_t> mf_rank_host;     // C3D8 meshes, while mf_state == 2
     int mf_kind = 0;
     int32_t mf_patches = 0, mf_cap_nodes = 0, mf_cap_elem = 0;
     int64_t mf_elem_slots = 0;
@@ -174,6 +177,10 @@ void system_prepare(svx_system* sys, int precision, bool matrix_free = false);
 InstDesc system_desc(svx_system* sys);
 // Plan and upload the mesh's matrix-free patches once; false if it has none.
 bool mesh_mf_prepare(svx_mesh* m);
+// The host half of mesh_mf_prepare (no device calls; coords [n_nodes][3]),
+// for callers that plan off the device thread; false if the mesh has no
+// matrix-free operator.
+bool mesh_mf_plan(svx_mesh* m, const double* coords);
 double spmv_bytes(const svx_mesh* m, int precision);
 // Algorithmic bytes of one matrix-free product (direction step fused in).
 double mf_spmv_bytes(const svx_mesh* m);

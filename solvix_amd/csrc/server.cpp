@@ -187,6 +187,13 @@ void svx_server::prep_loop() {
                 const svx_model_desc& md = r->model->desc;
                 mesh_build_host(r->mesh.get(), r->pm.ndof, md.n_nodes, md.n_elem, md.kinds, md.conn_ptr,
                                 r->pm.conn_pos.data());
+                // the matrix-free plan (host work) here rather than on the
+                // device thread; only systems of one material use it
+                const std::vector<double>& E = r->pm.E;
+                const std::vector<double>& nu = r->pm.nu;
+                bool uniform = tune.matrix_free && cfg.precision == SVX_FP64 && !E.empty();
+                for (size_t e = 1; e < E.size() && uniform; ++e) uniform = E[e] == E[0] && nu[e] == nu[0];
+                if (uniform) mesh_mf_plan(r->mesh.get(), md.coords);
                 r->bytes = footprint(*r->mesh, cfg.precision);
             }
         } catch (const std::exception& e) {
