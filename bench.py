@@ -527,6 +527,8 @@ def main():
     ap.add_argument("--arms", default="both", choices=["both", "fp64", "fp32_ir"],
                     help="run one precision only (profiling); the JSON line needs both")
     ap.add_argument("--spmv-reps", type=int, default=100, help="launches of the timed SpMV kernel")
+    ap.add_argument("--rotation", type=int, default=2,
+                    help="single C3D8 solve: distinct matrices the SpMV rotation leg cycles through (0: skip)")
     ap.add_argument("--batch-arms", default="all", choices=["all", "fp64_batched", "fp64_batched_mf", "fp32_ir_batched"],
                     help="batched workload: every arm, or one batched engine alone (profiling)")
     ap.add_argument("--operator", default="assembled", choices=["assembled", "matrix_free"],
@@ -656,9 +658,45 @@ def main():
         "speedup_fp32_ir_vs_fp64": round(a["dofs_per_s"] / b["dofs_per_s"], 3),
         "max_rel_err_u_fp32_ir_vs_fp64": err,
     }
+    if args.config == "cfg2" and not args.plane_size and args.rotation > 1:
+        out["spmv_rotation"] = spmv_rotation(args, mesh, arms, E, nu, model, load_rows, cons_rows)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model, int(b["its"]), args.cpu_cap)
     print(json.dumps(out), flush=True)
+
+
+def spmv_rotation(args, mesh, arms, E, nu, model, load_rows, cons_rows):
+    """SpMV bandwidth beyond the Infinity Cache: the solver's SpMV launched in
+    turn over `args.rotation` distinct 200x20x20 C3D8 matrices (the bench
+    deck plus decks of other jitter seeds, each on its own mesh), so each
+    launch streams a matrix that the previous launches evicted."""
+    from solvix_amd import decks
+    from solvix_amd.core import Mesh, System
+    from solvix_amd.model import Precision
+
+    extra = []
+    for k in range(1, args.rotation):
+        m = decks.cfg2(seed=decks.SEED_BASE + 2 + 0x100 * k)
+        mk = Mesh.from_model(m, mesh.ctx)
+        sk = mk.assemble(E, nu, m.thickness, load_rows, m.cload_value)
+        sk.apply_dirichlet(cons_rows)
+        extra.append((mk, sk))
+    res = {"systems": args.rotation, "reps": args.spmv_reps}
+    for prec, label in ((Precision.FP64, "fp64"), (Precision.FP32_IR, "fp32_ir")):
+        for _, sk in extra:
+            sk.solve(1e-10, 0, prec)
+        systems = [arms[label]["sys"]] + [sk for _, sk in extra]
+        ms, by = System.time_spmv_rotation(systems, prec, args.spmv_reps)
+        ach = by / (ms * 1e-3) / 1e9
+        res[label] = {"avg_launch_us": round(ms * 1e3, 2), "bytes_per_launch": by, "achieved": round(ach, 1),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                      "matrix_bytes_all": round(by * len(systems))}
+        print(f"[bench] SpMV rotation {label}: {len(systems)} matrices, {ms * 1e3:.1f} us per launch, "
+              f"{ach:.0f} GB/s", file=sys.stderr, flush=True)
+    for mk, sk in extra:
+        sk.close()
+        mk.close()
+    return res
 
 
 if __name__ == "__main__":

@@ -377,6 +377,16 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol,
 int svx_system_time_spmv(svx_system* sys, int precision, int reps,
                          double* avg_ms, double* bytes_per_launch);
 
+/* Measurement hook (no reference counterpart): the SpMV of every system in
+ * systems[0..count) launched in turn, reps rounds, on their shared context's
+ * stream, timed with HIP events. avg_ms receives the average duration of one
+ * launch and bytes_per_launch the average algorithmic bytes of one launch.
+ * With systems whose matrices together exceed the 256 MB Infinity Cache,
+ * every launch streams its matrix from HBM. Every system must have been
+ * solved with that precision on the same context. */
+int svx_time_spmv_rotation(svx_system* const* systems, int count, int precision,
+                           int reps, double* avg_ms, double* bytes_per_launch);
+
 /* Measurement hook (no reference counterpart): average device time of each
  * phase of one solver iteration, each launched alone on this system with the
  * solve's launch shape, reps times: avg_ms[0] SpMV, [1] update (alpha, u, r,
@@ -424,6 +434,12 @@ typedef struct svx_server_summary {
     int64_t solved_dof;        /* DOF of completed requests */
     double p50_ms, p99_ms, max_ms; /* latency of completed requests */
     double dof_per_s;          /* solved DOF / (last completion - first submit) */
+    /* device thread's wall time (ms) by activity: admission (upload,
+     * assembly, constraints, solver set-up), Krylov chunks (tile tables and
+     * graph capture included), retirement (results, recovery, freeing) and
+     * waiting for work; chunk launches and tile-table rebuilds */
+    double admit_ms, solve_ms, retire_ms, idle_ms;
+    int64_t chunks, retiles;
 } svx_server_summary;
 
 int svx_server_create(svx_ctx* ctx, const svx_server_config* cfg,

@@ -110,6 +110,12 @@ class SvxServerSummary(C.Structure):
         ("p99_ms", C.c_double),
         ("max_ms", C.c_double),
         ("dof_per_s", C.c_double),
+        ("admit_ms", C.c_double),
+        ("solve_ms", C.c_double),
+        ("retire_ms", C.c_double),
+        ("idle_ms", C.c_double),
+        ("chunks", C.c_int64),
+        ("retiles", C.c_int64),
     ]
 
 

@@ -53,6 +53,7 @@ SIGNATURES = [
     ("svx_csr_pcg", C.c_int, [_vp, _dp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _errp]),
     ("svx_system_time_spmv", C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     ("svx_system_time_phases", C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    ("svx_time_spmv_rotation", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, _dp]),
     ("svx_batch_create", C.c_int, [_vp, C.POINTER(_vp)]),
     ("svx_batch_destroy", None, [_vp]),
     ("svx_batch_add", C.c_int, [_vp, _vp, C.c_double, _i64, _i64p]),

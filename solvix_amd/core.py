@@ -347,6 +347,18 @@ class System:
         _raise(lib().svx_system_time_spmv(self._h, precision, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    @staticmethod
+    def time_spmv_rotation(systems, precision: int, reps: int = 20) -> Tuple[float, float]:
+        """(average ms, average algorithmic bytes) of one SpMV launch when the
+        systems' SpMV kernels run in turn, reps rounds, on their shared
+        context's stream: with matrices together larger than the Infinity
+        Cache, each launch streams its matrix from HBM."""
+        arr = (C.c_void_p * len(systems))(*[s._h for s in systems])
+        ms = C.c_double()
+        by = C.c_double()
+        _raise(lib().svx_time_spmv_rotation(arr, len(systems), precision, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
     def time_phases(self, precision: int, reps: int = 20) -> Tuple[np.ndarray, np.ndarray]:
         """(average ms, algorithmic bytes) of [SpMV, update, direction update,
         FP32-IR refresh], each phase launched alone. Overwrites the solver

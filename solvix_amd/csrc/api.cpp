@@ -352,12 +352,16 @@ void system_prepare(svx_system* sys, int precision, bool matrix_free) {
     if (!sys->dinv.p) {
         sys->dinv.alloc(n);
         sys->dinv32.alloc(n);
-        DevBuf<unsigned long long> ext(2);
+        // the diagonal's extremes in the context's scratch: a temporary
+        // buffer's free would wait for every stream of the device
+        void* tmp[1];
+        m->ctx->scratch.carve(s, {2 * sizeof(unsigned long long)}, tmp);
+        unsigned long long* ext = (unsigned long long*)tmp[0];
         unsigned long long init[2] = {0ull, ~0ull};
-        ext.upload(init, 2, s);
-        launch_diag_inv(n, m->row_ptr.p, m->col.p, sys->vals.p, sys->dinv.p, sys->dinv32.p, ext.p, ext.p + 1, s);
+        SVX_HIP(hipMemcpyAsync(ext, init, sizeof init, hipMemcpyHostToDevice, s));
+        launch_diag_inv(n, m->row_ptr.p, m->col.p, sys->vals.p, sys->dinv.p, sys->dinv32.p, ext, ext + 1, s);
         unsigned long long h[2];
-        ext.download(h, 2, s);
+        SVX_HIP(hipMemcpyAsync(h, ext, sizeof h, hipMemcpyDeviceToHost, s));
         SVX_HIP(hipStreamSynchronize(s));
         double hi, lo;
         std::memcpy(&hi, &h[0], 8);
@@ -907,6 +911,41 @@ int svx_system_time_spmv(svx_system* sys, int precision, int reps, double* avg_m
         double ms = sys->engine->time_spmv(precision, reps);
         if (avg_ms) *avg_ms = ms;
         if (bytes_per_launch) *bytes_per_launch = sys->engine_mf ? mf_spmv_bytes(m) : spmv_bytes(m, precision);
+        return SVX_OK;
+    });
+}
+
+int svx_time_spmv_rotation(svx_system* const* systems, int count, int precision, int reps, double* avg_ms,
+                           double* bytes_per_launch) {
+    if (!systems || count <= 0 || reps <= 0 || (precision != SVX_FP64 && precision != SVX_FP32_IR))
+        return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        svx_ctx* ctx = nullptr;
+        double bytes = 0.0;
+        for (int i = 0; i < count; ++i) {
+            svx_system* s = systems[i];
+            if (!s || !s->engine || s->engine_precision != precision)
+                throw std::invalid_argument("solve every system with this precision first");
+            if (ctx && s->mesh->ctx != ctx) throw std::invalid_argument("systems of different contexts");
+            ctx = s->mesh->ctx;
+            bytes += s->engine_mf ? mf_spmv_bytes(s->mesh) : spmv_bytes(s->mesh, precision);
+        }
+        for (int i = 0; i < count; ++i) systems[i]->engine->spmv_prepare();
+        for (int i = 0; i < count; ++i) systems[i]->engine->spmv_launch(precision);  // warm
+        hipEvent_t a, b;
+        SVX_HIP(hipEventCreate(&a));
+        SVX_HIP(hipEventCreate(&b));
+        SVX_HIP(hipEventRecord(a, ctx->stream));
+        for (int r = 0; r < reps; ++r)
+            for (int i = 0; i < count; ++i) systems[i]->engine->spmv_launch(precision);
+        SVX_HIP(hipEventRecord(b, ctx->stream));
+        SVX_HIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        SVX_HIP(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (avg_ms) *avg_ms = (double)ms / ((double)reps * count);
+        if (bytes_per_launch) *bytes_per_launch = bytes / count;
         return SVX_OK;
     });
 }

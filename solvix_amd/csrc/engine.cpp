@@ -294,6 +294,7 @@ int Engine::run_phase(const EngineOptions& opt_in, int32_t* h_flag, int32_t* d_f
         done_chunks++;
         launch_count_running(descs_.p, (int32_t)h_descs_.size(), d_flag, stream_);
         SVX_HIP(hipMemcpyAsync(h_flag, d_flag, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+        if (!opt.wait) return -1;
         SVX_HIP(hipStreamSynchronize(stream_));
         const int running = h_flag[0], live_tiles = h_flag[1];
         if (running == 0) return 0;
@@ -348,15 +349,20 @@ void Engine::time_phases(int precision, int reps, double ms[4]) {
     (void)hipEventDestroy(b);
 }
 
+void Engine::spmv_prepare() { launch_set_running(descs_.p, (int32_t)h_descs_.size(), stream_); }
+
+void Engine::spmv_launch(int precision) {
+    for (int bs = 1; bs <= 3; ++bs) spmv_phase(groups_[bs], precision, stream_);
+}
+
 double Engine::time_spmv(int precision, int reps) {
     hipEvent_t a, b;
     SVX_HIP(hipEventCreate(&a));
     SVX_HIP(hipEventCreate(&b));
-    launch_set_running(descs_.p, (int32_t)h_descs_.size(), stream_);
-    for (int bs = 1; bs <= 3; ++bs) spmv_phase(groups_[bs], precision, stream_);  // warm
+    spmv_prepare();
+    spmv_launch(precision);  // warm
     SVX_HIP(hipEventRecord(a, stream_));
-    for (int i = 0; i < reps; ++i)
-        for (int bs = 1; bs <= 3; ++bs) spmv_phase(groups_[bs], precision, stream_);
+    for (int i = 0; i < reps; ++i) spmv_launch(precision);
     SVX_HIP(hipEventRecord(b, stream_));
     SVX_HIP(hipEventSynchronize(b));
     float ms = 0.0f;

@@ -19,6 +19,9 @@ struct EngineOptions {
     int chunk = 32;        // iterations per captured graph
     bool use_graph = true;
     bool compact = true;   // drop finished instances' tiles as they converge
+    // false: enqueue one chunk (and the running count) and return -1 at once;
+    // the caller synchronizes the stream (read_scalars does) before reading.
+    bool wait = true;
 };
 
 struct EngineShape {
@@ -44,6 +47,11 @@ class Engine {
     // Download the per-instance scalars (status, iterations, residual).
     void read_scalars(std::vector<InstScal>& out) const;
     double time_spmv(int precision, int reps);
+    // One SpMV launch (direction step and product) over the current tiles,
+    // for timing several engines' products back to back on one stream;
+    // spmv_prepare() forces every instance to run first.
+    void spmv_prepare();
+    void spmv_launch(int precision);
     // Put every instance back into the tile tables (after a solve compacted
     // them), e.g. before timing the whole set.
     void tile_all() { build_tiles(std::vector<char>(h_descs_.size(), 1)); }

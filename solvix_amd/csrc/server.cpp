@@ -143,6 +143,9 @@ int64_t footprint(const svx_mesh& m, int precision) {
 
 struct svx_server {
     svx_ctx* ctx = nullptr;
+    // Second context on the same device: its stream and scratch carry the
+    // requests' admission and retirement while Krylov chunks run on ctx's.
+    svx_ctx* aux = nullptr;
     svx_server_config cfg{};
     svx::TuneParams tune;  // the context's tuning and operator when the server was created
     std::mutex mu;
@@ -158,8 +161,15 @@ struct svx_server {
     int64_t submitted = 0, completed = 0, failed = 0, resident = 0, peak_resident = 0, peak_active = 0;
     int64_t dof_done = 0;
     double t_first = -1, t_last_done = 0;
+    double admit_ms = 0, solve_ms = 0, retire_ms = 0, idle_ms = 0;  // device thread
+    int64_t chunks = 0, retiles = 0;
     std::vector<double> latencies;
     std::string fatal;
+
+    // systems and meshes of finished requests, freed between chunks (device
+    // thread only; systems before their meshes)
+    std::vector<std::unique_ptr<svx_mesh>> graveyard_mesh;
+    std::vector<std::unique_ptr<svx_system>> graveyard_sys;  // destroyed first
 
     void prep_loop();
     void dev_loop();
@@ -183,7 +193,7 @@ void svx_server::prep_loop() {
             rc = prepare_model(&r->model->desc, r->pm, &err);
             if (rc == SVX_OK) {
                 r->mesh = std::make_unique<svx_mesh>();
-                r->mesh->ctx = ctx;
+                r->mesh->ctx = aux;
                 const svx_model_desc& md = r->model->desc;
                 mesh_build_host(r->mesh.get(), r->pm.ndof, md.n_nodes, md.n_elem, md.kinds, md.conn_ptr,
                                 r->pm.conn_pos.data());
@@ -227,8 +237,10 @@ void svx_server::prep_loop() {
 // Called on the device thread, without the lock, for a request that left
 // the engines (or failed during admission).
 void svx_server::finish(Request* r) {
-    r->mesh.reset();
-    r->sys.reset();
+    // Device memory is released by the device thread once no chunk is in
+    // flight (a free waits for every stream of the device).
+    if (r->sys) graveyard_sys.push_back(std::move(r->sys));
+    if (r->mesh) graveyard_mesh.push_back(std::move(r->mesh));
     std::lock_guard<std::mutex> lk(mu);
     r->t_done = wall_ms();
     r->state = St::Done;
@@ -247,7 +259,8 @@ void svx_server::finish(Request* r) {
 void svx_server::dev_loop() {
     try {
         SVX_HIP(hipSetDevice(ctx->device));
-        hipStream_t s = ctx->stream;
+        hipStream_t s = ctx->stream;   // Krylov chunks
+        hipStream_t sa = aux->stream;  // admission and retirement, beside the running chunk
         Engine eng32(s), eng64(s);
         std::vector<Request*> in32, in64;  // instance order of each engine
         bool dirty32 = false, dirty64 = false;
@@ -257,153 +270,200 @@ void svx_server::dev_loop() {
         opt.period = tune.period;
         opt.chunk = cfg.chunk > 0 ? cfg.chunk : tune.chunk;
         opt.use_graph = tune.use_graph;
+        // Admission on the aux stream: upload, assembly, constraints, solver
+        // set-up. Returns false (after finishing the request) on a model error.
+        auto admit_one = [&](Request* r) -> bool {
+            r->t_admit = wall_ms();
+            const svx_model_desc& md = r->model->desc;
+            mesh_upload(r->mesh.get(), md.coords);
+            r->sys = std::make_unique<svx_system>();
+            r->sys->mesh = r->mesh.get();
+            int64_t bad = -1;
+            int why = 0;
+            system_assemble(r->sys.get(), r->pm.E.data(), r->pm.nu.data(), md.thickness, &bad, &why);
+            if (bad >= 0) {
+                char buf[256];
+                snprintf(buf, sizeof buf, "element %u: %s", md.elem_ids[bad], reason_text(why));
+                r->rc = SVX_E_INVALID_MODEL;
+                put_err(&r->err, SVX_E_INVALID_MODEL, buf, why, bad);
+                finish(r);
+                return false;
+            }
+            const std::string& le = !r->pm.load_error.empty() ? r->pm.load_error : r->pm.bc_error;
+            if (!le.empty()) {
+                r->rc = SVX_E_INVALID_MODEL;
+                put_err(&r->err, SVX_E_INVALID_MODEL, le);
+                finish(r);
+                return false;
+            }
+            system_set_f(r->sys.get(), r->pm.f.data());
+            system_dirichlet(r->sys.get(), (int64_t)r->pm.cons_rows.size(), r->pm.cons_rows.data());
+            system_prepare(r->sys.get(), main_prec, mf);
+            const int64_t n = r->mesh->n;
+            InstScal sc = initial_scal(cfg.tol, std::max<int64_t>(1000, 10 * n), tune.ru_delta, r->mesh->max_row_len);
+            r->sys->scal.upload(&sc, 1, sa);
+            SVX_HIP(hipStreamSynchronize(sa));  // complete before a chunk on s reads it
+            r->phase_precision = main_prec;
+            return true;
+        };
+        // Retirement on the aux stream of a request that left the engines:
+        // status, u, recovery, results, memory.
+        auto retire = [&](Request* r, const InstScal& st) {
+            r->iterations = st.it;
+            r->refreshes = st.refreshes;
+            r->fp64_iterations = st.fp64_it != 0 ? st.it + st.fp64_it : 0;
+            r->residual = st.res;
+            if (st.status != ST_CONVERGED) {
+                char buf[256];
+                snprintf(buf, sizeof buf, "NotConverged { iterations: %lld, residual: %.6e }", (long long)st.it,
+                         st.res);
+                r->rc = SVX_E_NOT_CONVERGED;
+                put_err(&r->err, SVX_E_NOT_CONVERGED, buf, 0, -1, st.it, st.res);
+                finish(r);
+                return;
+            }
+            r->sys->has_u = true;
+            const svx_model_desc& md = r->model->desc;
+            const int nd = r->pm.ndof;
+            std::vector<double> u(r->mesh->n);
+            r->sys->u.download(u.data(), u.size(), sa);
+            SVX_HIP(hipStreamSynchronize(sa));
+            r->disp.assign(md.n_nodes * 3, 0.0);
+            for (int64_t k = 0; k < md.n_nodes; ++k)
+                for (int c = 0; c < nd; ++c) r->disp[k * 3 + c] = u[k * nd + c];
+            r->vm.resize(md.n_elem);
+            r->nvm.resize(md.n_nodes);
+            r->react.resize(r->pm.cons_rows.size());
+            system_recover(r->sys.get(), r->vm.data(), r->nvm.data(), (int64_t)r->pm.cons_rows.size(),
+                           r->pm.cons_rows.data(), r->react.data());
+            r->cond = r->sys->cond;
+            r->rc = SVX_OK;
+            finish(r);
+        };
+        // One chunk per non-empty engine, enqueued without waiting for it.
+        auto step = [&](Engine& eng, std::vector<Request*>& in, bool& dirty, int prec) -> int {
+            if (in.empty()) return 0;
+            int retiled = 0;
+            if (dirty) {
+                std::vector<InstDesc> descs;
+                std::vector<svx_system*> systems;
+                for (Request* r : in) {
+                    descs.push_back(system_desc(r->sys.get()));
+                    systems.push_back(r->sys.get());
+                }
+                eng.set_instances(descs, std::vector<char>(in.size(), 1), batch_shape(systems, tune, prec));
+                dirty = false;
+                retiled = 1;
+            }
+            EngineOptions o = opt;
+            o.precision = prec;
+            o.compact = false;  // membership is rebuilt here as instances retire
+            o.wait = false;
+            eng.run_chunks(o, ctx->h_flag, ctx->d_flag, 1);
+            return retiled;
+        };
+        std::vector<Request*> admitted;                       // set up on sa; join at the next chunk
+        std::vector<std::pair<Request*, InstScal>> retiring;  // left the engines at the last sweep
+        std::vector<Request*> moved;
+        // Reads back the membership an engine stepped over (instance i is
+        // ran[i]) once its chunk is done: running instances stay, finished
+        // ones retire, FP32-IR instances that fall back continue in FP64.
+        auto sweep = [&](Engine& eng, const std::vector<Request*>& in, std::vector<Request*>& next, bool& dirty,
+                         bool fp32) {
+            if (in.empty()) return;
+            std::vector<InstScal> sc;
+            eng.read_scalars(sc);  // waits for the chunk
+            if (sc.size() != in.size()) throw std::logic_error("server: engine membership out of step");
+            std::vector<Request*> keep;
+            for (size_t i = 0; i < in.size(); ++i) {
+                Request* r = in[i];
+                const InstScal& st = sc[i];
+                if (st.status == ST_RUNNING) {
+                    keep.push_back(r);
+                    continue;
+                }
+                dirty = true;
+                if (fp32 && st.status == ST_FALLBACK) {  // continue as FP64 from the current u
+                    InstScal moved_state = st;
+                    moved_state.status = ST_RUNNING;
+                    moved_state.flags = FL_FRESH | FL_WARM;
+                    moved_state.fp64_it = -st.it;
+                    r->sys->scal.upload(&moved_state, 1, s);
+                    SVX_HIP(hipStreamSynchronize(s));
+                    r->phase_precision = SVX_FP64;
+                    moved.push_back(r);
+                    continue;
+                }
+                retiring.emplace_back(r, st);
+            }
+            next.swap(keep);
+        };
         for (;;) {
-            // admit
-            std::vector<Request*> admit;
+            for (Request* r : admitted) {
+                (r->phase_precision == SVX_FP64 ? in64 : in32).push_back(r);
+                (r->phase_precision == SVX_FP64 ? dirty64 : dirty32) = true;
+            }
+            admitted.clear();
+            const bool any = !in32.empty() || !in64.empty();
+            const double t0 = wall_ms();
+            // Each engine steps over its current membership. Instances that
+            // left at the last sweep are not in it: a dirty engine rebuilds
+            // its tile tables before it launches, so from here on no kernel
+            // on s touches a retiring request and its memory may go.
+            const std::vector<Request*> ran32 = in32, ran64 = in64;
+            int64_t n_retiles = step(eng32, in32, dirty32, SVX_FP32_IR);
+            n_retiles += step(eng64, in64, dirty64, SVX_FP64);
+            const int64_t n_chunks = (int64_t)!ran32.empty() + (int64_t)!ran64.empty();
+            const double t1 = wall_ms();
+            // while the chunk runs: admit what fits, then retire
+            std::vector<Request*> take;
             {
-                std::unique_lock<std::mutex> lk(mu);
-                if (in32.empty() && in64.empty())
-                    cv_dev.wait(lk, [&] { return stop || !ready_q.empty(); });
-                if (stop) break;
+                std::lock_guard<std::mutex> lk(mu);
+                if (stop) {
+                    (void)hipStreamSynchronize(s);  // no chunk in flight past the loop
+                    break;
+                }
                 while (!ready_q.empty()) {
                     Request* r = ready_q.front();
-                    const int64_t n_active = (int64_t)(in32.size() + in64.size() + admit.size());
+                    const int64_t n_active = (int64_t)(in32.size() + in64.size() + take.size());
                     if (n_active >= cfg.max_active || resident + r->bytes > cfg.hbm_budget) break;
                     ready_q.pop_front();
                     resident += r->bytes;
                     peak_resident = std::max(peak_resident, resident);
-                    admit.push_back(r);
+                    take.push_back(r);
                 }
-                peak_active = std::max<int64_t>(peak_active, (int64_t)(in32.size() + in64.size() + admit.size()));
+                peak_active = std::max<int64_t>(peak_active, (int64_t)(in32.size() + in64.size() + take.size()));
             }
-            for (Request* r : admit) {
-                r->t_admit = wall_ms();
-                const svx_model_desc& md = r->model->desc;
-                mesh_upload(r->mesh.get(), md.coords);
-                r->sys = std::make_unique<svx_system>();
-                r->sys->mesh = r->mesh.get();
-                int64_t bad = -1;
-                int why = 0;
-                system_assemble(r->sys.get(), r->pm.E.data(), r->pm.nu.data(), md.thickness, &bad, &why);
-                if (bad >= 0) {
-                    char buf[256];
-                    snprintf(buf, sizeof buf, "element %u: %s", md.elem_ids[bad], reason_text(why));
-                    r->rc = SVX_E_INVALID_MODEL;
-                    put_err(&r->err, SVX_E_INVALID_MODEL, buf, why, bad);
-                    finish(r);
-                    continue;
-                }
-                const std::string& le = !r->pm.load_error.empty() ? r->pm.load_error : r->pm.bc_error;
-                if (!le.empty()) {
-                    r->rc = SVX_E_INVALID_MODEL;
-                    put_err(&r->err, SVX_E_INVALID_MODEL, le);
-                    finish(r);
-                    continue;
-                }
-                system_set_f(r->sys.get(), r->pm.f.data());
-                system_dirichlet(r->sys.get(), (int64_t)r->pm.cons_rows.size(), r->pm.cons_rows.data());
-                system_prepare(r->sys.get(), main_prec, mf);
-                const int64_t n = r->mesh->n;
-                InstScal sc = initial_scal(cfg.tol, std::max<int64_t>(1000, 10 * n), tune.ru_delta,
-                                           r->mesh->max_row_len);
-                r->sys->scal.upload(&sc, 1, s);
-                SVX_HIP(hipStreamSynchronize(s));
-                r->phase_precision = main_prec;
-                (main_prec == SVX_FP64 ? in64 : in32).push_back(r);
-                (main_prec == SVX_FP64 ? dirty64 : dirty32) = true;
-            }
-            // one chunk per non-empty engine
-            auto step = [&](Engine& eng, std::vector<Request*>& in, bool& dirty, int prec) {
-                if (in.empty()) return;
-                if (dirty) {
-                    std::vector<InstDesc> descs;
-                    std::vector<svx_system*> systems;
-                    for (Request* r : in) {
-                        descs.push_back(system_desc(r->sys.get()));
-                        systems.push_back(r->sys.get());
-                    }
-                    eng.set_instances(descs, std::vector<char>(in.size(), 1), batch_shape(systems, tune, prec));
-                    dirty = false;
-                }
-                EngineOptions o = opt;
-                o.precision = prec;
-                o.compact = false;  // membership is rebuilt here as instances retire
-                eng.run_chunks(o, ctx->h_flag, ctx->d_flag, 1);
-            };
-            // Each engine steps over its current membership; the sweep below
-            // reads back exactly that membership (instance i of the engine is
-            // ran[i]). An FP32-IR instance that falls back joins in64 for the
-            // next round, after the FP64 engine's own sweep.
-            const std::vector<Request*> ran32 = in32, ran64 = in64;
-            step(eng32, in32, dirty32, SVX_FP32_IR);
-            step(eng64, in64, dirty64, SVX_FP64);
-            std::vector<Request*> moved;
-            // retire / move
-            auto sweep = [&](Engine& eng, const std::vector<Request*>& in, std::vector<Request*>& next, bool& dirty,
-                             bool fp32) {
-                if (in.empty()) return;
-                std::vector<InstScal> sc;
-                eng.read_scalars(sc);
-                if (sc.size() != in.size()) throw std::logic_error("server: engine membership out of step");
-                std::vector<Request*> keep;
-                for (size_t i = 0; i < in.size(); ++i) {
-                    Request* r = in[i];
-                    const InstScal& st = sc[i];
-                    if (st.status == ST_RUNNING) {
-                        keep.push_back(r);
-                        continue;
-                    }
-                    dirty = true;
-                    if (fp32 && st.status == ST_FALLBACK) {  // continue as FP64 from the current u
-                        InstScal moved_state = st;
-                        moved_state.status = ST_RUNNING;
-                        moved_state.flags = FL_FRESH | FL_WARM;
-                        moved_state.fp64_it = -st.it;
-                        r->sys->scal.upload(&moved_state, 1, s);
-                        SVX_HIP(hipStreamSynchronize(s));
-                        r->phase_precision = SVX_FP64;
-                        moved.push_back(r);
-                        continue;
-                    }
-                    r->iterations = st.it;
-                    r->refreshes = st.refreshes;
-                    r->fp64_iterations = st.fp64_it != 0 ? st.it + st.fp64_it : 0;
-                    r->residual = st.res;
-                    if (st.status != ST_CONVERGED) {
-                        char buf[256];
-                        snprintf(buf, sizeof buf, "NotConverged { iterations: %lld, residual: %.6e }",
-                                 (long long)st.it, st.res);
-                        r->rc = SVX_E_NOT_CONVERGED;
-                        put_err(&r->err, SVX_E_NOT_CONVERGED, buf, 0, -1, st.it, st.res);
-                        finish(r);
-                        continue;
-                    }
-                    r->sys->has_u = true;
-                    const svx_model_desc& md = r->model->desc;
-                    const int nd = r->pm.ndof;
-                    std::vector<double> u(r->mesh->n);
-                    r->sys->u.download(u.data(), u.size(), s);
-                    SVX_HIP(hipStreamSynchronize(s));
-                    r->disp.assign(md.n_nodes * 3, 0.0);
-                    for (int64_t k = 0; k < md.n_nodes; ++k)
-                        for (int c = 0; c < nd; ++c) r->disp[k * 3 + c] = u[k * nd + c];
-                    r->vm.resize(md.n_elem);
-                    r->nvm.resize(md.n_nodes);
-                    r->react.resize(r->pm.cons_rows.size());
-                    system_recover(r->sys.get(), r->vm.data(), r->nvm.data(), (int64_t)r->pm.cons_rows.size(),
-                                   r->pm.cons_rows.data(), r->react.data());
-                    r->cond = r->sys->cond;
-                    r->rc = SVX_OK;
-                    finish(r);
-                }
-                next.swap(keep);
-            };
+            for (Request* r : take)
+                if (admit_one(r)) admitted.push_back(r);
+            const double t2 = wall_ms();
+            for (auto& e : retiring) retire(e.first, e.second);
+            retiring.clear();
+            const double t3 = wall_ms();
             sweep(eng32, ran32, in32, dirty32, true);
             sweep(eng64, ran64, in64, dirty64, false);
+            graveyard_sys.clear();  // both streams idle: the frees wait for nothing
+            graveyard_mesh.clear();
             if (!moved.empty()) {
                 in64.insert(in64.end(), moved.begin(), moved.end());
                 dirty64 = true;
+                moved.clear();
             }
+            const double t4 = wall_ms();
+            double t5 = t4;
+            if (!any && admitted.empty() && retiring.empty() && in32.empty() && in64.empty()) {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_dev.wait(lk, [&] { return stop || !ready_q.empty(); });
+                if (stop) break;
+                t5 = wall_ms();
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            solve_ms += (t1 - t0) + (t4 - t3);
+            admit_ms += t2 - t1;
+            retire_ms += t3 - t2;
+            idle_ms += t5 - t4;
+            chunks += n_chunks;
+            retiles += n_retiles;
         }
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu);
@@ -431,6 +491,8 @@ int svx_server_create(svx_ctx* ctx, const svx_server_config* cfg, svx_server** o
         s->ctx = ctx;
         s->cfg = *cfg;
         s->tune = ctx->tune;
+        if (svx_ctx_create(ctx->device, &s->aux) != SVX_OK) return SVX_E_DEVICE;
+        s->aux->tune = ctx->tune;
         svx_server* p = s.get();
         const int workers = std::max(1, std::min(4, (int)std::thread::hardware_concurrency() / 4));
         for (int i = 0; i < workers; ++i) p->prep_threads.emplace_back([p] { p->prep_loop(); });
@@ -453,7 +515,9 @@ void svx_server_destroy(svx_server* srv) {
     for (std::thread& t : srv->prep_threads)
         if (t.joinable()) t.join();
     if (srv->dev_thread.joinable()) srv->dev_thread.join();
-    delete srv;
+    svx_ctx* aux = srv->aux;
+    delete srv;  // requests (and their device memory on aux) first
+    svx_ctx_destroy(aux);
 }
 
 int svx_server_submit(svx_server* srv, const svx_model_desc* model, int64_t* ticket) {
@@ -548,6 +612,12 @@ int svx_server_stats(svx_server* srv, svx_server_summary* out) {
     out->peak_resident_bytes = srv->peak_resident;
     out->peak_active = srv->peak_active;
     out->solved_dof = srv->dof_done;
+    out->admit_ms = srv->admit_ms;
+    out->solve_ms = srv->solve_ms;
+    out->retire_ms = srv->retire_ms;
+    out->idle_ms = srv->idle_ms;
+    out->chunks = srv->chunks;
+    out->retiles = srv->retiles;
     if (!srv->latencies.empty()) {
         std::vector<double> l = srv->latencies;
         std::sort(l.begin(), l.end());

@@ -192,3 +192,29 @@ def test_every_launch_shape_matches_oracle(precision, ctx):
                             np.testing.assert_array_equal(s.displacements, first)
     finally:
         ctx.set_tuning()
+
+
+def test_spmv_rotation_hook(ctx):
+    """The rotation timing hook launches each system's own SpMV: it reports
+    the mean of the systems' algorithmic bytes, leaves their solutions as
+    they were, and refuses a system not solved with that precision."""
+    systems, meshes = [], []
+    for k in range(2):
+        m = decks.cfg2(seed=decks.SEED_BASE + 2 + 0x100 * k, nx=20, ny=4, nz=4)
+        mesh, s = assemble(m, ctx)
+        rows, _, _ = constrained_dofs(m)
+        s.apply_dirichlet(rows)
+        meshes.append(mesh)
+        systems.append(s)
+    us = [s.solve(1e-10, 0, Precision.FP64)[0].copy() for s in systems]
+    single = [s.time_spmv(Precision.FP64, 3)[1] for s in systems]
+    ms, by = type(systems[0]).time_spmv_rotation(systems, Precision.FP64, 5)
+    assert ms > 0.0 and by == pytest.approx(sum(single) / 2, rel=1e-12)
+    for s, u in zip(systems, us):
+        assert np.array_equal(s.solution(), u)
+    with pytest.raises(SolveError):
+        type(systems[0]).time_spmv_rotation(systems, Precision.FP32_IR, 5)
+    for s in systems:
+        s.close()
+    for mesh in meshes:
+        mesh.close()
