@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the FP64 SpMV on the 200x20x20 C3D8 block, from the
 # FETCH_SIZE / WRITE_SIZE counters of rocprofv3 scaled by a 1 GiB device copy
 # (profiles/cg64_spmv_c3d8_200x20x20.md); None where no counter run exists.
-TRAFFIC_BYTES = {("cfg2", "fp64"): 203.8e6}
+TRAFFIC_BYTES = {("cfg2", "fp64"): 207.0e6}
 CONFIGS = {
     "cfg2": ("C3D8 200x20x20 block 20x2x2 m, jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20", "cfg2"),
     "cfg1": ("CPS4 400x40 cantilever 10x1 m, jitter 0.15h, -1e4 N tip load", "cfg1"),

@@ -142,12 +142,12 @@ int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* unroll, i
         EngineShape best = batch_shape(b->sys, b->ctx->tune, precision);
         b->engine->set_instances(descs, all, best);
         double best_ms = iteration_ms(*b->engine, precision, reps);
-        // blocks in flight per lane x ELL chunks per tile (one, split over the
-        // wavefronts; or one per wavefront). Longer tiles time well with every
+        // blocks in flight per lane x ELL chunks per tile (one or two, split
+        // over the wavefronts; or one per wavefront). Longer tiles time well with every
         // instance running but leave the last, largest instances of a batch
         // with few workgroups (measured: FP32-IR batch of 512 plane meshes,
         // 16-chunk tiles 6.5 s against 6.1 s with 4-chunk tiles).
-        static const int kChunksPerTile[] = {1, kBlock / 64};
+        static const int kChunksPerTile[] = {1, 2, kBlock / 64};
         for (int bs = 2; bs <= 3; ++bs) {
             if (!present[bs]) continue;
             for (int u = 1; u <= kMaxUnroll; ++u)

@@ -104,7 +104,7 @@ int default_lanes(double avg_row_len);
 // unroll of a node-block size and precision: functions of the instance alone,
 // so an instance gets the same launch shape, and the same bits, alone and
 // inside any batch.
-int default_rows_per_tile(int bs, int64_t n_rows);
+int default_rows_per_tile(int bs, int64_t n_rows, bool matrix_free = false);
 int unroll_for(int bs, int precision);
 
 }  // namespace svx

@@ -279,7 +279,8 @@ __device__ __forceinline__ void ell_row(int32_t nb, int32_t k0, int32_t ks, gptr
 // row 64c + l; every load of a wavefront covers 64 consecutive words (values,
 // columns) and the column gathers hit the caches. Four or more chunks: one
 // chunk per wavefront at a time. One or two chunks (the tiles of a small
-// instance): the S = 4 or 2 wavefronts of a chunk take every S-th block each
+// instance, and the two-chunk tiles of a large assembled 3x3-block one):
+// the S = 4 or 2 wavefronts of a chunk take every S-th block each
 // and the owner adds their sums in LDS in wavefront order, so a small
 // instance still spreads over all four wavefronts. The split is a function of
 // the tile, hence of the instance alone.
