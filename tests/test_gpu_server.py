@@ -73,6 +73,10 @@ def _check_mix(models, alone, srv, precision):
         assert st["completed"] == len(models) and st["failed"] == 0
         assert 1 <= st["peak_active"] <= 4
         assert st["p99_ms"] >= st["p50_ms"] > 0
+        # device-thread accounting: every request was admitted and retired,
+        # and the engines ran at least one chunk per request's join
+        assert st["chunks"] >= 1 and st["retiles"] >= 1 and st["retiles"] <= st["chunks"]
+        assert st["admit_ms"] > 0 and st["solve_ms"] > 0 and st["retire_ms"] > 0 and st["idle_ms"] >= 0
     finally:
         srv.close()
 
