@@ -341,30 +341,37 @@ __global__ __launch_bounds__(kHexPerBlock * 24) void ke_c3d8_kernel(
     }
 }
 
-// One wavefront per CSR row; lanes walk the row's entries.
+// One wavefront per CSR row; lanes walk the row's entries. The row, and so
+// its node's incident elements, their connectivity and Ke row offsets, is
+// uniform over the wavefront: taken through readfirstlane, those reads are
+// scalar loads, and each lane loads only the Ke entries its column matches.
+// Contributions are added in incidence order, then corner order (a node
+// listed twice in a degenerate element contributes twice).
 __global__ void gather_assembly_kernel(int64_t n_rows, int ndof, const int32_t* __restrict__ row_ptr,
                                        const int32_t* __restrict__ col, const int64_t* __restrict__ inc_ptr,
                                        const int32_t* __restrict__ inc_elem, const int8_t* __restrict__ inc_slot,
                                        const int64_t* __restrict__ conn_ptr, const int32_t* __restrict__ conn,
                                        const uint8_t* __restrict__ kinds, const double* __restrict__ ke,
                                        const int64_t* __restrict__ ke_off, double* __restrict__ vals) {
-    int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    int lane = threadIdx.x % kWave;
+    // n_rows < 2^31 (int32 CSR indices)
+    const int32_t r = __builtin_amdgcn_readfirstlane(
+        (int32_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+    const int lane = threadIdx.x % kWave;
     if (r >= n_rows) return;
-    int64_t i = r / ndof;
-    int a = (int)(r % ndof);
-    int64_t ib = inc_ptr[i], ie = inc_ptr[i + 1];
+    const int32_t i = r / ndof;
+    const int a = r % ndof;
+    const int64_t ib = inc_ptr[i], ie = inc_ptr[i + 1];
     for (int32_t k = row_ptr[r] + lane; k < row_ptr[r + 1]; k += kWave) {
-        int32_t c = col[k];
-        int32_t j = c / ndof;
-        int b = c % ndof;
+        const int32_t c = col[k];
+        const int32_t j = c / ndof;
+        const int b = c % ndof;
         double v = 0.0;
         for (int64_t qq = ib; qq < ie; ++qq) {
-            int32_t e = inc_elem[qq];
-            int li = inc_slot[qq];
-            int64_t cb = conn_ptr[e];
-            int nen = (int)(conn_ptr[e + 1] - cb);
-            int N = nen * ndof;
+            const int32_t e = inc_elem[qq];
+            const int li = inc_slot[qq];
+            const int64_t cb = conn_ptr[e];
+            const int nen = (int)(conn_ptr[e + 1] - cb);
+            const int N = nen * ndof;
             const double* kr = ke + ke_off[e] + (int64_t)(li * ndof + a) * N;
             for (int lj = 0; lj < nen; ++lj)
                 if (conn[cb + lj] == j) v += kr[lj * ndof + b];
