@@ -434,7 +434,7 @@ def bench_serve(args):
     ctx = Context(0)
     prec = Precision.FP64 if args.serve_precision == "fp64" else Precision.FP32_IR
     ctx.set_operator(args.serve_operator)  # the server keeps the operator in force at its creation
-    srv = Server(ctx, prec, hbm_budget=200 << 30, max_active=1024)
+    srv = Server(ctx, prec, hbm_budget=200 << 30, max_active=1024, use_graph=args.serve_graph == "on")
     ctx.set_operator("assembled")  # the FP64 references below multiply the assembled K
     subs = [None] * len(stream)
     t0 = time.perf_counter()
@@ -502,7 +502,7 @@ def bench_serve(args):
         "data": "synthetic (seeded SplitMix64 deck generator)",
         "config": {"workload": CONFIGS["cfg5"][0], "precision": args.serve_precision,
                    "operator": args.serve_operator if prec == Precision.FP64 else "assembled",
-                   "rate_per_s": args.rate, "measured_s": args.duration,
+                   "rate_per_s": args.rate, "measured_s": args.duration, "graphs": args.serve_graph,
                    "lead_s": args.lead, "requests": len(stream), "measured_requests": len(meas),
                    "offered_dof_per_s": round(sum(r.model.n_dofs() for r in stream) / (args.lead + args.duration), 1)},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
@@ -542,6 +542,8 @@ def main():
                     help="served stream: operator of the server's FP64 iterations")
     ap.add_argument("--serve-precision", default="fp64", choices=["fp32_ir", "fp64"],
                     help="served stream: precision of the server's solves")
+    ap.add_argument("--serve-graph", default="off", choices=["on", "off"],
+                    help="served stream: run the server's chunks as captured hipGraphs (on) or kernel by kernel")
     ap.add_argument("--duration", type=float, default=60.0, help="served stream: measured seconds")
     ap.add_argument("--lead", type=float, default=10.0, help="served stream: unmeasured seconds before")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -416,7 +416,9 @@ typedef struct svx_server_config {
     int64_t hbm_budget; /* device bytes the admitted requests may hold */
     double tol;         /* relative residual; 1e-10 is the reference's */
     int32_t chunk;      /* iterations between admissions; 0 = context default */
-    int32_t pad;
+    int32_t use_graph;  /* 1: each chunk as a captured hipGraph; 0: kernel by
+                           kernel (membership changes at most chunk boundaries
+                           of a busy server, so a capture is rarely reused) */
 } svx_server_config;
 
 typedef struct svx_request_info {

@@ -79,7 +79,7 @@ class SvxServerConfig(C.Structure):
         ("hbm_budget", C.c_int64),
         ("tol", C.c_double),
         ("chunk", C.c_int32),
-        ("pad", C.c_int32),
+        ("use_graph", C.c_int32),
     ]
 
 

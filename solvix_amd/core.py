@@ -500,9 +500,9 @@ class Server:
     server owns its context: use a context of its own."""
 
     def __init__(self, ctx: Context, precision: int = Precision.FP32_IR, hbm_budget: int = 200 << 30,
-                 max_active: int = 256, tol: float = 1e-10, chunk: int = 0):
+                 max_active: int = 256, tol: float = 1e-10, chunk: int = 0, use_graph: bool = False):
         self.ctx = ctx
-        cfg = _abi.SvxServerConfig(precision, max_active, hbm_budget, tol, chunk, 0)
+        cfg = _abi.SvxServerConfig(precision, max_active, hbm_budget, tol, chunk, int(use_graph))
         self._h = C.c_void_p()
         _raise(lib().svx_server_create(ctx.handle, C.byref(cfg), C.byref(self._h)))
         self._pending = {}
