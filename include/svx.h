@@ -145,7 +145,10 @@ typedef struct svx_model_desc {
 
 /* Caller-owned output buffers for svx_solve_model (Solution, lib.rs:486-513).
  * Array lengths: displacements n_nodes*3, von_mises n_elem, nodal_von_mises
- * n_nodes, reactions svx_model_reaction_count(). A NULL array is skipped. */
+ * n_nodes, reactions svx_model_reaction_count(). A NULL array is skipped.
+ * reaction_capacity is the number of entries each non-NULL reaction array
+ * holds; a model with more reactions than that is refused with
+ * SVX_E_ARGUMENT before anything is written. */
 typedef struct svx_solution {
     double* displacements;
     double* von_mises;
@@ -153,6 +156,7 @@ typedef struct svx_solution {
     uint32_t* reaction_node;
     int32_t* reaction_dof;
     double* reaction_value;
+    int64_t reaction_capacity;
     int64_t iterations;
     double final_residual;
     double cond_estimate;
@@ -271,20 +275,37 @@ int svx_solve(svx_system* sys, double tol, int64_t max_iter, int precision,
               double* residual, svx_error* err);
 /* What the last svx_solve or svx_batch_solve of this system did (no
  * reference counterpart: the reference reports iterations and residual
- * only, lib.rs:706-716). For SVX_FP32_IR, residual is the FP64 true
- * residual ||f - K u|| / ||f|| at the last check and floor the FP64
- * evaluation floor sqrt(max row length) * eps64 * || |K||u| + |f| || / ||f||
- * at that check; the solve converges when residual <= tol + floor. */
+ * only, lib.rs:706-716).
+ *
+ * Stopping rules. A solve converges only at residual <= tol, by one of two
+ * tests (converged_by): the recursive residual of FP64 PCG, the test of
+ * solver.rs:129-131, or the true residual ||f - K u|| / ||f|| evaluated in
+ * FP64 at an FP32-IR refresh. FP64 cannot evaluate f - K u below its floor
+ * sqrt(max row length) * eps64 * || |K||u| + |f| || / ||f||; an FP32-IR
+ * instance whose true residual stops improving at that floor while still
+ * above tol is finished by FP64 PCG from its current u (fallback_reason 2),
+ * which stops on its recursive residual as the reference's PCG does. After
+ * every solve the library evaluates the true residual once more in FP64
+ * (true_residual, true_floor), whatever stopped it, so both precisions
+ * report the same quantity. */
 typedef struct svx_solve_stats {
     int64_t iterations;      /* all Krylov iterations, FP64 fallback included */
     int64_t refreshes;       /* FP32-IR: FP64 residual replacements */
     int64_t fp64_iterations; /* FP32-IR: iterations run by the FP64 fallback */
     int64_t residual_evals;  /* FP32-IR: FP64 residuals f - K u evaluated */
-    double residual;
-    double floor;
+    double residual;         /* the residual the stopping test read */
+    double floor;            /* FP32-IR: evaluation floor at the last refresh */
     int32_t status;    /* 0 converged, 1 breakdown, 2 iteration budget spent */
     int32_t precision; /* SVX_FP64 / SVX_FP32_IR; -1 before any solve */
+    double true_residual;    /* ||f - K u|| / ||f|| of the final u, in FP64 */
+    double true_floor;       /* its FP64 evaluation floor */
+    int32_t converged_by;    /* SVX_STOP_* */
+    int32_t fallback_reason; /* FP32-IR: 0 none, 1 stalled above the floor,
+                                2 at the floor above tol */
 } svx_solve_stats;
+
+/* Stopping test that ended a converged solve (svx_solve_stats). */
+enum { SVX_STOP_NONE = 0, SVX_STOP_RECURSIVE = 1, SVX_STOP_TRUE = 2 };
 int svx_system_stats(const svx_system* sys, svx_solve_stats* out);
 
 /* Recovery from the last solution: centroid von Mises (stress.rs:22-42),
@@ -446,7 +467,9 @@ typedef struct svx_server_summary {
 
 int svx_server_create(svx_ctx* ctx, const svx_server_config* cfg,
                       svx_server** out);
-/* Stops the server; requests still in flight are dropped. */
+/* Stops the server. Requests still in flight are dropped: every waiter
+ * blocked in svx_server_wait returns SVX_E_DEVICE, and destroy returns once
+ * no waiter is inside the server. */
 void svx_server_destroy(svx_server* srv);
 /* Copies the deck; returns at once with a ticket. */
 int svx_server_submit(svx_server* srv, const svx_model_desc* model,
@@ -455,7 +478,10 @@ int svx_server_submit(svx_server* srv, const svx_model_desc* model,
  * expires, the ticket stays valid). On completion fills out (arrays sized as
  * for svx_solve_model) and info, returns the request's status (errors as
  * svx_solve_model reports them, SVX_E_CAPACITY for a deck larger than the
- * budget), and releases the ticket. */
+ * budget, SVX_E_DEVICE for requests the server dropped when it failed or was
+ * destroyed), and releases the ticket. A done request whose reactions exceed
+ * out->reaction_capacity returns SVX_E_ARGUMENT, writes nothing and keeps
+ * the ticket, so the caller can wait again with larger arrays. */
 int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms,
                     svx_solution* out, svx_request_info* info,
                     svx_error* err);

@@ -28,6 +28,7 @@ class OracleOpts(C.Structure):
         ("tol", C.c_double),
         ("max_iter", C.c_int64),
         ("stop_after_assembly", C.c_int32),
+        ("dup_seed", C.c_uint64),
     ]
 
 
@@ -113,11 +114,13 @@ def _ip(a):
 
 
 def solve(model, threads: int = 0, faithful: bool = False, tol: float = 0.0, max_iter: int = 0,
-          stop_after_assembly: bool = False, want_csr: bool = True) -> OracleResult:
-    """solve_with_method(model, Method::Pcg) restated on the CPU."""
+          stop_after_assembly: bool = False, want_csr: bool = True, dup_seed: int = 0) -> OracleResult:
+    """solve_with_method(model, Method::Pcg) restated on the CPU. dup_seed != 0
+    sums each row's duplicate contributions in a seeded shuffled order (the
+    order the reference leaves to its unstable sort, assembly.rs:97)."""
     L = lib()
     desc, keep = model.to_desc()
-    opts = OracleOpts(threads, 1 if faithful else 0, tol, max_iter, 1 if stop_after_assembly else 0)
+    opts = OracleOpts(threads, 1 if faithful else 0, tol, max_iter, 1 if stop_after_assembly else 0, dup_seed)
     h = L.oracle_solve(C.byref(desc), C.byref(opts))
     try:
         t = np.zeros(4)

@@ -28,7 +28,9 @@
 // One deliberate difference: csr_from_triplets sums duplicate (row, col)
 // contributions in triplet order (a stable sort), where the reference's
 // sort_unstable_by_key leaves that order to Rust's sort implementation;
-// the two agree to a few ULP per entry.
+// the two agree to a few ULP per entry. oracle_opts::dup_seed != 0 shuffles
+// each row's triplets (seeded) before the stable sort, so tests can measure
+// how far the duplicate order alone moves iteration counts and residuals.
 // Stress recovery recomputes the per-element B matrices from the coordinates
 // instead of storing them (identical arithmetic, less memory).
 
@@ -377,8 +379,17 @@ struct Trip {
     double v;
 };
 
-// assembly.rs:87-116 (stable order for duplicates, see header)
-Csr csr_from_triplets(int64_t n, const std::vector<Trip>& trips) {
+// SplitMix64 step (the decks' generator), for the duplicate-order shuffle.
+static uint64_t mix64(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// assembly.rs:87-116 (stable order for duplicates, see header; dup_seed != 0:
+// each row's triplets shuffled first, a different duplicate order)
+Csr csr_from_triplets(int64_t n, const std::vector<Trip>& trips, uint64_t dup_seed = 0) {
     std::vector<int64_t> cnt(n + 1, 0);
     for (const auto& t : trips) cnt[t.r + 1]++;
     for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
@@ -392,6 +403,10 @@ Csr csr_from_triplets(int64_t n, const std::vector<Trip>& trips) {
     k.row_ptr.push_back(0);
     for (int64_t r = 0; r < n; ++r) {
         auto b = rows.begin() + cnt[r], e = rows.begin() + cnt[r + 1];
+        if (dup_seed) {  // Fisher-Yates with a per-row stream
+            uint64_t st = dup_seed ^ (0x5DEECE66Dull * (uint64_t)(r + 1));
+            for (int64_t i = (e - b) - 1; i > 0; --i) std::swap(b[i], b[(int64_t)(mix64(st) % (uint64_t)(i + 1))]);
+        }
         std::stable_sort(b, e, [](const auto& x, const auto& y) { return x.first < y.first; });
         auto it = b;
         while (it != e) {
@@ -543,7 +558,7 @@ struct Assembled {
 };
 
 // assembly.rs:291-386 (CLOADs only: the path's decks carry no DLOAD/gravity)
-bool assemble_with(const Model& M, int ndof, int threads, Assembled& out, Err& err) {
+bool assemble_with(const Model& M, int ndof, int threads, Assembled& out, Err& err, uint64_t dup_seed = 0) {
     const svx_model_desc* m = M.m;
     int64_t ndof_total = m->n_nodes * ndof;
     int64_t n_elem = m->n_elem;
@@ -610,7 +625,7 @@ bool assemble_with(const Model& M, int ndof, int threads, Assembled& out, Err& e
         }
         out.f[idx * ndof + dof - 1] += m->cload_value[l];
     }
-    out.k = csr_from_triplets(ndof_total, trips);
+    out.k = csr_from_triplets(ndof_total, trips, dup_seed);
     return true;
 }
 
@@ -807,6 +822,7 @@ typedef struct oracle_opts {
     double tol;        // <= 0: 1e-10 (lib.rs:666)
     int64_t max_iter;  // <= 0: max(1000, 10 n) (lib.rs:632)
     int32_t stop_after_assembly;
+    uint64_t dup_seed;  // != 0: duplicate triplets summed in a seeded shuffled order
 } oracle_opts;
 
 struct oracle_result {
@@ -856,7 +872,7 @@ oracle_result* oracle_solve(const svx_model_desc* m, const oracle_opts* o) {
     R->ndof = ndof;
 
     Assembled A;
-    if (!assemble_with(M, ndof, threads, A, err)) return R;
+    if (!assemble_with(M, ndof, threads, A, err, o ? o->dup_seed : 0)) return R;
     std::vector<ConstrainedDof>& cons = R->cons;
     if (!constrained_dofs(M, ndof, cons, err)) return R;
     R->k_orig = A.k;  // lib.rs:623-628

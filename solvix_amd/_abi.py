@@ -25,6 +25,7 @@ SVX_ELEM_NONPOSITIVE_JACOBIAN = 5
 SVX_METHOD_AUTO, SVX_METHOD_PCG, SVX_METHOD_LDLT = 0, 1, 2
 SVX_FP64, SVX_FP32_IR = 0, 1
 SVX_OP_ASSEMBLED, SVX_OP_MATRIX_FREE = 0, 1
+SVX_STOP_NONE, SVX_STOP_RECURSIVE, SVX_STOP_TRUE = 0, 1, 2
 
 _p = C.POINTER
 
@@ -129,6 +130,10 @@ class SvxSolveStats(C.Structure):
         ("floor", C.c_double),
         ("status", C.c_int32),
         ("precision", C.c_int32),
+        ("true_residual", C.c_double),
+        ("true_floor", C.c_double),
+        ("converged_by", C.c_int32),
+        ("fallback_reason", C.c_int32),
     ]
 
 
@@ -140,6 +145,7 @@ class SvxSolution(C.Structure):
         ("reaction_node", _p(C.c_uint32)),
         ("reaction_dof", _p(C.c_int32)),
         ("reaction_value", _p(C.c_double)),
+        ("reaction_capacity", C.c_int64),
         ("iterations", C.c_int64),
         ("final_residual", C.c_double),
         ("cond_estimate", C.c_double),

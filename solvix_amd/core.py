@@ -105,7 +105,7 @@ def solve_with_method(model: Model, method: int = Method.AUTO, precision: int = 
     rd = np.zeros(nr, np.int32)
     rv = np.zeros(nr)
     sol = _abi.SvxSolution(_d(disp), _d(vm), _d(nvm), rn.ctypes.data_as(C.POINTER(C.c_uint32)),
-                           rd.ctypes.data_as(_i32p), _d(rv))
+                           rd.ctypes.data_as(_i32p), _d(rv), nr)
     err = _abi.SvxError()
     rc = lib().svx_solve_model(ctx.handle, C.byref(desc), method, precision, C.byref(sol), C.byref(err))
     del keep
@@ -520,7 +520,7 @@ class Server:
         (kind "Timeout" if the request is not done within timeout_ms)."""
         model = self._pending[ticket]
         nn, ne = model.n_nodes, model.n_elem
-        nr = max(0, int(lib().svx_model_reaction_count(C.byref(model.to_desc()[0]))))
+        nr = max(0, int(model.reaction_count()))
         disp = np.zeros((nn, 3))
         vm = np.zeros(ne)
         nvm = np.zeros(nn)
@@ -528,11 +528,11 @@ class Server:
         rd = np.zeros(nr, np.int32)
         rv = np.zeros(nr)
         sol = _abi.SvxSolution(_d(disp), _d(vm), _d(nvm), rn.ctypes.data_as(C.POINTER(C.c_uint32)),
-                               rd.ctypes.data_as(_i32p), _d(rv))
+                               rd.ctypes.data_as(_i32p), _d(rv), nr)
         info = _abi.SvxRequestInfo()
         err = _abi.SvxError()
         rc = lib().svx_server_wait(self._h, ticket, timeout_ms, C.byref(sol), C.byref(info), C.byref(err))
-        if rc != _abi.SVX_E_TIMEOUT:
+        if rc not in (_abi.SVX_E_TIMEOUT, _abi.SVX_E_ARGUMENT):
             del self._pending[ticket]
         _raise(rc, err)
         out = Solution(

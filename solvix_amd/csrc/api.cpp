@@ -525,6 +525,25 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     sys->last_precision = precision;
     sys->solved = out_scal->status == ST_CONVERGED;
     sys->has_u = true;
+    system_true_residual(sys);
+}
+
+void system_true_residual(svx_system* sys) {
+    svx_mesh* m = sys->mesh;
+    hipStream_t s = m->ctx->stream;
+    const int64_t nb = true_residual_blocks(m->n);
+    void* tmp[2];
+    m->ctx->scratch.carve(s, {sizeof(double) * 3 * (size_t)std::max<int64_t>(1, nb), 4 * sizeof(double)}, tmp);
+    double* d_out = static_cast<double*>(tmp[1]);
+    launch_true_residual(m->n, m->row_ptr.p, m->col.p, sys->vals.p, sys->u.p, sys->f.p, static_cast<double*>(tmp[0]),
+                         d_out, s);
+    double h[3] = {0.0, 0.0, 0.0};
+    SVX_HIP(hipMemcpyAsync(h, d_out, sizeof h, hipMemcpyDeviceToHost, s));
+    SVX_HIP(hipStreamSynchronize(s));
+    const double bnorm = std::sqrt(h[2]);
+    const double scale = std::sqrt((double)std::max<int64_t>(1, m->max_row_len)) * std::numeric_limits<double>::epsilon();
+    sys->true_res = bnorm > 0.0 ? std::sqrt(h[0]) / bnorm : 0.0;
+    sys->true_floor = bnorm > 0.0 ? scale * std::sqrt(h[1]) / bnorm : 0.0;
 }
 
 void system_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n_rows, const int64_t* rows,
@@ -766,6 +785,11 @@ int svx_system_stats(const svx_system* sys, svx_solve_stats* out) {
     out->residual = sc.res;
     out->floor = sc.floor;
     out->status = sc.status == ST_CONVERGED ? 0 : sc.status == ST_BREAKDOWN ? 1 : 2;
+    out->true_residual = sys->true_res;
+    out->true_floor = sys->true_floor;
+    out->converged_by = sc.status != ST_CONVERGED ? SVX_STOP_NONE
+                        : sc.stop_rule == SVX_STOP_TRUE ? SVX_STOP_TRUE : SVX_STOP_RECURSIVE;
+    out->fallback_reason = sc.fb_reason;
     return SVX_OK;
 }
 
@@ -1055,6 +1079,11 @@ int svx_solve_model(svx_ctx* ctx, const svx_model_desc* md, int method, int prec
     }
     if (precision != SVX_FP64 && precision != SVX_FP32_IR) {
         set_err(err, SVX_E_ARGUMENT, "unknown precision");
+        return SVX_E_ARGUMENT;
+    }
+    if ((out->reaction_node || out->reaction_dof || out->reaction_value) &&
+        svx_model_reaction_count(md) > out->reaction_capacity) {
+        set_err(err, SVX_E_ARGUMENT, "svx_solve_model: reaction arrays smaller than the model's reaction count");
         return SVX_E_ARGUMENT;
     }
     return guarded(err, [&]() -> int {

@@ -127,6 +127,7 @@ struct svx_system {
     bool solved = false, has_u = false;
     svx::InstScal last{};    // scalars of the last solve
     int last_precision = -1;
+    double true_res = 0.0, true_floor = 0.0;  // FP64 ||f - K u|| / ||f|| of the last solve's u, its floor
 };
 
 struct svx_csr {
@@ -189,5 +190,9 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
                   const TuneParams& tp);
 void system_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n_rows, const int64_t* rows,
                     double* reactions);
+// ||f - K u|| / ||f|| of the system's current u on its constrained K and f,
+// evaluated in FP64, and its FP64 evaluation floor; stored in sys->true_res /
+// sys->true_floor.
+void system_true_residual(svx_system* sys);
 
 }  // namespace svx

@@ -204,6 +204,7 @@ int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
             b->sys[i]->last_precision = precision;
             b->sys[i]->solved = b->h_scal[i].status == ST_CONVERGED;
             failed += !b->sys[i]->solved;
+            system_true_residual(b->sys[i]);
         }
         if (failed && err) {
             err->code = SVX_E_NOT_CONVERGED;

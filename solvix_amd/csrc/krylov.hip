@@ -27,13 +27,14 @@
 // the last replacement (or below tol, or that has not been checked for
 // kForcedRefresh iterations) folds its FP32 correction into the FP64 u,
 // recomputes r = f - K u in FP64, and continues with p = z + beta p built from
-// the replaced residual. The solve ends when the FP64 residual satisfies
-// ||f - K u|| <= tol ||f|| + floor_scale || |K||u| + |f| ||, i.e. reaches tol
-// or the rounding level at which FP64 can still resolve f - K u (within
-// kFloorSlack of that level once two replacements in a row failed to halve
-// the residual). An instance whose FP64 residual stops improving above that
-// level (four replacements without halving) falls back to FP64 PCG from its
-// current u.
+// the replaced residual. The solve ends when the FP64 true residual
+// ||f - K u|| / ||f|| reaches tol. FP64 cannot evaluate f - K u below the
+// floor floor_scale || |K||u| + |f| || / ||f||: an instance whose true
+// residual is within that floor of tol (within kFloorSlack floors once two
+// replacements in a row failed to halve it) while still above tol, or whose
+// residual stops improving above it (four replacements without halving),
+// hands over to FP64 PCG from its current u, which stops on its recursive
+// residual <= tol as the reference's pcg does (solver.rs:129-131).
 #include <algorithm>
 
 #include "mf_element.hpp"
@@ -512,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
         S->res = sqrt(v[0]) / S->bnorm;
         S->flags = 0;
         S->dir = 1;  // p = z is a complete direction
-        if (S->res <= S->tol) S->status = ST_CONVERGED;
+        if (S->res <= S->tol) S->status = ST_CONVERGED, S->stop_rule = SVX_STOP_TRUE;  // r = f - K u in FP64
         return;
     }
     S->bnorm = sqrt(v[0]);
@@ -526,6 +527,8 @@ __global__ __launch_bounds__(kBlock) void init_finish_kernel(const Tile* __restr
     S->it_refresh = 0;
     S->fp64_it = 0;
     S->evals = 0;
+    S->stop_rule = 0;
+    S->fb_reason = 0;
     S->floor = 0.0;
     S->flags = 0;
     S->dir = 1;  // p = z is a complete direction
@@ -1442,12 +1445,15 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
     const double floor = S->floor_scale * sqrt(v[2]) / S->bnorm;
     const bool improved = res < 0.5 * S->snap_res_best;
     const int32_t stall = improved ? 0 : S->snap_stall + 1;
-    // Converged at tol, or at the FP64 evaluation floor: within the floor, or
-    // within kFloorSlack floors once the residual has stopped halving (the
-    // FP64 product K u cannot resolve f - K u any further).
-    const bool conv = res <= S->tol + floor || (stall >= 2 && res <= S->tol + kFloorSlack * floor);
+    // Converged only at tol. Within the FP64 evaluation floor of tol (within
+    // kFloorSlack floors once the residual has stopped halving) the FP64
+    // product K u cannot resolve f - K u any further, so the FP32 phase ends
+    // there and FP64 PCG finishes on its recursive residual; so does an
+    // instance whose residual stopped improving above the floor.
+    const bool conv = res <= S->tol;
+    const bool at_floor = !conv && (res <= S->tol + floor || (stall >= 2 && res <= S->tol + kFloorSlack * floor));
     const bool hold = (S->snap_flags & FL_HOLD) != 0;
-    const bool fallback = !conv && !hold && stall >= kStallLimit;
+    const bool fallback = !conv && !hold && (at_floor || stall >= kStallLimit);
     const bool cont = !conv && !hold && !fallback;
     const double beta = v[1] / S->snap_rz;
     if (is_leader(t)) {
@@ -1460,10 +1466,12 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
         S->stall = stall;
         if (conv) {
             S->status = ST_CONVERGED;
+            S->stop_rule = SVX_STOP_TRUE;
         } else if (hold) {
             S->status = ST_MAXITER;
         } else if (fallback) {
             S->status = ST_FALLBACK;  // FP32 cannot resolve this system further
+            S->fb_reason = at_floor ? 2 : 1;
         } else {
             S->rmax = sqrt(v[0]);
             S->rz = v[1];
@@ -1605,6 +1613,86 @@ void launch_spmv_ordered(int64_t n, const int32_t* row_ptr, const int32_t* col, 
                          double* y, hipStream_t s) {
     if (n <= 0) return;
     spmv_ordered_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, row_ptr, col, val, x, y);
+    SVX_HIP(hipGetLastError());
+}
+
+namespace {
+constexpr int kTrLanes = 16;  // lanes per CSR row of the true-residual pass
+
+// One kTrLanes-lane group per row; block partials of (f - K u)^2,
+// (|K||u| + |f|)^2 and f^2, summed in a fixed wave / LDS order.
+__global__ __launch_bounds__(kBlock) void true_residual_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                               const int32_t* __restrict__ col,
+                                                               const double* __restrict__ val,
+                                                               const double* __restrict__ u,
+                                                               const double* __restrict__ f,
+                                                               double* __restrict__ partials) {
+    const int lane = threadIdx.x % kTrLanes;
+    const int64_t row = (int64_t)blockIdx.x * (kBlock / kTrLanes) + threadIdx.x / kTrLanes;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (row < n) {
+        double ku = 0.0, aku = 0.0;
+        for (int32_t k = rp[row] + lane; k < rp[row + 1]; k += kTrLanes) {
+            const double w = val[k] * u[col[k]];
+            ku += w;
+            aku += fabs(w);
+        }
+#pragma unroll
+        for (int off = kTrLanes / 2; off > 0; off >>= 1) {
+            ku += __shfl_xor(ku, off, kTrLanes);
+            aku += __shfl_xor(aku, off, kTrLanes);
+        }
+        if (lane == 0) {
+            const double fi = f[row], ri = fi - ku, a = aku + fabs(fi);
+            v[0] = ri * ri;
+            v[1] = a * a;
+            v[2] = fi * fi;
+        }
+    }
+    __shared__ double sh[3][kBlock];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// One block: each thread sums blocks j, j + kBlock, ... in order, then a
+// fixed tree over the threads.
+__global__ __launch_bounds__(kBlock) void true_residual_finish_kernel(int64_t n_blocks,
+                                                                      const double* __restrict__ partials,
+                                                                      double* __restrict__ out) {
+    __shared__ double sh[3][kBlock];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        for (int64_t j = threadIdx.x; j < n_blocks; j += kBlock) acc += partials[(size_t)c * n_blocks + j];
+        sh[c][threadIdx.x] = acc;
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+}  // namespace
+
+int64_t true_residual_blocks(int64_t n) { return (n + kBlock / kTrLanes - 1) / (kBlock / kTrLanes); }
+
+void launch_true_residual(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val, const double* u,
+                          const double* f, double* partials, double* out, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t nb = true_residual_blocks(n);
+    true_residual_kernel<<<(unsigned)nb, kBlock, 0, s>>>(n, row_ptr, col, val, u, f, partials);
+    true_residual_finish_kernel<<<1, kBlock, 0, s>>>(nb, partials, out);
     SVX_HIP(hipGetLastError());
 }
 

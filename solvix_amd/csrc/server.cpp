@@ -103,7 +103,10 @@ struct OwnedModel {
     }
 };
 
-enum class St { Queued, Prepared, Active, Done };
+// Queued: in prep_q. Preparing: a host worker holds it (nobody else may
+// touch it until the worker publishes it under the lock). Prepared: in
+// ready_q. Active: admitted by the device thread. Done: a waiter may take it.
+enum class St { Queued, Preparing, Prepared, Active, Done };
 
 struct Request {
     int64_t ticket = 0;
@@ -149,12 +152,12 @@ struct svx_server {
     svx_server_config cfg{};
     svx::TuneParams tune;  // the context's tuning and operator when the server was created
     std::mutex mu;
-    std::condition_variable cv_prep, cv_dev, cv_done;
+    std::condition_variable cv_prep, cv_dev, cv_done, cv_waiters;
     std::deque<Request*> prep_q, ready_q;
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
-    std::vector<Request*> active;  // admitted, in an engine
     int64_t next_ticket = 1;
     bool stop = false;
+    int waiters = 0;  // threads inside svx_server_wait
     std::vector<std::thread> prep_threads;
     std::thread dev_thread;
     // statistics (guarded by mu)
@@ -174,7 +177,26 @@ struct svx_server {
     void prep_loop();
     void dev_loop();
     void finish(Request* r);
+    // With mu held: every request not Done and not held by a host worker is
+    // failed with SVX_E_DEVICE (the device thread is gone or going), the
+    // queues are emptied and waiters are woken.
+    void fail_pending(const std::string& why);
 };
+
+void svx_server::fail_pending(const std::string& why) {
+    for (auto& kv : reqs) {
+        Request* r = kv.second.get();
+        if (r->state == St::Done || r->state == St::Preparing) continue;
+        r->rc = SVX_E_DEVICE;
+        put_err(&r->err, SVX_E_DEVICE, why);
+        r->state = St::Done;
+        r->t_done = wall_ms();
+        failed++;
+    }
+    prep_q.clear();
+    ready_q.clear();
+    cv_done.notify_all();
+}
 
 void svx_server::prep_loop() {
     for (;;) {
@@ -185,6 +207,7 @@ void svx_server::prep_loop() {
             if (stop) return;
             r = prep_q.front();
             prep_q.pop_front();
+            r->state = St::Preparing;
         }
         svx_error err{};
         err.index = -1;
@@ -212,7 +235,14 @@ void svx_server::prep_loop() {
         }
         std::lock_guard<std::mutex> lk(mu);
         r->t_prepared = wall_ms();
-        if (rc != SVX_OK) {
+        if (stop) {  // the server failed or is being destroyed: nothing will admit it
+            r->rc = SVX_E_DEVICE;
+            put_err(&r->err, SVX_E_DEVICE, fatal.empty() ? "server stopped" : fatal);
+            r->state = St::Done;
+            r->t_done = r->t_prepared;
+            failed++;
+            cv_done.notify_all();
+        } else if (rc != SVX_OK) {
             r->rc = rc;
             r->err = err;
             r->state = St::Done;
@@ -430,6 +460,7 @@ void svx_server::dev_loop() {
                     const int64_t n_active = (int64_t)(in32.size() + in64.size() + take.size());
                     if (n_active >= cfg.max_active || resident + r->bytes > cfg.hbm_budget) break;
                     ready_q.pop_front();
+                    r->state = St::Active;
                     resident += r->bytes;
                     peak_resident = std::max(peak_resident, resident);
                     take.push_back(r);
@@ -470,14 +501,10 @@ void svx_server::dev_loop() {
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu);
         fatal = e.what();
-        for (auto& kv : reqs)
-            if (kv.second->state != St::Done) {
-                kv.second->rc = SVX_E_DEVICE;
-                put_err(&kv.second->err, SVX_E_DEVICE, fatal);
-                kv.second->state = St::Done;
-            }
         stop = true;
-        cv_done.notify_all();
+        // Requests a host worker still prepares are failed by that worker
+        // when it publishes them (it sees stop under the lock).
+        fail_pending(fatal);
         cv_prep.notify_all();
     }
 }
@@ -517,6 +544,13 @@ void svx_server_destroy(svx_server* srv) {
     for (std::thread& t : srv->prep_threads)
         if (t.joinable()) t.join();
     if (srv->dev_thread.joinable()) srv->dev_thread.join();
+    {
+        // No worker or device thread is left: fail what they left behind and
+        // wait for every waiter to leave before the server goes.
+        std::unique_lock<std::mutex> lk(srv->mu);
+        srv->fail_pending("server destroyed with the request in flight");
+        srv->cv_waiters.wait(lk, [&] { return srv->waiters == 0; });
+    }
     svx_ctx* aux = srv->aux;
     delete srv;  // requests (and their device memory on aux) first
     svx_ctx_destroy(aux);
@@ -554,6 +588,13 @@ int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms, svx_solu
     std::unique_ptr<Request> r;
     {
         std::unique_lock<std::mutex> lk(srv->mu);
+        srv->waiters++;
+        struct Leave {  // every return path leaves the waiter count
+            svx_server* s;
+            ~Leave() {
+                if (--s->waiters == 0) s->cv_waiters.notify_all();
+            }
+        } leave{srv};
         auto it = srv->reqs.find(ticket);
         if (it == srv->reqs.end()) return SVX_E_ARGUMENT;
         Request* q = it->second.get();
@@ -562,6 +603,11 @@ int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms, svx_solu
             srv->cv_done.wait(lk, done);
         else if (!srv->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done))
             return SVX_E_TIMEOUT;
+        const bool wants_reactions = out && (out->reaction_node || out->reaction_dof || out->reaction_value);
+        if (q->rc == SVX_OK && wants_reactions && (int64_t)q->pm.cons_rows.size() > out->reaction_capacity) {
+            if (err) put_err(err, SVX_E_ARGUMENT, "svx_server_wait: reaction arrays smaller than the request's reaction count");
+            return SVX_E_ARGUMENT;  // the ticket stays valid
+        }
         r = std::move(it->second);
         srv->reqs.erase(it);
     }

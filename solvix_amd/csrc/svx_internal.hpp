@@ -98,6 +98,8 @@ struct alignas(16) InstScal {
     int32_t flags;      // FL_*
     int32_t stall;      // FP32-IR refreshes without halving res_best
     int32_t dir;        // 1: the direction in P[0] is complete (after init / refresh); 0: the next SpMV takes the step
+    int32_t stop_rule;  // SVX_STOP_TRUE when an FP64 true residual <= tol ended the solve; 0 otherwise
+    int32_t fb_reason;  // FP32-IR: why FP64 took over (0 none, 1 stalled above the floor, 2 at the floor above tol)
     // snapshots
     double snap_rz;
     double snap_rmax;
@@ -261,6 +263,12 @@ void launch_nodal_average(int64_t n_nodes, const int64_t* inc_ptr, const int32_t
 // krylov.hip
 void launch_spmv_ordered(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
                          const double* x, double* y, hipStream_t s);
+// ||f - K u||^2, || |K||u| + |f| ||^2 and ||f||^2 over the CSR rows, in FP64
+// with a fixed summation order: per-block partials (3 * true_residual_blocks(n)
+// doubles of `partials`) reduced by one block into out[3].
+int64_t true_residual_blocks(int64_t n);
+void launch_true_residual(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val, const double* u,
+                          const double* f, double* partials, double* out, hipStream_t s);
 void launch_spmv_csr64(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
                        const double* x, double* y, int lanes, hipStream_t s);
 

@@ -48,11 +48,6 @@ def small_models():
     return [x for x in out if x is not None]
 
 
-# FP32-IR stops at tol + floor, or at tol + FLOOR_SLACK * floor once its FP64
-# residual has stopped halving (the device solver's documented rule).
-FLOOR_SLACK = 4.0
-
-
 def eval_floor(row_ptr, col_idx, vals, f, u):
     """sqrt(m) * eps * || |K||u| + |f| || / ||f||: the rounding level of an FP64
     evaluation of f - K u (m = longest row). The reference's own solutions of

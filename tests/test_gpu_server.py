@@ -28,20 +28,24 @@ def _decks():
     return ms
 
 
-@pytest.mark.parametrize("precision,operator", [(Precision.FP64, "assembled"), (Precision.FP32_IR, "assembled"),
-                                                (Precision.FP64, "matrix_free")],
-                         ids=["fp64", "fp32ir", "fp64_matrix_free"])
-def test_results_alone_equal_results_in_the_mix(precision, operator):
+@pytest.mark.parametrize("precision,operator,graph", [(Precision.FP64, "assembled", False),
+                                                      (Precision.FP32_IR, "assembled", False),
+                                                      (Precision.FP64, "matrix_free", False),
+                                                      (Precision.FP64, "assembled", True),
+                                                      (Precision.FP32_IR, "assembled", True)],
+                         ids=["fp64", "fp32ir", "fp64_matrix_free", "fp64_graph", "fp32ir_graph"])
+def test_results_alone_equal_results_in_the_mix(precision, operator, graph):
     """matrix_free: a server created on a context with that operator runs its
     FP64 instances (plane meshes and C3D8 blocks of one material) on it, as
-    the same context's solve does alone."""
+    the same context's solve does alone. graph: the server's chunks run as
+    captured hipGraphs (svx_server_config.use_graph = 1)."""
     models = _decks()
     actx, sctx = Context(0), Context(0)
     actx.set_operator(operator)
     sctx.set_operator(operator)
     try:
         alone = [solve_with_method(m, Method.AUTO, precision, actx) for m in models]
-        srv = Server(sctx, precision, hbm_budget=1 << 30, max_active=4)
+        srv = Server(sctx, precision, hbm_budget=1 << 30, max_active=4, use_graph=graph)
         sctx.set_operator("assembled")  # the server keeps the operator it was created with
         _check_mix(models, alone, srv, precision)
     finally:
@@ -121,3 +125,60 @@ def test_admission_respects_the_budget_and_reports_errors(ctx):
     finally:
         srv.close()
         sctx.close()
+
+
+def test_wait_refuses_short_reaction_arrays_and_keeps_the_ticket():
+    """svx_server_wait writes reactions only into arrays that hold them all;
+    a short capacity returns SVX_E_ARGUMENT and the ticket stays valid."""
+    import ctypes as C
+    from solvix_amd import _abi
+    from solvix_amd._lib import lib
+    m = small_models()[0]
+    sctx = Context(0)
+    srv = Server(sctx, Precision.FP64, hbm_budget=1 << 30, max_active=4)
+    try:
+        t = srv.submit(m)
+        nr = m.reaction_count()
+        assert nr > 1
+        disp = np.zeros((m.n_nodes, 3))
+        rv = np.zeros(nr)
+        sol = _abi.SvxSolution(disp.ctypes.data_as(C.POINTER(C.c_double)), None, None, None, None,
+                               rv.ctypes.data_as(C.POINTER(C.c_double)), nr - 1)
+        err = _abi.SvxError()
+        rc = lib().svx_server_wait(srv._h, t, -1.0, C.byref(sol), None, C.byref(err))
+        assert rc == _abi.SVX_E_ARGUMENT and not rv.any()
+        s, info = srv.wait(t, 60000.0)  # same ticket, full arrays
+        assert len(s.reactions) == nr
+    finally:
+        srv.close()
+        sctx.close()
+
+
+def test_destroy_fails_pending_requests_and_releases_waiters():
+    """Destroying a server with requests queued and in flight returns every
+    blocked waiter with a Device error instead of leaving it waiting."""
+    import threading
+    models = _decks()
+    sctx = Context(0)
+    srv = Server(sctx, Precision.FP64, hbm_budget=1 << 30, max_active=1)
+    tickets = [srv.submit(m) for m in models]
+    outcomes = []
+
+    def waiter(t):
+        try:
+            srv.wait(t, -1.0)
+            outcomes.append("done")
+        except SolveError as e:
+            outcomes.append(e.kind)
+
+    threads = [threading.Thread(target=waiter, args=(t,)) for t in tickets[-3:]]
+    for th in threads:
+        th.start()
+    time.sleep(0.2)  # the waiters are blocked inside svx_server_wait
+    srv.close()  # returns once every waiter has left the server
+    for th in threads:
+        th.join(timeout=60.0)
+        assert not th.is_alive()
+    print("outcomes", outcomes)
+    assert len(outcomes) == 3 and all(o in ("done", "Device") for o in outcomes)
+    sctx.close()

@@ -51,23 +51,127 @@ def test_bundled_decks_match_published_results(name):
         assert r3(r.nodal_von_mises.max() / 1e6) == ka["max_nodal_von_mises_mpa"]
     # Iteration counts: the reference sums duplicate (row, col) triplets in
     # the order Rust's sort_unstable_by_key leaves them (assembly.rs:97),
-    # the restatement in triplet order. Reordering those sums moves K by a
-    # few ULP, and CG's iteration count at 1e-10 with it: over shuffled
-    # orders the counts span 390-406 (cantilever), 544-611 (cantilever3d)
-    # and exactly 77 (plate_hole). The published 2D counts are reproduced
-    # exactly; the 3D one lies inside the measured band.
+    # the restatement in triplet order. The 2D counts are reproduced exactly;
+    # the 3D one differs, and test_duplicate_order_explains_the_published_digits
+    # shows the duplicate order alone spans it.
     if name == "cantilever3d":
-        assert 540 <= r.iterations <= 615
+        lo, hi = _dup_spread(m, 16)[:2]
+        assert lo <= r.iterations <= hi
     else:
         assert r.iterations == ka["iterations"]
 
 
+def _gate_80x8():
+    """The v0.1.0 gate deck (docs/qa/v0.1.0-gate.md:154-156): the README 80x8
+    cantilever with -111.11111111 N on each of the 9 tip nodes."""
+    g = decks.readme_cantilever(80, 8)
+    g.cload_value = np.full(g.cload_value.size, -111.11111111)
+    return g
+
+
+def _dup_spread(model, seeds):
+    """(min, max iterations, min, max residual, printed mm set, printed MPa
+    set) of the oracle over `seeds` orders of the duplicate sums: the
+    stable order (seed 0) and seeded shuffles of each row's triplets."""
+    its, res, mm, mpa = [], [], set(), set()
+    for seed in range(seeds):
+        r = oracle.solve(model, dup_seed=seed, want_csr=False)
+        assert r.status == 0
+        its.append(r.iterations)
+        res.append(r.residual)
+        mm.add(r3(r.max_disp() * 1e3))
+        mpa.add(r3(r.max_von_mises() / 1e6))
+    return min(its), max(its), min(res), max(res), mm, mpa
+
+
+@pytest.mark.parametrize("name,seeds", [("cantilever", 64), ("plate_hole", 300), ("cantilever3d", 16),
+                                        ("gate_80x8", 32)])
+def test_duplicate_order_explains_the_published_digits(name, seeds):
+    """The duplicate-sum order of csr_from_triplets (assembly.rs:97) is the
+    only part of the path that only running Rust fixes. Summing the
+    duplicates in other orders moves K by ULPs: the printed millimetres and
+    megapascals never change, while the iteration count at 1e-10 and the
+    residual's digits do, and the published count and residual fall inside
+    that spread. So the restatement's differences in those digits are
+    explained by the summation order, and a mistake elsewhere (which would
+    move the printed mm / MPa) would fail here."""
+    ka = KNOWN[name]
+    m = _gate_80x8() if name == "gate_80x8" else read_inp(os.path.join(GOLDEN, ka["deck"]))
+    lo, hi, rlo, rhi, mm, mpa = _dup_spread(m, seeds)
+    print(f"{name}: iterations {lo}..{hi} (published {ka['iterations']}), residual {rlo:.3e}..{rhi:.3e} "
+          f"(published {ka['residual']:.3e}), mm {mm}, MPa {mpa}")
+    assert mm == {ka["max_disp_mm"]} and mpa == {ka["max_von_mises_mpa"]}
+    assert lo <= ka["iterations"] <= hi
+    assert rlo <= ka["residual"] <= rhi
+
+
+def test_gate_80x8_published_answers():
+    ka = KNOWN["gate_80x8"]
+    g = _gate_80x8()
+    assert (g.n_nodes, g.n_elem) == (ka["nodes"], ka["elements"])
+    r = oracle.solve(g, faithful=True)
+    assert r.status == 0 and r.residual <= 1e-10
+    assert r3(r.max_disp() * 1e3) == ka["max_disp_mm"]
+    assert r3(r.max_von_mises() / 1e6) == ka["max_von_mises_mpa"]
+
+
 @pytest.mark.parametrize("nx,ny,its,mm", KNOWN["readme_family"]["meshes"])
 def test_readme_cantilever_family(nx, ny, its, mm):
-    r = oracle.solve(decks.readme_cantilever(nx, ny))
+    m = decks.readme_cantilever(nx, ny)
+    r = oracle.solve(m)
     assert r.status == 0
     assert r3(r.max_disp() * 1e3) == mm
-    assert abs(r.iterations - its) <= 0.05 * its  # ULP-order band, see above
+    lo, hi = _dup_spread(m, 16)[:2]  # the duplicate-order spread, as above
+    assert lo <= its <= hi and lo <= r.iterations <= hi
+
+
+# ------------------------------------------------------------- cond_estimate
+def _column(E=210e9, nu=0.3, t=1.0):
+    """The CPS4 column of docs/qa/v0.3.0-gate.md:91-104 as far as the text
+    gives it: 0.1 x 1.0 m, one element wide and five tall (12 nodes), base
+    fixed in dofs 1-2, CLOAD 11, 2, -2500 (the step-2 load that OP=NEW leaves).
+    The gate's t = 1 follows from its 0.025 MPa (P / (w t) at the top
+    element's centroid)."""
+    m = decks.quad_model(1, 5, 0.1, 1.0, E, nu, t)
+    m.bcs = [(np.array([1, 2], np.uint32), np.array([1, 2], np.int32))]
+    m.cload_node, m.cload_dof, m.cload_value = np.array([11], np.uint32), np.array([2], np.int32), np.array([-2500.0])
+    return m
+
+
+def test_cond_estimate_closed_forms():
+    """cond_estimate = max |K_ii| / min |K_ii| of the constrained K
+    (lib.rs:634-644), where every constrained row holds 1.0.
+    Column (a x b = 0.1 x 0.2 elements): the largest diagonal is an interior
+    node's x dof, 2 t (D11 b / 3a + D33 a / 3b) from the exact bilinear
+    integrals. Unit cube (C3D8, the patch-test block): every diagonal entry
+    is (lambda + 4 mu) / 9. Unit square (CPS4): E t (3 - nu) / (6 (1 - nu^2))."""
+    E, nu = 210e9, 0.3
+    D11, D33 = E / (1 - nu * nu), E / (2 * (1 + nu))
+    a, b, t = 0.1, 0.2, 1.0
+    r = oracle.solve(_column())
+    assert r.status == 0
+    assert r.cond_estimate == pytest.approx(2 * t * (D11 * b / (3 * a) + D33 * a / (3 * b)), rel=1e-13)
+    # What the column reproduces of the gate's published line: 23 iterations
+    # and 0.025 MPa. Its cond 4.062e11 and 0.002 mm do not follow from the
+    # stated dimensions (a 0.1 x 0.2 element gives 3.346e11 at t = 1), so the
+    # gate's deck differed in something the text does not state.
+    ka = KNOWN["two_step_column"]
+    assert r.iterations == ka["iterations"]
+    assert r3(r.max_von_mises() / 1e6) == ka["max_von_mises_mpa"]
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    cube = decks.hex_model(1, 1, 1, 1, 1, 1, E=E, nu=nu)
+    cube.bcs = [(np.array([1, 2, 3, 4], np.uint32), np.array([3], np.int32)),
+                (np.array([1], np.uint32), np.array([1], np.int32)),
+                (np.array([1, 2], np.uint32), np.array([2], np.int32))]
+    cube.cload_node, cube.cload_dof, cube.cload_value = (np.array([5, 6, 7, 8], np.uint32), np.full(4, 3, np.int32),
+                                                         np.full(4, 2.5e5))
+    r = oracle.solve(cube)
+    assert r.status == 0 and r.cond_estimate == pytest.approx((lam + 4 * mu) / 9, rel=1e-13)
+    sq = decks.quad_model(1, 1, 1.0, 1.0, E, nu, 0.5)
+    sq.bcs = [(np.array([1], np.uint32), np.array([1, 2], np.int32)), (np.array([2], np.uint32), np.array([2], np.int32))]
+    sq.cload_node, sq.cload_dof, sq.cload_value = np.array([3], np.uint32), np.array([1], np.int32), np.array([1e3])
+    r = oracle.solve(sq)
+    assert r.status == 0 and r.cond_estimate == pytest.approx(E * 0.5 * (3 - nu) / (6 * (1 - nu * nu)), rel=1e-13)
 
 
 # ------------------------------------------------------------- element level
