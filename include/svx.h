@@ -331,6 +331,18 @@ int svx_csr_spmv(svx_csr* csr, const double* x, double* y, int ordered);
 int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter,
                 int precision, double* u, int64_t* iterations,
                 double* residual, svx_error* err);
+/* svx_csr_pcg in FP64 without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops). The Krylov vectors, scalar
+ * state and launch tables live in a fixed device array of the library,
+ * about 38 MB per GPU, held until the process ends; matrices of more than
+ * 524,288 rows return SVX_E_CAPACITY. Every call on one GPU uses the same
+ * array: calls that overlap on a GPU (from several threads or contexts)
+ * race and must be serialised by the caller. Same kernels and launch shape
+ * as svx_csr_pcg with SVX_FP64, run kernel by kernel. */
+int svx_csr_pcg_noalloc(svx_csr* csr, const double* f, double tol,
+                        int64_t max_iter, double* u, int64_t* iterations,
+                        double* residual, svx_error* err);
 
 /* ---------------------------------------------------------------------------
  * Batched solves (no reference counterpart: the reference solves one model

@@ -51,6 +51,7 @@ SIGNATURES = [
     ("svx_csr_destroy", None, [_vp]),
     ("svx_csr_spmv", C.c_int, [_vp, _dp, _dp, C.c_int]),
     ("svx_csr_pcg", C.c_int, [_vp, _dp, C.c_double, _i64, C.c_int, _dp, _i64p, _dp, _errp]),
+    ("svx_csr_pcg_noalloc", C.c_int, [_vp, _dp, C.c_double, _i64, _dp, _i64p, _dp, _errp]),
     ("svx_system_time_spmv", C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     ("svx_system_time_phases", C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     ("svx_time_spmv_rotation", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _dp, _dp]),

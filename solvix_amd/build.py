@@ -32,6 +32,7 @@ SOURCES = [
     ("batch_api.cpp", []),
     ("block_api.cpp", []),
     ("server.cpp", []),
+    ("noalloc.cpp", []),
 ]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE,

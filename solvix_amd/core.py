@@ -480,6 +480,20 @@ class CsrMatrix:
         _raise(rc, err)
         return u, its.value, res.value
 
+    def pcg_noalloc(self, f, tol: float = 1e-10, max_iter: int = 0):
+        """FP64 pcg without device allocation (svx_csr_pcg_noalloc): the
+        library's fixed per-GPU scratch holds the solver state, so calls on one
+        GPU must not overlap."""
+        f = np.ascontiguousarray(f, np.float64)
+        u = np.zeros(self.n)
+        its = C.c_int64()
+        res = C.c_double()
+        err = _abi.SvxError()
+        rc = lib().svx_csr_pcg_noalloc(self._h, _d(f), tol, max_iter, _d(u), C.byref(its), C.byref(res),
+                                       C.byref(err))
+        _raise(rc, err)
+        return u, its.value, res.value
+
     def close(self):
         if self._h:
             lib().svx_csr_destroy(self._h)

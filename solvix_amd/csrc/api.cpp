@@ -609,6 +609,7 @@ int svx_ctx_create(int device, svx_ctx** out) {
         SVX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         SVX_HIP(hipHostMalloc((void**)&c->h_flag, sizeof(int32_t) * 4, hipHostMallocDefault));
         SVX_HIP(hipMalloc((void**)&c->d_flag, sizeof(int32_t) * 4));
+        noalloc_register_device(device);
         *out = c.release();
         return SVX_OK;
     });

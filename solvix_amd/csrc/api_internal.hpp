@@ -194,5 +194,9 @@ void system_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n
 // evaluated in FP64, and its FP64 evaluation floor; stored in sys->true_res /
 // sys->true_floor.
 void system_true_residual(svx_system* sys);
+// Resolve this device's copy of the no-allocation scratch array (with the
+// device current); svx_ctx_create calls it once per context.
+void noalloc_register_device(int device);
+int64_t noalloc_max_rows();
 
 }  // namespace svx

@@ -40,6 +40,25 @@ int unroll_for(int bs, int precision) {
 
 Engine::~Engine() { release_graph(); }
 
+template <typename T>
+void Engine::place(DevBuf<T>& b, size_t count) {
+    if (!arena_) {
+        b.alloc(count);
+        return;
+    }
+    const size_t bytes = (count * sizeof(T) + 255) / 256 * 256;
+    if (arena_off_ + bytes > arena_bytes_) throw std::length_error("engine tables exceed the arena");
+    b.view(reinterpret_cast<T*>(arena_ + arena_off_), count);
+    arena_off_ += bytes;
+}
+
+size_t Engine::arena_bytes_for_csr(int64_t n) {
+    const size_t tiles = (size_t)std::max<int64_t>(1, (n + 255) / 256);
+    auto r = [](size_t b) { return (b + 255) / 256 * 256; };
+    return r(tiles * sizeof(Tile)) + 3 * r(sizeof(Tile)) + r(sizeof(InstDesc)) +
+           r(std::max<size_t>(kPartialSlots, kPartialSlots * tiles) * sizeof(double)) + r(sizeof(InstScal));
+}
+
 void Engine::release_graph() {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     exec_ = nullptr;
@@ -70,6 +89,8 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         old_off[bs] = groups_[bs].partials ? (size_t)(groups_[bs].partials - partials_.p) : 0;
     }
     carry = carry && tiled_.size() == h_descs_.size() && partials_.p;
+    if (arena_ && carry) throw std::logic_error("an engine on an arena does not re-tile mid-solve");
+    arena_off_ = 0;  // an arena's tables are rebuilt in place
     if (carry)
         for (size_t i = 0; i < h_descs_.size(); ++i)
             if (tiled_[i]) old_begin[i] = h_descs_[i].tile_begin;
@@ -166,17 +187,18 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
             }
     DevBuf<double> old_partials;
     if (any_src) old_partials = std::move(partials_);
-    tiles_.alloc(std::max<size_t>(1, tiles.size()));
+    place(tiles_, std::max<size_t>(1, tiles.size()));
     tiles_.upload(tiles.data(), tiles.size(), stream_);
-    mf_tiles_.alloc(std::max<size_t>(1, mf_tiles.size()));
+    place(mf_tiles_, std::max<size_t>(1, mf_tiles.size()));
     mf_tiles_.upload(mf_tiles.data(), mf_tiles.size(), stream_);
-    mf_leaders_.alloc(std::max<size_t>(1, mf_leaders.size()));
+    place(mf_leaders_, std::max<size_t>(1, mf_leaders.size()));
     mf_leaders_.upload(mf_leaders.data(), mf_leaders.size(), stream_);
-    mf3_tiles_.alloc(std::max<size_t>(1, mf3_tiles.size()));
+    place(mf3_tiles_, std::max<size_t>(1, mf3_tiles.size()));
     mf3_tiles_.upload(mf3_tiles.data(), mf3_tiles.size(), stream_);
-    descs_.alloc(std::max<size_t>(1, h_descs_.size()));
+    place(descs_, std::max<size_t>(1, h_descs_.size()));
     descs_.upload(h_descs_.data(), h_descs_.size(), stream_);
-    partials_.alloc(std::max<size_t>(kPartialSlots, total));
+    place(partials_, std::max<size_t>(kPartialSlots, total));
+    if (arena_) place(scal_tmp_, std::max<size_t>(1, h_descs_.size()));
     if (any_src) {
         DevBuf<int32_t> d_src(src.size());
         d_src.upload(src.data(), src.size(), stream_);
@@ -265,7 +287,7 @@ int Engine::run_phase(const EngineOptions& opt_in, int32_t* h_flag, int32_t* d_f
     int done_chunks = 0;
     for (;;) {
         enqueue_init(opt);  // initialises FL_FRESH instances only
-        bool graph_ok = opt.use_graph;
+        bool graph_ok = opt.use_graph && !arena_;  // instantiating a graph allocates
         if (graph_ok && (exec_ == nullptr || exec_precision_ != opt.precision || exec_period_ != opt.period ||
                          exec_chunk_ != chunk)) {
             release_graph();
@@ -325,7 +347,10 @@ int Engine::run_phase(const EngineOptions& opt_in, int32_t* h_flag, int32_t* d_f
 void Engine::read_scalars(std::vector<InstScal>& out) const {
     out.resize(h_descs_.size());
     if (h_descs_.empty()) return;
-    if (scal_tmp_.n < h_descs_.size()) scal_tmp_.alloc(h_descs_.size());
+    if (scal_tmp_.n < h_descs_.size()) {
+        if (arena_) throw std::logic_error("arena staging smaller than the instance set");
+        scal_tmp_.alloc(h_descs_.size());
+    }
     launch_gather_scalars(descs_.p, (int32_t)h_descs_.size(), scal_tmp_.p, stream_);
     scal_tmp_.download(out.data(), out.size(), stream_);
     SVX_HIP(hipStreamSynchronize(stream_));

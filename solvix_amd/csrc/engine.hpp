@@ -62,6 +62,19 @@ class Engine {
     // instances' solver state undefined.
     void time_phases(int precision, int reps, double ms[4]);
     int64_t launches() const { return launches_; }
+    // Carve the engine's device tables (tiles, descriptors, partial sums,
+    // read-back staging) from [base, base + bytes) instead of allocating
+    // them: no device allocation after this call. Tile tables are then
+    // rebuilt only by set_instances (no partial carry on compaction: run with
+    // EngineOptions::compact = false, and without graphs, whose
+    // instantiation allocates). Throws if a table does not fit.
+    void use_arena(void* base, size_t bytes) {
+        arena_ = static_cast<char*>(base);
+        arena_bytes_ = bytes;
+    }
+    // Device bytes the tables of one instance of n rows (raw CSR, 256-row
+    // tiles) take in an arena.
+    static size_t arena_bytes_for_csr(int64_t n);
     int n_instances() const { return (int)h_descs_.size(); }
 
     // Run chunks of opt.precision only (no FP64 fallback phase) until no
@@ -97,6 +110,10 @@ class Engine {
     hipGraphExec_t exec_ = nullptr;
     int exec_precision_ = -1, exec_period_ = -1, exec_chunk_ = -1;
     int64_t launches_ = 0;
+    char* arena_ = nullptr;  // use_arena: tables carved here, nothing allocated
+    size_t arena_bytes_ = 0, arena_off_ = 0;
+    template <typename T>
+    void place(DevBuf<T>& b, size_t count);
 };
 
 int default_lanes(double avg_row_len);

@@ -30,16 +30,17 @@ template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool own = true;  // false: a view of memory someone else holds (never freed here)
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), own(o.own) { o.p = nullptr, o.n = 0, o.own = true; }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             release();
-            p = o.p, n = o.n;
-            o.p = nullptr, o.n = 0;
+            p = o.p, n = o.n, own = o.own;
+            o.p = nullptr, o.n = 0, o.own = true;
         }
         return *this;
     }
@@ -49,9 +50,14 @@ struct DevBuf {
         n = count;
         if (count) SVX_HIP(hipMalloc(&p, count * sizeof(T)));
     }
+    // Use `count` elements at `mem` (device memory owned elsewhere).
+    void view(T* mem, size_t count) {
+        release();
+        p = mem, n = count, own = false;
+    }
     void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr, n = 0;
+        if (p && own) (void)hipFree(p);
+        p = nullptr, n = 0, own = true;
     }
     size_t bytes() const { return n * sizeof(T); }
     void upload(const T* h, size_t count, hipStream_t s) {
