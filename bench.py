@@ -43,7 +43,7 @@ METRIC = "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 
 # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters of rocprofv3,
 # each scaled by what the counter reads for a 1 GiB device copy (profiles/);
 # None where no counter run exists.
-TRAFFIC_BYTES = {("cfg2", "fp64"): 207.0e6}
+TRAFFIC_BYTES = {("cfg2", "fp64"): 207.0e6, ("cfg2", "fp32_ir"): 108.2e6, ("cfg3", "fp32_ir_batched"): 2758.5e6}
 CONFIGS = {
     "cfg2": "C3D8 200x20x20 block 20x2x2 m (265,923 DOF), jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20",
     "cfg1": "CPS4 400x40 cantilever 10x1 m (32,882 DOF), jitter 0.15h, -1e4 N tip load",
@@ -738,6 +738,61 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
 
 
+def profile_spmv(args):
+    """The roofline kernel alone, for profiler runs: the solver set-up of the
+    workload without a solve (svx_batch_prepare / svx_system_prepare_solver:
+    the same tile tables and launch shape as the solve), then the SpMV
+    launched args.spmv_reps times on the solver's stream and timed with HIP
+    events, as the bench's roofline does after its solves. No graph is
+    captured in this mode."""
+    import numpy as np
+
+    from solvix_amd import decks
+    from solvix_amd.core import Batch, Context, assemble, constrained_dofs
+    from solvix_amd.model import Precision
+
+    prec = Precision.FP64 if args.profile_spmv == "fp64" else Precision.FP32_IR
+    ctx = Context(0)
+    ctx.set_operator(args.operator)
+    keep = []
+    if args.config == "cfg3":
+        b = Batch(ctx)
+        for m in decks.cfg3():
+            mesh, sy = assemble(m, ctx)
+            sy.apply_dirichlet(constrained_dofs(m)[0])
+            keep.append((mesh, sy))
+            b.add(sy)
+        if args.profile_shape:  # the autotuned shape of a bench run: no tuning launches here
+            u, r = (int(v) for v in args.profile_shape.split(","))
+            tuned = {"unroll": [0, 0, u, 0], "rows_per_tile": [0, 0, r, 0], "spmv_ms": None}
+        else:
+            tuned = b.autotune(prec, 3)
+        ctx.set_tuning(unroll=tuned["unroll"][2], rows_per_tile=tuned["rows_per_tile"][2])
+        b.prepare(prec)
+        ms, by = b.time_spmv(args.spmv_reps)
+        kernel = ("cg64_mf_spmv_kernel<4|3>" if args.operator == "matrix_free" else
+                  ("cg64_spmv_kernel<2,%d>" if prec == Precision.FP64 else "ir32_spmv_kernel<2,%d>") % tuned["unroll"][2])
+        extra = {"instances": len(keep), "autotune": tuned}
+    else:
+        model = build_model(args.config)
+        mesh, sy = assemble(model, ctx)
+        sy.apply_dirichlet(constrained_dofs(model)[0])
+        keep.append((mesh, sy))
+        sy.prepare_solver(prec)
+        ms, by = sy.time_spmv(prec, args.spmv_reps)
+        kernel = ("cg64_spmv_kernel<%d,...>" if prec == Precision.FP64 else "ir32_spmv_kernel<%d,...>") % model.ndof()
+        extra = {"n_dof": model.n_dofs()}
+    ach = by / (ms * 1e-3) / 1e9
+    print(json.dumps({"profile_spmv": args.profile_spmv, "config": args.config, "operator": args.operator,
+                      "kernel": kernel, "launches": args.spmv_reps + 1, "avg_launch_us": round(ms * 1e3, 2),
+                      "bytes_per_launch": by, "achieved_GBps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                      **extra}), flush=True)
+    for mesh, sy in keep:
+        sy.close()
+        mesh.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -773,6 +828,12 @@ def main():
                     help="served stream: run the server's chunks as captured hipGraphs (on) or kernel by kernel")
     ap.add_argument("--duration", type=float, default=60.0, help="served stream: measured seconds")
     ap.add_argument("--lead", type=float, default=10.0, help="served stream: unmeasured seconds before")
+    ap.add_argument("--profile-spmv", default=None, choices=["fp64", "fp32_ir"],
+                    help="cfg3 / cfg2: time the solver's SpMV of this precision alone after the solver set-up, "
+                         "without solving (profiler runs)")
+    ap.add_argument("--profile-shape", default=None, metavar="UNROLL,ROWS",
+                    help="--profile-spmv on cfg3: the plane launch shape (blocks in flight, rows per tile) instead "
+                         "of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (counter calibration) and exit")
     args = ap.parse_args()
@@ -783,7 +844,9 @@ def main():
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the solver path is a single-GPU "
               f"build; multi-GPU sharding is design-only.", file=sys.stderr)
         sys.exit(2)
-    if args.config == "cfg3":
+    if args.profile_spmv:
+        profile_spmv(args)
+    elif args.config == "cfg3":
         bench_batch(args)
     elif args.config == "cfg5":
         bench_serve(args)

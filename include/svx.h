@@ -379,8 +379,14 @@ int svx_batch_result(const svx_batch* batch, int64_t slot,
 int svx_batch_autotune(svx_batch* batch, int precision, int reps,
                        int32_t* unroll /* [4] */, int32_t* rows_per_tile /* [4] */,
                        double* spmv_ms);
+/* Solver set-up of svx_batch_solve without the solve (vectors, instance
+ * state, tile tables with the tuned or default launch shape and the
+ * operator in force), so that svx_batch_time_spmv can time the batch's SpMV
+ * alone. No reference counterpart: a measurement entry. */
+int svx_batch_prepare(svx_batch* batch, int precision);
 /* Average duration and algorithmic bytes of one SpMV phase over the whole
- * batch (all instances), timed like svx_system_time_spmv. */
+ * batch (all instances), timed like svx_system_time_spmv, after a solve or
+ * svx_batch_prepare with the same precision. */
 int svx_batch_time_spmv(svx_batch* batch, int reps, double* avg_ms,
                         double* bytes_per_launch);
 
@@ -409,6 +415,11 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol,
  * ------------------------------------------------------------------------- */
 int svx_system_time_spmv(svx_system* sys, int precision, int reps,
                          double* avg_ms, double* bytes_per_launch);
+/* Solver set-up of svx_solve for `precision` (vectors, scalar state, tile
+ * tables, with the context's operator) without solving, after which
+ * svx_system_time_spmv times that precision's SpMV. No reference
+ * counterpart: a measurement entry. */
+int svx_system_prepare_solver(svx_system* sys, int precision);
 
 /* Measurement hook (no reference counterpart): the SpMV of every system in
  * systems[0..count) launched in turn, reps rounds, on their shared context's

@@ -340,6 +340,11 @@ class System:
                                  C.byref(cond)))
         return vm, vn, re, cond.value
 
+    def prepare_solver(self, precision: int):
+        """The solver set-up of solve() without solving (svx_system_prepare_solver),
+        so that time_spmv can time that precision's SpMV alone."""
+        _raise(lib().svx_system_prepare_solver(self._h, precision))
+
     def time_spmv(self, precision: int, reps: int = 50) -> Tuple[float, float]:
         """(average ms, algorithmic bytes) of the solver's SpMV kernel."""
         ms = C.c_double()
@@ -427,6 +432,11 @@ class Batch:
         by = C.c_double()
         _raise(lib().svx_batch_time_spmv(self._h, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def prepare(self, precision: int = Precision.FP64):
+        """The solver set-up of solve() without solving (svx_batch_prepare),
+        so that time_spmv can time the batch's SpMV alone."""
+        _raise(lib().svx_batch_prepare(self._h, precision))
 
     def autotune(self, precision: int = Precision.FP64, reps: int = 5) -> dict:
         """Pick the fastest SpMV launch shape for this batch (svx_batch_autotune);

@@ -495,11 +495,11 @@ int status_to_code(const InstScal& sc) {
     return SVX_E_NOT_CONVERGED;
 }
 
-void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,
-                  const TuneParams& tp) {
+// Solver set-up of system_solve without the solve: vectors, scalar state
+// and the engine's tile tables for `precision`.
+void system_engine_setup(svx_system* sys, double tol, int64_t max_iter, int precision, const TuneParams& tp) {
     svx_mesh* m = sys->mesh;
-    svx_ctx* ctx = m->ctx;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = m->ctx->stream;
     system_prepare(sys, precision, tp.matrix_free);
     InstScal sc = initial_scal(tol, max_iter, tp.ru_delta, m->max_row_len);
     sys->scal.upload(&sc, 1, s);
@@ -513,6 +513,14 @@ void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, 
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;
     sys->engine_mf = shape.matrix_free && sys->mf_ready;
+}
+
+void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,
+                  const TuneParams& tp) {
+    svx_mesh* m = sys->mesh;
+    svx_ctx* ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    system_engine_setup(sys, tol, max_iter, precision, tp);
     EngineOptions opt;
     opt.precision = precision;
     opt.period = tp.period;
@@ -926,6 +934,15 @@ int svx_csr_pcg(svx_csr* csr, const double* f, double tol, int64_t max_iter, int
     });
 }
 
+
+int svx_system_prepare_solver(svx_system* sys, int precision) {
+    if (!sys || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
+    return guarded(nullptr, [&] {
+        if (!sys->dirichlet) throw std::invalid_argument("svx_system_prepare_solver needs svx_apply_dirichlet first");
+        system_engine_setup(sys, 1e-10, std::max<int64_t>(1000, 10 * sys->mesh->n), precision, sys->mesh->ctx->tune);
+        return SVX_OK;
+    });
+}
 
 int svx_system_time_spmv(svx_system* sys, int precision, int reps, double* avg_ms, double* bytes_per_launch) {
     if (!sys || reps <= 0 || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;

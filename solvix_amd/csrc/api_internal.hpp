@@ -186,6 +186,8 @@ double spmv_bytes(const svx_mesh* m, int precision);
 // Algorithmic bytes of one matrix-free product (direction step fused in).
 double mf_spmv_bytes(const svx_mesh* m);
 InstScal initial_scal(double tol, int64_t max_iter, double ru_delta, int64_t max_row_len);
+// The set-up half of system_solve: vectors, scalar state, engine tables.
+void system_engine_setup(svx_system* sys, double tol, int64_t max_iter, int precision, const TuneParams& tp);
 void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,
                   const TuneParams& tp);
 void system_recover(svx_system* sys, double* vm_elem, double* vm_node, int64_t n_rows, const int64_t* rows,

@@ -214,6 +214,22 @@ int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
     });
 }
 
+int svx_batch_prepare(svx_batch* b, int precision) {
+    if (!b || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
+    return guarded_b(nullptr, [&]() -> int {
+        const size_t n = b->sys.size();
+        if (n == 0) return SVX_E_ARGUMENT;
+        std::vector<InstDesc> descs = batch_setup(b, precision);
+        EngineShape shape =
+            b->tuned_precision == precision ? b->tuned_shape : batch_shape(b->sys, b->ctx->tune, precision);
+        shape.matrix_free = b->ctx->tune.matrix_free && precision == SVX_FP64;
+        b->engine->set_instances(descs, std::vector<char>(n, 1), shape);
+        b->matrix_free = shape.matrix_free;
+        b->precision = precision;
+        return SVX_OK;
+    });
+}
+
 int svx_batch_result(const svx_batch* b, int64_t slot, int64_t* iterations, double* residual, int32_t* converged) {
     if (!b || slot < 0 || slot >= (int64_t)b->h_scal.size()) return SVX_E_ARGUMENT;
     const InstScal& sc = b->h_scal[slot];

@@ -621,3 +621,29 @@ def test_inp_round_trip():
     m2 = parse_inp(write_inp(m))
     assert np.array_equal(m.coords, m2.coords) and np.array_equal(m.conn_ids, m2.conn_ids)
     assert np.array_equal(m.cload_value, m2.cload_value)
+
+
+# ------------------------------------------------------------- error wording
+# The reference's messages, as its source writes them: validate()'s
+# ModelError through format!("{e:?}") (lib.rs:604-606, 406-437), the
+# thickness and dimensionality checks (lib.rs:607-611, 747), the element
+# errors "element {id}: {reason}" (element.rs:1056-1058, assembly.rs:145-199)
+# and the constraint errors (solver.rs:21-33).
+REFERENCE_MESSAGES = {
+    "empty_mesh": "EmptyMesh", "no_material": "NoMaterial", "malformed_element": "MalformedElement(2)",
+    "unknown_node": "UnknownNode(999)", "mixed_dimensionality": "mixed dimensionality",
+    "thickness": "thickness must be positive", "degenerate_triangle": "element 2: degenerate (zero-area) triangle",
+    "poisson_solid": "element 1: poisson ratio must be < 0.5 for solids and plane strain",
+    "bad_material": "element 1: invalid material constants", "no_material_assigned": "element 2: no material assigned",
+    "inverted_quad": "element 2: non-positive Jacobian determinant at a Gauss point",
+    "unknown_bc_node": "boundary condition on unknown node 77",
+    "bad_dof": "boundary condition on node 1 has unsupported dof 3", "no_constraints": "NoConstraints",
+}
+
+
+@pytest.mark.parametrize("name", sorted(REFERENCE_MESSAGES))
+def test_oracle_errors_use_the_reference_wording(name):
+    import test_gpu_errors as cases  # the models the device error tests use
+    r = oracle.solve(cases.CASES[name]())
+    expect_code = _abi.SVX_E_NO_CONSTRAINTS if name == "no_constraints" else _abi.SVX_E_INVALID_MODEL
+    assert (r.status, r.message) == (expect_code, REFERENCE_MESSAGES[name])
