@@ -41,7 +41,9 @@ enum {
     SVX_E_DEVICE = 5,   /* HIP runtime failure; message holds the HIP string */
     SVX_E_ARGUMENT = 6, /* bad handle, size, option or index */
     SVX_E_CAPACITY = 7, /* batch / server budget or slot limit exceeded */
-    SVX_E_TIMEOUT = 8   /* svx_server_wait: the request is not done yet */
+    SVX_E_TIMEOUT = 8,  /* svx_server_wait: the request is not done yet */
+    SVX_E_PARSE = 9     /* deck text the .inp reader rejects: message "line N: ...",
+                           svx_error.index = N */
 };
 
 /* Element kinds on the path (lib.rs:53-73, subset CPS3 / CPS4 / C3D8). */
@@ -168,6 +170,7 @@ typedef struct svx_solution {
 } svx_solution;
 
 typedef struct svx_ctx svx_ctx;
+typedef struct svx_deck svx_deck;
 typedef struct svx_mesh svx_mesh;
 typedef struct svx_system svx_system;
 typedef struct svx_csr svx_csr;
@@ -220,6 +223,22 @@ int svx_system_operator(const svx_system* sys);
 int svx_mf_plan_check(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem,
                       const uint8_t* kinds, const int64_t* conn_ptr, const int32_t* conn_idx,
                       int64_t* stats, double* max_rel_diff);
+
+/* ---------------------------------------------------------------------------
+ * Deck text — replaces solvix_io::parse_inp (crates/solvix-io/src/parser.rs:87)
+ * for the keywords the linear path's decks use: *NODE (146-173), *ELEMENT
+ * TYPE=CPS3|CPS4|C3D8 [ELSET=] with comma continuation (175-256), *NSET /
+ * *ELSET [GENERATE] (258-312, names upper-cased, duplicates kept), *MATERIAL
+ * + *ELASTIC (496-535), *SOLID SECTION (537-584, model-global thickness),
+ * *STEP [OP=NEW] / *END STEP (1114-1150), *BOUNDARY node|set, dof[, dof2]
+ * (742-796), *CLOAD node|set, dof, value (798-844); the last step's
+ * constraints and loads, else those outside any step (1325-1336). Other
+ * keywords are skipped. Errors: SVX_E_PARSE.
+ * ------------------------------------------------------------------------- */
+int svx_deck_parse(const char* text, size_t len, svx_deck** out, svx_error* err);
+/* The parsed model; valid until svx_deck_destroy. */
+const svx_model_desc* svx_deck_model(const svx_deck* deck);
+void svx_deck_destroy(svx_deck* deck);
 
 /* ---------------------------------------------------------------------------
  * Whole pipeline — replaces solve_with_method(&Model, Method) (lib.rs:586),
@@ -472,6 +491,9 @@ typedef struct svx_request_info {
     int64_t iterations, refreshes, fp64_iterations;
     int64_t device_bytes; /* admitted footprint estimate */
     int64_t n_dof;
+    /* array lengths the result needs (for decks submitted as text): nodes,
+     * elements, reactions; 0 before the deck is read */
+    int64_t n_nodes, n_elem, n_reactions;
 } svx_request_info;
 
 typedef struct svx_server_summary {
@@ -497,6 +519,11 @@ void svx_server_destroy(svx_server* srv);
 /* Copies the deck; returns at once with a ticket. */
 int svx_server_submit(svx_server* srv, const svx_model_desc* model,
                       int64_t* ticket);
+/* Copies the deck text; a host worker reads it (svx_deck_parse) before
+ * preparing the request. Read errors come back from svx_server_wait as
+ * SVX_E_PARSE. */
+int svx_server_submit_deck(svx_server* srv, const char* text, size_t len,
+                           int64_t* ticket);
 /* Waits for a ticket (timeout_ms < 0: no limit; SVX_E_TIMEOUT when it
  * expires, the ticket stays valid). On completion fills out (arrays sized as
  * for svx_solve_model) and info, returns the request's status (errors as
@@ -504,7 +531,9 @@ int svx_server_submit(svx_server* srv, const svx_model_desc* model,
  * budget, SVX_E_DEVICE for requests the server dropped when it failed or was
  * destroyed), and releases the ticket. A done request whose reactions exceed
  * out->reaction_capacity returns SVX_E_ARGUMENT, writes nothing and keeps
- * the ticket, so the caller can wait again with larger arrays. */
+ * the ticket, so the caller can wait again with larger arrays. With out NULL
+ * the call waits, fills info (with the result's array lengths) and returns
+ * the status, and keeps the ticket. */
 int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms,
                     svx_solution* out, svx_request_info* info,
                     svx_error* err);

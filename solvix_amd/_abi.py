@@ -10,6 +10,7 @@ SVX_E_DEVICE = 5
 SVX_E_ARGUMENT = 6
 SVX_E_CAPACITY = 7
 SVX_E_TIMEOUT = 8
+SVX_E_PARSE = 9
 
 SVX_CPS3, SVX_CPS4, SVX_C3D8 = 0, 1, 2
 KIND_NAMES = {"CPS3": SVX_CPS3, "CPS4": SVX_CPS4, "C3D8": SVX_C3D8}
@@ -94,6 +95,9 @@ class SvxRequestInfo(C.Structure):
         ("fp64_iterations", C.c_int64),
         ("device_bytes", C.c_int64),
         ("n_dof", C.c_int64),
+        ("n_nodes", C.c_int64),
+        ("n_elem", C.c_int64),
+        ("n_reactions", C.c_int64),
     ]
 
 

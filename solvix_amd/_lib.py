@@ -68,6 +68,10 @@ SIGNATURES = [
     ("svx_server_create", C.c_int, [_vp, C.POINTER(_abi.SvxServerConfig), C.POINTER(_vp)]),
     ("svx_server_destroy", None, [_vp]),
     ("svx_server_submit", C.c_int, [_vp, C.POINTER(_abi.SvxModelDesc), _i64p]),
+    ("svx_server_submit_deck", C.c_int, [_vp, C.c_char_p, C.c_size_t, _i64p]),
+    ("svx_deck_parse", C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(_vp), _errp]),
+    ("svx_deck_model", C.POINTER(_abi.SvxModelDesc), [_vp]),
+    ("svx_deck_destroy", None, [_vp]),
     ("svx_server_wait", C.c_int, [_vp, _i64, C.c_double, C.POINTER(_abi.SvxSolution),
                                   C.POINTER(_abi.SvxRequestInfo), _errp]),
     ("svx_server_stats", C.c_int, [_vp, C.POINTER(_abi.SvxServerSummary)]),

@@ -33,6 +33,7 @@ SOURCES = [
     ("block_api.cpp", []),
     ("server.cpp", []),
     ("noalloc.cpp", []),
+    ("deck.cpp", []),
 ]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE,

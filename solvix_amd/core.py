@@ -123,6 +123,20 @@ def solve(model: Model, precision: int = Precision.FP64, ctx: Optional[Context] 
     return solve_with_method(model, Method.AUTO, precision, ctx)
 
 
+def parse_deck(text: str, name: str = "model") -> Model:
+    """parse_inp (crates/solvix-io/src/parser.rs:87) for the linear path's
+    keyword subset, read by the library's C++ reader (svx_deck_parse; no
+    device needed); raises SolveError kind "Parse" with "line N: ..."."""
+    raw = text.encode()
+    h = C.c_void_p()
+    err = _abi.SvxError()
+    _raise(lib().svx_deck_parse(raw, len(raw), C.byref(h), C.byref(err)), err)
+    try:
+        return Model.from_desc(lib().svx_deck_model(h).contents, name)
+    finally:
+        lib().svx_deck_destroy(h)
+
+
 def constrained_dofs(model: Model) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
     """constrained_dofs (solver.rs:15-41) -> (rows, node_ids, dofs), BC order."""
     ndof = model.ndof()
@@ -539,12 +553,31 @@ class Server:
         self._pending[t.value] = model
         return t.value
 
+    def submit_deck(self, text: str) -> int:
+        """Deck text (svx_server_submit_deck): a host worker of the server reads
+        it; read errors come back from wait() as SolveError kind "Parse"."""
+        raw = text.encode()
+        t = C.c_int64()
+        _raise(lib().svx_server_submit_deck(self._h, raw, len(raw), C.byref(t)))
+        self._pending[t.value] = None
+        return t.value
+
     def wait(self, ticket: int, timeout_ms: float = -1.0) -> Tuple[Solution, dict]:
         """(Solution, request info); raises SolveError like solve_with_method
         (kind "Timeout" if the request is not done within timeout_ms)."""
         model = self._pending[ticket]
-        nn, ne = model.n_nodes, model.n_elem
-        nr = max(0, int(model.reaction_count()))
+        if model is None:  # deck text: the server reports the result's sizes
+            info = _abi.SvxRequestInfo()
+            err = _abi.SvxError()
+            rc = lib().svx_server_wait(self._h, ticket, timeout_ms, None, C.byref(info), C.byref(err))
+            if rc not in (_abi.SVX_OK, _abi.SVX_E_TIMEOUT, _abi.SVX_E_ARGUMENT):
+                del self._pending[ticket]
+                lib().svx_server_wait(self._h, ticket, 0.0, C.byref(_abi.SvxSolution()), None, None)  # release
+            _raise(rc, err)
+            nn, ne, nr = int(info.n_nodes), int(info.n_elem), int(info.n_reactions)
+        else:
+            nn, ne = model.n_nodes, model.n_elem
+            nr = max(0, int(model.reaction_count()))
         disp = np.zeros((nn, 3))
         vm = np.zeros(ne)
         nvm = np.zeros(nn)

@@ -27,6 +27,7 @@
 #include <unordered_map>
 
 #include "api_internal.hpp"
+#include "deck.hpp"
 
 using namespace svx;
 
@@ -47,61 +48,6 @@ void put_err(svx_error* err, int code, const std::string& msg, int reason = 0, i
     snprintf(err->message, sizeof err->message, "%s", msg.c_str());
 }
 
-// A deck owned by the server (the caller's arrays may go away after submit).
-struct OwnedModel {
-    std::vector<uint32_t> node_ids, elem_ids, conn_ids, elset_members, bc_nodes, cload_node;
-    std::vector<double> coords, youngs, poisson, cload_value;
-    std::vector<uint8_t> kinds;
-    std::vector<int64_t> conn_ptr, elset_ptr, bc_node_ptr, bc_dof_ptr;
-    std::vector<int32_t> assign_elset, assign_material, bc_dofs, cload_dof;
-    svx_model_desc desc{};
-
-    template <typename T>
-    static void copy(std::vector<T>& v, const T* p, int64_t n) {
-        v.assign(p ? p : nullptr, p ? p + std::max<int64_t>(0, n) : nullptr);
-    }
-    explicit OwnedModel(const svx_model_desc& m) {
-        copy(node_ids, m.node_ids, m.n_nodes);
-        copy(coords, m.coords, m.n_nodes * 3);
-        copy(elem_ids, m.elem_ids, m.n_elem);
-        copy(kinds, m.kinds, m.n_elem);
-        copy(conn_ptr, m.conn_ptr, m.n_elem + 1);
-        copy(conn_ids, m.conn_ids, m.conn_ptr ? m.conn_ptr[m.n_elem] : 0);
-        copy(youngs, m.youngs, m.n_materials);
-        copy(poisson, m.poisson, m.n_materials);
-        copy(elset_ptr, m.elset_ptr, m.n_elsets + 1);
-        copy(elset_members, m.elset_members, m.elset_ptr ? m.elset_ptr[m.n_elsets] : 0);
-        copy(assign_elset, m.assign_elset, m.n_assign);
-        copy(assign_material, m.assign_material, m.n_assign);
-        copy(bc_node_ptr, m.bc_node_ptr, m.n_bcs + 1);
-        copy(bc_nodes, m.bc_nodes, m.bc_node_ptr ? m.bc_node_ptr[m.n_bcs] : 0);
-        copy(bc_dof_ptr, m.bc_dof_ptr, m.n_bcs + 1);
-        copy(bc_dofs, m.bc_dofs, m.bc_dof_ptr ? m.bc_dof_ptr[m.n_bcs] : 0);
-        copy(cload_node, m.cload_node, m.n_cloads);
-        copy(cload_dof, m.cload_dof, m.n_cloads);
-        copy(cload_value, m.cload_value, m.n_cloads);
-        desc = m;
-        desc.node_ids = node_ids.data();
-        desc.coords = coords.data();
-        desc.elem_ids = elem_ids.data();
-        desc.kinds = kinds.data();
-        desc.conn_ptr = conn_ptr.data();
-        desc.conn_ids = conn_ids.data();
-        desc.youngs = youngs.data();
-        desc.poisson = poisson.data();
-        desc.elset_ptr = elset_ptr.data();
-        desc.elset_members = elset_members.data();
-        desc.assign_elset = assign_elset.data();
-        desc.assign_material = assign_material.data();
-        desc.bc_node_ptr = bc_node_ptr.data();
-        desc.bc_nodes = bc_nodes.data();
-        desc.bc_dof_ptr = bc_dof_ptr.data();
-        desc.bc_dofs = bc_dofs.data();
-        desc.cload_node = cload_node.data();
-        desc.cload_dof = cload_dof.data();
-        desc.cload_value = cload_value.data();
-    }
-};
 
 // Queued: in prep_q. Preparing: a host worker holds it (nobody else may
 // touch it until the worker publishes it under the lock). Prepared: in
@@ -110,7 +56,8 @@ enum class St { Queued, Preparing, Prepared, Active, Done };
 
 struct Request {
     int64_t ticket = 0;
-    std::unique_ptr<OwnedModel> model;
+    std::unique_ptr<OwnedModel> model;  // null until a host worker reads deck_text
+    std::string deck_text;              // svx_server_submit_deck
     PreparedModel pm;
     std::unique_ptr<svx_mesh> mesh;
     std::unique_ptr<svx_system> sys;
@@ -213,7 +160,20 @@ void svx_server::prep_loop() {
         err.index = -1;
         int rc = SVX_OK;
         try {
-            rc = prepare_model(&r->model->desc, r->pm, &err);
+            if (!r->model) {  // deck text: read it here, off the submitting thread
+                auto om = std::make_unique<OwnedModel>();
+                std::string msg;
+                const int64_t line = parse_inp(r->deck_text.data(), r->deck_text.size(), *om, &msg);
+                std::string().swap(r->deck_text);
+                if (line) {
+                    rc = SVX_E_PARSE;
+                    put_err(&err, SVX_E_PARSE, "line " + std::to_string(line) + ": " + msg, 0, line);
+                } else {
+                    std::lock_guard<std::mutex> lk(mu);  // waiters read model sizes under the lock
+                    r->model = std::move(om);
+                }
+            }
+            if (rc == SVX_OK) rc = prepare_model(&r->model->desc, r->pm, &err);
             if (rc == SVX_OK) {
                 r->mesh = std::make_unique<svx_mesh>();
                 r->mesh->ctx = aux;
@@ -578,6 +538,28 @@ int svx_server_submit(svx_server* srv, const svx_model_desc* model, int64_t* tic
     }
 }
 
+int svx_server_submit_deck(svx_server* srv, const char* text, size_t len, int64_t* ticket) {
+    if (!srv || !ticket || (!text && len)) return SVX_E_ARGUMENT;
+    try {
+        auto r = std::make_unique<Request>();
+        r->deck_text.assign(text ? text : "", len);
+        r->err.index = -1;
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (srv->stop) return SVX_E_DEVICE;
+        r->ticket = srv->next_ticket++;
+        r->t_submit = wall_ms();
+        if (srv->t_first < 0) srv->t_first = r->t_submit;
+        *ticket = r->ticket;
+        srv->prep_q.push_back(r.get());
+        srv->reqs[r->ticket] = std::move(r);
+        srv->submitted++;
+        srv->cv_prep.notify_one();
+        return SVX_OK;
+    } catch (const std::exception&) {
+        return SVX_E_DEVICE;
+    }
+}
+
 int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms, svx_solution* out, svx_request_info* info,
                     svx_error* err) {
     if (err) {
@@ -603,7 +585,20 @@ int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms, svx_solu
             srv->cv_done.wait(lk, done);
         else if (!srv->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done))
             return SVX_E_TIMEOUT;
-        const bool wants_reactions = out && (out->reaction_node || out->reaction_dof || out->reaction_value);
+        if (!out) {  // sizes and status only; the ticket stays valid
+            if (info) {
+                *info = svx_request_info{};
+                if (q->model) {
+                    info->n_nodes = q->model->desc.n_nodes;
+                    info->n_elem = q->model->desc.n_elem;
+                    info->n_reactions = svx_model_reaction_count(&q->model->desc);
+                }
+                info->n_dof = q->model ? info->n_nodes * q->pm.ndof : 0;
+            }
+            if (q->rc != SVX_OK && err) *err = q->err;
+            return q->rc;
+        }
+        const bool wants_reactions = out->reaction_node || out->reaction_dof || out->reaction_value;
         if (q->rc == SVX_OK && wants_reactions && (int64_t)q->pm.cons_rows.size() > out->reaction_capacity) {
             if (err) put_err(err, SVX_E_ARGUMENT, "svx_server_wait: reaction arrays smaller than the request's reaction count");
             return SVX_E_ARGUMENT;  // the ticket stays valid
@@ -619,7 +614,10 @@ int svx_server_wait(svx_server* srv, int64_t ticket, double timeout_ms, svx_solu
         info->refreshes = r->refreshes;
         info->fp64_iterations = r->fp64_iterations;
         info->device_bytes = r->bytes;
-        info->n_dof = (int64_t)r->model->desc.n_nodes * r->pm.ndof;
+        info->n_dof = r->model ? (int64_t)r->model->desc.n_nodes * r->pm.ndof : 0;
+        info->n_nodes = r->model ? r->model->desc.n_nodes : 0;
+        info->n_elem = r->model ? r->model->desc.n_elem : 0;
+        info->n_reactions = r->model ? svx_model_reaction_count(&r->model->desc) : 0;
     }
     if (r->rc != SVX_OK) {
         if (err) *err = r->err;

@@ -41,6 +41,7 @@ _KIND_OF_CODE = {
     _abi.SVX_E_ARGUMENT: "Argument",
     _abi.SVX_E_CAPACITY: "Capacity",
     _abi.SVX_E_TIMEOUT: "Timeout",
+    _abi.SVX_E_PARSE: "Parse",
 }
 
 
@@ -180,6 +181,42 @@ class Model:
             P(cl_val, C.c_double),
         )
         return d, keep
+
+    @classmethod
+    def from_desc(cls, d, name: str = "model") -> "Model":
+        """A Model holding copies of an svx_model_desc's arrays. The
+        descriptor carries no names: element sets and materials are named by
+        position ("ES0", "MAT0", ...), and an assignment index of -1 by a name
+        that resolves to nothing, so resolution is unchanged."""
+        def get(p, n, dt):
+            return np.ctypeslib.as_array(p, shape=(int(n),)).astype(dt, copy=True) if n > 0 else np.zeros(0, dt)
+        n_nodes, n_elem = int(d.n_nodes), int(d.n_elem)
+        conn_ptr = get(d.conn_ptr, n_elem + 1, np.int64)
+        es_ptr = get(d.elset_ptr, d.n_elsets + 1, np.int64)
+        es_mem = get(d.elset_members, es_ptr[-1] if es_ptr.size else 0, np.uint32)
+        E, nu = get(d.youngs, d.n_materials, np.float64), get(d.poisson, d.n_materials, np.float64)
+        a_es, a_mat = get(d.assign_elset, d.n_assign, np.int32), get(d.assign_material, d.n_assign, np.int32)
+        bn = get(d.bc_node_ptr, d.n_bcs + 1, np.int64)
+        bd = get(d.bc_dof_ptr, d.n_bcs + 1, np.int64)
+        bc_nodes = get(d.bc_nodes, bn[-1] if bn.size else 0, np.uint32)
+        bc_dofs = get(d.bc_dofs, bd[-1] if bd.size else 0, np.int32)
+        return cls(
+            node_ids=get(d.node_ids, n_nodes, np.uint32),
+            coords=get(d.coords, 3 * n_nodes, np.float64).reshape(-1, 3),
+            elem_ids=get(d.elem_ids, n_elem, np.uint32),
+            kinds=get(d.kinds, n_elem, np.uint8),
+            conn_ptr=conn_ptr,
+            conn_ids=get(d.conn_ids, conn_ptr[-1] if conn_ptr.size else 0, np.uint32),
+            materials=[(f"MAT{k}", float(E[k]), float(nu[k])) for k in range(E.size)],
+            elsets=[(f"ES{k}", es_mem[es_ptr[k]:es_ptr[k + 1]]) for k in range(es_ptr.size - 1)],
+            assignments=[(f"ES{e}" if e >= 0 else "?", f"MAT{m}" if m >= 0 else "?") for e, m in zip(a_es, a_mat)],
+            thickness=float(d.thickness),
+            bcs=[(bc_nodes[bn[k]:bn[k + 1]], bc_dofs[bd[k]:bd[k + 1]]) for k in range(bn.size - 1)],
+            cload_node=get(d.cload_node, d.n_cloads, np.uint32),
+            cload_dof=get(d.cload_dof, d.n_cloads, np.int32),
+            cload_value=get(d.cload_value, d.n_cloads, np.float64),
+            name=name,
+        )
 
 
 def simple_model(node_ids, coords, elem_ids, kinds, conn, E, nu, thickness=1.0,

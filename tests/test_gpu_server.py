@@ -182,3 +182,34 @@ def test_destroy_fails_pending_requests_and_releases_waiters():
     print("outcomes", outcomes)
     assert len(outcomes) == 3 and all(o in ("done", "Device") for o in outcomes)
     sctx.close()
+
+
+def test_deck_text_submissions_equal_solves_of_the_parsed_decks(ctx):
+    """svx_server_submit_deck: the server's host workers read the deck text;
+    each result equals solve_with_method on the deck read by inp.py, and a
+    deck the reader rejects comes back as a Parse error with its line."""
+    import os
+    from solvix_amd.inp import parse_inp, write_inp
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    texts = [open(os.path.join(golden, f"{n}.inp")).read() for n in ("cantilever", "cantilever3d", "plate_hole")]
+    texts += [write_inp(m) for m in small_models()[:3]]
+    sctx = Context(0)
+    srv = Server(sctx, Precision.FP64, hbm_budget=1 << 30, max_active=4)
+    try:
+        tickets = [srv.submit_deck(t) for t in texts]
+        bad = srv.submit_deck("*NODE\n1, 0.0\n")
+        for text, t in zip(texts, tickets):
+            s, info = srv.wait(t, 60000.0)
+            a = solve_with_method(parse_inp(text), Method.AUTO, Precision.FP64, ctx)
+            print(f"deck: {info['n_nodes']} nodes, {info['n_elem']} elements, its {s.iterations} "
+                  f"(alone {a.iterations}), bitwise {np.array_equal(s.displacements, a.displacements)}")
+            assert rel_err(s.displacements, a.displacements) <= 1e-8
+            assert rel_err(s.von_mises, a.von_mises) <= 1e-8
+            assert [r.node_id for r in s.reactions] == [r.node_id for r in a.reactions]
+        with pytest.raises(SolveError) as e:
+            srv.wait(bad, 60000.0)
+        assert e.value.kind == "Parse" and e.value.message.startswith("line 2: *NODE expects")
+        assert srv.stats()["failed"] == 1
+    finally:
+        srv.close()
+        sctx.close()
