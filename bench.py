@@ -198,11 +198,13 @@ def bench_batch(args):
             b.add(sy)
         t = tuned[(prec, mf)]  # the autotuned shape of this (2D-only) batch, node-block size 2
         ctx.set_tuning(unroll=t["unroll"][2], rows_per_tile=t["rows_per_tile"][2])
+        ctx.set_tile_order("xcd" if t["tile_order"][2] else "linear")
         ctx.set_operator("matrix_free" if mf else "assembled")
         try:
             res = b.solve(prec, raise_on_failure=False)
         finally:
             ctx.set_tuning()
+            ctx.set_tile_order("xcd")
             ctx.set_operator("assembled")
         clock("solve", t0)
         t0 = time.perf_counter()
@@ -599,7 +601,7 @@ def bench_multi(args):
         for nid, d, v in zip(nodes, dofs, vals):
             F[k, pos[int(nid)] * 3 + int(d) - 1] += v
     out = {}
-    arms = [a for a in ("single", "block") if args.multi_arm in ("both", a)]
+    arms = [a for a in ("single", "block", "block_fp32_ir") if args.multi_arm in ("both", a)]
     for arm in arms:
         for k in range(args.warmup + args.steps):
             t0 = time.perf_counter()
@@ -611,7 +613,8 @@ def bench_multi(args):
                     U[j], its[j], _ = sy.solve(1e-10, 0, Precision.FP64)
                 ok = np.ones(len(cases), bool)
             else:
-                U, its, res, ok, dev_ms = sy.solve_multi(F, 1e-10, 0, Precision.FP64, raise_on_failure=False)
+                prec = Precision.FP64 if arm == "block" else Precision.FP32_IR
+                U, its, res, ok, dev_ms = sy.solve_multi(F, 1e-10, 0, prec, raise_on_failure=False)
             dt = time.perf_counter() - t0
             log(f"{arm}: pass {k + 1}/{args.warmup + args.steps} {dt:.3f} s, converged {int(ok.sum())}/{ok.size}, "
                 f"iterations max {int(its.max())} mean {its.mean():.0f}")
@@ -619,22 +622,31 @@ def bench_multi(args):
                 rec = out.setdefault(arm, {"dt": 0.0})
                 rec["dt"] += dt
                 rec.update(U=U, its=its, ok=ok)
-    if len(arms) < 2:
+    if len(arms) < 3:
         return
-    a, b = out["block"], out["single"]
+    mixed, blk, b = out["block_fp32_ir"], out["block"], out["single"]
     dof = float(n) * len(cases) * args.steps
-    err = float(max(rel_err(a["U"][j], b["U"][j]) for j in range(len(cases))))
+
+    def err(a):
+        return float(max(rel_err(a["U"][j], b["U"][j]) for j in range(len(cases))))
+
+    def arm(a):
+        return {"value": round(dof / a["dt"], 1), "ms_per_step": round(a["dt"] / args.steps * 1e3, 3),
+                "iterations_max": int(a["its"].max()), "iterations_mean": round(float(a["its"].mean()), 1),
+                "converged": int(a["ok"].sum()), "max_rel_err_u_vs_single_fp64": err(a),
+                "speedup_vs_single_fp64": round(b["dt"] / a["dt"], 3)}
     line = {
-        "metric": METRIC, "value": round(dof / a["dt"], 1), "unit": "solved DOF/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(a["dt"] / args.steps * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": METRIC, "value": round(dof / mixed["dt"], 1), "unit": "solved DOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mixed["dt"] / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 Krylov + f64 refinement",
         "data": "synthetic (seeded SplitMix64 deck generator)",
         "config": {"workload": CONFIGS["cfg4"], "n_dof": n, "nnz": int(mesh.nnz), "load_cases": len(cases),
                    "tolerance": 1e-10},
-        "iterations_max": int(a["its"].max()), "converged": int(a["ok"].sum()), "max_rel_err_u_vs_single": err,
+        "arm": "block_fp32_ir", "comparator": "single_fp64",
+        "speedup_vs_fp64": round(b["dt"] / mixed["dt"], 3),
+        "block_fp32_ir": arm(mixed), "block_fp64": arm(blk),
         "single_fp64": {"value": round(dof / b["dt"], 1), "ms_per_step": round(b["dt"] / args.steps * 1e3, 3),
-                        "iterations_max": int(b["its"].max())},
-        "speedup_vs_fp64": round(b["dt"] / a["dt"], 3),
+                        "iterations_max": int(b["its"].max()), "iterations_mean": round(float(b["its"].mean()), 1)},
     }
     print(json.dumps(line), flush=True)
 
@@ -645,7 +657,9 @@ def bench_serve(args):
     times from a client thread while the server solves; latency is submit to
     result. Requests arriving in the first `lead` seconds are served but not
     measured. Solved DOF/s counts the measured requests' DOF over the span
-    from the first measured submission to the last measured completion."""
+    from the first measured submission to the last measured completion. The
+    samples (oracle errors, plain FP64 one by one) are chosen before the run;
+    every other request's deck is dropped once the server has copied it."""
     import threading
 
     import numpy as np
@@ -654,14 +668,31 @@ def bench_serve(args):
     from solvix_amd.core import Context, Server, solve_with_method
     from solvix_amd.model import Method, Precision
 
+    t_gen = time.perf_counter()
     stream = list(decks.cfg5_stream(duration_s=args.lead + args.duration, rate=args.rate))
-    log(f"{len(stream)} requests generated ({args.rate}/s over {args.lead + args.duration} s)")
+    dofs = [r.model.n_dofs() for r in stream]
+    offered = sum(dofs) / (args.lead + args.duration)
+    measured = [i for i, r in enumerate(stream) if r.arrival_s >= args.lead]
+    # per-request error against the FP64 oracle: a seeded sample of the
+    # measured requests bounded by DOF; plain FP64 one by one: every k-th
+    rng = decks.SplitMix64(decks.SEED_BASE + 5 ^ 0xC0FFEE)
+    order = sorted(measured, key=lambda k: int(rng.next_u64(1)[0]))
+    oracle_ids, tot = [], 0
+    for i in order:
+        if tot + dofs[i] <= args.cpu_sample_dof:
+            oracle_ids.append(i)
+            tot += dofs[i]
+    plain_ids = measured[::max(1, len(measured) // 60)]
+    keep = set(oracle_ids) | set(plain_ids)
+    log(f"{len(stream)} requests generated ({args.rate}/s over {args.lead + args.duration} s, offered "
+        f"{offered / 1e6:.2f} M DOF/s) in {time.perf_counter() - t_gen:.1f} s")
     ctx = Context(0)
     prec = Precision.FP64 if args.serve_precision == "fp64" else Precision.FP32_IR
     ctx.set_operator(args.serve_operator)  # the server keeps the operator in force at its creation
     srv = Server(ctx, prec, hbm_budget=200 * 10 ** 9, max_active=1024, use_graph=args.serve_graph == "on")
     ctx.set_operator("assembled")  # the FP64 references below multiply the assembled K
     subs = [None] * len(stream)
+    models = {i: stream[i].model for i in keep}
     t0 = time.perf_counter()
 
     def client():
@@ -670,51 +701,42 @@ def bench_serve(args):
             if delay > 0:
                 time.sleep(delay)
             subs[i] = (time.perf_counter(), srv.submit(req.model))
+            req.model = None  # the server holds its own copy
 
     th = threading.Thread(target=client)
     th.start()
-    done = []
+    done = {}
+    sols = {}
     last_print = time.perf_counter()
-    for i, req in enumerate(stream):
+    for i in range(len(stream)):
         while subs[i] is None:
             time.sleep(0.001)
         t_sub, ticket = subs[i]
         sol, info = srv.wait(ticket)
-        done.append((i, t_sub - t0, time.perf_counter() - t0, info, req.model.n_dofs(), sol))
+        done[i] = (t_sub - t0, time.perf_counter() - t0, info["latency_ms"])
+        if i in oracle_ids:
+            sols[i] = sol
         if time.perf_counter() - last_print > 5.0:
             st = srv.stats()
-            log(f"{i + 1}/{len(stream)} done at {time.perf_counter() - t0:.1f} s, active {st['peak_active']} peak, "
-                f"queued {st['queued']}")
+            log(f"{i + 1}/{len(stream)} done at {time.perf_counter() - t0:.1f} s, peak active {st['peak_active']}, "
+                f"queued {st['queued']}, resident {st['resident_bytes'] / 1e9:.1f} GB")
             last_print = time.perf_counter()
     th.join()
     st = srv.stats()
     srv.close()
-    meas = [d for d in done if stream[d[0]].arrival_s >= args.lead]
-    lat = np.array([d[3]["latency_ms"] for d in meas])
-    span = max(d[2] for d in meas) - min(d[1] for d in meas)
-    dof = float(sum(d[4] for d in meas))
-    # per-request error against the FP64 oracle on a seeded sample of the
-    # measured requests, bounded by DOF (the oracle's reference-mode lookups)
+    lat = np.array([done[i][2] for i in measured])
+    span = max(done[i][1] for i in measured) - min(done[i][0] for i in measured)
+    dof = float(sum(dofs[i] for i in measured))
     import oracle
-    rng = decks.SplitMix64(decks.SEED_BASE + 5 ^ 0xC0FFEE)
-    order = sorted(range(len(meas)), key=lambda k: int(rng.next_u64(1)[0]))
-    errs, tot = [], 0
-    for k in order:
-        if tot >= args.cpu_sample_dof:
-            break
-        d = meas[k]
-        r = oracle.solve(stream[d[0]].model, want_csr=False)
-        errs.append((rel_err(d[5].displacements, r.displacements), rel_err(d[5].von_mises, r.von_mises)))
-        tot += d[4]
-    # plain FP64 PCG, one request after another, on a sample of the measured
-    # requests (every k-th, about 60): the same mix solved without the server
-    k = max(1, len(meas) // 60)
-    plain = meas[::k]
+    errs = []
+    for i in oracle_ids:
+        r = oracle.solve(models[i], want_csr=False)
+        errs.append((rel_err(sols[i].displacements, r.displacements), rel_err(sols[i].von_mises, r.von_mises)))
     t1 = time.perf_counter()
-    for d in plain:
-        solve_with_method(stream[d[0]].model, Method.AUTO, Precision.FP64, ctx)
-    plain_dps = float(sum(d[4] for d in plain)) / (time.perf_counter() - t1)
-    log(f"plain FP64 one by one on {len(plain)} sampled requests: {plain_dps / 1e6:.2f} M DOF/s")
+    for i in plain_ids:
+        solve_with_method(models[i], Method.AUTO, Precision.FP64, ctx)
+    plain_dps = float(sum(dofs[i] for i in plain_ids)) / (time.perf_counter() - t1)
+    log(f"plain FP64 one by one on {len(plain_ids)} sampled requests: {plain_dps / 1e6:.2f} M DOF/s")
     line = {
         "metric": METRIC, "value": round(dof / span, 1), "unit": "solved DOF/s", "n_gpus": 1, "steps": 1,
         "warmup": 0, "ms_per_step": round(span * 1e3, 1), "higher_is_better": True, "scaling": "weak",
@@ -723,16 +745,15 @@ def bench_serve(args):
         "config": {"workload": CONFIGS["cfg5"], "precision": args.serve_precision,
                    "operator": args.serve_operator if prec == Precision.FP64 else "assembled",
                    "rate_per_s": args.rate, "measured_s": args.duration, "graphs": args.serve_graph,
-                   "lead_s": args.lead, "requests": len(stream), "measured_requests": len(meas),
-                   "hbm_budget_bytes": 200 * 10 ** 9,
-                   "offered_dof_per_s": round(sum(r.model.n_dofs() for r in stream) / (args.lead + args.duration), 1)},
+                   "lead_s": args.lead, "requests": len(stream), "measured_requests": len(measured),
+                   "hbm_budget_bytes": 200 * 10 ** 9, "offered_dof_per_s": round(offered, 1)},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
         "server": st,
         "max_rel_err_u_vs_oracle_sample": max(e[0] for e in errs) if errs else None,
         "max_rel_err_vm_vs_oracle_sample": max(e[1] for e in errs) if errs else None,
         "oracle_sample": {"requests": len(errs), "dof": tot},
-        "fp64_sequential_sample": {"value": round(plain_dps, 1), "requests": len(plain)},
+        "fp64_sequential_sample": {"value": round(plain_dps, 1), "requests": len(plain_ids)},
         "speedup_vs_fp64": round(dof / span / plain_dps, 3),
     }
     print(json.dumps(line), flush=True)
@@ -763,16 +784,25 @@ def profile_spmv(args):
             keep.append((mesh, sy))
             b.add(sy)
         if args.profile_shape:  # the autotuned shape of a bench run: no tuning launches here
-            u, r = (int(v) for v in args.profile_shape.split(","))
-            tuned = {"unroll": [0, 0, u, 0], "rows_per_tile": [0, 0, r, 0], "spmv_ms": None}
+            u, r, *x = (int(v) for v in args.profile_shape.split(","))
+            tuned = {"unroll": [0, 0, u, 0], "rows_per_tile": [0, 0, r, 0], "tile_order": [1, 1, x[0] if x else 1, 1],
+                     "spmv_ms": None}
         else:
             tuned = b.autotune(prec, 3)
         ctx.set_tuning(unroll=tuned["unroll"][2], rows_per_tile=tuned["rows_per_tile"][2])
+        ctx.set_tile_order("xcd" if tuned["tile_order"][2] else "linear")
         b.prepare(prec)
         ms, by = b.time_spmv(args.spmv_reps)
+        if args.profile_orders:  # the other tile order in the same process, after the first
+            ctx.set_tile_order("linear" if tuned["tile_order"][2] else "xcd")
+            b.prepare(prec)
+            ms2, _ = b.time_spmv(args.spmv_reps)
+            extra_orders = {"other_tile_order_avg_launch_us": round(ms2 * 1e3, 2)}
+        else:
+            extra_orders = {}
         kernel = ("cg64_mf_spmv_kernel<4|3>" if args.operator == "matrix_free" else
                   ("cg64_spmv_kernel<2,%d>" if prec == Precision.FP64 else "ir32_spmv_kernel<2,%d>") % tuned["unroll"][2])
-        extra = {"instances": len(keep), "autotune": tuned}
+        extra = {"instances": len(keep), "autotune": tuned, **extra_orders}
     else:
         model = build_model(args.config)
         mesh, sy = assemble(model, ctx)
@@ -780,8 +810,13 @@ def profile_spmv(args):
         keep.append((mesh, sy))
         sy.prepare_solver(prec)
         ms, by = sy.time_spmv(prec, args.spmv_reps)
+        extra = {}
+        if args.profile_orders:  # the linear tile order in the same process
+            ctx.set_tile_order("linear")
+            sy.prepare_solver(prec)
+            extra["other_tile_order_avg_launch_us"] = round(sy.time_spmv(prec, args.spmv_reps)[0] * 1e3, 2)
         kernel = ("cg64_spmv_kernel<%d,...>" if prec == Precision.FP64 else "ir32_spmv_kernel<%d,...>") % model.ndof()
-        extra = {"n_dof": model.n_dofs()}
+        extra["n_dof"] = model.n_dofs()
     ach = by / (ms * 1e-3) / 1e9
     print(json.dumps({"profile_spmv": args.profile_spmv, "config": args.config, "operator": args.operator,
                       "kernel": kernel, "launches": args.spmv_reps + 1, "avg_launch_us": round(ms * 1e3, 2),
@@ -817,7 +852,7 @@ def main():
                     help="single solves: operator of the FP64 arm (matrix_free: plane meshes)")
     ap.add_argument("--plane-size", default=None, metavar="NXxNY",
                     help="cfg1 only: the cantilever on an NX x NY mesh (profiling at other sizes)")
-    ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block"],
+    ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block", "block_fp32_ir"],
                     help="multi-RHS workload: run one arm only (profiling)")
     ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
     ap.add_argument("--serve-operator", default="matrix_free", choices=["assembled", "matrix_free"],
@@ -831,9 +866,11 @@ def main():
     ap.add_argument("--profile-spmv", default=None, choices=["fp64", "fp32_ir"],
                     help="cfg3 / cfg2: time the solver's SpMV of this precision alone after the solver set-up, "
                          "without solving (profiler runs)")
-    ap.add_argument("--profile-shape", default=None, metavar="UNROLL,ROWS",
-                    help="--profile-spmv on cfg3: the plane launch shape (blocks in flight, rows per tile) instead "
-                         "of autotuning")
+    ap.add_argument("--profile-shape", default=None, metavar="UNROLL,ROWS[,XCD]",
+                    help="--profile-spmv on cfg3: the plane launch shape (blocks in flight, rows per tile, tile "
+                         "order 1 = XCD / 0 = linear) instead of autotuning")
+    ap.add_argument("--profile-orders", action="store_true",
+                    help="--profile-spmv: also time the other tile order in the same process")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (counter calibration) and exit")
     args = ap.parse_args()

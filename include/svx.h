@@ -89,6 +89,15 @@ enum { SVX_FP64 = 0, SVX_FP32_IR = 1 };
  *                     evaluations keep the assembled K. */
 enum { SVX_OP_ASSEMBLED = 0, SVX_OP_MATRIX_FREE = 1 };
 
+/* Order in which the workgroups of the gathering kernels (SpMV, FP64
+ * residual, matrix-free product) take their tiles (svx_set_tile_order).
+ * SVX_TILES_XCD:    each of the eight XCDs takes a contiguous run of tiles, so
+ *                   tiles whose rows gather each other's vectors share an L2
+ *                   (workgroups are dealt to the XCDs round robin);
+ * SVX_TILES_LINEAR: tile = workgroup index.
+ * Either order gives the same results bit for bit. */
+enum { SVX_TILES_LINEAR = 0, SVX_TILES_XCD = 1 };
+
 typedef struct svx_error {
     int32_t code;       /* SVX_* status */
     int32_t reason;     /* SVX_ELEM_* for element errors, else 0 */
@@ -207,6 +216,9 @@ int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile,
  * afterwards): SVX_OP_ASSEMBLED (default) or SVX_OP_MATRIX_FREE. No
  * reference counterpart: the reference multiplies its assembled CSR only. */
 int svx_set_operator(svx_ctx* ctx, int op);
+/* Tile order of this context's subsequent solves (default SVX_TILES_XCD).
+ * No reference counterpart: a launch-shape knob the autotuner also sets. */
+int svx_set_tile_order(svx_ctx* ctx, int order);
 /* SVX_OP_MATRIX_FREE once a matrix-free solve has prepared the system's
  * operator (and until its constraints change), else SVX_OP_ASSEMBLED; -1 for
  * NULL. No reference counterpart. */
@@ -296,17 +308,19 @@ int svx_solve(svx_system* sys, double tol, int64_t max_iter, int precision,
  * reference counterpart: the reference reports iterations and residual
  * only, lib.rs:706-716).
  *
- * Stopping rules. A solve converges only at residual <= tol, by one of two
- * tests (converged_by): the recursive residual of FP64 PCG, the test of
- * solver.rs:129-131, or the true residual ||f - K u|| / ||f|| evaluated in
- * FP64 at an FP32-IR refresh. FP64 cannot evaluate f - K u below its floor
- * sqrt(max row length) * eps64 * || |K||u| + |f| || / ||f||; an FP32-IR
- * instance whose true residual stops improving at that floor while still
- * above tol is finished by FP64 PCG from its current u (fallback_reason 2),
- * which stops on its recursive residual as the reference's PCG does. After
- * every solve the library evaluates the true residual once more in FP64
- * (true_residual, true_floor), whatever stopped it, so both precisions
- * report the same quantity. */
+ * Stopping rules. A solve converges only at residual <= tol, on the
+ * recursive residual of FP64 PCG, the test of solver.rs:129-131
+ * (converged_by SVX_STOP_RECURSIVE), or, for a warm start whose FP64 true
+ * residual ||f - K u|| / ||f|| is already there, on that (SVX_STOP_TRUE).
+ * FP64 cannot evaluate f - K u below its floor sqrt(max row length) * eps64 *
+ * || |K||u| + |f| || / ||f||, and on ill-conditioned systems a true residual
+ * <= tol does not give the accuracy of the reference's test. So an SVX_FP32_IR
+ * solve never stops in its FP32 phase: once its FP64 true residual is within
+ * the floor of tol, or below it (fallback_reason 2), FP64 PCG finishes from
+ * the current u to min(tol, that residual / 10); once it stops improving
+ * above the floor (fallback_reason 1), to tol. After every solve the library
+ * evaluates the true residual once more in FP64 (true_residual, true_floor),
+ * whatever stopped it, so both precisions report the same quantity. */
 typedef struct svx_solve_stats {
     int64_t iterations;      /* all Krylov iterations, FP64 fallback included */
     int64_t refreshes;       /* FP32-IR: FP64 residual replacements */
@@ -386,18 +400,19 @@ int svx_batch_result(const svx_batch* batch, int64_t slot,
                      int32_t* converged);
 /* SpMV autotuner: times one solver iteration (SpMV with its direction step,
  * update, and the FP32-IR period end per iteration) over the whole batch for
- * blocks in flight per lane in {1, 2, 3, 4} and 1 or 4 ELL chunks (64 node
- * rows each) per workgroup tile, one node-block size at a time, and
- * keeps the fastest shape for later svx_batch_solve calls with this precision
- * (until svx_batch_add). unroll[bs] / rows_per_tile[bs] (arrays of 4,
- * indexed by block size 1..3, or NULL; rows_per_tile 0 = each instance's
- * default) receive the choice, spmv_ms the iteration time.
+ * blocks in flight per lane in {1, 2, 3, 4} x 1, 2 or 4 ELL chunks (64 node
+ * rows each) per workgroup tile x both tile orders, one node-block size at
+ * a time, and keeps the fastest shape for later svx_batch_solve calls with
+ * this precision (until svx_batch_add). unroll[bs] / rows_per_tile[bs] /
+ * tile_order[bs] (arrays of 4, indexed by block size 1..3, or NULL;
+ * rows_per_tile 0 = each instance's default) receive the choice, spmv_ms the
+ * iteration time.
  * The unroll does not change results; a tile size other than the default
  * changes the order of the partial sums of the dot products, so results then
  * differ from solo solves at rounding level instead of bit for bit. */
 int svx_batch_autotune(svx_batch* batch, int precision, int reps,
                        int32_t* unroll /* [4] */, int32_t* rows_per_tile /* [4] */,
-                       double* spmv_ms);
+                       int32_t* tile_order /* [4] */, double* spmv_ms);
 /* Solver set-up of svx_batch_solve without the solve (vectors, instance
  * state, tile tables with the tuned or default launch shape and the
  * operator in force), so that svx_batch_time_spmv can time the batch's SpMV
@@ -418,7 +433,11 @@ int svx_batch_time_spmv(svx_batch* batch, int reps, double* avg_ms,
  * the matrix is read once per iteration for all of them. U[k*n ..] receives
  * the solutions; iterations / residuals / converged are per load case;
  * elapsed_ms is device time from the upload of F to the download of U.
- * FP64 only (SVX_FP32_IR returns SVX_E_ARGUMENT).
+ * SVX_FP64: one block PCG, vectors [n][R], the matrix read once per
+ * iteration for all cases. SVX_FP32_IR: every case runs the FP32-IR solve of
+ * svx_solve (FP32 inner iterations, FP64 refinement, residual and final
+ * test) as one instance of a batched engine over shared matrix copies; each
+ * case's result equals its own svx_solve in SVX_FP32_IR bit for bit.
  * ------------------------------------------------------------------------- */
 int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol,
                     int64_t max_iter, int precision, double* U,

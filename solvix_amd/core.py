@@ -62,6 +62,12 @@ class Context:
     def set_tuning(self, lanes=0, unroll=0, rows_per_tile=0, period=0, chunk=0, use_graph=True, ru_delta=0.0):
         _raise(lib().svx_set_tuning(self._h, lanes, unroll, rows_per_tile, period, chunk, int(use_graph), ru_delta))
 
+    def set_tile_order(self, order: str):
+        """'xcd' (default: each XCD takes a contiguous run of tiles) or 'linear'
+        (tile = workgroup) for the gathering kernels of subsequent solves."""
+        code = {"xcd": _abi.SVX_TILES_XCD, "linear": _abi.SVX_TILES_LINEAR}[order]
+        _raise(lib().svx_set_tile_order(self._h, code))
+
     def set_operator(self, op: str):
         """'assembled' (default) or 'matrix_free': the operator of subsequent
         FP64 solves on plane meshes of one element kind and one material."""
@@ -457,10 +463,12 @@ class Batch:
         later solves with this precision use it."""
         unroll = np.zeros(4, np.int32)
         rows = np.zeros(4, np.int32)
+        order = np.zeros(4, np.int32)
         ms = C.c_double()
         _raise(lib().svx_batch_autotune(self._h, precision, reps, unroll.ctypes.data_as(_i32p),
-                                        rows.ctypes.data_as(_i32p), C.byref(ms)))
-        return {"unroll": unroll.tolist(), "rows_per_tile": rows.tolist(), "spmv_ms": ms.value}
+                                        rows.ctypes.data_as(_i32p), order.ctypes.data_as(_i32p), C.byref(ms)))
+        return {"unroll": unroll.tolist(), "rows_per_tile": rows.tolist(), "tile_order": order.tolist(),
+                "spmv_ms": ms.value}
 
     def close(self):
         if self._h:

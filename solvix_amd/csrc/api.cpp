@@ -510,6 +510,7 @@ void system_engine_setup(svx_system* sys, double tol, int64_t max_iter, int prec
     shape.rows_per_tile[bs] = tp.rows_per_tile;  // 0: the instance's default
     shape.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
     shape.matrix_free = tp.matrix_free && precision == SVX_FP64;
+    shape.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;
     sys->engine_mf = shape.matrix_free && sys->mf_ready;
@@ -1034,6 +1035,12 @@ int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile, int p
     if (chunk > 0) t.chunk = chunk;
     t.use_graph = use_graph != 0;
     if (ru_delta > 0) t.ru_delta = ru_delta;
+    return SVX_OK;
+}
+
+int svx_set_tile_order(svx_ctx* ctx, int order) {
+    if (!ctx || (order != SVX_TILES_XCD && order != SVX_TILES_LINEAR)) return SVX_E_ARGUMENT;
+    ctx->tune.tile_order = order;
     return SVX_OK;
 }
 

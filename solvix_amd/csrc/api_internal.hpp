@@ -21,6 +21,7 @@ struct TuneParams {
     bool use_graph = true;
     double ru_delta = 0.1;  // FP32-IR: refresh when ||r|| < ru_delta * ||r_last||
     bool matrix_free = false;  // FP64 iterations on the matrix-free operator where a system has one
+    int tile_order = SVX_TILES_XCD;  // order of the gathering kernels' tiles over the XCDs
 };
 
 // Grow-only device scratch of one context for the temporaries of a call

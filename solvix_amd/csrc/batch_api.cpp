@@ -39,6 +39,7 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
         sh.rows_per_tile[bs] = tp.rows_per_tile;  // 0: each instance's default
         sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : 16;  // CSR rows of the FP64 residual and warm start
         sh.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
+        sh.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
     }
     sh.matrix_free = tp.matrix_free && precision == SVX_FP64;
     return sh;
@@ -130,7 +131,7 @@ std::vector<InstDesc> batch_setup(svx_batch* b, int precision) {
 extern "C" {
 
 int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* unroll, int32_t* rows_per_tile,
-                       double* spmv_ms) {
+                       int32_t* tile_order, double* spmv_ms) {
     if (!b || reps <= 0 || (precision != SVX_FP64 && precision != SVX_FP32_IR)) return SVX_E_ARGUMENT;
     return guarded_b(nullptr, [&]() -> int {
         const size_t n = b->sys.size();
@@ -150,21 +151,24 @@ int svx_batch_autotune(svx_batch* b, int precision, int reps, int32_t* unroll, i
         static const int kChunksPerTile[] = {1, 2, kBlock / 64};
         for (int bs = 2; bs <= 3; ++bs) {
             if (!present[bs]) continue;
-            for (int u = 1; u <= kMaxUnroll; ++u)
-                for (int c : kChunksPerTile) {
-                    EngineShape sh = best;
-                    sh.unroll[bs] = u;
-                    sh.rows_per_tile[bs] = c * kEllRows * bs;
-                    b->engine->set_instances(descs, all, sh);
-                    const double ms = iteration_ms(*b->engine, precision, reps);
-                    if (ms < best_ms) best_ms = ms, best = sh;
-                }
+            for (int x = 0; x <= 1; ++x)
+                for (int u = 1; u <= kMaxUnroll; ++u)
+                    for (int c : kChunksPerTile) {
+                        EngineShape sh = best;
+                        sh.unroll[bs] = u;
+                        sh.rows_per_tile[bs] = c * kEllRows * bs;
+                        sh.xcd[bs] = x;
+                        b->engine->set_instances(descs, all, sh);
+                        const double ms = iteration_ms(*b->engine, precision, reps);
+                        if (ms < best_ms) best_ms = ms, best = sh;
+                    }
         }
         b->tuned_shape = best;
         b->tuned_precision = precision;
         for (int bs = 0; bs < 4; ++bs) {
             if (unroll) unroll[bs] = best.unroll[bs];
             if (rows_per_tile) rows_per_tile[bs] = best.rows_per_tile[bs];
+            if (tile_order) tile_order[bs] = best.xcd[bs] ? SVX_TILES_XCD : SVX_TILES_LINEAR;
         }
         if (spmv_ms) *spmv_ms = best_ms;
         return SVX_OK;

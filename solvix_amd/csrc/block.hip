@@ -25,6 +25,17 @@ constexpr int kWavesPerBlock = kBlock / kWave;  // rows per block pass
 constexpr int kReduceThreads = 1024;
 constexpr int kReduceWaves = kReduceThreads / kWave;
 
+// Tile of an SpMV workgroup: with xcd set, each of the eight XCDs (which the
+// hardware deals workgroups to round robin) takes a contiguous run of tiles,
+// so neighbouring nodes' rows of P are gathered through one L2. Tiles and
+// their partial-sum columns are unchanged.
+__device__ __forceinline__ int spmv_tile(int xcd) {
+    const int n = (int)gridDim.x, b = (int)blockIdx.x;
+    if (!xcd) return b;
+    const int q = n / 8, r = n % 8, x = b % 8;
+    return x * q + (x < r ? x : r) + b / 8;
+}
+
 // Per-load-case partials of this block (waves summed in order) into tile's slot.
 __device__ __forceinline__ void write_rhs_partials(double a, double b, double* partials, int tile, int n_tiles,
                                                    int nv) {
@@ -113,7 +124,7 @@ __global__ __launch_bounds__(kReduceThreads) void blk_init_finish_kernel(BlockAr
 __global__ __launch_bounds__(kBlock) void blk_spmv_kernel(BlockArgs A) {
     if (!A.S->any_running) return;
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int tile = blockIdx.x;
+    const int tile = spmv_tile(A.xcd);
     const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
     const bool on = lane < A.R;
     const int R = A.R;
@@ -200,7 +211,7 @@ template <int BS>
 __global__ __launch_bounds__(kBlock) void blk_spmv_node_kernel(BlockArgs A) {
     if (!A.S->any_running) return;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
-    const int tile = blockIdx.x;
+    const int tile = spmv_tile(A.xcd);
     const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
     const double pq = blk_node_rows<BS>(r0 / BS, r1 / BS, w, lane, A.R, lane < A.R, A.row_ptr, A.brow_ptr, A.bcol,
                                         A.val, A.P, A.Q);

@@ -27,6 +27,7 @@ struct BlockArgs {
     int32_t rows_per_tile;  // a multiple of bs
     int32_t n_tiles;
     int32_t bs;  // node block size (2, 3) for the node-row SpMV; 1 = row form
+    int32_t xcd; // SpMV tiles in XCD-contiguous order (1) or tile = workgroup (0)
     const int32_t* row_ptr;
     const int32_t* brow_ptr;  // [n/bs + 1] (bs > 1)
     const int32_t* bcol;      // node-block columns (bs > 1)

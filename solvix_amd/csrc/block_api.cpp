@@ -8,6 +8,86 @@
 
 using namespace svx;
 
+namespace {
+
+// FP32-IR over the load cases: each case is an instance of the Krylov
+// engine whose descriptor shares the system's matrix copies (CSR, FP64 and
+// FP32 ELL, preconditioner) and owns its vectors, so every case runs the
+// FP32-IR recurrence of a single solve (FP32 inner iterations, FP64
+// residual replacement and final test, FP64 finish from the evaluation
+// floor or a stall) with its own scalars, in one launch per phase.
+int solve_multi_ir32(svx_system* sys, int32_t R, const double* F, double tol, int64_t max_iter, double* U,
+                     int64_t* iterations, double* residuals, int32_t* converged, double* elapsed_ms, svx_error* err) {
+    svx_mesh* m = sys->mesh;
+    svx_ctx* ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t n = m->n;
+    hipEvent_t e0, e1;
+    SVX_HIP(hipEventCreate(&e0));
+    SVX_HIP(hipEventCreate(&e1));
+    system_prepare(sys, SVX_FP32_IR);
+    const size_t nr = (size_t)n * R;
+    DevBuf<double> f(nr), u(nr), r(nr), p(nr), z(nr), q(nr), p_alt(nr);
+    DevBuf<float> x32(nr), r32(nr), p32(nr), z32(nr), q32(nr), p32_alt(nr);
+    DevBuf<InstScal> scal(R);
+    std::vector<double> fh(F, F + nr);
+    for (int32_t k = 0; k < R; ++k)
+        for (int64_t row : sys->cons_rows) fh[(size_t)k * n + row] = 0.0;  // apply_dirichlet (solver.rs:59-61)
+    const int64_t mi = max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * n);
+    std::vector<InstScal> init(R, initial_scal(tol, mi, ctx->tune.ru_delta, m->max_row_len));
+    std::vector<InstDesc> descs(R);
+    const InstDesc base = system_desc(sys);
+    for (int32_t k = 0; k < R; ++k) {
+        const size_t o = (size_t)k * n;
+        InstDesc d = base;
+        d.f = f.p + o, d.u = u.p + o, d.r = r.p + o, d.p = p.p + o, d.z = z.p + o, d.q = q.p + o;
+        d.p_alt = p_alt.p + o;
+        d.x32 = x32.p + o, d.r32 = r32.p + o, d.p32 = p32.p + o, d.z32 = z32.p + o, d.q32 = q32.p + o;
+        d.p32_alt = p32_alt.p + o;
+        d.s = scal.p + k;
+        d.mf_kind = 0;  // the FP64 finish multiplies the assembled K
+        descs[k] = d;
+    }
+    SVX_HIP(hipEventRecord(e0, s));
+    f.upload(fh.data(), nr, s);
+    scal.upload(init.data(), R, s);
+    Engine eng(s);
+    eng.set_instances(descs, std::vector<char>(R, 1), batch_shape({sys}, ctx->tune, SVX_FP32_IR));
+    EngineOptions opt;
+    opt.precision = SVX_FP32_IR;
+    opt.period = ctx->tune.period;
+    opt.chunk = ctx->tune.chunk;
+    opt.use_graph = ctx->tune.use_graph;
+    eng.run(opt, ctx->h_flag, ctx->d_flag);
+    if (U) u.download(U, nr, s);
+    SVX_HIP(hipEventRecord(e1, s));
+    std::vector<InstScal> out(R);
+    scal.download(out.data(), R, s);
+    SVX_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    SVX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    int failed = 0;
+    for (int32_t k = 0; k < R; ++k) {
+        if (iterations) iterations[k] = out[k].it;
+        if (residuals) residuals[k] = out[k].res;
+        if (converged) converged[k] = out[k].status == ST_CONVERGED;
+        failed += out[k].status != ST_CONVERGED;
+    }
+    if (failed) {
+        if (err) {
+            err->code = SVX_E_NOT_CONVERGED;
+            snprintf(err->message, sizeof err->message, "%d of %d load cases did not converge", failed, R);
+        }
+        return SVX_E_NOT_CONVERGED;
+    }
+    return SVX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, int64_t max_iter, int precision,
@@ -18,12 +98,17 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
         err->index = -1;
     }
     if (!sys || nrhs <= 0 || nrhs > kMaxRhs || !F || !(tol > 0.0) || !sys->dirichlet) return SVX_E_ARGUMENT;
-    if (precision != SVX_FP64) {
-        if (err) {
-            err->code = SVX_E_ARGUMENT;
-            snprintf(err->message, sizeof err->message, "multi-RHS solves run in FP64");
+    if (precision != SVX_FP64 && precision != SVX_FP32_IR) return SVX_E_ARGUMENT;
+    if (precision == SVX_FP32_IR) {
+        try {
+            return solve_multi_ir32(sys, nrhs, F, tol, max_iter, U, iterations, residuals, converged, elapsed_ms, err);
+        } catch (const std::exception& e) {
+            if (err) {
+                err->code = SVX_E_DEVICE;
+                snprintf(err->message, sizeof err->message, "%s", e.what());
+            }
+            return SVX_E_DEVICE;
         }
-        return SVX_E_ARGUMENT;
     }
     try {
         svx_mesh* m = sys->mesh;
@@ -47,6 +132,7 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
         a.n = n;
         a.R = R;
         a.bs = m->ndof > 1 ? m->ndof : 1;
+        a.xcd = ctx->tune.tile_order == SVX_TILES_XCD;
         a.rows_per_tile = a.bs > 1 ? 96 : 64;  // 96 rows = 32 / 48 nodes, 8 / 12 per wavefront
         a.n_tiles = (n + a.rows_per_tile - 1) / a.rows_per_tile;
         a.row_ptr = m->row_ptr.p;

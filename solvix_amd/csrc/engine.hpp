@@ -28,6 +28,7 @@ struct EngineShape {
     int lanes[4] = {0, 0, 0, 0};   // lanes per row (raw CSR, block size 1)
     int unroll[4] = {0, 0, 0, 0};  // blocks in flight per lane (ELL, block sizes 2, 3)
     int rows_per_tile[4] = {0, 0, 0, 0};  // 0: per-instance default; block sizes 2, 3: whole 64-node-row chunks
+    int xcd[4] = {1, 1, 1, 1};  // gathering kernels: XCD-contiguous tile order (1) or tile = workgroup (0)
     // FP64 SpMV of instances that carry a matrix-free operator (InstDesc::mf_kind)
     // on that operator instead of the assembled matrix.
     bool matrix_free = false;

@@ -27,14 +27,19 @@
 // the last replacement (or below tol, or that has not been checked for
 // kForcedRefresh iterations) folds its FP32 correction into the FP64 u,
 // recomputes r = f - K u in FP64, and continues with p = z + beta p built from
-// the replaced residual. The solve ends when the FP64 true residual
-// ||f - K u|| / ||f|| reaches tol. FP64 cannot evaluate f - K u below the
-// floor floor_scale || |K||u| + |f| || / ||f||: an instance whose true
-// residual is within that floor of tol (within kFloorSlack floors once two
-// replacements in a row failed to halve it) while still above tol, or whose
-// residual stops improving above it (four replacements without halving),
-// hands over to FP64 PCG from its current u, which stops on its recursive
-// residual <= tol as the reference's pcg does (solver.rs:129-131).
+// the replaced residual. FP64 cannot evaluate f - K u below the floor
+// floor_scale || |K||u| + |f| || / ||f||, and on these ill-conditioned
+// systems the norm of the true residual does not bound the error the way the
+// reference's test does: FP64 PCG stops on its recursive residual <= tol
+// (solver.rs:129-131) with its true residual at the floor and its smooth
+// error components well below it. So the FP32 phase ends when its FP64 true
+// residual is within the floor of tol (within kFloorSlack floors once two
+// replacements in a row failed to halve it), or below tol, and FP64 PCG
+// finishes from the current u with the reference's recursive test, to
+// min(tol, residual at the hand-over / 10): at least a decade of FP64 CG
+// recursion, which reduces the error as the reference's last iterations do.
+// An instance whose residual stops improving above that level (four
+// replacements without halving) hands over the same way, to tol.
 #include <algorithm>
 
 #include "mf_element.hpp"
@@ -49,6 +54,24 @@ constexpr int32_t FL_HOLD = 2;          // FP32-IR: budget reached, awaiting the
 constexpr int64_t kForcedRefresh = 256; // FP32-IR: check the FP64 residual at least this often
 constexpr double kFloorSlack = 4.0;     // FP32-IR: floors accepted once the residual stagnates
 constexpr int kStallLimit = 4;          // FP32-IR: refreshes without halving the best residual before FP64 takes over
+
+// Tile of this workgroup in the gathering kernels (SpMV, FP64 residual, warm
+// start, matrix-free product). The hardware deals workgroups out to the
+// eight XCDs round robin (workgroup b runs on XCD b % 8), and each XCD has
+// its own L2. Mapping workgroup b to tile (b % 8) * (n / 8) + b / 8 (the
+// first n % 8 XCDs take one tile more) gives each XCD a contiguous run of
+// tiles, so tiles whose rows gather each other's z and p (neighbouring
+// chunks of one mesh, neighbouring node patches) share an L2. Tiles, their
+// partial-sum columns and hence every sum are unchanged: only which XCD runs
+// a tile moves.
+// xcd = 0 keeps tile = workgroup (the launch shape's choice).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int xcd) {
+    const int n = (int)gridDim.x, b = (int)blockIdx.x;
+    if (!xcd) return b;
+    const int q = n / kXcds, r = n % kXcds, x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
 
 // Instance arrays typed as global memory. The pointers come out of the
 // descriptor table, so the compiler cannot infer their address space: as
@@ -426,8 +449,8 @@ __global__ __launch_bounds__(kBlock) void cg64_init_kernel(const Tile* __restric
 template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __restrict__ tiles,
                                                                 const InstDesc* __restrict__ descs,
-                                                                double* __restrict__ partials, int pstride) {
-    const Tile t = tiles[blockIdx.x];
+                                                                double* __restrict__ partials, int pstride, int xcd) {
+    const Tile t = tiles[xcd_tile(xcd)];
     const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || (S->flags & (FL_FRESH | FL_WARM)) != (FL_FRESH | FL_WARM)) return;
@@ -467,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void cg64_init_warm_kernel(const Tile* __re
                                        });
     }
     const int slot[2] = {P_IFF, P_IRZ};
-    write_partials<2>(v, slot, partials, pstride, blockIdx.x);
+    write_partials<2>(v, slot, partials, pstride, xcd_tile(xcd));
 }
 
 // Cold start (u = 0), FP32-IR vectors.
@@ -609,8 +632,8 @@ template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride, int par,
-                                                           int force_dir) {
-    const Tile t = tiles[blockIdx.x];
+                                                           int force_dir, int xcd) {
+    const Tile t = tiles[xcd_tile(xcd)];
     const gptr<InstScal> S = global(t.s);
     const int32_t status = S->status, flags = S->flags, dir = S->dir;
     const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
@@ -651,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
         v[0] = tile_spmv<BS, W, double>(t, D, D.ell_val, D.val, Step<double>{D.z, cur, beta}, nxt, D.q);
     }
     const int slot[1] = {P_PQ};
-    write_partials<1>(v, slot, partials, pstride, blockIdx.x);
+    write_partials<1>(v, slot, partials, pstride, xcd_tile(xcd));
 }
 
 // ------------------------------------------------ matrix-free plane stress
@@ -836,14 +859,14 @@ __global__ __launch_bounds__(kBlock) void cg64_mf_step_kernel(const Tile* __rest
 // Direction step and q = K p of one FP64 iteration on the matrix-free
 // operator (cg64_spmv_kernel with node patches for row tiles), after
 // cg64_mf_step_kernel has tested convergence and set beta. Tile t is
-// mf_tiles[base + blockIdx.x]; its p.q partial goes to slot P_MPQ, column
-// base + blockIdx.x.
+// mf_tiles[base + xcd_tile(xcd)]; its p.q partial goes to slot P_MPQ, column
+// base + xcd_tile(xcd).
 template <int NEN>
 __global__ __launch_bounds__(kBlock) void cg64_mf_spmv_kernel(const Tile* __restrict__ mf_tiles, int base,
                                                               const InstDesc* __restrict__ descs,
                                                               double* __restrict__ partials, int pstride, int par,
-                                                              int force_dir, int cap_nodes, int cap_elem) {
-    const Tile t = mf_tiles[base + blockIdx.x];
+                                                              int force_dir, int cap_nodes, int cap_elem, int xcd) {
+    const Tile t = mf_tiles[base + xcd_tile(xcd)];
     // the instance's state, its descriptor and the patch record are read
     // together, before the status test, so that their latencies overlap
     const gptr<InstScal> S = global(t.s);
@@ -872,7 +895,7 @@ __global__ __launch_bounds__(kBlock) void cg64_mf_spmv_kernel(const Tile* __rest
         v[0] = mf_patch_spmv<NEN>(M, P, Step<double>{D.z, cur, beta}, nxt, D.q, cap_nodes, cap_elem);
     }
     const int slot[1] = {P_MPQ};
-    write_partials<1>(v, slot, partials, pstride, base + blockIdx.x);
+    write_partials<1>(v, slot, partials, pstride, base + xcd_tile(xcd));
 }
 
 // ------------------------------------------------ matrix-free C3D8
@@ -1144,8 +1167,8 @@ template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride, int par,
-                                                           int force_dir) {
-    const Tile t = tiles[blockIdx.x];
+                                                           int force_dir, int xcd) {
+    const Tile t = tiles[xcd_tile(xcd)];
     const gptr<InstScal> S = global(t.s);
     const int32_t status = S->status, flags = S->flags;
     const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
@@ -1185,7 +1208,7 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
         v[0] = tile_spmv<BS, W, float>(t, D, D.ell_val32, D.bval, Step<float>{D.z32, cur, beta}, nxt, D.q32);
     }
     const int slot[1] = {P_PQ};
-    write_partials<1>(v, slot, partials, pstride, blockIdx.x);
+    write_partials<1>(v, slot, partials, pstride, xcd_tile(xcd));
 }
 
 __device__ __forceinline__ void ir32_update_rows(int32_t b, int32_t e, float alpha, gptr<const float> __restrict__ p,
@@ -1411,8 +1434,8 @@ __device__ __forceinline__ void ell_residual_rows(int32_t c0, int32_t c1, const 
 template <int BS, int W>
 __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __restrict__ tiles,
                                                                const InstDesc* __restrict__ descs,
-                                                               double* __restrict__ partials, int pstride) {
-    const Tile t = tiles[blockIdx.x];
+                                                               double* __restrict__ partials, int pstride, int xcd) {
+    const Tile t = tiles[xcd_tile(xcd)];
     const Dev D = view(descs[t.inst]);
     const gptr<InstScal> S = global(t.s);
     if (S->status != ST_RUNNING || !(S->flags & FL_REFRESH)) return;
@@ -1426,7 +1449,7 @@ __global__ __launch_bounds__(kBlock) void ir32_residual_kernel(const Tile* __res
         ell_residual_rows<BS, W>(c0, c1, D, v);
     }
     const int slot[3] = {P_TRR, P_TRZ, P_TFL};
-    write_partials<3>(v, slot, partials, pstride, blockIdx.x);
+    write_partials<3>(v, slot, partials, pstride, xcd_tile(xcd));
 }
 
 // Refresh step 3: convergence / budget / fallback decision, then
@@ -1445,16 +1468,14 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
     const double floor = S->floor_scale * sqrt(v[2]) / S->bnorm;
     const bool improved = res < 0.5 * S->snap_res_best;
     const int32_t stall = improved ? 0 : S->snap_stall + 1;
-    // Converged only at tol. Within the FP64 evaluation floor of tol (within
-    // kFloorSlack floors once the residual has stopped halving) the FP64
-    // product K u cannot resolve f - K u any further, so the FP32 phase ends
-    // there and FP64 PCG finishes on its recursive residual; so does an
-    // instance whose residual stopped improving above the floor.
-    const bool conv = res <= S->tol;
-    const bool at_floor = !conv && (res <= S->tol + floor || (stall >= 2 && res <= S->tol + kFloorSlack * floor));
+    // The FP32 phase never ends the solve: at or below tol plus the FP64
+    // evaluation floor (kFloorSlack floors once the residual has stopped
+    // halving), or once the residual stopped improving above it, FP64 PCG
+    // finishes on its recursive residual (the reference's test).
+    const bool at_floor = res <= S->tol + floor || (stall >= 2 && res <= S->tol + kFloorSlack * floor);
     const bool hold = (S->snap_flags & FL_HOLD) != 0;
-    const bool fallback = !conv && !hold && (at_floor || stall >= kStallLimit);
-    const bool cont = !conv && !hold && !fallback;
+    const bool fallback = !hold && (at_floor || stall >= kStallLimit);
+    const bool cont = !hold && !fallback;
     const double beta = v[1] / S->snap_rz;
     if (is_leader(t)) {
         S->res = res;
@@ -1464,14 +1485,12 @@ __global__ __launch_bounds__(kBlock) void ir32_repoint_kernel(const Tile* __rest
         S->it_refresh = S->it;
         S->res_best = improved ? res : S->snap_res_best;
         S->stall = stall;
-        if (conv) {
-            S->status = ST_CONVERGED;
-            S->stop_rule = SVX_STOP_TRUE;
-        } else if (hold) {
+        if (hold) {
             S->status = ST_MAXITER;
         } else if (fallback) {
-            S->status = ST_FALLBACK;  // FP32 cannot resolve this system further
+            S->status = ST_FALLBACK;  // FP64 PCG finishes from the current u
             S->fb_reason = at_floor ? 2 : 1;
+            if (at_floor) S->tol = fmin(S->tol, 0.1 * res);  // at least a decade of FP64 recursion
         } else {
             S->rmax = sqrt(v[0]);
             S->rz = v[1];
@@ -1560,15 +1579,15 @@ void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir)
     const int g = k.n_ell_tiles;
     if (g > 0) {
         if (k.bs == 3) {
-#define L(U) cg64_spmv_kernel<3, U><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+#define L(U) cg64_spmv_kernel<3, U><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
             SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
         } else if (k.bs == 2) {
-#define L(U) cg64_spmv_kernel<2, U><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+#define L(U) cg64_spmv_kernel<2, U><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
             SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
         } else {
-#define L(LN) cg64_spmv_kernel<1, LN><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+#define L(LN) cg64_spmv_kernel<1, LN><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
             SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
         }
@@ -1582,26 +1601,26 @@ void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir)
     }
     if (k.n_mf4 > 0)
         cg64_mf_spmv_kernel<4><<<k.n_mf4, kBlock, mf_lds_bytes(4, k.mf_cap_nodes[0], k.mf_cap_elem[0]), s>>>(
-            k.mf_tiles, 0, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[0], k.mf_cap_elem[0]);
+            k.mf_tiles, 0, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[0], k.mf_cap_elem[0], k.xcd);
     if (k.n_mf3 > 0)
         cg64_mf_spmv_kernel<3><<<k.n_mf3, kBlock, mf_lds_bytes(3, k.mf_cap_nodes[1], k.mf_cap_elem[1]), s>>>(
-            k.mf_tiles, k.n_mf4, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[1], k.mf_cap_elem[1]);
+            k.mf_tiles, k.n_mf4, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[1], k.mf_cap_elem[1], k.xcd);
 }
 
 void launch_ir32_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {
     if (k.bs == 3) {
 #define L(U) \
-    ir32_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+    ir32_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
 #define L(U) \
-    ir32_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+    ir32_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
 #define L(LN) \
-    ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir)
+    ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }
@@ -1709,15 +1728,15 @@ void cg64_init(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     cg64_init_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride);
     if (k.bs == 3) {
-#define L(U) cg64_init_warm_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
+#define L(U) cg64_init_warm_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, k.xcd)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(U) cg64_init_warm_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
+#define L(U) cg64_init_warm_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, k.xcd)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
-#define L(LN) cg64_init_warm_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
+#define L(LN) cg64_init_warm_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, k.xcd)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }
@@ -1762,15 +1781,15 @@ void ir32_refresh(const KrylovSet& k, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     ir32_fold_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs);
     if (k.bs == 3) {
-#define L(U) ir32_residual_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
+#define L(U) ir32_residual_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, k.xcd)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else if (k.bs == 2) {
-#define L(U) ir32_residual_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
+#define L(U) ir32_residual_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, k.xcd)
         SVX_UNROLL_DISPATCH(k.unroll, L)
 #undef L
     } else {
-#define L(LN) ir32_residual_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride)
+#define L(LN) ir32_residual_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, k.xcd)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }

@@ -49,6 +49,12 @@ void put_err(svx_error* err, int code, const std::string& msg, int reason = 0, i
 }
 
 
+// Prepared requests waiting for admission at most: a prepared request holds
+// its CSR pattern and incidence lists on the host (tens of MB for a
+// 1e5-DOF C3D8 block), so under overload the host workers stop preparing
+// once this many wait, and the backlog stays as the submitted decks.
+constexpr size_t kReadyCap = 128;
+
 // Queued: in prep_q. Preparing: a host worker holds it (nobody else may
 // touch it until the worker publishes it under the lock). Prepared: in
 // ready_q. Active: admitted by the device thread. Done: a waiter may take it.
@@ -105,6 +111,7 @@ struct svx_server {
     int64_t next_ticket = 1;
     bool stop = false;
     int waiters = 0;  // threads inside svx_server_wait
+    size_t preparing = 0;  // requests the host workers hold
     std::vector<std::thread> prep_threads;
     std::thread dev_thread;
     // statistics (guarded by mu)
@@ -150,11 +157,12 @@ void svx_server::prep_loop() {
         Request* r = nullptr;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_prep.wait(lk, [&] { return stop || !prep_q.empty(); });
+            cv_prep.wait(lk, [&] { return stop || (!prep_q.empty() && ready_q.size() + preparing < kReadyCap); });
             if (stop) return;
             r = prep_q.front();
             prep_q.pop_front();
             r->state = St::Preparing;
+            preparing++;
         }
         svx_error err{};
         err.index = -1;
@@ -194,6 +202,7 @@ void svx_server::prep_loop() {
             put_err(&err, rc, e.what());
         }
         std::lock_guard<std::mutex> lk(mu);
+        preparing--;
         r->t_prepared = wall_ms();
         if (stop) {  // the server failed or is being destroyed: nothing will admit it
             r->rc = SVX_E_DEVICE;
@@ -420,6 +429,7 @@ void svx_server::dev_loop() {
                     const int64_t n_active = (int64_t)(in32.size() + in64.size() + take.size());
                     if (n_active >= cfg.max_active || resident + r->bytes > cfg.hbm_budget) break;
                     ready_q.pop_front();
+                    cv_prep.notify_one();  // room for another prepared request
                     r->state = St::Active;
                     resident += r->bytes;
                     peak_resident = std::max(peak_resident, resident);

@@ -291,6 +291,7 @@ struct KrylovSet {
     int lanes = 16;              // lanes per row (raw CSR instances, bs = 1)
     int unroll = 2;              // blocks in flight per lane (ELL instances, bs = 2, 3)
     int bs = 0;                  // node-block size shared by the set's instances (1, 2, 3)
+    int xcd = 1;                 // gathering kernels: 1 = XCD-contiguous tile order, 0 = tile = workgroup
     // FP64 SpMV: row tiles [0, n_ell_tiles) multiply the assembled matrix;
     // instances with matrix-free tiles (mf_tiles, CPS4 first, then CPS3) own
     // the row tiles after them. FP32 SpMV launches cover every row tile.

@@ -81,6 +81,7 @@ def test_autotuned_batch_matches_oracle(precision, ctx):
     for bs in (2, 3):
         assert tuned["unroll"][bs] in (1, 2, 3, 4)
         assert tuned["rows_per_tile"][bs] % (64 * bs) == 0  # whole ELL chunks
+        assert tuned["tile_order"][bs] in (0, 1)
     res = b.solve(precision)
     for (mesh, s), m, (its, r, ok) in zip(keep, models, res):
         ref = oracle.solve(m)
@@ -118,3 +119,34 @@ def test_many_tiles_equal_solo(precision, rows_per_tile, ctx):
         print(f"{m.name} [{precision}] rows/tile {rows_per_tile}: its {its} (solo {its0}) res {res:.2e}")
         assert ok and its == its0 and res == res0, m.name
         assert np.array_equal(s.solution().view(np.uint64), u0.view(np.uint64)), m.name
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+def test_tile_order_changes_no_bits(precision):
+    """XCD-contiguous and linear tile orders run the same tiles with the same
+    partial-sum columns: single solves, a batch and the matrix-free operator
+    give identical bits either way."""
+    from solvix_amd.core import Context
+    models = small_models() + [decks.plane_instance(decks.SplitMix64(0x77 ^ i), 2e4, 6e4) for i in range(3)]
+    results = {}
+    for order in ("xcd", "linear"):
+        ctx = Context(0)
+        ctx.set_tile_order(order)
+        if precision == Precision.FP64:
+            ctx.set_operator("matrix_free")
+        try:
+            solo = []
+            for m in models:
+                mesh, s = _system(m, ctx)
+                solo.append(s.solve(1e-10, 0, precision)[0])
+            keep, b = [], Batch(ctx)
+            for m in models:
+                mesh, s = _system(m, ctx)
+                keep.append((mesh, s))
+                b.add(s)
+            b.solve(precision)
+            results[order] = (solo, [s.solution() for _, s in keep])
+        finally:
+            ctx.close()
+    for a, c in zip(results["xcd"][0] + results["xcd"][1], results["linear"][0] + results["linear"][1]):
+        assert np.array_equal(a.view(np.uint64), c.view(np.uint64))
