@@ -98,6 +98,17 @@ enum { SVX_OP_ASSEMBLED = 0, SVX_OP_MATRIX_FREE = 1 };
  * Either order gives the same results bit for bit. */
 enum { SVX_TILES_LINEAR = 0, SVX_TILES_XCD = 1 };
 
+/* How the FP64 assembled SpMV loads the matrix values (svx_set_load_hint).
+ * SVX_LOADS_STREAMING: nontemporal loads; the values, read once per product,
+ *                      do not evict the gathered vectors from the caches;
+ * SVX_LOADS_CACHED:    ordinary loads, so a matrix that fits the 256 MB
+ *                      last-level cache is read from there on the next product;
+ * SVX_LOADS_AUTO:      streaming when the FP64 matrix values one launch reads
+ *                      (all its instances of one node-block size) exceed
+ *                      256 MiB, else cached.
+ * Every hint gives the same results bit for bit. */
+enum { SVX_LOADS_AUTO = 0, SVX_LOADS_CACHED = 1, SVX_LOADS_STREAMING = 2 };
+
 typedef struct svx_error {
     int32_t code;       /* SVX_* status */
     int32_t reason;     /* SVX_ELEM_* for element errors, else 0 */
@@ -219,6 +230,9 @@ int svx_set_operator(svx_ctx* ctx, int op);
 /* Tile order of this context's subsequent solves (default SVX_TILES_XCD).
  * No reference counterpart: a launch-shape knob the autotuner also sets. */
 int svx_set_tile_order(svx_ctx* ctx, int order);
+/* Load hint of this context's subsequent FP64 solves (default
+ * SVX_LOADS_AUTO). No reference counterpart: a launch knob. */
+int svx_set_load_hint(svx_ctx* ctx, int hint);
 /* SVX_OP_MATRIX_FREE once a matrix-free solve has prepared the system's
  * operator (and until its constraints change), else SVX_OP_ASSEMBLED; -1 for
  * NULL. No reference counterpart. */

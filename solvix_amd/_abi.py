@@ -28,6 +28,7 @@ SVX_FP64, SVX_FP32_IR = 0, 1
 SVX_OP_ASSEMBLED, SVX_OP_MATRIX_FREE = 0, 1
 SVX_STOP_NONE, SVX_STOP_RECURSIVE, SVX_STOP_TRUE = 0, 1, 2
 SVX_TILES_LINEAR, SVX_TILES_XCD = 0, 1
+SVX_LOADS_AUTO, SVX_LOADS_CACHED, SVX_LOADS_STREAMING = 0, 1, 2
 
 _p = C.POINTER
 

@@ -68,6 +68,14 @@ class Context:
         code = {"xcd": _abi.SVX_TILES_XCD, "linear": _abi.SVX_TILES_LINEAR}[order]
         _raise(lib().svx_set_tile_order(self._h, code))
 
+    def set_load_hint(self, hint: str):
+        """'auto' (default: nontemporal matrix loads once a launch's FP64 matrix
+        values exceed 256 MiB), 'cached' or 'streaming' for the FP64 assembled
+        SpMV of subsequent solves; results are the same bit for bit."""
+        code = {"auto": _abi.SVX_LOADS_AUTO, "cached": _abi.SVX_LOADS_CACHED,
+                "streaming": _abi.SVX_LOADS_STREAMING}[hint]
+        _raise(lib().svx_set_load_hint(self._h, code))
+
     def set_operator(self, op: str):
         """'assembled' (default) or 'matrix_free': the operator of subsequent
         FP64 solves on plane meshes of one element kind and one material."""

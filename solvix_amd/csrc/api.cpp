@@ -511,6 +511,7 @@ void system_engine_setup(svx_system* sys, double tol, int64_t max_iter, int prec
     shape.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
     shape.matrix_free = tp.matrix_free && precision == SVX_FP64;
     shape.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
+    shape.nt[bs] = streaming_loads({sys}, bs, tp);
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;
     sys->engine_mf = shape.matrix_free && sys->mf_ready;
@@ -1041,6 +1042,13 @@ int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile, int p
 int svx_set_tile_order(svx_ctx* ctx, int order) {
     if (!ctx || (order != SVX_TILES_XCD && order != SVX_TILES_LINEAR)) return SVX_E_ARGUMENT;
     ctx->tune.tile_order = order;
+    return SVX_OK;
+}
+
+int svx_set_load_hint(svx_ctx* ctx, int hint) {
+    if (!ctx || (hint != SVX_LOADS_AUTO && hint != SVX_LOADS_CACHED && hint != SVX_LOADS_STREAMING))
+        return SVX_E_ARGUMENT;
+    ctx->tune.load_hint = hint;
     return SVX_OK;
 }
 

@@ -22,6 +22,7 @@ struct TuneParams {
     double ru_delta = 0.1;  // FP32-IR: refresh when ||r|| < ru_delta * ||r_last||
     bool matrix_free = false;  // FP64 iterations on the matrix-free operator where a system has one
     int tile_order = SVX_TILES_XCD;  // order of the gathering kernels' tiles over the XCDs
+    int load_hint = SVX_LOADS_AUTO;  // FP64 ELL SpMV matrix loads
 };
 
 // Grow-only device scratch of one context for the temporaries of a call
@@ -145,6 +146,11 @@ int nodes_of(uint8_t kind);
 const char* reason_text(int reason);
 // The per-block-size launch shape every batched solve uses by default.
 EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp, int precision);
+// The load hint (EngineShape::nt) of the FP64 ELL SpMV over `systems`'
+// instances of node-block size bs: tp.load_hint, or for SVX_LOADS_AUTO
+// streaming once their FP64 ELL values exceed kStreamBytes.
+constexpr int64_t kStreamBytes = int64_t(256) << 20;
+int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp);
 
 // A model validated and resolved on the host (lib.rs:593-632 up to the
 // element pass): node positions, per-element material constants, the load

@@ -32,14 +32,23 @@ namespace svx {
 
 // Launch shape for a set of systems: the per-block-size shape every solve
 // uses, so batched results equal solo ones bit for bit.
+int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp) {
+    if (tp.load_hint != SVX_LOADS_AUTO) return tp.load_hint == SVX_LOADS_STREAMING;
+    int64_t bytes = 0;
+    for (const svx_system* s : systems)
+        if (s->mesh->ndof == bs && bs > 1 && !(tp.matrix_free && s->mf_ready))
+            bytes += s->mesh->ell_blocks * bs * bs * (int64_t)sizeof(double);
+    return bytes > kStreamBytes;
+}
+
 EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp, int precision) {
-    (void)systems;
     EngineShape sh;
     for (int bs = 1; bs <= 3; ++bs) {
         sh.rows_per_tile[bs] = tp.rows_per_tile;  // 0: each instance's default
         sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : 16;  // CSR rows of the FP64 residual and warm start
         sh.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
         sh.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
+        sh.nt[bs] = streaming_loads(systems, bs, tp);
     }
     sh.matrix_free = tp.matrix_free && precision == SVX_FP64;
     return sh;

@@ -220,6 +220,7 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         k.unroll = shape_.unroll[bs] > 0 ? shape_.unroll[bs] : 2;
         k.bs = bs;
         k.xcd = shape_.xcd[bs];
+        k.nt = shape_.nt[bs];
         k.mf_tiles = bs == 2 ? mf_tiles_.p : nullptr;
         k.mf_leaders = bs == 2 ? mf_leaders_.p : bs == 3 ? mf_leaders_.p + n_lead2 : nullptr;
         k.n_mf_inst = bs == 2 ? n_lead2 : bs == 3 ? (int32_t)mf_leaders.size() - n_lead2 : 0;

@@ -29,6 +29,7 @@ struct EngineShape {
     int unroll[4] = {0, 0, 0, 0};  // blocks in flight per lane (ELL, block sizes 2, 3)
     int rows_per_tile[4] = {0, 0, 0, 0};  // 0: per-instance default; block sizes 2, 3: whole 64-node-row chunks
     int xcd[4] = {1, 1, 1, 1};  // gathering kernels: XCD-contiguous tile order (1) or tile = workgroup (0)
+    int nt[4] = {0, 0, 0, 0};   // FP64 ELL SpMV: nontemporal matrix loads (1) or cached (0)
     // FP64 SpMV of instances that carry a matrix-free operator (InstDesc::mf_kind)
     // on that operator instead of the assembled matrix.
     bool matrix_free = false;

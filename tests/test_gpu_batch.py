@@ -122,6 +122,38 @@ def test_many_tiles_equal_solo(precision, rows_per_tile, ctx):
 
 
 @pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+def test_load_hint_changes_no_bits(precision):
+    """Nontemporal and ordinary matrix loads read the same values in the same
+    order: single solves and a batch of 2D and 3D systems on the assembled
+    operator give identical bits and iteration counts under either hint."""
+    from solvix_amd.core import Context
+    models = small_models() + [decks.plane_instance(decks.SplitMix64(0x78 ^ i), 2e4, 6e4) for i in range(2)]
+    models.append(decks.cfg2(nx=40, ny=6, nz=6))
+    results = {}
+    for hint in ("cached", "streaming"):
+        ctx = Context(0)
+        ctx.set_load_hint(hint)
+        try:
+            solo = []
+            for m in models:
+                mesh, s = _system(m, ctx)
+                solo.append(s.solve(1e-10, 0, precision))
+            keep, b = [], Batch(ctx)
+            for m in models:
+                mesh, s = _system(m, ctx)
+                keep.append((mesh, s))
+                b.add(s)
+            its = [r[0] for r in b.solve(precision)]
+            results[hint] = ([u for u, _, _ in solo], [k for _, k, _ in solo], [s.solution() for _, s in keep], its)
+        finally:
+            ctx.close()
+    a, c = results["cached"], results["streaming"]
+    assert a[1] == c[1] and a[3] == c[3]
+    for x, y in zip(a[0] + a[2], c[0] + c[2]):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
 def test_tile_order_changes_no_bits(precision):
     """XCD-contiguous and linear tile orders run the same tiles with the same
     partial-sum columns: single solves, a batch and the matrix-free operator
