@@ -43,7 +43,7 @@ METRIC = "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 
 # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters of rocprofv3,
 # each scaled by what the counter reads for a 1 GiB device copy (profiles/);
 # None where no counter run exists.
-TRAFFIC_BYTES = {("cfg2", "fp64"): 207.0e6, ("cfg2", "fp32_ir"): 108.2e6, ("cfg3", "fp32_ir_batched"): 2758.5e6}
+TRAFFIC_BYTES = {("cfg2", "fp64"): 184.9e6, ("cfg2", "fp32_ir"): 108.2e6, ("cfg3", "fp32_ir_batched"): 2758.5e6}
 CONFIGS = {
     "cfg2": "C3D8 200x20x20 block 20x2x2 m (265,923 DOF), jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20",
     "cfg1": "CPS4 400x40 cantilever 10x1 m (32,882 DOF), jitter 0.15h, -1e4 N tip load",
@@ -420,11 +420,18 @@ def bench_batch(args):
 
 
 # ---------------------------------------------------------------- single
-def build_model(name, plane_size=None):
+def build_model(name, size=None):
+    """cfg2 / cfg1, or the same block / cantilever at another mesh size
+    (`size` = (nx, ny, nz) or (nx, ny), the element size kept)."""
     from solvix_amd import decks
     if name == "cfg2":
-        return decks.cfg2()
-    return decks.cfg1(nx=plane_size[0], ny=plane_size[1]) if plane_size else decks.cfg1()
+        return decks.cfg2(nx=size[0], ny=size[1], nz=size[2]) if size else decks.cfg2()
+    return decks.cfg1(nx=size[0], ny=size[1]) if size else decks.cfg1()
+
+
+def custom_size(args, config):
+    spec = args.solid_size if config == "cfg2" else args.plane_size
+    return tuple(int(v) for v in spec.split("x")) if spec else None
 
 
 def bench_single(args, config, emit=True):
@@ -436,7 +443,7 @@ def bench_single(args, config, emit=True):
     from solvix_amd.core import Context, Mesh, constrained_dofs, element_materials
     from solvix_amd.model import Precision
 
-    size = tuple(int(v) for v in args.plane_size.split("x")) if args.plane_size else None
+    size = custom_size(args, config)
     model = build_model(config, size)
     n_dofs = model.n_dofs()
     ctx = Context(0)
@@ -484,7 +491,7 @@ def bench_single(args, config, emit=True):
         ach = a["spmv_bytes"] / (a["spmv_ms"] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": None if (args.plane_size or (label == "fp64" and mf64)) else TRAFFIC_BYTES.get((config, label)),
+                "traffic": None if (size or (label == "fp64" and mf64)) else TRAFFIC_BYTES.get((config, label)),
                 "kernel": name, "bytes_per_launch": a["spmv_bytes"], "avg_launch_us": round(a["spmv_ms"] * 1e3, 2)}
 
     def arm(a, label, kernel, dtype):
@@ -516,7 +523,8 @@ def bench_single(args, config, emit=True):
         "vs_baseline": None,
         "dtype": "f32 Krylov + f64 refinement",
         "data": "synthetic (seeded SplitMix64 deck generator)",
-        "config": {"workload": CONFIGS[config] if not args.plane_size else f"CPS4 cantilever {args.plane_size}",
+        "config": {"workload": CONFIGS[config] if not size else
+                   (f"C3D8 block {args.solid_size}" if config == "cfg2" else f"CPS4 cantilever {args.plane_size}"),
                    "n_dof": n_dofs, "nnz": int(mesh.nnz), "tolerance": 1e-10},
         "arm": "fp32_ir",
         "roofline": arm_out["fp32_ir"]["roofline"],
@@ -534,7 +542,7 @@ def bench_single(args, config, emit=True):
                        f"{ref.times_ms['assemble'] / 1e3:.2f} s, PCG {ref.times_ms['pcg'] / 1e3:.2f} s, recovery "
                        f"{ref.times_ms['recover'] / 1e3:.2f} s, wall {wall:.1f} s"),
             **host_info()}
-    if config == "cfg2" and not args.plane_size and args.rotation > 1:
+    if config == "cfg2" and not size and args.rotation > 1:
         out["spmv_rotation"] = spmv_rotation(args, mesh, arms, E, nu, model, load_rows, cons_rows)
     for a in arms.values():
         a["sys"].close()
@@ -715,7 +723,7 @@ def bench_serve(args):
     order = sorted(measured, key=lambda k: int(rng.next_u64(1)[0]))
     oracle_ids, tot = [], 0
     for i in order:
-        if tot + dofs[i] <= args.cpu_sample_dof:
+        if tot + dofs[i] <= args.serve_oracle_dof:
             oracle_ids.append(i)
             tot += dofs[i]
     plain_ids = measured[::max(1, len(measured) // 60)]
@@ -840,7 +848,7 @@ def profile_spmv(args):
                   ("cg64_spmv_kernel<2,%d>" if prec == Precision.FP64 else "ir32_spmv_kernel<2,%d>") % tuned["unroll"][2])
         extra = {"instances": len(keep), "autotune": tuned, **extra_orders}
     else:
-        model = build_model(args.config)
+        model = build_model(args.config, custom_size(args, args.config))
         mesh, sy = assemble(model, ctx)
         sy.apply_dirichlet(constrained_dofs(model)[0])
         keep.append((mesh, sy))
@@ -853,6 +861,9 @@ def profile_spmv(args):
             extra["other_tile_order_avg_launch_us"] = round(sy.time_spmv(prec, args.spmv_reps)[0] * 1e3, 2)
         kernel = ("cg64_spmv_kernel<%d,...>" if prec == Precision.FP64 else "ir32_spmv_kernel<%d,...>") % model.ndof()
         extra["n_dof"] = model.n_dofs()
+        extra["load_hint"] = "auto"
+        if custom_size(args, args.config):
+            extra["mesh_size"] = args.solid_size if args.config == "cfg2" else args.plane_size
     ach = by / (ms * 1e-3) / 1e9
     print(json.dumps({"profile_spmv": args.profile_spmv, "config": args.config, "operator": args.operator,
                       "kernel": kernel, "launches": args.spmv_reps + 1, "avg_launch_us": round(ms * 1e3, 2),
@@ -886,11 +897,15 @@ def main():
                     help="batched workload: every arm, or one alone")
     ap.add_argument("--operator", default="assembled", choices=["assembled", "matrix_free"],
                     help="single solves: operator of the FP64 arm (matrix_free: plane meshes)")
+    ap.add_argument("--solid-size", default=None, metavar="NXxNYxNZ",
+                    help="cfg2 (and its SpMV profile) on a C3D8 block of this mesh size, same element size")
     ap.add_argument("--plane-size", default=None, metavar="NXxNY",
                     help="cfg1 only: the cantilever on an NX x NY mesh (profiling at other sizes)")
     ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block", "block_fp32_ir"],
                     help="multi-RHS workload: run one arm only (profiling)")
     ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
+    ap.add_argument("--serve-oracle-dof", type=int, default=1000000,
+                    help="served stream: DOF budget of the seeded request sample checked against the oracle")
     ap.add_argument("--serve-operator", default="matrix_free", choices=["assembled", "matrix_free"],
                     help="served stream: operator of the server's FP64 iterations")
     ap.add_argument("--serve-precision", default="fp64", choices=["fp32_ir", "fp64"],
