@@ -10,12 +10,17 @@ in all). A step is one pass over the batch with every instance's mesh
 every K and f, Dirichlet constraints, one batched Krylov engine solving all
 instances to 1e-10 (autotuned SpMV launch shape), stress recovery and
 reactions, u and the von Mises fields copied back. `value` is the FP32-IR
-batched arm; the comparator is plain FP64 PCG one instance after another over
-the same 512 instances, timed once in the same run; FP64 batched arms (the
-assembled K, and the matrix-free operator) are reported beside them.
+batched arm with the matrix-free operator in force (FP32 inner iterations on
+the assembled FP32 K, FP64 refreshes on the assembled K, the FP64 phase on the
+operator); the comparator is plain FP64 PCG one instance after another over
+the same 512 instances, timed once in the same run; FP32-IR with the assembled
+FP64 phase and the FP64 batched arms (the assembled K, and the matrix-free
+operator) are reported beside them.
 
-The roofline is the dominant kernel's: the batch SpMV of the FP32 inner
-iteration over all instances, timed with HIP events on the solver's stream.
+The roofline is the dominant kernel's: of the value arm's two products (the
+batch SpMV of the FP32 inner iteration, the matrix-free product of its FP64
+phase), the one with the larger share of the solve, each timed with HIP
+events on the solver's stream.
 The CPU baseline is the oracle (the restated reference CPU path: linear-scan
 lookups, min(nproc, 8) threads spawned per SpMV, serial dots) on a seeded
 sample of the instances, which also gives every arm's error against the
@@ -43,7 +48,8 @@ METRIC = "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 
 # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters of rocprofv3,
 # each scaled by what the counter reads for a 1 GiB device copy (profiles/);
 # None where no counter run exists.
-TRAFFIC_BYTES = {("cfg2", "fp64"): 184.9e6, ("cfg2", "fp32_ir"): 108.2e6, ("cfg3", "fp32_ir_batched"): 2758.5e6}
+TRAFFIC_BYTES = {("cfg2", "fp64"): 184.9e6, ("cfg2", "fp32_ir"): 108.2e6, ("cfg3", "fp32_ir_batched"): 2758.5e6,
+                 ("cfg3", "fp32_ir_batched_mf"): 2758.5e6}  # the same FP32 batch SpMV
 CONFIGS = {
     "cfg2": "C3D8 200x20x20 block 20x2x2 m (265,923 DOF), jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20",
     "cfg1": "CPS4 400x40 cantilever 10x1 m (32,882 DOF), jitter 0.15h, -1e4 N tip load",
@@ -220,9 +226,11 @@ def bench_batch(args):
         stats = [sy.stats() for sy in systems]
         return b, systems, us, its, ok, vms, stats
 
-    ARMS = {"fp32_ir_batched": (Precision.FP32_IR, False), "fp64_batched": (Precision.FP64, False),
-            "fp64_batched_mf": (Precision.FP64, True)}
-    labels = ["fp32_ir_batched", "fp64_sequential", "fp64_batched", "fp64_batched_mf"]
+    # FP32-IR with the matrix-free operator in force: FP32 inner iterations on
+    # the assembled FP32 K, the FP64 phase (finish, fallback) on the operator
+    ARMS = {"fp32_ir_batched_mf": (Precision.FP32_IR, True), "fp32_ir_batched": (Precision.FP32_IR, False),
+            "fp64_batched": (Precision.FP64, False), "fp64_batched_mf": (Precision.FP64, True)}
+    labels = ["fp32_ir_batched_mf", "fp32_ir_batched", "fp64_sequential", "fp64_batched", "fp64_batched_mf"]
     if args.batch_arms != "all":  # one arm alone
         labels = [args.batch_arms]
     for label in labels:  # autotune once per batched arm (untimed; the shape depends on the batch only)
@@ -270,6 +278,14 @@ def bench_batch(args):
                 if b is not None and k == sum(passes) - 1:
                     ms, by = b.time_spmv(args.spmv_reps)
                     rec.update(spmv_ms=ms, spmv_bytes=by)
+                    if ARMS[label] == (Precision.FP32_IR, True):  # the product of its FP64 phase too
+                        ctx.set_operator("matrix_free")
+                        try:
+                            b.prepare(Precision.FP64)
+                            ms64, by64 = b.time_spmv(args.spmv_reps)
+                        finally:
+                            ctx.set_operator("assembled")
+                        rec.update(spmv64_ms=ms64, spmv64_bytes=by64)
             if b is not None:
                 b.close()
                 for sy in systems:
@@ -341,11 +357,11 @@ def bench_batch(args):
         return {"algorithmic_bytes": by, "solve_s": round(t, 3), "GBps": round(by / t / 1e9, 1),
                 "frac": round(by / t / 1e9 / HBM_PEAK_GBS, 4)}
 
-    def roof(rec, name, key):
-        ach = rec["spmv_bytes"] / (rec["spmv_ms"] * 1e-3) / 1e9
+    def roof(rec, name, key, which="spmv"):
+        ach = rec[which + "_bytes"] / (rec[which + "_ms"] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES.get(key), "kernel": name,
-                "bytes_per_launch": rec["spmv_bytes"], "avg_launch_us": round(rec["spmv_ms"] * 1e3, 2)}
+                "bytes_per_launch": rec[which + "_bytes"], "avg_launch_us": round(rec[which + "_ms"] * 1e3, 2)}
 
     seq = out.get("fp64_sequential")
 
@@ -365,9 +381,26 @@ def bench_batch(args):
         if kernel:
             a["roofline"] = roof(rec, kernel, ("cfg3", label))
             a["whole_solve"] = whole(rec, prec, mf)
+        if "spmv64_ms" in rec:
+            # FP32 phase and FP64 phase: each product's share of the solve,
+            # from the DOF-weighted iterations of each phase times its launch
+            # time per DOF; the roofline is the dominant one's
+            it64 = np.array([st["fp64_iterations"] for st in rec["stats"]], float)
+            it32 = rec["its"] - it64
+            share32 = float((it32 * dofs).sum()) * rec["spmv_ms"]
+            share64 = float((it64 * dofs).sum()) * rec["spmv64_ms"]
+            r64 = roof(rec, "cg64_mf_spmv_kernel<4|3> (whole batch)", None, "spmv64")
+            r64["traffic"] = None
+            a["product_time_share"] = {"fp32_phase": round(share32 / (share32 + share64), 3),
+                                       "fp64_phase": round(share64 / (share32 + share64), 3)}
+            if share64 > share32:
+                a["roofline"], a["roofline_fp32_phase"] = r64, a["roofline"]
+            else:
+                a["roofline_fp64_phase"] = r64
         return a
 
     kernels = {"fp32_ir_batched": "ir32_spmv_kernel<2,...> (whole batch)",
+               "fp32_ir_batched_mf": "ir32_spmv_kernel<2,...> (whole batch)",
                "fp64_batched": "cg64_spmv_kernel<2,...> (whole batch)",
                "fp64_batched_mf": "cg64_mf_spmv_kernel<4|3> (whole batch)"}
     arms = {}
@@ -389,7 +422,7 @@ def bench_batch(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 Krylov + f64 refinement" if labels[0] == "fp32_ir_batched" else "f64",
+        "dtype": "f32 Krylov + f64 refinement" if labels[0].startswith("fp32_ir") else "f64",
         "data": "synthetic (seeded SplitMix64 deck generator)",
         "config": {"workload": CONFIGS["cfg3"], "instances": len(models), "n_dof_total": int(dofs.sum()),
                    "tolerance": 1e-10},
@@ -819,6 +852,7 @@ def profile_spmv(args):
     prec = Precision.FP64 if args.profile_spmv == "fp64" else Precision.FP32_IR
     ctx = Context(0)
     ctx.set_operator(args.operator)
+    ctx.set_load_hint(args.load_hint)
     keep = []
     if args.config == "cfg3":
         b = Batch(ctx)
@@ -846,7 +880,7 @@ def profile_spmv(args):
             extra_orders = {}
         kernel = ("cg64_mf_spmv_kernel<4|3>" if args.operator == "matrix_free" else
                   ("cg64_spmv_kernel<2,%d>" if prec == Precision.FP64 else "ir32_spmv_kernel<2,%d>") % tuned["unroll"][2])
-        extra = {"instances": len(keep), "autotune": tuned, **extra_orders}
+        extra = {"instances": len(keep), "autotune": tuned, "load_hint": args.load_hint, **extra_orders}
     else:
         model = build_model(args.config, custom_size(args, args.config))
         mesh, sy = assemble(model, ctx)
@@ -861,7 +895,7 @@ def profile_spmv(args):
             extra["other_tile_order_avg_launch_us"] = round(sy.time_spmv(prec, args.spmv_reps)[0] * 1e3, 2)
         kernel = ("cg64_spmv_kernel<%d,...>" if prec == Precision.FP64 else "ir32_spmv_kernel<%d,...>") % model.ndof()
         extra["n_dof"] = model.n_dofs()
-        extra["load_hint"] = "auto"
+        extra["load_hint"] = args.load_hint
         if custom_size(args, args.config):
             extra["mesh_size"] = args.solid_size if args.config == "cfg2" else args.plane_size
     ach = by / (ms * 1e-3) / 1e9
@@ -893,7 +927,8 @@ def main():
     ap.add_argument("--rotation", type=int, default=3,
                     help="single C3D8 solve: distinct matrices the SpMV rotation leg cycles through (0: skip)")
     ap.add_argument("--batch-arms", default="all",
-                    choices=["all", "fp32_ir_batched", "fp64_batched", "fp64_batched_mf", "fp64_sequential"],
+                    choices=["all", "fp32_ir_batched_mf", "fp32_ir_batched", "fp64_batched", "fp64_batched_mf",
+                             "fp64_sequential"],
                     help="batched workload: every arm, or one alone")
     ap.add_argument("--operator", default="assembled", choices=["assembled", "matrix_free"],
                     help="single solves: operator of the FP64 arm (matrix_free: plane meshes)")
@@ -920,6 +955,8 @@ def main():
     ap.add_argument("--profile-shape", default=None, metavar="UNROLL,ROWS[,XCD]",
                     help="--profile-spmv on cfg3: the plane launch shape (blocks in flight, rows per tile, tile "
                          "order 1 = XCD / 0 = linear) instead of autotuning")
+    ap.add_argument("--load-hint", default="auto", choices=["auto", "cached", "streaming"],
+                    help="SpMV profile: load hint of the FP64 ELL product (svx_set_load_hint)")
     ap.add_argument("--profile-orders", action="store_true",
                     help="--profile-spmv: also time the other tile order in the same process")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -336,7 +336,8 @@ void system_prepare(svx_system* sys, int precision, bool matrix_free) {
     svx_mesh* m = sys->mesh;
     hipStream_t s = m->ctx->stream;
     const int64_t n = m->n;
-    if (matrix_free && precision == SVX_FP64 && !sys->mf_ready && sys->mat_uniform && sys->dirichlet &&
+    (void)precision;  // FP64 solves multiply on the operator, FP32-IR solves in their FP64 phase
+    if (matrix_free && !sys->mf_ready && sys->mat_uniform && sys->dirichlet &&
         mesh_mf_prepare(m)) {
         std::vector<uint8_t> mask(m->n_nodes, 0);
         const int nd = m->ndof;
@@ -509,12 +510,12 @@ void system_engine_setup(svx_system* sys, double tol, int64_t max_iter, int prec
     const int bs = m->ndof;
     shape.rows_per_tile[bs] = tp.rows_per_tile;  // 0: the instance's default
     shape.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
-    shape.matrix_free = tp.matrix_free && precision == SVX_FP64;
+    shape.matrix_free = tp.matrix_free;
     shape.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
     shape.nt[bs] = streaming_loads({sys}, bs, tp);
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;
-    sys->engine_mf = shape.matrix_free && sys->mf_ready;
+    sys->engine_mf = shape.matrix_free && sys->mf_ready && precision == SVX_FP64;  // the timed SpMV's operator
 }
 
 void system_solve(svx_system* sys, double tol, int64_t max_iter, int precision, InstScal* out_scal,

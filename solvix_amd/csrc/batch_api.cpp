@@ -30,8 +30,6 @@ struct svx_batch {
 
 namespace svx {
 
-// Launch shape for a set of systems: the per-block-size shape every solve
-// uses, so batched results equal solo ones bit for bit.
 int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp) {
     if (tp.load_hint != SVX_LOADS_AUTO) return tp.load_hint == SVX_LOADS_STREAMING;
     int64_t bytes = 0;
@@ -41,6 +39,8 @@ int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneP
     return bytes > kStreamBytes;
 }
 
+// Launch shape for a set of systems: the per-block-size shape every solve
+// uses, so batched results equal solo ones bit for bit.
 EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp, int precision) {
     EngineShape sh;
     for (int bs = 1; bs <= 3; ++bs) {
@@ -50,7 +50,7 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
         sh.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
         sh.nt[bs] = streaming_loads(systems, bs, tp);
     }
-    sh.matrix_free = tp.matrix_free && precision == SVX_FP64;
+    sh.matrix_free = tp.matrix_free;  // FP32-IR: the operator of its FP64 phase
     return sh;
 }
 
@@ -197,7 +197,7 @@ int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
         std::vector<InstDesc> descs = batch_setup(b, precision);
         EngineShape shape =
             b->tuned_precision == precision ? b->tuned_shape : batch_shape(b->sys, b->ctx->tune, precision);
-        shape.matrix_free = b->ctx->tune.matrix_free && precision == SVX_FP64;  // the operator in force now
+        shape.matrix_free = b->ctx->tune.matrix_free;  // the operator in force now
         b->engine->set_instances(descs, std::vector<char>(n, 1), shape);
         b->matrix_free = shape.matrix_free;
         EngineOptions opt;
@@ -235,7 +235,7 @@ int svx_batch_prepare(svx_batch* b, int precision) {
         std::vector<InstDesc> descs = batch_setup(b, precision);
         EngineShape shape =
             b->tuned_precision == precision ? b->tuned_shape : batch_shape(b->sys, b->ctx->tune, precision);
-        shape.matrix_free = b->ctx->tune.matrix_free && precision == SVX_FP64;
+        shape.matrix_free = b->ctx->tune.matrix_free;
         b->engine->set_instances(descs, std::vector<char>(n, 1), shape);
         b->matrix_free = shape.matrix_free;
         b->precision = precision;
@@ -258,7 +258,7 @@ int svx_batch_time_spmv(svx_batch* b, int reps, double* avg_ms, double* bytes_pe
         b->engine->tile_all();
         double ms = b->engine->time_spmv(b->precision, reps);
         double bytes = 0.0;
-        const bool mf = b->matrix_free;
+        const bool mf = b->matrix_free && b->precision == SVX_FP64;  // the FP32 SpMV multiplies the FP32 copy
         for (svx_system* sy : b->sys)
             bytes += mf && sy->mf_ready ? mf_spmv_bytes(sy->mesh) : spmv_bytes(sy->mesh, b->precision);
         if (avg_ms) *avg_ms = ms;

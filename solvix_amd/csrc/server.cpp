@@ -92,6 +92,13 @@ int64_t footprint(const svx_mesh& m, int precision) {
     b += 8 * nnz + 8 * ell_vals + 8 * n * 10 + 4 * n;                             // CSR + ELL values, vectors, dinv32
     if (precision == SVX_FP32_IR) b += 4 * ell_vals + 4 * n * 6;                   // FP32 ELL values + vectors
     b += 8 * std::max(m.ke_total, m.ns_total + nn * 6);                            // transient assembly / recovery
+    if (m.mf_state == 2 && m.mf_kind == 8) {  // matrix-free C3D8: element ranks, force slots, constraint mask
+        b += 8 * ne + 24 * nn * m.mf_width + nn;
+    } else if (m.mf_state == 2 && m.mf_plan) {  // matrix-free plane: patch lists, packed coordinates, mask
+        const MfPlan& p = *m.mf_plan;
+        b += (int64_t)(sizeof(MfPatch) * p.patches.size() + 4 * p.nodes.size() + 2 * p.econn.size() + 2 * p.adj.size());
+        b += 17 * nn;
+    }
     return b + (1 << 16);
 }
 
@@ -192,7 +199,7 @@ void svx_server::prep_loop() {
                 // device thread; only systems of one material use it
                 const std::vector<double>& E = r->pm.E;
                 const std::vector<double>& nu = r->pm.nu;
-                bool uniform = tune.matrix_free && cfg.precision == SVX_FP64 && !E.empty();
+                bool uniform = tune.matrix_free && !E.empty();
                 for (size_t e = 1; e < E.size() && uniform; ++e) uniform = E[e] == E[0] && nu[e] == nu[0];
                 if (uniform) mesh_mf_plan(r->mesh.get(), md.coords);
                 r->bytes = footprint(*r->mesh, cfg.precision);

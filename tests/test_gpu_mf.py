@@ -174,14 +174,37 @@ def test_matrix_free_solid_matches_oracle(mfctx):
         assert its <= 1.5 * ref.iterations + 10, m.name
 
 
-def test_fp32_ir_ignores_the_operator(mfctx, ctx):
-    """FP32-IR inner iterations and residuals use the assembled K whatever the
-    context's operator: the same bits as on an assembled-operator context."""
-    for m in (_plane_models()[2], _solid_models()[0]):
+def test_fp32_ir_finishes_on_the_operator(mfctx, ctx):
+    """FP32-IR on a matrix-free context: the FP32 inner iterations and the
+    FP64 residual refreshes use the assembled K, the FP64 phase (finish and
+    fallback) multiplies on the operator. Solutions agree with the oracle
+    within the north-star tolerance, the FP32 phase runs the same iterations
+    as on an assembled context, and a batch equals its solo solves bit for
+    bit."""
+    models = [_plane_models()[2], _plane_models()[4], _solid_models()[0]]
+    solo = []
+    for m in models:
         out = []
         for c in (ctx, mfctx):
             mesh, s = _system(m, c)
             u, its, res = s.solve(1e-10, 0, Precision.FP32_IR)
-            out.append((u, its, res))
-            assert s.operator == "assembled", m.name
-        assert out[0][1] == out[1][1] and np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64)), m.name
+            st = s.stats()
+            out.append((u, its, st))
+        (ua, ia, sa), (um, im, sm) = out
+        assert sm["iterations"] - sm["fp64_iterations"] == sa["iterations"] - sa["fp64_iterations"], m.name
+        assert s.operator == "matrix_free", m.name
+        ref = oracle.solve(m)
+        e = rel_err(um, ref.u)
+        print(f"{m.name}: assembled {ia} its ({sa['fp64_iterations']} FP64), matrix-free finish {im} "
+              f"({sm['fp64_iterations']} FP64), err {e:.1e}")
+        assert e <= 1e-8, m.name
+        solo.append((um, im))
+    keep, b = [], Batch(mfctx)
+    for m in models:
+        mesh, s = _system(m, mfctx)
+        keep.append((mesh, s))
+        b.add(s)
+    res = b.solve(Precision.FP32_IR)
+    for (mesh, s), (u0, its0), (its, _, ok), m in zip(keep, solo, res, models):
+        assert ok and its == its0, m.name
+        assert np.array_equal(s.solution().view(np.uint64), u0.view(np.uint64)), m.name

@@ -31,13 +31,16 @@ def _decks():
 @pytest.mark.parametrize("precision,operator,graph", [(Precision.FP64, "assembled", False),
                                                       (Precision.FP32_IR, "assembled", False),
                                                       (Precision.FP64, "matrix_free", False),
+                                                      (Precision.FP32_IR, "matrix_free", False),
                                                       (Precision.FP64, "assembled", True),
                                                       (Precision.FP32_IR, "assembled", True)],
-                         ids=["fp64", "fp32ir", "fp64_matrix_free", "fp64_graph", "fp32ir_graph"])
+                         ids=["fp64", "fp32ir", "fp64_matrix_free", "fp32ir_matrix_free", "fp64_graph",
+                              "fp32ir_graph"])
 def test_results_alone_equal_results_in_the_mix(precision, operator, graph):
-    """matrix_free: a server created on a context with that operator runs its
-    FP64 instances (plane meshes and C3D8 blocks of one material) on it, as
-    the same context's solve does alone. graph: the server's chunks run as
+    """matrix_free: a server created on a context with that operator runs the
+    FP64 iterations of its instances (plane meshes and C3D8 blocks of one
+    material; FP32-IR instances in their FP64 phase) on it, as the same
+    context's solve does alone. graph: the server's chunks run as
     captured hipGraphs (svx_server_config.use_graph = 1)."""
     models = _decks()
     actx, sctx = Context(0), Context(0)
