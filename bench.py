@@ -820,7 +820,7 @@ def bench_serve(args):
         "vs_baseline": None, "dtype": "f64" if prec == Precision.FP64 else "f32 Krylov + f64 refinement",
         "data": "synthetic (seeded SplitMix64 deck generator)",
         "config": {"workload": CONFIGS["cfg5"], "precision": args.serve_precision,
-                   "operator": args.serve_operator if prec == Precision.FP64 else "assembled",
+                   "operator": args.serve_operator,  # FP32-IR: the operator of its FP64 phase
                    "rate_per_s": args.rate, "measured_s": args.duration, "graphs": args.serve_graph,
                    "lead_s": args.lead, "requests": len(stream), "measured_requests": len(measured),
                    "hbm_budget_bytes": 200 * 10 ** 9, "offered_dof_per_s": round(offered, 1)},
