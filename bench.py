@@ -145,6 +145,8 @@ def bench_batch(args):
     models = decks.cfg3()
     t_setup = time.perf_counter()
     ctx = Context(0)
+    if args.ru_delta > 0:
+        ctx.set_tuning(ru_delta=args.ru_delta)
     prep = []
     for m in models:
         E, nu = element_materials(m)
@@ -955,6 +957,9 @@ def main():
     ap.add_argument("--profile-shape", default=None, metavar="UNROLL,ROWS[,XCD]",
                     help="--profile-spmv on cfg3: the plane launch shape (blocks in flight, rows per tile, tile "
                          "order 1 = XCD / 0 = linear) instead of autotuning")
+    ap.add_argument("--ru-delta", type=float, default=0.0,
+                    help="FP32-IR: refresh once the recursive residual falls below this fraction of its maximum "
+                         "since the last refresh (0: the library default, 0.1)")
     ap.add_argument("--load-hint", default="auto", choices=["auto", "cached", "streaming"],
                     help="SpMV profile: load hint of the FP64 ELL product (svx_set_load_hint)")
     ap.add_argument("--profile-orders", action="store_true",

@@ -84,7 +84,7 @@ __device__ __forceinline__ double reduce_slot(const double* __restrict__ partial
 __global__ __launch_bounds__(kBlock) void blk_init_kernel(BlockArgs A) {
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int tile = blockIdx.x;
-    const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
     const bool on = lane < A.R;
     double ff = 0.0, rz = 0.0;
     for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
@@ -99,13 +99,13 @@ __global__ __launch_bounds__(kBlock) void blk_init_kernel(BlockArgs A) {
         ff += fi * fi;
         rz += fi * zi;
     }
-    write_rhs_partials(ff, rz, A.partials, tile, A.n_tiles, 2);
+    write_rhs_partials(ff, rz, A.partials, tile, A.n_vtiles, 2);
 }
 
 // One workgroup: initial scalars per load case.
 __global__ __launch_bounds__(kReduceThreads) void blk_init_finish_kernel(BlockArgs A) {
-    const double ff = reduce_slot(A.partials, A.n_tiles, 0);
-    const double rz = reduce_slot(A.partials, A.n_tiles, 1);
+    const double ff = reduce_slot(A.partials, A.n_vtiles, 0);
+    const double rz = reduce_slot(A.partials, A.n_vtiles, 1);
     const int lane = threadIdx.x;
     if (lane >= kWave) return;
     const bool on = lane < A.R;
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void blk_update_kernel(BlockArgs A) {
     if (!A.S->any_running) return;
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int tile = blockIdx.x;
-    const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
     const bool on = lane < A.R && A.S->status[lane] == ST_RUNNING;
     const double alpha = on ? A.S->alpha[lane] : 0.0;
     double rr = 0.0, rz = 0.0;
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void blk_update_kernel(BlockArgs A) {
             rr += ri * ri;
             rz += ri * zi;
         }
-    write_rhs_partials(rr, rz, A.partials, tile, A.n_tiles, 2);
+    write_rhs_partials(rr, rz, A.partials, tile, A.n_vtiles, 2);
 }
 
 // One workgroup: residual, convergence, budget and beta per load case; the
@@ -263,8 +263,8 @@ __global__ __launch_bounds__(kReduceThreads) void blk_beta_kernel(BlockArgs A, i
         if (threadIdx.x == 0 && running_out) *running_out = 0;
         return;
     }
-    const double rr = reduce_slot(A.partials, A.n_tiles, 0);
-    const double rz = reduce_slot(A.partials, A.n_tiles, 1);
+    const double rr = reduce_slot(A.partials, A.n_vtiles, 0);
+    const double rz = reduce_slot(A.partials, A.n_vtiles, 1);
     const int lane = threadIdx.x;
     if (lane >= kWave) return;
     bool run = false;
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void blk_pupdate_kernel(BlockArgs A) {
     if (!A.S->any_running) return;
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int tile = blockIdx.x;
-    const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
     const bool on = lane < A.R && A.S->status[lane] == ST_RUNNING;
     if (!on) return;
     const double beta = A.S->beta[lane];
@@ -347,7 +347,7 @@ void blk_unpack(const double* in, double* out, int32_t n, int32_t R, hipStream_t
 }
 
 void blk_init(const BlockArgs& a, hipStream_t s) {
-    blk_init_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
+    blk_init_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
     blk_init_finish_kernel<<<1, kReduceThreads, 0, s>>>(a);
     SVX_HIP(hipGetLastError());
 }
@@ -364,9 +364,9 @@ void launch_blk_spmv(const BlockArgs& a, hipStream_t s) {
 void blk_iteration(const BlockArgs& a, hipStream_t s) {
     launch_blk_spmv(a, s);
     blk_alpha_kernel<<<1, kReduceThreads, 0, s>>>(a);
-    blk_update_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
+    blk_update_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
     blk_beta_kernel<<<1, kReduceThreads, 0, s>>>(a, nullptr);
-    blk_pupdate_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
+    blk_pupdate_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
     SVX_HIP(hipGetLastError());
 }
 

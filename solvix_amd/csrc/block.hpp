@@ -24,8 +24,10 @@ struct alignas(16) BlockScal {
 struct BlockArgs {
     int32_t n;
     int32_t R;
-    int32_t rows_per_tile;  // a multiple of bs
+    int32_t rows_per_tile;  // SpMV tiles: a multiple of bs
     int32_t n_tiles;
+    int32_t vrows_per_tile;  // tiles of the vector kernels (initialisation, updates)
+    int32_t n_vtiles;
     int32_t bs;  // node block size (2, 3) for the node-row SpMV; 1 = row form
     int32_t xcd; // SpMV tiles in XCD-contiguous order (1) or tile = workgroup (0)
     const int32_t* row_ptr;
@@ -40,7 +42,7 @@ struct BlockArgs {
     double* P;
     double* Z;
     double* Q;
-    double* partials;  // [2][n_tiles][64]: slot-major per-tile, per-load-case sums
+    double* partials;  // slot-major per-tile, per-load-case sums: [1][n_tiles][64] (p.q), [2][n_vtiles][64] (r.r, r.z)
     BlockScal* S;
 };
 

@@ -141,6 +141,13 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
         const int rpt = ctx->tune.rows_per_tile > 0 ? ctx->tune.rows_per_tile : (a.bs > 1 ? 48 : 64);
         a.rows_per_tile = (rpt + a.bs - 1) / a.bs * a.bs;
         a.n_tiles = (n + a.rows_per_tile - 1) / a.rows_per_tile;
+        // vector kernels on tiles of two SpMV tiles: half the partial sums for
+        // the one-workgroup reduction of r.r and r.z, and still enough
+        // workgroups to stream (measured with 48-row SpMV tiles: 96-row vector
+        // tiles give update 33 us and the r.r / r.z reduction 12 us per
+        // iteration, 192-row ones 48 us and 9 us)
+        a.vrows_per_tile = 2 * a.rows_per_tile;
+        a.n_vtiles = (n + a.vrows_per_tile - 1) / a.vrows_per_tile;
         a.row_ptr = m->row_ptr.p;
         a.brow_ptr = m->brow_ptr.p;
         a.bcol = m->bcol.p;
@@ -149,7 +156,7 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
         a.dinv = sys->dinv.p;
         a.F = Fp.p;
         a.U = Uv.p, a.Rv = Rv.p, a.P = P.p, a.Z = Z.p, a.Q = Q.p;
-        DevBuf<double> partials((size_t)a.n_tiles * 2 * kMaxRhs);
+        DevBuf<double> partials((size_t)std::max(a.n_tiles, a.n_vtiles) * 2 * kMaxRhs);
         DevBuf<BlockScal> S(1);
         BlockScal hs{};
         hs.tol = tol;
