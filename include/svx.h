@@ -233,6 +233,14 @@ int svx_set_tile_order(svx_ctx* ctx, int order);
 /* Load hint of this context's subsequent FP64 solves (default
  * SVX_LOADS_AUTO). No reference counterpart: a launch knob. */
 int svx_set_load_hint(svx_ctx* ctx, int hint);
+/* Assembled 2D / 3D systems whose FP64 solve has more row tiles than this
+ * take the pending direction step of each iteration (r.r and r.z over all
+ * their tiles, the convergence and budget tests of solver.rs:129-148, beta)
+ * in one workgroup per system, where smaller ones take it in every SpMV tile
+ * (whose reduction grows with the square of the tile count). 0: the default,
+ * 1024; negative: never. Either way gives the same results bit for bit. No
+ * reference counterpart: a launch knob. */
+int svx_set_step_kernel_tiles(svx_ctx* ctx, int32_t tiles);
 /* SVX_OP_MATRIX_FREE once a matrix-free solve has prepared the system's
  * operator (and until its constraints change), else SVX_OP_ASSEMBLED; -1 for
  * NULL. No reference counterpart. */

@@ -67,6 +67,7 @@ SIGNATURES = [
     ("svx_batch_autotune", C.c_int, [_vp, C.c_int, C.c_int, _i32p, _i32p, _i32p, _dp]),
     ("svx_set_tile_order", C.c_int, [_vp, C.c_int]),
     ("svx_set_load_hint", C.c_int, [_vp, C.c_int]),
+    ("svx_set_step_kernel_tiles", C.c_int, [_vp, C.c_int32]),
     ("svx_server_create", C.c_int, [_vp, C.POINTER(_abi.SvxServerConfig), C.POINTER(_vp)]),
     ("svx_server_destroy", None, [_vp]),
     ("svx_server_submit", C.c_int, [_vp, C.POINTER(_abi.SvxModelDesc), _i64p]),

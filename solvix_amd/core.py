@@ -68,6 +68,12 @@ class Context:
         code = {"xcd": _abi.SVX_TILES_XCD, "linear": _abi.SVX_TILES_LINEAR}[order]
         _raise(lib().svx_set_tile_order(self._h, code))
 
+    def set_step_kernel_tiles(self, tiles: int):
+        """Systems with more FP64 row tiles than `tiles` take each pending
+        direction step in one per-system kernel (0: the default 1024,
+        negative: never); same bits either way."""
+        _raise(lib().svx_set_step_kernel_tiles(self._h, int(tiles)))
+
     def set_load_hint(self, hint: str):
         """'auto' (default: nontemporal matrix loads once a launch's FP64 matrix
         values exceed 256 MiB), 'cached' or 'streaming' for the FP64 assembled

@@ -513,6 +513,7 @@ void system_engine_setup(svx_system* sys, double tol, int64_t max_iter, int prec
     shape.matrix_free = tp.matrix_free;
     shape.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
     shape.nt[bs] = streaming_loads({sys}, bs, tp);
+    if (tp.step_tiles != 0) shape.step_tiles = tp.step_tiles;
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;
     sys->engine_mf = shape.matrix_free && sys->mf_ready && precision == SVX_FP64;  // the timed SpMV's operator
@@ -1043,6 +1044,12 @@ int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile, int p
 int svx_set_tile_order(svx_ctx* ctx, int order) {
     if (!ctx || (order != SVX_TILES_XCD && order != SVX_TILES_LINEAR)) return SVX_E_ARGUMENT;
     ctx->tune.tile_order = order;
+    return SVX_OK;
+}
+
+int svx_set_step_kernel_tiles(svx_ctx* ctx, int32_t tiles) {
+    if (!ctx) return SVX_E_ARGUMENT;
+    ctx->tune.step_tiles = tiles;
     return SVX_OK;
 }
 

@@ -23,6 +23,7 @@ struct TuneParams {
     bool matrix_free = false;  // FP64 iterations on the matrix-free operator where a system has one
     int tile_order = SVX_TILES_XCD;  // order of the gathering kernels' tiles over the XCDs
     int load_hint = SVX_LOADS_AUTO;  // FP64 ELL SpMV matrix loads
+    int step_tiles = 0;              // EngineShape::step_tiles; 0: the engine's default
 };
 
 // Grow-only device scratch of one context for the temporaries of a call

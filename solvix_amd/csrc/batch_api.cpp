@@ -50,6 +50,7 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
         sh.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
         sh.nt[bs] = streaming_loads(systems, bs, tp);
     }
+    if (tp.step_tiles != 0) sh.step_tiles = tp.step_tiles;
     sh.matrix_free = tp.matrix_free;  // FP32-IR: the operator of its FP64 phase
     return sh;
 }

@@ -151,11 +151,23 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         t.tile_begin = h_descs_[t.inst].tile_begin;
         t.tile_count = h_descs_[t.inst].tile_count;
     }
-    // one leader per matrix-free instance for the step kernel: plane
-    // instances (group 2) first, then C3D8 ones (group 3); C3D8 element tiles
+    // one leader per instance whose FP64 pending step the step kernel takes:
+    // group 2 (plane matrix-free instances, then large assembled 2x2-block
+    // ones) first, then group 3 (C3D8 matrix-free, then large assembled
+    // 3x3-block ones); C3D8 element tiles
     std::vector<Tile> mf_leaders, mf3_tiles;
     for (const Tile& t : mf_tiles)
         if (t.row_begin == 0) mf_leaders.push_back(t);
+    auto large_assembled = [&](size_t i) {
+        const InstDesc& d = h_descs_[i];
+        return running[i] && d.bs >= 2 && !uses_mf(d) && shape_.step_tiles >= 0 && d.tile_count > shape_.step_tiles;
+    };
+    for (size_t i = 0; i < h_descs_.size(); ++i) {
+        InstDesc& d = h_descs_[i];
+        d.step_kernel = large_assembled(i) ? 1 : 0;
+        if (d.step_kernel && d.bs == 2)
+            mf_leaders.push_back(Tile{(int32_t)i, 0, 0, d.tile_begin, d.tile_count, P_PQ, d.tile_begin, d.tile_count, d.s});
+    }
     const int32_t n_lead2 = (int32_t)mf_leaders.size();
     for (size_t i = 0; i < h_descs_.size(); ++i) {
         const InstDesc& d = h_descs_[i];
@@ -164,6 +176,11 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         for (int32_t e = 0; e < d.mf_n_elem; e += kMf3Elems)
             mf3_tiles.push_back(Tile{(int32_t)i, e, std::min(d.mf_n_elem, e + kMf3Elems), d.tile_begin, d.tile_count,
                                      P_PQ, d.tile_begin, d.tile_count, d.s});
+    }
+    for (size_t i = 0; i < h_descs_.size(); ++i) {
+        const InstDesc& d = h_descs_[i];
+        if (d.step_kernel && d.bs == 3)
+            mf_leaders.push_back(Tile{(int32_t)i, 0, 0, d.tile_begin, d.tile_count, P_PQ, d.tile_begin, d.tile_count, d.s});
     }
     tiled_tiles_ = (int)tiles.size();
     int stride[4] = {0, 0, 0, 0};

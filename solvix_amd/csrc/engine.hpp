@@ -24,12 +24,21 @@ struct EngineOptions {
     bool wait = true;
 };
 
+constexpr int kStepKernelTiles = 1024;
+
 struct EngineShape {
     int lanes[4] = {0, 0, 0, 0};   // lanes per row (raw CSR, block size 1)
     int unroll[4] = {0, 0, 0, 0};  // blocks in flight per lane (ELL, block sizes 2, 3)
     int rows_per_tile[4] = {0, 0, 0, 0};  // 0: per-instance default; block sizes 2, 3: whole 64-node-row chunks
     int xcd[4] = {1, 1, 1, 1};  // gathering kernels: XCD-contiguous tile order (1) or tile = workgroup (0)
     int nt[4] = {0, 0, 0, 0};   // FP64 ELL SpMV: nontemporal matrix loads (1) or cached (0)
+    // Assembled instances with more row tiles than this take their FP64
+    // pending direction step (r.r and r.z reduced over the instance's tiles,
+    // convergence, budget, beta) in the step kernel, once per instance,
+    // instead of in every SpMV tile, whose reduction reads every tile's
+    // partials (a cost that grows with the square of the tile count) at one
+    // launch more per iteration. Negative: never.
+    int step_tiles = kStepKernelTiles;
     // FP64 SpMV of instances that carry a matrix-free operator (InstDesc::mf_kind)
     // on that operator instead of the assembled matrix.
     bool matrix_free = false;

@@ -107,6 +107,7 @@ __device__ __forceinline__ T load_hint(gptr<const T> p) {
 
 struct Dev {
     int32_t n;
+    int32_t stepk;  // InstDesc::step_kernel
     gptr<const int32_t> row_ptr, col, brow_ptr, bcol;
     gptr<const int32_t> ell_off, ell_nb, ell_col;
     gptr<const double> val, ell_val, dinv, f;
@@ -118,6 +119,7 @@ struct Dev {
 __device__ __forceinline__ Dev view(const InstDesc& d) {
     Dev v;
     v.n = d.n;
+    v.stepk = d.step_kernel;
     v.row_ptr = global(d.row_ptr);
     v.col = global(d.col);
     v.brow_ptr = global(d.brow_ptr);
@@ -669,8 +671,9 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
     const Tile t = tiles[xcd_tile(xcd)];
     const gptr<InstScal> S = global(t.s);
     const int32_t status = S->status, flags = S->flags, dir = S->dir;
+    const double sbeta = S->beta;
     const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
-    issue_before_test(D.p, D.p_alt, D.z, D.q, D.ell_off, D.ell_nb, D.ell_col, D.ell_val, par, force_dir);
+    issue_before_test(D.p, D.p_alt, D.z, D.q, D.ell_off, D.ell_nb, D.ell_col, D.ell_val, par, force_dir, D.stepk);
     if (status != ST_RUNNING || (flags & FL_FRESH)) return;
     const gptr<const double> cur = par ? D.p_alt : D.p;
     const gptr<double> nxt = par ? D.p : D.p_alt;
@@ -679,7 +682,9 @@ __global__ __launch_bounds__(kBlock) void cg64_spmv_kernel(const Tile* __restric
         v[0] = tile_spmv<BS, W, double, NT>(t, D, D.ell_val, D.val, Plain<double>{cur}, nxt, D.q);
     } else {
         double beta = 1.0;
-        if (!force_dir) {
+        if (!force_dir && D.stepk) {
+            beta = sbeta;  // cg64_mf_step_kernel took the pending step (convergence, budget, beta)
+        } else if (!force_dir) {
             const double snap_rz = S->snap_rz;
             const int64_t snap_it = S->snap_it;
             const int sl[2] = {P_RR, P_RZ};
@@ -1610,6 +1615,10 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 
 void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {
     const int g = k.n_ell_tiles;
+    // the pending step of the matrix-free and the large assembled instances,
+    // before any product that reads it
+    if (k.n_mf_inst > 0 && !force_dir)
+        cg64_mf_step_kernel<<<k.n_mf_inst, kBlock, 0, s>>>(k.mf_leaders, k.partials, k.pstride);
     if (g > 0) {
         if (k.bs == 3) {
 #define L(U) cg64_spmv_kernel<3, U, false><<<g, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
@@ -1637,8 +1646,6 @@ void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir)
 #undef L
         }
     }
-    if (k.n_mf_inst > 0 && !force_dir)
-        cg64_mf_step_kernel<<<k.n_mf_inst, kBlock, 0, s>>>(k.mf_leaders, k.partials, k.pstride);
     if (k.n_mf3_tiles > 0) {
         cg64_mf3_elem_kernel<<<k.n_mf3_tiles, kMf3Elems * 8, 0, s>>>(k.mf3_tiles, k.descs, par, force_dir);
         cg64_mf3_node_kernel<<<k.n_tiles - k.n_ell_tiles, kBlock, 0, s>>>(k.tiles, k.n_ell_tiles, k.descs,

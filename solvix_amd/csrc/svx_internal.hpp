@@ -137,6 +137,10 @@ struct InstDesc {
     int32_t tile_begin;  // first tile of this instance in the tile table
     int32_t tile_count;
     int32_t bs;          // node-block size (ndof); 1 = raw CSR
+    // set by the engine: the FP64 SpMV of this (large, assembled) instance
+    // reads the pending direction step that cg64_mf_step_kernel took once
+    // for it, instead of every tile reducing the instance's partial sums
+    int32_t step_kernel;
     const int32_t* row_ptr;
     const int32_t* col;
     const double* val;
@@ -299,7 +303,9 @@ struct KrylovSet {
     int32_t n_ell_tiles = 0;
     const Tile* mf_tiles = nullptr;
     int32_t n_mf4 = 0, n_mf3 = 0;
-    const Tile* mf_leaders = nullptr;  // one per matrix-free instance (its first patch tile)
+    // one per instance whose FP64 pending step runs in cg64_mf_step_kernel:
+    // the matrix-free instances, then the large assembled ones
+    const Tile* mf_leaders = nullptr;
     int32_t n_mf_inst = 0;
     const Tile* mf3_tiles = nullptr;  // C3D8 matrix-free element tiles (row_begin / row_end: elements)
     int32_t n_mf3_tiles = 0;

@@ -121,6 +121,37 @@ def test_many_tiles_equal_solo(precision, rows_per_tile, ctx):
         assert np.array_equal(s.solution().view(np.uint64), u0.view(np.uint64)), m.name
 
 
+def test_step_kernel_changes_no_bits():
+    """Instances with more row tiles than the threshold take each pending
+    direction step in the per-instance step kernel: the same reduction over
+    the same partial sums as every SpMV tile takes otherwise, so the bits and
+    iteration counts do not change (one-chunk tiles give the 265,923-DOF block
+    1,385 tiles and a 1000x80 plane cantilever 1,267)."""
+    from solvix_amd.core import Context
+    models = [decks.cfg2(), decks.cfg1(nx=1000, ny=80)]
+    runs = [(m, Precision.FP64) for m in models] + [(models[0], Precision.FP32_IR)]
+    out = {}
+    for tiles in (0, -1):
+        ctx = Context(0)
+        ctx.set_tuning(rows_per_tile=1)  # one ELL chunk per tile
+        ctx.set_step_kernel_tiles(tiles)
+        try:
+            res = []
+            for m, prec in runs:
+                mesh, s = _system(m, ctx)
+                u, its, r = s.solve(1e-10, 0, prec)
+                res.append((u, its, r))
+                s.close()
+                mesh.close()
+            out[tiles] = res
+        finally:
+            ctx.close()
+    for (ua, ia, ra), (ub, ib, rb), (m, prec) in zip(out[0], out[-1], runs):
+        print(f"{m.name} [{prec}]: {ia} / {ib} iterations, residual {ra:.3e} / {rb:.3e}")
+        assert ia == ib and ra == rb, m.name
+        assert np.array_equal(ua.view(np.uint64), ub.view(np.uint64)), m.name
+
+
 @pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
 def test_load_hint_changes_no_bits(precision):
     """Nontemporal and ordinary matrix loads read the same values in the same
