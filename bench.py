@@ -48,7 +48,7 @@ METRIC = "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 
 # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters of rocprofv3,
 # each scaled by what the counter reads for a 1 GiB device copy (profiles/);
 # None where no counter run exists.
-TRAFFIC_BYTES = {("cfg2", "fp64"): 184.9e6, ("cfg2", "fp32_ir"): 108.2e6, ("cfg3", "fp32_ir_batched"): 2758.5e6,
+TRAFFIC_BYTES = {("cfg2", "fp64"): 184.9e6, ("cfg2", "fp32_ir"): 96.6e6, ("cfg3", "fp32_ir_batched"): 2758.5e6,
                  ("cfg3", "fp32_ir_batched_mf"): 2758.5e6}  # the same FP32 batch SpMV
 CONFIGS = {
     "cfg2": "C3D8 200x20x20 block 20x2x2 m (265,923 DOF), jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20",
