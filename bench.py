@@ -48,8 +48,9 @@ METRIC = "solved DOF/s at 1e-10 rel. residual: mixed-prec batched vs plain FP64 
 # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters of rocprofv3,
 # each scaled by what the counter reads for a 1 GiB device copy (profiles/);
 # None where no counter run exists.
-TRAFFIC_BYTES = {("cfg2", "fp64"): 184.9e6, ("cfg2", "fp32_ir"): 96.6e6, ("cfg3", "fp32_ir_batched"): 2758.5e6,
-                 ("cfg3", "fp32_ir_batched_mf"): 2758.5e6}  # the same FP32 batch SpMV
+TRAFFIC_BYTES = {("cfg2", "fp64"): 184.9e6, ("cfg2", "fp32_ir"): 96.6e6,
+                 ("cfg3", "fp32_ir_batched"): 2797.0e6, ("cfg3", "fp32_ir_batched_mf"): 2797.0e6,  # one FP32 SpMV
+                 ("cfg3", "fp64_batched"): 5020.4e6, ("cfg3", "fp64_batched_mf"): 1281.8e6}
 CONFIGS = {
     "cfg2": "C3D8 200x20x20 block 20x2x2 m (265,923 DOF), jitter 0.1h, x=0 clamped, -1e6 Pa z-traction on x=20",
     "cfg1": "CPS4 400x40 cantilever 10x1 m (32,882 DOF), jitter 0.15h, -1e4 N tip load",
@@ -391,8 +392,7 @@ def bench_batch(args):
             it32 = rec["its"] - it64
             share32 = float((it32 * dofs).sum()) * rec["spmv_ms"]
             share64 = float((it64 * dofs).sum()) * rec["spmv64_ms"]
-            r64 = roof(rec, "cg64_mf_spmv_kernel<4|3> (whole batch)", None, "spmv64")
-            r64["traffic"] = None
+            r64 = roof(rec, "cg64_mf_spmv_kernel<4|3> (whole batch)", ("cfg3", "fp64_batched_mf"), "spmv64")
             a["product_time_share"] = {"fp32_phase": round(share32 / (share32 + share64), 3),
                                        "fp64_phase": round(share64 / (share32 + share64), 3)}
             if share64 > share32:
