@@ -84,9 +84,10 @@ enum { SVX_FP64 = 0, SVX_FP32_IR = 1 };
  *                     with the formulas of element.rs (91-211 plane, 427-561
  *                     solid, the reference's J^-1 sign pattern included), the
  *                     same product up to rounding, so iterates differ from the
- *                     assembled ones in the last bits. Other systems, the FP32
- *                     inner iterations of SVX_FP32_IR and residual
- *                     evaluations keep the assembled K. */
+ *                     assembled ones in the last bits. It serves FP64 solves
+ *                     and the FP64 phase (finish, fallback) of SVX_FP32_IR
+ *                     solves; other systems, the FP32 inner iterations and
+ *                     residual evaluations keep the assembled K. */
 enum { SVX_OP_ASSEMBLED = 0, SVX_OP_MATRIX_FREE = 1 };
 
 /* Order in which the workgroups of the gathering kernels (SpMV, FP64
@@ -222,10 +223,11 @@ const char* svx_last_error(void);
  * counterpart: the autotuner drives it. */
 int svx_set_tuning(svx_ctx* ctx, int lanes, int unroll, int rows_per_tile,
                    int period, int chunk, int use_graph, double ru_delta);
-/* Operator of this context's subsequent FP64 solves (svx_solve,
- * svx_batch_solve, and the FP64 engine of a server created on the context
- * afterwards): SVX_OP_ASSEMBLED (default) or SVX_OP_MATRIX_FREE. No
- * reference counterpart: the reference multiplies its assembled CSR only. */
+/* Operator of this context's subsequent FP64 iterations (svx_solve,
+ * svx_batch_solve, and the engines of a server created on the context
+ * afterwards; for SVX_FP32_IR, its FP64 phase): SVX_OP_ASSEMBLED (default) or
+ * SVX_OP_MATRIX_FREE. No reference counterpart: the reference multiplies its
+ * assembled CSR only. */
 int svx_set_operator(svx_ctx* ctx, int op);
 /* Tile order of this context's subsequent solves (default SVX_TILES_XCD).
  * No reference counterpart: a launch-shape knob the autotuner also sets. */

@@ -84,7 +84,8 @@ class Context:
 
     def set_operator(self, op: str):
         """'assembled' (default) or 'matrix_free': the operator of subsequent
-        FP64 solves on plane meshes of one element kind and one material."""
+        FP64 iterations (FP64 solves, and FP32-IR's FP64 phase) on plane meshes
+        of one element kind and C3D8 meshes, each of one material."""
         code = {"assembled": _abi.SVX_OP_ASSEMBLED, "matrix_free": _abi.SVX_OP_MATRIX_FREE}[op]
         _raise(lib().svx_set_operator(self._h, code))
 
