@@ -649,6 +649,10 @@ def bench_multi(args):
 
     model, cases = decks.cfg4()
     ctx = Context(0)
+    if args.multi_group:
+        ctx.set_multi_group(args.multi_group)
+    if args.no_graph:
+        ctx.set_tuning(use_graph=False)
     mesh, sy = assemble(model, ctx)
     sy.apply_dirichlet(constrained_dofs(model)[0])
     n = mesh.n
@@ -658,6 +662,11 @@ def bench_multi(args):
         for nid, d, v in zip(nodes, dofs, vals):
             F[k, pos[int(nid)] * 3 + int(d) - 1] += v
     out = {}
+    if args.multi_arm == "spmv":  # the block SpMV alone (profiler runs: no solve, no graphs)
+        us, by = sy.multi_spmv_timing(len(cases), args.spmv_reps)
+        print(json.dumps({"block_spmv_us": round(us, 2), "bytes_per_launch": by,
+                          "GBps": round(by / us / 1e3, 1)}), flush=True)
+        return
     arms = [a for a in ("single", "block", "block_fp32_ir") if args.multi_arm in ("both", a)]
     for arm in arms:
         for k in range(args.warmup + args.steps):
@@ -679,7 +688,21 @@ def bench_multi(args):
                 rec = out.setdefault(arm, {"dt": 0.0})
                 rec["dt"] += dt
                 rec.update(U=U, its=its, ok=ok)
+    # the block FP64 solve's SpMV (direction step fused in), timed alone in
+    # the solve's launch shape
+    spmv = None
+    if "block" in arms:
+        us, by = sy.multi_spmv_timing(len(cases), args.spmv_reps)
+        grouped = args.multi_group >= 0
+        spmv = {"bound": "hbm", "achieved": round(by / us / 1e3, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(by / us / 1e3 / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"blk_spmv_group_kernel<3,{args.multi_group or 4}>" if grouped else "blk_spmv_node_kernel<3>",
+                "bytes_per_launch": by, "avg_launch_us": round(us, 2)}
+        log(f"block SpMV: {us:.2f} us per launch, {by / 1e6:.1f} MB algorithmic, {by / us / 1e3:.1f} GB/s")
     if len(arms) < 3:
+        if spmv:
+            print(json.dumps({"block_spmv": spmv, "block_ms": round(out["block"]["dt"] / args.steps * 1e3, 3)}),
+                  flush=True)
         return
     mixed, blk, b = out["block_fp32_ir"], out["block"], out["single"]
     dof = float(n) * len(cases) * args.steps
@@ -715,7 +738,7 @@ def bench_multi(args):
                    "tolerance": 1e-10},
         "arm": "block_fp32_ir", "comparator": "single_fp64",
         "speedup_vs_fp64": round(b["dt"] / mixed["dt"], 3),
-        "block_fp32_ir": arm(mixed), "block_fp64": arm(blk),
+        "block_fp32_ir": arm(mixed), "block_fp64": dict(arm(blk), roofline=spmv),
         "single_fp64": {"value": round(dof / b["dt"], 1), "ms_per_step": round(b["dt"] / args.steps * 1e3, 3),
                         "iterations_max": int(b["its"].max()), "iterations_mean": round(float(b["its"].mean()), 1)},
     }
@@ -938,7 +961,11 @@ def main():
                     help="cfg2 (and its SpMV profile) on a C3D8 block of this mesh size, same element size")
     ap.add_argument("--plane-size", default=None, metavar="NXxNY",
                     help="cfg1 only: the cantilever on an NX x NY mesh (profiling at other sizes)")
-    ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block", "block_fp32_ir"],
+    ap.add_argument("--no-graph", action="store_true",
+                    help="multi-RHS workload: enqueue each chunk kernel by kernel instead of as a captured graph")
+    ap.add_argument("--multi-group", type=int, default=0, choices=[-1, 0, 4, 8],
+                    help="multi-RHS workload: nodes per group of the block SpMV (0: default, -1: no groups)")
+    ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block", "block_fp32_ir", "spmv"],
                     help="multi-RHS workload: run one arm only (profiling)")
     ap.add_argument("--rate", type=float, default=200.0, help="served stream: arrivals per second")
     ap.add_argument("--serve-oracle-dof", type=int, default=1000000,

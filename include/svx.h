@@ -243,6 +243,13 @@ int svx_set_load_hint(svx_ctx* ctx, int hint);
  * 1024; negative: never. Either way gives the same results bit for bit. No
  * reference counterpart: a launch knob. */
 int svx_set_step_kernel_tiles(svx_ctx* ctx, int32_t tiles);
+/* Node groups of the FP64 multi-RHS block SpMV (svx_solve_multi): one
+ * wavefront multiplies the block rows of `nodes` consecutive nodes in Morton
+ * order (4 or 8; 0: the default, 4), gathering each column of their union
+ * once; negative: no groups, the gathering SpMV that loads each column once
+ * per block referencing it. Every setting adds each row's entries in its
+ * column order. No reference counterpart: a launch knob. */
+int svx_set_multi_group(svx_ctx* ctx, int32_t nodes);
 /* SVX_OP_MATRIX_FREE once a matrix-free solve has prepared the system's
  * operator (and until its constraints change), else SVX_OP_ASSEMBLED; -1 for
  * NULL. No reference counterpart. */
@@ -259,6 +266,20 @@ int svx_system_operator(const svx_system* sys);
 int svx_mf_plan_check(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem,
                       const uint8_t* kinds, const int64_t* conn_ptr, const int32_t* conn_idx,
                       int64_t* stats, double* max_rel_diff);
+
+/* Host-only check of the multi-RHS block SpMV's node groups for a mesh (no
+ * device use): plans the groups (`group` 4 or 8; 0 for the default), checks
+ * that every node is in exactly one group and every CSR entry is packed
+ * exactly once, and evaluates the product of a seeded matrix on the mesh's
+ * pattern with 3 seeded columns group by group, indexed as the kernel indexes
+ * it, against the plain CSR product. stats[3] = {groups, union columns over
+ * all groups, node blocks}; *max_rel_diff = max |difference| / max |product|
+ * (unset, and the reason in svx_last_error, when the mesh has no plan:
+ * groups 0). SVX_E_INVALID_MODEL if a check fails. No reference counterpart:
+ * a test and diagnostics entry. */
+int svx_blk_plan_check(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem,
+                       const int64_t* conn_ptr, const int32_t* conn_idx, int32_t group,
+                       int64_t* stats, double* max_rel_diff);
 
 /* ---------------------------------------------------------------------------
  * Deck text — replaces solvix_io::parse_inp (crates/solvix-io/src/parser.rs:87)
@@ -467,6 +488,14 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol,
                     int64_t max_iter, int precision, double* U,
                     int64_t* iterations, double* residuals, int32_t* converged,
                     double* elapsed_ms, svx_error* err);
+/* Timing of the FP64 block solve's SpMV (with its fused direction step) for
+ * `nrhs` load cases on this system, in the launch shape svx_solve_multi
+ * uses: `reps` launches on the context's stream after one warm launch,
+ * *avg_us the average per launch (HIP events), *bytes its algorithmic bytes
+ * (8 nnz + 4 nblk + 32 n nrhs + 8 n: matrix values, block columns, r and the
+ * previous direction in, the diagonal, p' and q out). No reference
+ * counterpart: a measurement entry. */
+int svx_multi_spmv_timing(svx_system* sys, int32_t nrhs, int32_t reps, double* avg_us, double* bytes);
 
 /* ---------------------------------------------------------------------------
  * Measurement hook (no reference counterpart): average duration of the

@@ -29,6 +29,7 @@ SIGNATURES = [
     ("svx_set_operator", C.c_int, [_vp, C.c_int]),
     ("svx_system_operator", C.c_int, [_vp]),
     ("svx_mf_plan_check", C.c_int, [C.c_int, _i64, _dp, _i64, C.POINTER(C.c_uint8), _i64p, _i32p, _i64p, _dp]),
+    ("svx_blk_plan_check", C.c_int, [C.c_int, _i64, _dp, _i64, _i64p, _i32p, C.c_int32, _i64p, _dp]),
     ("svx_model_reaction_count", _i64, [C.POINTER(_abi.SvxModelDesc)]),
     ("svx_solve_model", C.c_int, [_vp, C.POINTER(_abi.SvxModelDesc), C.c_int, C.c_int,
                                   C.POINTER(_abi.SvxSolution), _errp]),
@@ -68,6 +69,8 @@ SIGNATURES = [
     ("svx_set_tile_order", C.c_int, [_vp, C.c_int]),
     ("svx_set_load_hint", C.c_int, [_vp, C.c_int]),
     ("svx_set_step_kernel_tiles", C.c_int, [_vp, C.c_int32]),
+    ("svx_set_multi_group", C.c_int, [_vp, C.c_int32]),
+    ("svx_multi_spmv_timing", C.c_int, [_vp, C.c_int32, C.c_int32, _dp, _dp]),
     ("svx_server_create", C.c_int, [_vp, C.POINTER(_abi.SvxServerConfig), C.POINTER(_vp)]),
     ("svx_server_destroy", None, [_vp]),
     ("svx_server_submit", C.c_int, [_vp, C.POINTER(_abi.SvxModelDesc), _i64p]),

@@ -74,6 +74,11 @@ class Context:
         negative: never); same bits either way."""
         _raise(lib().svx_set_step_kernel_tiles(self._h, int(tiles)))
 
+    def set_multi_group(self, nodes: int = 0):
+        """Nodes per group of the multi-RHS block SpMV: 4 or 8 (0: default
+        4), negative: no groups (the gathering SpMV)."""
+        _raise(lib().svx_set_multi_group(self._h, int(nodes)))
+
     def set_load_hint(self, hint: str):
         """'auto' (default: nontemporal matrix loads once a launch's FP64 matrix
         values exceed 256 MiB), 'cached' or 'streaming' for the FP64 assembled
@@ -231,6 +236,24 @@ def mf_plan_check(ndof: int, coords, kinds, conn_ptr, conn_idx):
     return st, (diff.value if st["kind"] else None)
 
 
+def blk_plan_check(ndof: int, coords, conn_ptr, conn_idx, group: int = 0):
+    """Host-only check of the multi-RHS block SpMV's node groups (see
+    svx_blk_plan_check): returns (stats dict, max relative difference of the
+    group-wise product against the CSR product, or None when the mesh has no
+    plan)."""
+    coords = np.ascontiguousarray(coords, np.float64)
+    conn_ptr = np.ascontiguousarray(conn_ptr, np.int64)
+    conn_idx = np.ascontiguousarray(conn_idx, np.int32)
+    stats = np.zeros(3, np.int64)
+    diff = C.c_double(-1.0)
+    rc = lib().svx_blk_plan_check(ndof, coords.shape[0], _d(coords), conn_ptr.size - 1,
+                                  _abi.ptr(conn_ptr, C.c_int64), _abi.ptr(conn_idx, C.c_int32), int(group),
+                                  _abi.ptr(stats, C.c_int64), C.byref(diff))
+    _raise(rc)
+    st = dict(zip(("groups", "cols", "blocks"), (int(v) for v in stats)))
+    return st, (diff.value if st["groups"] else None)
+
+
 class Mesh:
     """Device-resident mesh with its CSR pattern (built once per mesh)."""
 
@@ -324,6 +347,13 @@ class System:
         if rc != _abi.SVX_OK and (raise_on_failure or rc != _abi.SVX_E_NOT_CONVERGED):
             _raise(rc, err)
         return U, its, res, ok.astype(bool), ms.value
+
+    def multi_spmv_timing(self, nrhs: int = 64, reps: int = 100):
+        """(average us per launch, algorithmic bytes per launch) of the FP64
+        block solve's SpMV for `nrhs` load cases (svx_multi_spmv_timing)."""
+        us, by = C.c_double(), C.c_double()
+        _raise(lib().svx_multi_spmv_timing(self._h, int(nrhs), int(reps), C.byref(us), C.byref(by)))
+        return us.value, by.value
 
     @property
     def operator(self) -> str:

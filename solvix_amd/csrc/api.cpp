@@ -12,6 +12,7 @@
 #include <unordered_set>
 
 #include "api_internal.hpp"
+#include "block.hpp"
 
 using namespace svx;
 
@@ -1053,6 +1054,12 @@ int svx_set_step_kernel_tiles(svx_ctx* ctx, int32_t tiles) {
     return SVX_OK;
 }
 
+int svx_set_multi_group(svx_ctx* ctx, int32_t nodes) {
+    if (!ctx || !(nodes <= 0 || nodes == 4 || nodes == 8)) return SVX_E_ARGUMENT;
+    ctx->tune.multi_group = nodes < 0 ? -1 : nodes;
+    return SVX_OK;
+}
+
 int svx_set_load_hint(svx_ctx* ctx, int hint) {
     if (!ctx || (hint != SVX_LOADS_AUTO && hint != SVX_LOADS_CACHED && hint != SVX_LOADS_STREAMING))
         return SVX_E_ARGUMENT;
@@ -1085,6 +1092,33 @@ int svx_mf_plan_check(int ndof, int64_t n_nodes, const double* coords, int64_t n
             return SVX_OK;
         }
         const double d = verify_mf_plan(plan, n_nodes, coords, n_elem, conn_ptr, conn_idx);
+        if (max_rel_diff) *max_rel_diff = d;
+        return SVX_OK;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SVX_E_INVALID_MODEL;
+    }
+}
+
+int svx_blk_plan_check(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const int64_t* conn_ptr,
+                       const int32_t* conn_idx, int32_t group, int64_t* stats, double* max_rel_diff) {
+    if (ndof < 1 || n_nodes <= 0 || n_elem < 0 || !coords || !conn_ptr || !conn_idx || !stats ||
+        !(group == 0 || group == 4 || group == 8))
+        return SVX_E_ARGUMENT;
+    try {
+        HostPattern hp;
+        build_pattern(ndof, n_nodes, n_elem, conn_ptr, conn_idx, hp);
+        BlkPlan plan;
+        build_blk_plan(n_nodes, coords, hp, group > 0 ? group : kGroupDefault, plan);
+        stats[0] = (int64_t)(plan.grp.size() / 4);
+        stats[1] = plan.cols;
+        stats[2] = (int64_t)hp.bcol.size();
+        if (*plan.why) {
+            stats[0] = 0;
+            g_last_error = plan.why;
+            return SVX_OK;
+        }
+        const double d = verify_blk_plan(plan, hp, 3);
         if (max_rel_diff) *max_rel_diff = d;
         return SVX_OK;
     } catch (const std::exception& e) {

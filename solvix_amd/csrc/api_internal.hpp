@@ -24,6 +24,7 @@ struct TuneParams {
     int tile_order = SVX_TILES_XCD;  // order of the gathering kernels' tiles over the XCDs
     int load_hint = SVX_LOADS_AUTO;  // FP64 ELL SpMV matrix loads
     int step_tiles = 0;              // EngineShape::step_tiles; 0: the engine's default
+    int multi_group = 0;  // multi-RHS block SpMV: nodes per group (0: default, -1: gathering SpMV)
 };
 
 // Grow-only device scratch of one context for the temporaries of a call
@@ -98,6 +99,10 @@ struct svx_mesh {
     svx::DevBuf<double> mf_xy;
     svx::DevBuf<uint8_t> mf_rank;  // C3D8: rank of each (element, corner) among the corner node's elements
     int32_t mf_width = 0;          // C3D8: most elements at one node
+    // multi-RHS block SpMV node groups, planned on the first block solve for
+    // the group size in force (blk_group); blk_groups 0: no plan
+    int32_t blk_group = 0, blk_groups = 0;
+    svx::DevBuf<int32_t> blk_grp, blk_nodes, blk_ucol, blk_pk_src, blk_pk_stride;
 };
 
 struct svx_system {

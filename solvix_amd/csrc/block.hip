@@ -1,20 +1,32 @@
 // Multi-RHS Jacobi-PCG: R <= 64 load cases on one shared matrix.
 //
-// Vectors are stored row-major [n][R]: one 64-wide wavefront owns one matrix
-// row and its lane j owns load case j. The SpMV reads each row's values and
-// column indices once (wave-uniform loads) and multiplies them with R
-// right-hand sides at once, so the matrix streams from HBM once per iteration
-// for all load cases (block SpMV). Every load case runs the recurrence of
+// Vectors are stored row-major [n][R]: lane j of a 64-wide wavefront owns load
+// case j of whichever row the wavefront is on. The SpMV reads each row's
+// values and column indices once (wave-uniform loads) and multiplies them
+// with all R right-hand sides at once, so the matrix streams from HBM once per
+// iteration for every load case. Each load case runs the recurrence of
 // solver.rs:82-149 with its own alpha, beta, residual and convergence mask.
 //
-// Plane and solid meshes use a node-row SpMV (one wavefront per node, each
-// gathered row of P serving the node's ndof rows).
-// One iteration is five launches: SpMV (partial p.q per tile and load case),
-// a one-workgroup reduction (alpha, breakdown), the update of u, r, z
-// (partials r.r, r.z), a second reduction (residual, convergence, beta) and
-// the p update. Reductions are per load case in a fixed order (lane-local
-// sums, tiles strided over the reduction's waves, waves in order), with no
-// atomics or fences; the kernel boundaries order the data.
+// One iteration is six launches:
+//  1. SpMV with the pending direction step fused in: p' = z + beta p with
+//     z = r / diag evaluated from r wherever a row of p' is needed (p is
+//     double-buffered: iteration parity `par` reads P[par] and the tile that
+//     owns a row stores its p' into P[1 - par]), q = K p', partial p'.q per
+//     tile and load case;
+//  2-3. a first-level reduction over groups of tiles, then one wavefront:
+//     alpha, breakdown (solver.rs:118-123);
+//  4. the update of u and r (z is not stored), partials r.r and r.z;
+//  5-6. a first-level reduction, then one wavefront: residual, convergence,
+//     budget and beta (solver.rs:126-148).
+// Reductions are per load case in a fixed order (first-level groups of
+// consecutive tiles with the group's waves taking tiles in turn, then the
+// groups in order), with no atomics or fences; kernel boundaries order the
+// data.
+//
+// SpMV forms: meshes with a group plan take the group SpMV
+// (blk_spmv_group_kernel, one wavefront per group of nodes); otherwise the
+// node-row gathering SpMV (one wavefront per node) or, for scalar systems,
+// the row form.
 #include "block.hpp"
 
 namespace svx {
@@ -22,8 +34,6 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kBlock / kWave;  // rows per block pass
-constexpr int kReduceThreads = 1024;
-constexpr int kReduceWaves = kReduceThreads / kWave;
 
 // Tile of an SpMV workgroup: with xcd set, each of the eight XCDs (which the
 // hardware deals workgroups to round robin) takes a contiguous run of tiles,
@@ -36,7 +46,11 @@ __device__ __forceinline__ int spmv_tile(int xcd) {
     return x * q + (x < r ? x : r) + b / 8;
 }
 
-// Per-load-case partials of this block (waves summed in order) into tile's slot.
+// Direction step of one row for this lane's load case: p' = z + beta p with
+// z = r / diag, the values pupdate-style PCG stores (solver.rs:139-143).
+__device__ __forceinline__ double dir_step(double beta, double p, double r, double d) { return fma(beta, p, r * d); }
+
+// Per-load-case partials of this 256-thread block (waves summed in order) into tile's slot.
 __device__ __forceinline__ void write_rhs_partials(double a, double b, double* partials, int tile, int n_tiles,
                                                    int nv) {
     __shared__ double sa[kWavesPerBlock][kWave], sb[kWavesPerBlock][kWave];
@@ -53,32 +67,48 @@ __device__ __forceinline__ void write_rhs_partials(double a, double b, double* p
     }
 }
 
-// Sums over tiles of slot s (0 or 1) for every load case: wave w adds tiles
-// w, w + 16, ... in turn; the 16 wave sums are then added in order.
-__device__ __forceinline__ double reduce_slot(const double* __restrict__ partials, int n_tiles, int s) {
-    __shared__ double sh[kReduceWaves][kWave];
-    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const double* base = partials + (size_t)s * n_tiles * kWave;
-    double acc = 0.0;
-    int t = w;
-    for (; t + 3 * kReduceWaves < n_tiles; t += 4 * kReduceWaves) {  // four loads in flight
-        const double v0 = base[(size_t)t * kWave + lane];
-        const double v1 = base[(size_t)(t + kReduceWaves) * kWave + lane];
-        const double v2 = base[(size_t)(t + 2 * kReduceWaves) * kWave + lane];
-        const double v3 = base[(size_t)(t + 3 * kReduceWaves) * kWave + lane];
-        acc += v0;
-        acc += v1;
-        acc += v2;
-        acc += v3;
-    }
-    for (; t < n_tiles; t += kReduceWaves) acc += base[(size_t)t * kWave + lane];
-    sh[w][lane] = acc;
-    __syncthreads();
-    double sum = sh[0][lane];
+// Sum of the first-level sums of slot s for this lane's load case, groups in order.
+__device__ __forceinline__ double l1_sum(const double* __restrict__ l1, int s, int lane) {
+    const double* b = l1 + (size_t)s * kReduceGroups * kWave + lane;
+    double v[kReduceGroups];
 #pragma unroll
-    for (int i = 1; i < kReduceWaves; ++i) sum += sh[i][lane];
-    __syncthreads();
-    return sum;
+    for (int g = 0; g < kReduceGroups; ++g) v[g] = b[g * kWave];
+    double acc = v[0];
+#pragma unroll
+    for (int g = 1; g < kReduceGroups; ++g) acc += v[g];
+    return acc;
+}
+
+// First-level sums: group g (a workgroup) adds the tiles [g T / G, (g + 1) T / G)
+// of each slot for every load case, its waves taking tiles in turn and then
+// added in order. `gate` (when given) skips the work once no case runs.
+__global__ __launch_bounds__(kBlock) void blk_reduce1_kernel(const double* __restrict__ partials, int n_tiles,
+                                                              int nslots, double* __restrict__ l1,
+                                                              const BlockScal* gate) {
+    if (gate && !gate->any_running) return;
+    __shared__ double sh[kWavesPerBlock][kWave];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave, g = blockIdx.x;
+    const int t0 = (int)((int64_t)g * n_tiles / kReduceGroups), t1 = (int)((int64_t)(g + 1) * n_tiles / kReduceGroups);
+    for (int s = 0; s < nslots; ++s) {
+        const double* base = partials + (size_t)s * n_tiles * kWave + lane;
+        double acc = 0.0;
+        int t = t0 + w;
+        for (; t + kWavesPerBlock < t1; t += 2 * kWavesPerBlock) {  // two loads in flight
+            const double v0 = base[(size_t)t * kWave], v1 = base[(size_t)(t + kWavesPerBlock) * kWave];
+            acc += v0;
+            acc += v1;
+        }
+        if (t < t1) acc += base[(size_t)t * kWave];
+        sh[w][lane] = acc;
+        __syncthreads();
+        if (w == 0) {
+            double sum = sh[0][lane];
+#pragma unroll
+            for (int i = 1; i < kWavesPerBlock; ++i) sum += sh[i][lane];
+            l1[((size_t)s * kReduceGroups + g) * kWave + lane] = sum;
+        }
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void blk_init_kernel(BlockArgs A) {
@@ -94,25 +124,24 @@ __global__ __launch_bounds__(kBlock) void blk_init_kernel(BlockArgs A) {
         double zi = fi * A.dinv[i];
         A.U[k] = 0.0;
         A.Rv[k] = fi;
-        A.Z[k] = zi;
-        A.P[k] = zi;
+        A.P[k] = 0.0;  // with beta = 0 the first direction step gives p' = z (solver.rs:90)
         ff += fi * fi;
         rz += fi * zi;
     }
     write_rhs_partials(ff, rz, A.partials, tile, A.n_vtiles, 2);
 }
 
-// One workgroup: initial scalars per load case.
-__global__ __launch_bounds__(kReduceThreads) void blk_init_finish_kernel(BlockArgs A) {
-    const double ff = reduce_slot(A.partials, A.n_vtiles, 0);
-    const double rz = reduce_slot(A.partials, A.n_vtiles, 1);
+// One wavefront: initial scalars per load case.
+__global__ void blk_init_finish_kernel(BlockArgs A) {
     const int lane = threadIdx.x;
-    if (lane >= kWave) return;
+    const double ff = l1_sum(A.l1, 0, lane);
+    const double rz = l1_sum(A.l1, 1, lane);
     const bool on = lane < A.R;
     if (on) {
         A.S->bnorm[lane] = sqrt(ff);
         A.S->rr[lane] = ff;
         A.S->rz[lane] = rz;
+        A.S->beta[lane] = 0.0;
         A.S->it[lane] = 0;
         A.S->status[lane] = (ff == 0.0) ? ST_CONVERGED : ST_RUNNING;  // solver.rs:93-95
         A.S->res[lane] = 0.0;
@@ -121,109 +150,214 @@ __global__ __launch_bounds__(kReduceThreads) void blk_init_finish_kernel(BlockAr
     if (lane == 0) A.S->any_running = m != 0ull;
 }
 
-__global__ __launch_bounds__(kBlock) void blk_spmv_kernel(BlockArgs A) {
-    if (!A.S->any_running) return;
-    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int tile = spmv_tile(A.xcd);
-    const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
-    const bool on = lane < A.R;
-    const int R = A.R;
-    double pq = 0.0;
-    for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
-        double acc = 0.0;
-        const int32_t kb = A.row_ptr[i], ke = A.row_ptr[i + 1];
-        if (on) {
-            int32_t k = kb;
-            for (; k + 3 < ke; k += 4) {  // four gathers in flight
-                const int32_t c0 = A.col[k], c1 = A.col[k + 1], c2 = A.col[k + 2], c3 = A.col[k + 3];
-                const double v0 = A.val[k], v1 = A.val[k + 1], v2 = A.val[k + 2], v3 = A.val[k + 3];
-                const double p0 = A.P[(size_t)c0 * R + lane], p1 = A.P[(size_t)c1 * R + lane];
-                const double p2 = A.P[(size_t)c2 * R + lane], p3 = A.P[(size_t)c3 * R + lane];
-                acc += v0 * p0;
-                acc += v1 * p1;
-                acc += v2 * p2;
-                acc += v3 * p3;
-            }
-            for (; k < ke; ++k) acc += A.val[k] * A.P[(size_t)A.col[k] * R + lane];
-            size_t o = (size_t)i * R + lane;
-            A.Q[o] = acc;
-            pq += A.P[o] * acc;
-        }
-    }
-    write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
-}
-
-// Node-row form of the block SpMV for plane / solid meshes: one wavefront
-// owns the BS rows of one node, which share their columns, so each gathered
-// row of P (one load case per lane) serves all BS rows; the values are read
-// as node blocks of the CSR storage (row a of the node holds block k's
-// entries at val[row_ptr[node*BS] + a*BS*nb + BS*k + b]). Each row still sums
-// its entries in column order, so the results equal the row form's.
-// The node is uniform over the wavefront and the arrays reach the loop as
-// restrict-qualified parameters (the matrix is not written in this kernel),
-// so the block columns and matrix values (the same for all 64 load cases)
-// are scalar loads and the vector memory pipeline carries only the gathers
-// of P and the stores of Q.
-template <int BS>
-__device__ __forceinline__ double blk_node_rows(int32_t n0, int32_t n1, int w, int lane, int R, bool on,
-                                                const int32_t* __restrict__ row_ptr,
-                                                const int32_t* __restrict__ brow_ptr,
-                                                const int32_t* __restrict__ bcol, const double* __restrict__ val,
-                                                const double* __restrict__ P, double* __restrict__ Q) {
-    double pq = 0.0;
-    for (int32_t node = n0 + w; node < n1; node += kWavesPerBlock) {
-        double acc[BS];
+// ---------------------------------------------------------------- group SpMV
+// One wavefront per node group (consecutive nodes in Morton order, see
+// build_blk_plan): it walks the union of the members' block columns in
+// ascending order, evaluates each column's rows of p' once (gathered from r
+// and the previous direction, one load case per lane) and applies them to
+// every member block in that column. Per block row the blocks are added in
+// ascending column order, the order of the CSR row, so each row's sum is the
+// gathering SpMV's. The union column, its member mask, the diagonal and the
+// packed values are wave-uniform (scalar loads, one contiguous run of blocks
+// per group); the gathers are branch-free (lanes past the last load case
+// read that case's entries and discard them), so a column's gathers are in
+// flight together.
+template <int BS, int G>
+__device__ __forceinline__ double group_rows(int gi, int lane, int R, bool on, double beta,
+                                             const int32_t* __restrict__ grp, const int32_t* __restrict__ gnodes,
+                                             const int2* __restrict__ ucol, const double* __restrict__ pval,
+                                             const double* __restrict__ dinv, const double* __restrict__ Rv,
+                                             const double* __restrict__ Pold, double* __restrict__ Pnew,
+                                             double* __restrict__ Q) {
+    const int32_t c0 = grp[4 * gi], c1 = grp[4 * gi + 1], t0 = grp[4 * gi + 2];
+    const int ln = on ? lane : R - 1;
+    double acc[G][BS];
 #pragma unroll
-        for (int a = 0; a < BS; ++a) acc[a] = 0.0;
-        const int32_t kb = brow_ptr[node], nb = brow_ptr[node + 1] - kb;
-        const int32_t rs = row_ptr[node * BS];
-        if (on) {
-            if (row_ptr[node * BS + BS] - rs != BS * BS * nb) {  // isolated node: diagonal entries only
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int a = 0; a < BS; ++a) acc[g][a] = 0.0;
+    const double* __restrict__ v = pval + (size_t)t0 * BS * BS;
+    for (int32_t c = c0; c < c1; ++c) {
+        const int2 um = ucol[c];
+        double po[BS], rv[BS], dv[BS];
+#pragma unroll
+        for (int b = 0; b < BS; ++b) {
+            const int32_t row = um.x * BS + b;
+            const size_t o = (size_t)row * R + ln;
+            po[b] = Pold[o];
+            rv[b] = Rv[o];
+            dv[b] = dinv[row];
+        }
+        double p[BS];
+#pragma unroll
+        for (int b = 0; b < BS; ++b) p[b] = dir_step(beta, po[b], rv[b], dv[b]);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if ((um.y >> g) & 1) {
 #pragma unroll
                 for (int a = 0; a < BS; ++a)
-                    acc[a] = val[row_ptr[node * BS + a]] * P[(size_t)(node * BS + a) * R + lane];
-            } else {
-                const double* v0 = val + rs;
-#pragma unroll 2
-                for (int32_t k = 0; k < nb; ++k) {
-                    const int32_t j = bcol[kb + k];
-                    double p[BS];
 #pragma unroll
-                    for (int b = 0; b < BS; ++b) p[b] = P[(size_t)(j * BS + b) * R + lane];
-#pragma unroll
-                    for (int a = 0; a < BS; ++a)
-#pragma unroll
-                        for (int b = 0; b < BS; ++b) acc[a] += v0[a * BS * nb + BS * k + b] * p[b];
-                }
+                    for (int b = 0; b < BS; ++b) acc[g][a] += v[a * BS + b] * p[b];
+                v += BS * BS;
             }
+        }
+    }
+    double pq = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int32_t node = gnodes[gi * G + g];
+        if (node >= 0) {
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
-                const size_t o = (size_t)(node * BS + a) * R + lane;
-                Q[o] = acc[a];
-                pq += P[o] * acc[a];
+                const int32_t row = node * BS + a;
+                const size_t o = (size_t)row * R + ln;
+                const double own = dir_step(beta, Pold[o], Rv[o], dinv[row]);
+                if (on) {
+                    Pnew[o] = own;
+                    Q[o] = acc[g][a];
+                    pq += own * acc[g][a];
+                }
             }
         }
     }
     return pq;
 }
 
+template <int BS, int G>
+__global__ __launch_bounds__(kBlock) void blk_spmv_group_kernel(BlockArgs A, int par) {
+    if (!A.S->any_running) return;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+    const int tile = spmv_tile(A.xcd);
+    const int gi = tile * kWavesPerBlock + w;
+    const bool on = lane < A.R;
+    const double beta = (on && A.S->status[lane] == ST_RUNNING) ? A.S->beta[lane] : 0.0;
+    double pq = 0.0;
+    if (gi < A.n_groups)
+        pq = group_rows<BS, G>(gi, lane, A.R, on, beta, A.grp, A.gnodes, (const int2*)A.ucol, A.pval, A.dinv, A.Rv,
+                               par ? A.P2 : A.P, par ? A.P : A.P2, A.Q);
+    write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
+}
+
+// Block values in group order: entry (a, b) of packed block t is
+// val[src + a * stride + b], or, for a diagonal-only row (stride 0), its
+// diagonal val[src + a] when a == b and 0 otherwise.
 template <int BS>
-__global__ __launch_bounds__(kBlock) void blk_spmv_node_kernel(BlockArgs A) {
+__global__ void blk_pack_values_kernel(const double* __restrict__ val, const int32_t* __restrict__ src,
+                                       const int32_t* __restrict__ stride, int64_t nblk, double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk * BS * BS) return;
+    const int64_t blk = t / (BS * BS);
+    const int e = (int)(t % (BS * BS)), a = e / BS, b = e % BS;
+    const int32_t st = stride[blk], s0 = src[blk];
+    out[t] = st == 0 ? (a == b ? val[s0 + a] : 0.0) : val[(int64_t)s0 + (int64_t)a * st + b];
+}
+
+// ------------------------------------------------------------ gathering SpMV
+// Without a group plan: each gathered column's p' is evaluated from r and the
+// previous direction on the fly (the owner of the row stores the same value).
+__global__ __launch_bounds__(kBlock) void blk_spmv_kernel(BlockArgs A, int par) {
+    if (!A.S->any_running) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = spmv_tile(A.xcd);
+    const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
+    const bool on = lane < A.R;
+    const int R = A.R;
+    const double beta = (on && A.S->status[lane] == ST_RUNNING) ? A.S->beta[lane] : 0.0;
+    const double* __restrict__ Pold = par ? A.P2 : A.P;
+    double* __restrict__ Pnew = par ? A.P : A.P2;
+    auto pstep = [&](int32_t c) {
+        const size_t o = (size_t)c * R + lane;
+        return dir_step(beta, Pold[o], A.Rv[o], A.dinv[c]);
+    };
+    double pq = 0.0;
+    for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
+        double acc = 0.0;
+        const int32_t kb = A.row_ptr[i], ke = A.row_ptr[i + 1];
+        if (on) {
+            int32_t k = kb;
+            for (; k + 1 < ke; k += 2) {
+                const int32_t c0 = A.col[k], c1 = A.col[k + 1];
+                const double v0 = A.val[k], v1 = A.val[k + 1];
+                const double p0 = pstep(c0), p1 = pstep(c1);
+                acc += v0 * p0;
+                acc += v1 * p1;
+            }
+            for (; k < ke; ++k) acc += A.val[k] * pstep(A.col[k]);
+            const size_t o = (size_t)i * R + lane;
+            const double own = pstep(i);
+            Pnew[o] = own;
+            A.Q[o] = acc;
+            pq += own * acc;
+        }
+    }
+    write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
+}
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void blk_spmv_node_kernel(BlockArgs A, int par) {
     if (!A.S->any_running) return;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
     const int tile = spmv_tile(A.xcd);
     const int32_t r0 = tile * A.rows_per_tile, r1 = min(A.n, r0 + A.rows_per_tile);
-    const double pq = blk_node_rows<BS>(r0 / BS, r1 / BS, w, lane, A.R, lane < A.R, A.row_ptr, A.brow_ptr, A.bcol,
-                                        A.val, A.P, A.Q);
+    const int R = A.R;
+    const bool on = lane < R;
+    const double beta = (on && A.S->status[lane] == ST_RUNNING) ? A.S->beta[lane] : 0.0;
+    const double* __restrict__ Pold = par ? A.P2 : A.P;
+    double* __restrict__ Pnew = par ? A.P : A.P2;
+    const double* __restrict__ Rv = A.Rv;
+    const double* __restrict__ dinv = A.dinv;
+    const double* __restrict__ val = A.val;
+    const int32_t* __restrict__ row_ptr = A.row_ptr;
+    const int32_t* __restrict__ brow_ptr = A.brow_ptr;
+    const int32_t* __restrict__ bcol = A.bcol;
+    auto pstep = [&](int32_t c) {
+        const size_t o = (size_t)c * R + lane;
+        return dir_step(beta, Pold[o], Rv[o], dinv[c]);
+    };
+    double pq = 0.0;
+    for (int32_t node = r0 / BS + w; node < r1 / BS; node += kWavesPerBlock) {
+        double acc[BS];
+#pragma unroll
+        for (int a = 0; a < BS; ++a) acc[a] = 0.0;
+        const int32_t kb = brow_ptr[node], nb = brow_ptr[node + 1] - kb;
+        const int32_t rs = row_ptr[node * BS];
+        if (!on) continue;
+        if (row_ptr[node * BS + BS] - rs != BS * BS * nb) {  // a node of no element: diagonal only
+#pragma unroll
+            for (int a = 0; a < BS; ++a) acc[a] = val[row_ptr[node * BS + a]] * pstep(node * BS + a);
+        } else {
+            const double* v0 = val + rs;
+#pragma unroll 2
+            for (int32_t k = 0; k < nb; ++k) {
+                const int32_t j = bcol[kb + k];
+                double p[BS];
+#pragma unroll
+                for (int b = 0; b < BS; ++b) p[b] = pstep(j * BS + b);
+#pragma unroll
+                for (int a = 0; a < BS; ++a)
+#pragma unroll
+                    for (int b = 0; b < BS; ++b) acc[a] += v0[a * BS * nb + BS * k + b] * p[b];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < BS; ++a) {
+            const size_t o = (size_t)(node * BS + a) * R + lane;
+            const double own = pstep(node * BS + a);
+            Pnew[o] = own;
+            A.Q[o] = acc[a];
+            pq += own * acc[a];
+        }
+    }
     write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
 }
 
-// One workgroup: alpha per load case, breakdown test (solver.rs:118-123).
-__global__ __launch_bounds__(kReduceThreads) void blk_alpha_kernel(BlockArgs A) {
+// ------------------------------------------------------------- scalars, update
+// One wavefront: alpha per load case, breakdown test (solver.rs:118-123).
+__global__ void blk_alpha_kernel(BlockArgs A) {
     if (!A.S->any_running) return;
-    const double pq = reduce_slot(A.partials, A.n_tiles, 0);
     const int lane = threadIdx.x;
-    if (lane >= kWave || lane >= A.R || A.S->status[lane] != ST_RUNNING) return;
+    const double pq = l1_sum(A.l1, 0, lane);
+    if (lane >= A.R || A.S->status[lane] != ST_RUNNING) return;
     if (pq <= 0.0 || pq != pq) {
         A.S->it[lane] += 1;
         A.S->res[lane] = sqrt(A.S->rr[lane]) / A.S->bnorm[lane];
@@ -234,7 +368,7 @@ __global__ __launch_bounds__(kReduceThreads) void blk_alpha_kernel(BlockArgs A) 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void blk_update_kernel(BlockArgs A) {
+__global__ __launch_bounds__(kBlock) void blk_update_kernel(BlockArgs A, const double* __restrict__ Pn) {
     if (!A.S->any_running) return;
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int tile = blockIdx.x;
@@ -245,28 +379,23 @@ __global__ __launch_bounds__(kBlock) void blk_update_kernel(BlockArgs A) {
     if (on)
         for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
             size_t k = (size_t)i * A.R + lane;
-            A.U[k] += alpha * A.P[k];
+            A.U[k] += alpha * Pn[k];
             double ri = A.Rv[k] - alpha * A.Q[k];
             A.Rv[k] = ri;
             double zi = ri * A.dinv[i];
-            A.Z[k] = zi;
             rr += ri * ri;
             rz += ri * zi;
         }
     write_rhs_partials(rr, rz, A.partials, tile, A.n_vtiles, 2);
 }
 
-// One workgroup: residual, convergence, budget and beta per load case; the
+// One wavefront: residual, convergence, budget and beta per load case; the
 // running flag every kernel reads.
-__global__ __launch_bounds__(kReduceThreads) void blk_beta_kernel(BlockArgs A, int32_t* running_out) {
-    if (!A.S->any_running) {
-        if (threadIdx.x == 0 && running_out) *running_out = 0;
-        return;
-    }
-    const double rr = reduce_slot(A.partials, A.n_vtiles, 0);
-    const double rz = reduce_slot(A.partials, A.n_vtiles, 1);
+__global__ void blk_beta_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
     const int lane = threadIdx.x;
-    if (lane >= kWave) return;
+    const double rr = l1_sum(A.l1, 0, lane);
+    const double rz = l1_sum(A.l1, 1, lane);
     bool run = false;
     if (lane < A.R && A.S->status[lane] == ST_RUNNING) {
         A.S->rr[lane] = rr;
@@ -285,24 +414,7 @@ __global__ __launch_bounds__(kReduceThreads) void blk_beta_kernel(BlockArgs A, i
         }
     }
     unsigned long long m = __ballot(run);
-    if (lane == 0) {
-        A.S->any_running = m != 0ull;
-        if (running_out) *running_out = __popcll(m);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void blk_pupdate_kernel(BlockArgs A) {
-    if (!A.S->any_running) return;
-    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int tile = blockIdx.x;
-    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
-    const bool on = lane < A.R && A.S->status[lane] == ST_RUNNING;
-    if (!on) return;
-    const double beta = A.S->beta[lane];
-    for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
-        size_t k = (size_t)i * A.R + lane;
-        A.P[k] = A.Z[k] + beta * A.P[k];
-    }
+    if (lane == 0) A.S->any_running = m != 0ull;
 }
 
 // One lane per load case: the number of running load cases.
@@ -332,7 +444,45 @@ __global__ void blk_unpack_kernel(const double* __restrict__ in, double* __restr
     out[t] = in[(size_t)i * R + j];
 }
 
+void launch_blk_spmv(const BlockArgs& a, int par, hipStream_t s) {
+    if (a.group > 0) {
+        const int bs = a.bs, g = a.group;
+        if (bs == 3 && g == 8)
+            blk_spmv_group_kernel<3, 8><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+        else if (bs == 3)
+            blk_spmv_group_kernel<3, 4><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+        else if (bs == 2 && g == 8)
+            blk_spmv_group_kernel<2, 8><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+        else if (bs == 2)
+            blk_spmv_group_kernel<2, 4><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+        else if (g == 8)
+            blk_spmv_group_kernel<1, 8><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+        else
+            blk_spmv_group_kernel<1, 4><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+    } else if (a.bs == 3) {
+        blk_spmv_node_kernel<3><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+    } else if (a.bs == 2) {
+        blk_spmv_node_kernel<2><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+    } else {
+        blk_spmv_kernel<<<a.n_tiles, kBlock, 0, s>>>(a, par);
+    }
+}
+
 }  // namespace
+
+void blk_pack_values(const double* val, const int32_t* pk_src, const int32_t* pk_stride, int64_t nblk, int bs,
+                     double* pval, hipStream_t s) {
+    const int64_t tot = nblk * bs * bs;
+    if (tot == 0) return;
+    const unsigned grid = (unsigned)((tot + kBlock - 1) / kBlock);
+    if (bs == 3)
+        blk_pack_values_kernel<3><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
+    else if (bs == 2)
+        blk_pack_values_kernel<2><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
+    else
+        blk_pack_values_kernel<1><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
+    SVX_HIP(hipGetLastError());
+}
 
 void blk_pack(const double* in, double* out, int32_t n, int32_t R, const uint8_t* mask, hipStream_t s) {
     int64_t tot = (int64_t)n * R;
@@ -348,25 +498,19 @@ void blk_unpack(const double* in, double* out, int32_t n, int32_t R, hipStream_t
 
 void blk_init(const BlockArgs& a, hipStream_t s) {
     blk_init_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
-    blk_init_finish_kernel<<<1, kReduceThreads, 0, s>>>(a);
+    blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_vtiles, 2, a.l1, nullptr);
+    blk_init_finish_kernel<<<1, kWave, 0, s>>>(a);
     SVX_HIP(hipGetLastError());
 }
 
-void launch_blk_spmv(const BlockArgs& a, hipStream_t s) {
-    if (a.bs == 3)
-        blk_spmv_node_kernel<3><<<a.n_tiles, kBlock, 0, s>>>(a);
-    else if (a.bs == 2)
-        blk_spmv_node_kernel<2><<<a.n_tiles, kBlock, 0, s>>>(a);
-    else
-        blk_spmv_kernel<<<a.n_tiles, kBlock, 0, s>>>(a);
-}
-
-void blk_iteration(const BlockArgs& a, hipStream_t s) {
-    launch_blk_spmv(a, s);
-    blk_alpha_kernel<<<1, kReduceThreads, 0, s>>>(a);
-    blk_update_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
-    blk_beta_kernel<<<1, kReduceThreads, 0, s>>>(a, nullptr);
-    blk_pupdate_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
+void blk_iteration(const BlockArgs& a, int it, hipStream_t s) {
+    const int par = it & 1;
+    launch_blk_spmv(a, par, s);
+    blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_tiles, 1, a.l1, a.S);
+    blk_alpha_kernel<<<1, kWave, 0, s>>>(a);
+    blk_update_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a, par ? a.P : a.P2);
+    blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_vtiles, 2, a.l1, a.S);
+    blk_beta_kernel<<<1, kWave, 0, s>>>(a);
     SVX_HIP(hipGetLastError());
 }
 
@@ -376,7 +520,7 @@ void blk_count_running(const BlockArgs& a, int32_t* out, hipStream_t s) {
 }
 
 void blk_spmv_only(const BlockArgs& a, hipStream_t s) {
-    launch_blk_spmv(a, s);
+    launch_blk_spmv(a, 0, s);
     SVX_HIP(hipGetLastError());
 }
 

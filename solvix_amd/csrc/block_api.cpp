@@ -86,6 +86,101 @@ int solve_multi_ir32(svx_system* sys, int32_t R, const double* F, double tol, in
     return SVX_OK;
 }
 
+// The node-group plan of the block SpMV for the group size in force (built
+// once per mesh and size). Returns the group size, or 0 when the mesh takes
+// the gathering SpMV.
+int32_t ensure_blk_plan(svx_mesh* m, const TuneParams& t, hipStream_t s) {
+    if (t.multi_group < 0) return 0;
+    const int group = t.multi_group > 0 ? t.multi_group : kGroupDefault;
+    if (m->blk_group != group) {
+        BlkPlan B;
+        build_blk_plan(m->n_nodes, m->h_coords.data(), m->hp, group, B);
+        m->blk_grp.release(), m->blk_nodes.release(), m->blk_ucol.release();
+        m->blk_pk_src.release(), m->blk_pk_stride.release();
+        m->blk_groups = 0;
+        m->blk_group = group;
+        if (*B.why) return 0;
+        m->blk_grp.alloc(B.grp.size());
+        m->blk_nodes.alloc(B.nodes.size());
+        m->blk_ucol.alloc(B.ucol.size());
+        m->blk_pk_src.alloc(B.pk_src.size());
+        m->blk_pk_stride.alloc(B.pk_stride.size());
+        m->blk_grp.upload(B.grp.data(), B.grp.size(), s);
+        m->blk_nodes.upload(B.nodes.data(), B.nodes.size(), s);
+        m->blk_ucol.upload(B.ucol.data(), B.ucol.size(), s);
+        m->blk_pk_src.upload(B.pk_src.data(), B.pk_src.size(), s);
+        m->blk_pk_stride.upload(B.pk_stride.data(), B.pk_stride.size(), s);
+        SVX_HIP(hipStreamSynchronize(s));  // the host arrays go out of scope
+        m->blk_groups = (int32_t)(B.grp.size() / 4);
+    }
+    return m->blk_groups > 0 ? group : 0;
+}
+
+// Device state of one FP64 block solve over R load cases: packed loads,
+// vectors, the launch shape (group SpMV when the mesh has a plan) and the
+// per-case scalars.
+struct BlockWork {
+    DevBuf<double> Fin, Fp, Uv, Rv, P, P2, Q, partials, l1, pval;
+    DevBuf<uint8_t> dmask;
+    DevBuf<BlockScal> S;
+    BlockArgs a{};
+    BlockWork(svx_system* sys, int32_t R, double tol, int64_t max_iter) {
+        svx_mesh* m = sys->mesh;
+        svx_ctx* ctx = m->ctx;
+        hipStream_t s = ctx->stream;
+        const int32_t n = (int32_t)m->n;
+        system_prepare(sys, SVX_FP64);
+        const size_t nr = (size_t)n * R;
+        for (DevBuf<double>* b : {&Fin, &Fp, &Uv, &Rv, &P, &P2, &Q}) b->alloc(nr);
+        std::vector<uint8_t> mask(n, 0);
+        for (int64_t r : sys->cons_rows) mask[r] = 1;
+        dmask.alloc(n);
+        dmask.upload(mask.data(), n, s);
+        a.n = n;
+        a.R = R;
+        a.bs = m->ndof > 1 ? m->ndof : 1;
+        a.xcd = ctx->tune.tile_order == SVX_TILES_XCD;
+        // gathering SpMV (no patch plan): 48 rows = 16 / 24 nodes, 4 / 6 per
+        // wavefront (measured on the 64-case 61,347-DOF C3D8 solve: 140.0 ms
+        // against 147.1 ms with 96-row tiles and 164.6 ms with 24-row tiles).
+        // svx_set_tuning's rows per tile override, rounded to whole nodes.
+        const int rpt = ctx->tune.rows_per_tile > 0 ? ctx->tune.rows_per_tile : (a.bs > 1 ? 48 : 64);
+        a.rows_per_tile = (rpt + a.bs - 1) / a.bs * a.bs;
+        a.group = ensure_blk_plan(m, ctx->tune, s);
+        a.n_groups = a.group > 0 ? m->blk_groups : 0;
+        a.n_tiles = a.group > 0 ? (a.n_groups + 3) / 4 : (n + a.rows_per_tile - 1) / a.rows_per_tile;
+        a.grp = m->blk_grp.p, a.gnodes = m->blk_nodes.p, a.ucol = m->blk_ucol.p;
+        if (a.group > 0) {
+            pval.alloc((size_t)m->nblk * a.bs * a.bs);
+            blk_pack_values(sys->vals.p, m->blk_pk_src.p, m->blk_pk_stride.p, m->nblk, a.bs, pval.p, s);
+            a.pval = pval.p;
+        }
+        // vector kernels (initialisation, update) on tiles of twice the
+        // gathering SpMV's rows
+        a.vrows_per_tile = 2 * a.rows_per_tile;
+        a.n_vtiles = (n + a.vrows_per_tile - 1) / a.vrows_per_tile;
+        a.row_ptr = m->row_ptr.p;
+        a.brow_ptr = m->brow_ptr.p;
+        a.bcol = m->bcol.p;
+        a.col = m->col.p;
+        a.val = sys->vals.p;
+        a.dinv = sys->dinv.p;
+        a.F = Fp.p;
+        a.U = Uv.p, a.Rv = Rv.p, a.P = P.p, a.P2 = P2.p, a.Q = Q.p;
+        partials.alloc((size_t)std::max(a.n_tiles, a.n_vtiles) * 2 * kMaxRhs);
+        l1.alloc((size_t)2 * kReduceGroups * kMaxRhs);
+        a.partials = partials.p;
+        a.l1 = l1.p;
+        S.alloc(1);
+        BlockScal hs{};
+        hs.tol = tol;
+        hs.max_iter = max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * (int64_t)n);
+        S.upload(&hs, 1, s);
+        a.S = S.p;
+        SVX_HIP(hipStreamSynchronize(s));  // the host arrays go out of scope
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -118,61 +213,25 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
         hipEvent_t e0, e1;
         SVX_HIP(hipEventCreate(&e0));
         SVX_HIP(hipEventCreate(&e1));
-        system_prepare(sys, SVX_FP64);
+        BlockWork W(sys, R, tol, max_iter);
         const size_t nr = (size_t)n * R;
-        DevBuf<double> Fin(nr), Fp(nr), Uv(nr), Rv(nr), P(nr), Z(nr), Q(nr);
-        std::vector<uint8_t> mask(n, 0);
-        for (int64_t r : sys->cons_rows) mask[r] = 1;
-        DevBuf<uint8_t> dmask(n);
-        dmask.upload(mask.data(), n, s);
         SVX_HIP(hipEventRecord(e0, s));
-        Fin.upload(F, nr, s);
-        blk_pack(Fin.p, Fp.p, n, R, dmask.p, s);
-        BlockArgs a{};
-        a.n = n;
-        a.R = R;
-        a.bs = m->ndof > 1 ? m->ndof : 1;
-        a.xcd = ctx->tune.tile_order == SVX_TILES_XCD;
-        // 48 rows = 16 / 24 nodes, 4 / 6 per wavefront (measured on the
-        // 64-case 61,347-DOF C3D8 solve: 140.0 ms against 147.1 ms with 96-row
-        // tiles and 164.6 ms with 24-row tiles, whose partial sums the
-        // one-workgroup reductions then read twice over). svx_set_tuning's
-        // rows per tile override, rounded to whole nodes.
-        const int rpt = ctx->tune.rows_per_tile > 0 ? ctx->tune.rows_per_tile : (a.bs > 1 ? 48 : 64);
-        a.rows_per_tile = (rpt + a.bs - 1) / a.bs * a.bs;
-        a.n_tiles = (n + a.rows_per_tile - 1) / a.rows_per_tile;
-        // vector kernels on tiles of two SpMV tiles: half the partial sums for
-        // the one-workgroup reduction of r.r and r.z, and still enough
-        // workgroups to stream (measured with 48-row SpMV tiles: 96-row vector
-        // tiles give update 33 us and the r.r / r.z reduction 12 us per
-        // iteration, 192-row ones 48 us and 9 us)
-        a.vrows_per_tile = 2 * a.rows_per_tile;
-        a.n_vtiles = (n + a.vrows_per_tile - 1) / a.vrows_per_tile;
-        a.row_ptr = m->row_ptr.p;
-        a.brow_ptr = m->brow_ptr.p;
-        a.bcol = m->bcol.p;
-        a.col = m->col.p;
-        a.val = sys->vals.p;
-        a.dinv = sys->dinv.p;
-        a.F = Fp.p;
-        a.U = Uv.p, a.Rv = Rv.p, a.P = P.p, a.Z = Z.p, a.Q = Q.p;
-        DevBuf<double> partials((size_t)std::max(a.n_tiles, a.n_vtiles) * 2 * kMaxRhs);
-        DevBuf<BlockScal> S(1);
+        W.Fin.upload(F, nr, s);
+        blk_pack(W.Fin.p, W.Fp.p, n, R, W.dmask.p, s);
+        const BlockArgs& a = W.a;
+        DevBuf<double>& Fin = W.Fin;
+        DevBuf<double>& Uv = W.Uv;
+        DevBuf<BlockScal>& S = W.S;
         BlockScal hs{};
-        hs.tol = tol;
-        hs.max_iter = max_iter > 0 ? max_iter : std::max<int64_t>(1000, 10 * (int64_t)n);
-        S.upload(&hs, 1, s);
-        a.partials = partials.p;
-        a.S = S.p;
         blk_init(a, s);
-        const int chunk = ctx->tune.chunk;
+        const int chunk = (ctx->tune.chunk + 1) / 2 * 2;  // even: every chunk starts on the same direction buffer
         hipGraphExec_t exec = nullptr;
         if (ctx->tune.use_graph) {
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                 bool ok = true;
                 try {
-                    for (int i = 0; i < chunk; ++i) blk_iteration(a, s);
+                    for (int i = 0; i < chunk; ++i) blk_iteration(a, i, s);
                 } catch (const DeviceError&) {
                     ok = false;
                 }
@@ -187,7 +246,7 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
             if (exec)
                 SVX_HIP(hipGraphLaunch(exec, s));
             else
-                for (int i = 0; i < chunk; ++i) blk_iteration(a, s);
+                for (int i = 0; i < chunk; ++i) blk_iteration(a, i, s);
             blk_count_running(a, ctx->d_flag, s);
             SVX_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
             SVX_HIP(hipStreamSynchronize(s));
@@ -224,6 +283,43 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
             err->code = SVX_E_DEVICE;
             snprintf(err->message, sizeof err->message, "%s", e.what());
         }
+        return SVX_E_DEVICE;
+    }
+}
+
+int svx_multi_spmv_timing(svx_system* sys, int32_t nrhs, int32_t reps, double* avg_us, double* bytes) {
+    if (!sys || nrhs <= 0 || nrhs > kMaxRhs || reps <= 0 || !avg_us || !sys->dirichlet) return SVX_E_ARGUMENT;
+    try {
+        svx_mesh* m = sys->mesh;
+        hipStream_t s = m->ctx->stream;
+        const int32_t n = (int32_t)m->n;
+        BlockWork W(sys, nrhs, 1e-10, 0);
+        // seeded load cases: unit loads on every free row, scaled per case
+        std::vector<double> F((size_t)n * nrhs);
+        for (int32_t k = 0; k < nrhs; ++k)
+            for (int32_t i = 0; i < n; ++i) F[(size_t)k * n + i] = 1.0 + 0.01 * k + 1e-4 * (i % 97);
+        W.Fin.upload(F.data(), F.size(), s);
+        blk_pack(W.Fin.p, W.Fp.p, n, nrhs, W.dmask.p, s);
+        blk_init(W.a, s);
+        blk_spmv_only(W.a, s);  // warm
+        hipEvent_t e0, e1;
+        SVX_HIP(hipEventCreate(&e0));
+        SVX_HIP(hipEventCreate(&e1));
+        SVX_HIP(hipEventRecord(e0, s));
+        for (int32_t r = 0; r < reps; ++r) blk_spmv_only(W.a, s);
+        SVX_HIP(hipEventRecord(e1, s));
+        SVX_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        SVX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_us = 1e3 * ms / reps;
+        // matrix values, block columns (patch positions or node columns), r
+        // and the previous direction in, the diagonal, p' and q out
+        if (bytes)
+            *bytes = 8.0 * (double)m->nnz + 4.0 * (double)m->nblk + 32.0 * (double)n * nrhs + 8.0 * (double)n;
+        return SVX_OK;
+    } catch (const std::exception&) {
         return SVX_E_DEVICE;
     }
 }

@@ -142,6 +142,185 @@ void build_pattern(int ndof, int64_t n_nodes, int64_t n_elem, const int64_t* con
 }
 
 namespace {
+uint64_t spread3(uint64_t v) {  // bit i -> bit 3i, 21 bits
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+}  // namespace
+
+void build_blk_plan(int64_t n_nodes, const double* coords, const HostPattern& hp, int group, BlkPlan& B) {
+    B = BlkPlan{};
+    B.group = group;
+    if (n_nodes <= 0) {
+        B.why = "empty mesh";
+        return;
+    }
+    if (group < 1 || group > 8) {
+        B.why = "group size";
+        return;
+    }
+    const int BS = hp.ndof;
+    // Morton order of (x, y, z), 21 bits per axis over the bounding cube
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = coords[a];
+    for (int64_t i = 0; i < n_nodes; ++i)
+        for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], coords[3 * i + a]), hi[a] = std::max(hi[a], coords[3 * i + a]);
+    double span = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(span > 0.0)) span = 1.0;
+    const double scale = 2097151.0 / span;
+    std::vector<uint64_t> key(n_nodes);
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        uint64_t k = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double qv = std::min(2097151.0, std::max(0.0, (coords[3 * i + a] - lo[a]) * scale));
+            k |= spread3((uint64_t)qv) << a;
+        }
+        key[i] = k;
+    }
+    std::vector<int32_t> order(n_nodes);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    const int64_t ng = (n_nodes + group - 1) / group;
+    B.nodes.assign(ng * group, -1);
+    for (int64_t i = 0; i < n_nodes; ++i) B.nodes[i] = order[i];
+    B.grp.resize(ng * 4);
+    B.pk_src.reserve(hp.bcol.size());
+    B.pk_stride.reserve(hp.bcol.size());
+    std::vector<std::pair<int32_t, int32_t>> cm;  // (column, member)
+    for (int64_t g = 0; g < ng; ++g) {
+        cm.clear();
+        for (int m = 0; m < group; ++m) {
+            const int32_t v = B.nodes[g * group + m];
+            if (v < 0) break;
+            for (int32_t k = hp.brow_ptr[v]; k < hp.brow_ptr[v + 1]; ++k) cm.push_back({hp.bcol[k], m});
+        }
+        std::sort(cm.begin(), cm.end());
+        B.grp[g * 4 + 0] = (int32_t)(B.ucol.size() / 2);
+        B.grp[g * 4 + 2] = (int32_t)B.pk_src.size();
+        for (size_t q = 0; q < cm.size();) {
+            const int32_t col = cm[q].first;
+            int32_t mask = 0;
+            for (; q < cm.size() && cm[q].first == col; ++q) {
+                const int m = cm[q].second;
+                const int32_t v = B.nodes[g * group + m];
+                mask |= 1 << m;
+                // block k of node v: row a of v holds it at row_ptr[v*BS + a] + k*BS (full blocks);
+                // a diagonal-only row holds its diagonal at row_ptr[v*BS + a]
+                const int32_t kb = hp.brow_ptr[v], nb = hp.brow_ptr[v + 1] - kb;
+                const int32_t k = (int32_t)(std::lower_bound(hp.bcol.begin() + kb, hp.bcol.begin() + kb + nb, col) -
+                                            (hp.bcol.begin() + kb));
+                const int32_t rs = hp.row_ptr[(int64_t)v * BS];
+                const bool lone = hp.row_ptr[(int64_t)v * BS + BS] - rs != BS * BS * nb;
+                B.pk_src.push_back(lone ? rs : rs + k * BS);
+                B.pk_stride.push_back(lone ? 0 : BS * nb);
+            }
+            B.ucol.push_back(col);
+            B.ucol.push_back(mask);
+        }
+        B.grp[g * 4 + 3] = (int32_t)B.pk_src.size();
+        B.max_cols = std::max(B.max_cols, (int32_t)(B.ucol.size() / 2) - B.grp[g * 4 + 0]);
+        B.max_blocks = std::max(B.max_blocks, (int32_t)B.pk_src.size() - B.grp[g * 4 + 2]);
+        B.grp[g * 4 + 1] = (int32_t)(B.ucol.size() / 2);
+        if (B.ucol.size() / 2 > (size_t)INT32_MAX) throw std::runtime_error("block SpMV plan exceeds 2^31 entries");
+    }
+    B.cols = (int64_t)(B.ucol.size() / 2);
+}
+
+double verify_blk_plan(const BlkPlan& B, const HostPattern& hp, int R) {
+    auto fail = [](const char* what) { throw std::logic_error(std::string("block SpMV plan: ") + what); };
+    const int BS = hp.ndof, G = B.group;
+    const int64_t nn = hp.n_nodes, n = nn * BS;
+    const int64_t ng = (int64_t)B.grp.size() / 4;
+    if ((int64_t)B.nodes.size() != ng * G || ng * G < nn) fail("group count");
+    if (B.pk_src.size() != hp.bcol.size() || B.pk_stride.size() != hp.bcol.size()) fail("packed block count");
+    std::vector<int> member(nn, 0);
+    for (int64_t i = 0; i < ng * G; ++i) {
+        const int32_t v = B.nodes[i];
+        if (v >= nn) fail("node index");
+        if (v >= 0) member[v]++;
+    }
+    for (int64_t v = 0; v < nn; ++v)
+        if (member[v] != 1) fail("a node is not in exactly one group");
+    // the CSR entries the packed blocks read: every entry exactly once
+    std::vector<int> hit(hp.col.size(), 0);
+    for (size_t t = 0; t < B.pk_src.size(); ++t)
+        for (int a = 0; a < BS; ++a)
+            for (int b = 0; b < BS; ++b) {
+                if (B.pk_stride[t] == 0 && a != b) continue;
+                const int64_t e = B.pk_stride[t] == 0 ? B.pk_src[t] + a : B.pk_src[t] + (int64_t)a * B.pk_stride[t] + b;
+                if (e < 0 || e >= (int64_t)hp.col.size()) fail("packed entry out of range");
+                hit[e]++;
+            }
+    for (int h : hit)
+        if (h != 1) fail("a CSR entry is not packed exactly once");
+    uint64_t st = 0x9e3779b97f4a7c15ull;
+    auto next = [&]() {
+        st += 0x9e3779b97f4a7c15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return (double)((z ^ (z >> 31)) >> 11) / 9007199254740992.0 - 0.5;
+    };
+    std::vector<double> val(hp.col.size()), x((size_t)n * R), ref((size_t)n * R, 0.0), out((size_t)n * R, 0.0);
+    for (double& v : val) v = next();
+    for (double& v : x) v = next();
+    for (int64_t r = 0; r < n; ++r)
+        for (int32_t k = hp.row_ptr[r]; k < hp.row_ptr[r + 1]; ++k)
+            for (int c = 0; c < R; ++c) ref[(size_t)r * R + c] += val[k] * x[(size_t)hp.col[k] * R + c];
+    // packed values as the packing kernel writes them, then the wavefront's loop
+    std::vector<double> pv(B.pk_src.size() * BS * BS);
+    for (size_t t = 0; t < B.pk_src.size(); ++t)
+        for (int a = 0; a < BS; ++a)
+            for (int b = 0; b < BS; ++b)
+                pv[(t * BS + a) * BS + b] = B.pk_stride[t] == 0 ? (a == b ? val[B.pk_src[t] + a] : 0.0)
+                                                                : val[B.pk_src[t] + (int64_t)a * B.pk_stride[t] + b];
+    std::vector<double> acc((size_t)G * BS * R);
+    for (int64_t g = 0; g < ng; ++g) {
+        std::fill(acc.begin(), acc.end(), 0.0);
+        int32_t prev = -1;
+        int64_t t = B.grp[g * 4 + 2];
+        const int32_t nc = B.grp[g * 4 + 1] - B.grp[g * 4];
+        if (nc > B.max_cols) fail("union column cap");
+        for (int32_t c = B.grp[g * 4]; c < B.grp[g * 4 + 1]; ++c) {
+            const int32_t u = B.ucol[2 * c], m = B.ucol[2 * c + 1];
+            int members = 0;
+            while (members < G && B.nodes[g * G + members] >= 0) ++members;
+            if (u <= prev || u < 0 || u >= nn || m == 0 || (m >> members) != 0) fail("union column");
+            prev = u;
+            for (int mb = 0; mb < G; ++mb) {
+                if (!((m >> mb) & 1)) continue;
+                for (int a = 0; a < BS; ++a)
+                    for (int b = 0; b < BS; ++b)
+                        for (int c2 = 0; c2 < R; ++c2)
+                            acc[((size_t)mb * BS + a) * R + c2] +=
+                                pv[(t * BS + a) * BS + b] * x[((size_t)u * BS + b) * R + c2];
+                ++t;
+            }
+        }
+        const int64_t t_end = g + 1 < ng ? B.grp[(g + 1) * 4 + 2] : (int64_t)B.pk_src.size();
+        if (t != t_end || t != B.grp[g * 4 + 3] || t - B.grp[g * 4 + 2] > B.max_blocks) fail("packed block run");
+        for (int mb = 0; mb < G; ++mb) {
+            const int32_t v = B.nodes[g * G + mb];
+            if (v < 0) break;
+            for (int a = 0; a < BS; ++a)
+                for (int c2 = 0; c2 < R; ++c2)
+                    out[((size_t)v * BS + a) * R + c2] = acc[((size_t)mb * BS + a) * R + c2];
+        }
+    }
+    double dmax = 0.0, qmax = 0.0;
+    for (size_t i = 0; i < ref.size(); ++i) {
+        dmax = std::max(dmax, std::fabs(out[i] - ref[i]));
+        qmax = std::max(qmax, std::fabs(ref[i]));
+    }
+    return qmax > 0.0 ? dmax / qmax : dmax;
+}
+
+namespace {
 uint64_t spread_bits(uint64_t v) {  // bit i -> bit 2i, 32 bits
     v &= 0xffffffffull;
     v = (v | (v << 16)) & 0x0000ffff0000ffffull;

@@ -47,6 +47,35 @@ struct MfPlan {
 // (CPS3) or 4 (CPS4) elements per node; `hp` supplies the incidence lists.
 void build_mf_plan(int ndof, int64_t n_nodes, const double* coords, int64_t n_elem, const uint8_t* kinds,
                    const int64_t* conn_ptr, const int32_t* conn, const HostPattern& hp, MfPlan& out);
+// Node groups of the multi-RHS block SpMV (block.hip): runs of `group`
+// consecutive nodes in Morton order of their coordinates (a 2x2x1 patch of a
+// structured solid mesh for group 4). One wavefront multiplies a group's
+// block rows: it gathers the direction row of each column in the union of the
+// members' block columns once and applies it to every member block in that
+// column, instead of gathering it once per block. The matrix values are
+// packed group by group in that order (union column ascending, then member),
+// so a group's blocks are one contiguous run.
+struct BlkPlan {
+    int group = 0;
+    std::vector<int32_t> nodes;      // [n_groups * group] members in Morton order, -1 past the last node
+    std::vector<int32_t> grp;        // [n_groups][4]: first union column, end, first packed block, end
+    std::vector<int32_t> ucol;       // [n_cols][2]: column node (ascending within a group), member mask
+    std::vector<int32_t> pk_src;     // [nblk] CSR offset of packed block t's (0, 0) entry
+    std::vector<int32_t> pk_stride;  // [nblk] CSR offset between its rows (0: a diagonal-only row)
+    int64_t cols = 0;                // union columns over all groups
+    int32_t max_cols = 0, max_blocks = 0;  // the most union columns and blocks of one group
+    const char* why = "";            // non-empty: no plan
+};
+void build_blk_plan(int64_t n_nodes, const double* coords, const HostPattern& hp, int group, BlkPlan& out);
+
+// Host check of a block SpMV plan: every node a member of exactly one group,
+// every block (and CSR entry) packed exactly once, and the product of a seeded matrix on the
+// pattern with `R` seeded columns evaluated group by group the way the
+// kernel indexes it (union columns, member masks, packed values read through
+// pk_src / pk_stride) against the plain CSR product. Returns max |difference|
+// / max |product|; throws std::logic_error naming a violated invariant.
+double verify_blk_plan(const BlkPlan& plan, const HostPattern& hp, int R);
+
 // Host check of a plan: its structural limits, every node owned by exactly
 // one patch, and the product q = K x evaluated patch by patch the way the
 // kernel does (LDS staging replaced by arrays) against the element-by-element

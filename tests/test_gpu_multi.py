@@ -65,10 +65,79 @@ def test_multi_rhs_fp32_ir_matches_oracle_and_single_solves(ctx):
 
 
 
+@pytest.mark.parametrize("group", [8, -1])
+def test_multi_rhs_group_sizes(group):
+    """The group SpMV with groups of 8 nodes and the gathering SpMV (no
+    groups) add each row's entries in the same column order as the default
+    groups of 4 but group the p.q partial sums by other tiles, so they agree
+    with the default to the stopping tolerance, with the same iteration
+    counts within 2%, and with the oracle within 1e-8."""
+    from solvix_amd.core import Context
+    model, cases = decks.cfg4(nrhs=8, nx=24, ny=4, nz=4)
+    pos = {int(nid): i for i, nid in enumerate(model.node_ids)}
+    out = {}
+    for g in (0, group):
+        ctx = Context(0)
+        try:
+            ctx.set_multi_group(g)
+            mesh, sys_ = assemble(model, ctx)
+            sys_.apply_dirichlet(constrained_dofs(model)[0])
+            F = np.zeros((len(cases), mesh.n))
+            for k, (nodes, dofs, vals) in enumerate(cases):
+                for nid, d, v in zip(nodes, dofs, vals):
+                    F[k, pos[int(nid)] * 3 + int(d) - 1] += v
+            U, its, res, ok, _ = sys_.solve_multi(F)
+            assert ok.all() and (res <= 1e-10).all()
+            out[g] = (U, its)
+        finally:
+            ctx.close()
+    U0, its0 = out[0]
+    U, its = out[group]
+    assert (np.abs(its - its0) <= np.maximum(2, 0.02 * its0)).all() and rel_err(U, U0) <= 1e-8
+    m = decks.cfg4(nrhs=8, nx=24, ny=4, nz=4)[0]
+    m.cload_node, m.cload_dof, m.cload_value = cases[3]
+    assert rel_err(U[3], oracle.solve(m).u) <= 1e-8
+
+
+def test_multi_rhs_plane_mesh_matches_oracle(ctx):
+    """Plane mesh (2x2 node blocks) with a node of no element: each of 5 load
+    cases against its oracle solve."""
+    base = decks.quad_model(30, 6, 6.0, 1.2, 70e9, 0.3, 0.01, jitter=0.15, rng=decks.SplitMix64(21))
+    rng = np.random.default_rng(4)
+    left = decks.left_edge_ids(30, 6)
+    base.bcs = [(left, np.array([1, 2], np.int32))]
+    free = [int(i) for i in base.node_ids if int(i) not in set(int(v) for v in left)]
+    # a node of no element (diagonal-only rows, assembly.rs:300-302), unloaded
+    base.node_ids = np.append(base.node_ids, np.uint32(base.node_ids.max() + 1))
+    base.coords = np.vstack([base.coords, [[7.0, 0.5, 0.0]]])
+    cases = []
+    for k in range(5):
+        nodes = rng.choice(free, size=1 + k % 3, replace=False)
+        dofs = rng.integers(1, 3, size=nodes.size)
+        vals = rng.uniform(1e2, 1e4, size=nodes.size) * rng.choice([-1.0, 1.0], size=nodes.size)
+        cases.append((nodes.astype(np.uint32), dofs.astype(np.int32), vals))
+    mesh, sys_ = assemble(base, ctx)
+    sys_.apply_dirichlet(constrained_dofs(base)[0])
+    pos = {int(nid): i for i, nid in enumerate(base.node_ids)}
+    F = np.zeros((len(cases), mesh.n))
+    for k, (nodes, dofs, vals) in enumerate(cases):
+        for nid, d, v in zip(nodes, dofs, vals):
+            F[k, pos[int(nid)] * 2 + int(d) - 1] += v
+    U, its, res, ok, _ = sys_.solve_multi(F)
+    assert ok.all() and (res <= 1e-10).all()
+    for k, (nodes, dofs, vals) in enumerate(cases):
+        base.cload_node, base.cload_dof, base.cload_value = nodes, dofs, vals
+        ref = oracle.solve(base)
+        e = rel_err(U[k], ref.u)
+        print(f"case {k}: its {its[k]} (oracle {ref.iterations}) err {e:.1e}")
+        assert e <= 1e-8
+
+
 def test_multi_rhs_tile_sizes():
-    """Other tile sizes of the block solve regroup the per-tile partial sums
-    (rows per tile rounded to whole nodes), so they agree with the default to
-    the stopping tolerance and with the same iteration counts within 2%."""
+    """Other vector-kernel tile sizes of the block solve regroup the per-tile
+    partial sums (rows per tile rounded to whole nodes), so they agree with
+    the default to the stopping tolerance and with the same iteration counts
+    within 2%."""
     from solvix_amd.core import Context
     model, cases = decks.cfg4(nrhs=8, nx=24, ny=4, nz=4)
     pos = {int(nid): i for i, nid in enumerate(model.node_ids)}
