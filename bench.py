@@ -146,8 +146,9 @@ def bench_batch(args):
     models = decks.cfg3()
     t_setup = time.perf_counter()
     ctx = Context(0)
-    if args.ru_delta > 0:
-        ctx.set_tuning(ru_delta=args.ru_delta)
+    ctx.set_tuning(ru_delta=args.ru_delta, use_graph=not args.no_graph)
+    if args.batch_streams:
+        ctx.set_batch_streams(args.batch_streams)
     prep = []
     for m in models:
         E, nu = element_materials(m)
@@ -206,13 +207,13 @@ def bench_batch(args):
         for sy in systems:
             b.add(sy)
         t = tuned[(prec, mf)]  # the autotuned shape of this (2D-only) batch, node-block size 2
-        ctx.set_tuning(unroll=t["unroll"][2], rows_per_tile=t["rows_per_tile"][2])
+        ctx.set_tuning(unroll=t["unroll"][2], rows_per_tile=t["rows_per_tile"][2], use_graph=not args.no_graph)
         ctx.set_tile_order("xcd" if t["tile_order"][2] else "linear")
         ctx.set_operator("matrix_free" if mf else "assembled")
         try:
             res = b.solve(prec, raise_on_failure=False)
         finally:
-            ctx.set_tuning()
+            ctx.set_tuning(use_graph=not args.no_graph)
             ctx.set_tile_order("xcd")
             ctx.set_operator("assembled")
         clock("solve", t0)
@@ -483,6 +484,8 @@ def bench_single(args, config, emit=True):
     n_dofs = model.n_dofs()
     ctx = Context(0)
     ctx.set_operator(args.operator)
+    if args.no_graph:
+        ctx.set_tuning(use_graph=False)
     mesh = Mesh.from_model(model, ctx)
     E, nu = element_materials(model)
     ndof = model.ndof()
@@ -606,11 +609,10 @@ def spmv_rotation(args, mesh, arms, E, nu, model, load_rows, cons_rows):
         extra.append((mk, sk))
     res = {"systems": args.rotation, "reps": args.spmv_reps}
     ctx = mesh.ctx
-    # FP64: the load hint the automatic policy gives a launch whose matrix
-    # exceeds the cache (streaming), and ordinary loads beside it; the FP32
-    # SpMV has one form
+    # the load hint the automatic policy gives a launch whose matrix exceeds
+    # the cache (streaming), and ordinary loads beside it, in both precisions
     runs = (("fp64", Precision.FP64, "streaming"), ("fp64", Precision.FP64, "cached"),
-            ("fp32_ir", Precision.FP32_IR, None))
+            ("fp32_ir", Precision.FP32_IR, "streaming"), ("fp32_ir", Precision.FP32_IR, "cached"))
     for label, prec, hint in runs:
         if hint:
             ctx.set_load_hint(hint)
@@ -961,8 +963,11 @@ def main():
                     help="cfg2 (and its SpMV profile) on a C3D8 block of this mesh size, same element size")
     ap.add_argument("--plane-size", default=None, metavar="NXxNY",
                     help="cfg1 only: the cantilever on an NX x NY mesh (profiling at other sizes)")
+    ap.add_argument("--batch-streams", type=int, default=0,
+                    help="batched workload: engines run side by side on their own streams (0: library default)")
     ap.add_argument("--no-graph", action="store_true",
-                    help="multi-RHS workload: enqueue each chunk kernel by kernel instead of as a captured graph")
+                    help="solves enqueue each chunk kernel by kernel instead of as a captured graph "
+                         "(profiler runs)")
     ap.add_argument("--multi-group", type=int, default=0, choices=[-1, 0, 4, 8],
                     help="multi-RHS workload: nodes per group of the block SpMV (0: default, -1: no groups)")
     ap.add_argument("--multi-arm", default="both", choices=["both", "single", "block", "block_fp32_ir", "spmv"],

@@ -99,14 +99,15 @@ enum { SVX_OP_ASSEMBLED = 0, SVX_OP_MATRIX_FREE = 1 };
  * Either order gives the same results bit for bit. */
 enum { SVX_TILES_LINEAR = 0, SVX_TILES_XCD = 1 };
 
-/* How the FP64 assembled SpMV loads the matrix values (svx_set_load_hint).
+/* How the assembled ELL SpMVs (FP64, and FP32 in FP32-IR) load the matrix
+ * values (svx_set_load_hint).
  * SVX_LOADS_STREAMING: nontemporal loads; the values, read once per product,
  *                      do not evict the gathered vectors from the caches;
  * SVX_LOADS_CACHED:    ordinary loads, so a matrix that fits the 256 MB
  *                      last-level cache is read from there on the next product;
- * SVX_LOADS_AUTO:      streaming when the FP64 matrix values one launch reads
- *                      (all its instances of one node-block size) exceed
- *                      256 MiB, else cached.
+ * SVX_LOADS_AUTO:      streaming when the matrix values one launch reads in
+ *                      its precision (all its instances of one node-block
+ *                      size) exceed 256 MiB, else cached.
  * Every hint gives the same results bit for bit. */
 enum { SVX_LOADS_AUTO = 0, SVX_LOADS_CACHED = 1, SVX_LOADS_STREAMING = 2 };
 
@@ -232,7 +233,7 @@ int svx_set_operator(svx_ctx* ctx, int op);
 /* Tile order of this context's subsequent solves (default SVX_TILES_XCD).
  * No reference counterpart: a launch-shape knob the autotuner also sets. */
 int svx_set_tile_order(svx_ctx* ctx, int order);
-/* Load hint of this context's subsequent FP64 solves (default
+/* Load hint of this context's subsequent solves' assembled SpMVs (default
  * SVX_LOADS_AUTO). No reference counterpart: a launch knob. */
 int svx_set_load_hint(svx_ctx* ctx, int hint);
 /* Assembled 2D / 3D systems whose FP64 solve has more row tiles than this
@@ -250,6 +251,14 @@ int svx_set_step_kernel_tiles(svx_ctx* ctx, int32_t tiles);
  * per block referencing it. Every setting adds each row's entries in its
  * column order. No reference counterpart: a launch knob. */
 int svx_set_multi_group(svx_ctx* ctx, int32_t nodes);
+/* Streams of a batch solve (svx_batch_solve): the instances are split over
+ * `streams` engines (1..8; 0: the default, 2), each on its own stream, and one
+ * chunk of iterations is enqueued on every stream before any is polled, so
+ * the engines' kernels run side by side (one set's latency-bound products
+ * beside another's bandwidth-bound updates). Every instance runs the same
+ * operations as with one engine: the same bits. No reference counterpart:
+ * a launch knob. */
+int svx_set_batch_streams(svx_ctx* ctx, int32_t streams);
 /* SVX_OP_MATRIX_FREE once a matrix-free solve has prepared the system's
  * operator (and until its constraints change), else SVX_OP_ASSEMBLED; -1 for
  * NULL. No reference counterpart. */

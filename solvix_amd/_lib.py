@@ -70,6 +70,7 @@ SIGNATURES = [
     ("svx_set_load_hint", C.c_int, [_vp, C.c_int]),
     ("svx_set_step_kernel_tiles", C.c_int, [_vp, C.c_int32]),
     ("svx_set_multi_group", C.c_int, [_vp, C.c_int32]),
+    ("svx_set_batch_streams", C.c_int, [_vp, C.c_int32]),
     ("svx_multi_spmv_timing", C.c_int, [_vp, C.c_int32, C.c_int32, _dp, _dp]),
     ("svx_server_create", C.c_int, [_vp, C.POINTER(_abi.SvxServerConfig), C.POINTER(_vp)]),
     ("svx_server_destroy", None, [_vp]),

@@ -79,6 +79,11 @@ class Context:
         4), negative: no groups (the gathering SpMV)."""
         _raise(lib().svx_set_multi_group(self._h, int(nodes)))
 
+    def set_batch_streams(self, streams: int = 0):
+        """Engines a batch solve runs side by side, each on its own stream
+        (0: the default); the same bits for every setting."""
+        _raise(lib().svx_set_batch_streams(self._h, int(streams)))
+
     def set_load_hint(self, hint: str):
         """'auto' (default: nontemporal matrix loads once a launch's FP64 matrix
         values exceed 256 MiB), 'cached' or 'streaming' for the FP64 assembled

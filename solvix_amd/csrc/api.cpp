@@ -513,7 +513,8 @@ void system_engine_setup(svx_system* sys, double tol, int64_t max_iter, int prec
     shape.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
     shape.matrix_free = tp.matrix_free;
     shape.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
-    shape.nt[bs] = streaming_loads({sys}, bs, tp);
+    shape.nt[bs] = streaming_loads({sys}, bs, tp, 8);
+    shape.nt32[bs] = streaming_loads({sys}, bs, tp, 4);
     if (tp.step_tiles != 0) shape.step_tiles = tp.step_tiles;
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;

@@ -12,6 +12,9 @@
 namespace svx {
 
 // Launch-shape and refinement knobs (0 = automatic choice).
+constexpr int kBatchStreamsDefault = 2;
+constexpr int kBatchStreamsMax = 8;
+
 struct TuneParams {
     int lanes = 0;          // lanes per row, raw CSR systems
     int unroll = 0;         // blocks in flight per lane, node-block (ELL) systems
@@ -25,6 +28,7 @@ struct TuneParams {
     int load_hint = SVX_LOADS_AUTO;  // FP64 ELL SpMV matrix loads
     int step_tiles = 0;              // EngineShape::step_tiles; 0: the engine's default
     int multi_group = 0;  // multi-RHS block SpMV: nodes per group (0: default, -1: gathering SpMV)
+    int batch_streams = 0;  // engines a batch solve runs side by side (0: default)
 };
 
 // Grow-only device scratch of one context for the temporaries of a call
@@ -152,11 +156,12 @@ int nodes_of(uint8_t kind);
 const char* reason_text(int reason);
 // The per-block-size launch shape every batched solve uses by default.
 EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp, int precision);
-// The load hint (EngineShape::nt) of the FP64 ELL SpMV over `systems`'
-// instances of node-block size bs: tp.load_hint, or for SVX_LOADS_AUTO
-// streaming once their FP64 ELL values exceed kStreamBytes.
+// The load hint (EngineShape::nt, nt32) of the FP64 (value_bytes 8) or
+// FP32 (4) ELL SpMV over `systems`' instances of node-block size bs:
+// tp.load_hint, or for SVX_LOADS_AUTO streaming once their ELL values in
+// that precision exceed kStreamBytes.
 constexpr int64_t kStreamBytes = int64_t(256) << 20;
-int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp);
+int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp, int value_bytes);
 
 // A model validated and resolved on the host (lib.rs:593-632 up to the
 // element pass): node positions, per-element material constants, the load

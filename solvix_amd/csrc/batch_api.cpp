@@ -14,6 +14,22 @@
 
 using namespace svx;
 
+// One of the engines a batch solve runs side by side: its own stream (lane 0
+// uses the context's), tile tables and running-count flags.
+struct BatchLane {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::unique_ptr<Engine> engine;
+    int32_t* h_flag = nullptr;
+    int32_t* d_flag = nullptr;
+    ~BatchLane() {
+        engine.reset();
+        if (h_flag) (void)hipHostFree(h_flag);
+        if (d_flag) (void)hipFree(d_flag);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 struct svx_batch {
     svx_ctx* ctx = nullptr;
     std::vector<svx_system*> sys;
@@ -22,6 +38,7 @@ struct svx_batch {
     DevBuf<InstScal> scal;
     std::vector<InstScal> h_scal;
     std::unique_ptr<Engine> engine;
+    std::vector<std::unique_ptr<BatchLane>> lanes;  // svx_batch_solve with more than one stream
     int precision = -1;
     EngineShape tuned_shape;   // from svx_batch_autotune
     int tuned_precision = -1;  // precision the tuned shape belongs to, -1 = none
@@ -30,12 +47,13 @@ struct svx_batch {
 
 namespace svx {
 
-int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp) {
+int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp, int value_bytes) {
     if (tp.load_hint != SVX_LOADS_AUTO) return tp.load_hint == SVX_LOADS_STREAMING;
     int64_t bytes = 0;
+    // the FP32 SpMV of FP32-IR multiplies the assembled FP32 copy whatever the operator
     for (const svx_system* s : systems)
-        if (s->mesh->ndof == bs && bs > 1 && !(tp.matrix_free && s->mf_ready))
-            bytes += s->mesh->ell_blocks * bs * bs * (int64_t)sizeof(double);
+        if (s->mesh->ndof == bs && bs > 1 && (value_bytes == 4 || !(tp.matrix_free && s->mf_ready)))
+            bytes += s->mesh->ell_blocks * bs * bs * (int64_t)value_bytes;
     return bytes > kStreamBytes;
 }
 
@@ -48,7 +66,8 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
         sh.lanes[bs] = tp.lanes > 0 ? tp.lanes : 16;  // CSR rows of the FP64 residual and warm start
         sh.unroll[bs] = tp.unroll > 0 ? tp.unroll : unroll_for(bs, precision);
         sh.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
-        sh.nt[bs] = streaming_loads(systems, bs, tp);
+        sh.nt[bs] = streaming_loads(systems, bs, tp, 8);
+        sh.nt32[bs] = streaming_loads(systems, bs, tp, 4);
     }
     if (tp.step_tiles != 0) sh.step_tiles = tp.step_tiles;
     sh.matrix_free = tp.matrix_free;  // FP32-IR: the operator of its FP64 phase
@@ -78,7 +97,58 @@ int guarded_b(svx_error* err, F&& fn) {
 }
 }  // namespace
 
+namespace {
+// Splits the batch over K engines on K streams: instances in decreasing size,
+// dealt to the lanes back and forth (0..K-1, K-1..0, ...) so each lane holds
+// a similar share of rows; each lane keeps its instances in batch order. The
+// lanes wait for the context stream's set-up work, and run_engines returns
+// with every lane's stream drained.
+void solve_lanes(svx_batch* b, const std::vector<InstDesc>& descs, const EngineShape& shape,
+                 const EngineOptions& opt, int K) {
+    svx_ctx* ctx = b->ctx;
+    while ((int)b->lanes.size() < K) {
+        auto l = std::make_unique<BatchLane>();
+        if (b->lanes.empty()) {
+            l->stream = ctx->stream;
+        } else {
+            SVX_HIP(hipStreamCreate(&l->stream));  // default priority
+            l->own_stream = true;
+        }
+        l->engine = std::make_unique<Engine>(l->stream);
+        SVX_HIP(hipHostMalloc((void**)&l->h_flag, sizeof(int32_t) * 4, hipHostMallocDefault));
+        SVX_HIP(hipMalloc((void**)&l->d_flag, sizeof(int32_t) * 4));
+        b->lanes.push_back(std::move(l));
+    }
+    const size_t n = descs.size();
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return descs[a].n > descs[c].n; });
+    std::vector<int> lane_of(n);
+    for (size_t r = 0; r < n; ++r) {
+        const size_t round = r / K, pos = r % K;
+        lane_of[order[r]] = (int)((round & 1) ? K - 1 - pos : pos);
+    }
+    SVX_HIP(hipStreamSynchronize(ctx->stream));  // batch set-up before any lane reads it
+    std::vector<EngineLane> run;
+    for (int k = 0; k < K; ++k) {
+        std::vector<InstDesc> mine;
+        for (size_t i = 0; i < n; ++i)
+            if (lane_of[i] == k) mine.push_back(descs[i]);
+        BatchLane& l = *b->lanes[k];
+        l.engine->set_instances(mine, std::vector<char>(mine.size(), 1), shape);
+        run.push_back(EngineLane{l.engine.get(), l.h_flag, l.d_flag});
+    }
+    run_engines(run, opt);
+}
+}  // namespace
+
 extern "C" {
+
+int svx_set_batch_streams(svx_ctx* ctx, int32_t streams) {
+    if (!ctx || streams < 0 || streams > kBatchStreamsMax) return SVX_E_ARGUMENT;
+    ctx->tune.batch_streams = streams;
+    return SVX_OK;
+}
 
 int svx_batch_create(svx_ctx* ctx, svx_batch** out) {
     if (!ctx || !out) return SVX_E_ARGUMENT;
@@ -199,14 +269,21 @@ int svx_batch_solve(svx_batch* b, int precision, svx_error* err) {
         EngineShape shape =
             b->tuned_precision == precision ? b->tuned_shape : batch_shape(b->sys, b->ctx->tune, precision);
         shape.matrix_free = b->ctx->tune.matrix_free;  // the operator in force now
-        b->engine->set_instances(descs, std::vector<char>(n, 1), shape);
         b->matrix_free = shape.matrix_free;
         EngineOptions opt;
         opt.precision = precision;
         opt.period = b->ctx->tune.period;
         opt.chunk = b->ctx->tune.chunk;
         opt.use_graph = b->ctx->tune.use_graph;
-        b->engine->run(opt, b->ctx->h_flag, b->ctx->d_flag);
+        const int K = (int)std::min<size_t>(n, (size_t)(b->ctx->tune.batch_streams > 0 ? b->ctx->tune.batch_streams
+                                                                                         : kBatchStreamsDefault));
+        // the batch's own engine holds every instance either way (the timing
+        // entries launch its tables)
+        b->engine->set_instances(descs, std::vector<char>(n, 1), shape);
+        if (K <= 1)
+            b->engine->run(opt, b->ctx->h_flag, b->ctx->d_flag);
+        else
+            solve_lanes(b, descs, shape, opt, K);
         b->precision = precision;
         b->h_scal.resize(n);
         b->scal.download(b->h_scal.data(), n, s);

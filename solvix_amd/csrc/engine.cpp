@@ -238,6 +238,7 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         k.bs = bs;
         k.xcd = shape_.xcd[bs];
         k.nt = shape_.nt[bs];
+        k.nt32 = shape_.nt32[bs];
         k.mf_tiles = bs == 2 ? mf_tiles_.p : nullptr;
         k.mf_leaders = bs == 2 ? mf_leaders_.p : bs == 3 ? mf_leaders_.p + n_lead2 : nullptr;
         k.n_mf_inst = bs == 2 ? n_lead2 : bs == 3 ? (int32_t)mf_leaders.size() - n_lead2 : 0;
@@ -283,6 +284,13 @@ void Engine::enqueue_chunk(const EngineOptions& opt, int iters) {
 int Engine::run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
     int left = run_phase(opt, h_flag, d_flag, max_chunks);
     if (left != 0 || opt.precision == SVX_FP64) return left;
+    if (!begin_fallback()) return 0;
+    EngineOptions o64 = opt;
+    o64.precision = SVX_FP64;
+    return run_phase(o64, h_flag, d_flag, -1);
+}
+
+bool Engine::begin_fallback() {
     // FP32-IR instances that stopped improving finish as FP64 PCG from their
     // current solution (their descriptors carry the FP64 vectors too).
     std::vector<InstScal> sc;
@@ -290,75 +298,126 @@ int Engine::run(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int 
     std::vector<char> fb(sc.size());
     bool any = false;
     for (size_t i = 0; i < sc.size(); ++i) any |= (fb[i] = sc[i].status == ST_FALLBACK) != 0;
-    if (!any) return 0;
+    if (!any) return false;
     launch_promote_fallback(descs_.p, (int32_t)h_descs_.size(), stream_);
     build_tiles(fb);
-    EngineOptions o64 = opt;
-    o64.precision = SVX_FP64;
-    return run_phase(o64, h_flag, d_flag, -1);
+    return true;
 }
 
-int Engine::run_phase(const EngineOptions& opt_in, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
-    EngineOptions opt = opt_in;
+namespace {
+// Chunk length of a phase: even, and a multiple of the (even) FP32-IR period.
+int phase_chunk(EngineOptions& opt) {
     opt.period = std::max(2, (opt.period + 1) / 2 * 2);  // even: the period ends with the direction in P[0]
     int chunk = std::max(2, (opt.chunk + 1) / 2 * 2);
     if (opt.precision != SVX_FP64) chunk = std::max(opt.period, (chunk / opt.period) * opt.period);
-    int done_chunks = 0;
-    for (;;) {
-        enqueue_init(opt);  // initialises FL_FRESH instances only
-        bool graph_ok = opt.use_graph && !arena_;  // instantiating a graph allocates
-        if (graph_ok && (exec_ == nullptr || exec_precision_ != opt.precision || exec_period_ != opt.period ||
-                         exec_chunk_ != chunk)) {
-            release_graph();
-            hipGraph_t g = nullptr;
-            if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    return chunk;
+}
+}  // namespace
+
+void Engine::enqueue_phase_chunk(const EngineOptions& opt_in, int32_t* h_flag, int32_t* d_flag) {
+    EngineOptions opt = opt_in;
+    const int chunk = phase_chunk(opt);
+    enqueue_init(opt);  // initialises FL_FRESH instances only
+    bool graph_ok = opt.use_graph && !arena_;  // instantiating a graph allocates
+    if (graph_ok && (exec_ == nullptr || exec_precision_ != opt.precision || exec_period_ != opt.period ||
+                     exec_chunk_ != chunk)) {
+        release_graph();
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            graph_ok = false;
+        } else {
+            bool enqueued = true;
+            try {
+                enqueue_chunk(opt, chunk);
+            } catch (const DeviceError&) {
+                enqueued = false;
+            }
+            if (hipStreamEndCapture(stream_, &g) != hipSuccess || g == nullptr || !enqueued) {
+                if (g) (void)hipGraphDestroy(g);
                 graph_ok = false;
             } else {
-                bool enqueued = true;
-                try {
-                    enqueue_chunk(opt, chunk);
-                } catch (const DeviceError&) {
-                    enqueued = false;
-                }
-                if (hipStreamEndCapture(stream_, &g) != hipSuccess || g == nullptr || !enqueued) {
-                    if (g) (void)hipGraphDestroy(g);
+                if (hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0) != hipSuccess) {
+                    exec_ = nullptr;
                     graph_ok = false;
-                } else {
-                    if (hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0) != hipSuccess) {
-                        exec_ = nullptr;
-                        graph_ok = false;
-                    }
-                    (void)hipGraphDestroy(g);
                 }
-            }
-            (void)hipGetLastError();
-            if (graph_ok) {
-                exec_precision_ = opt.precision;
-                exec_period_ = opt.period;
-                exec_chunk_ = chunk;
+                (void)hipGraphDestroy(g);
             }
         }
-        if (graph_ok)
-            SVX_HIP(hipGraphLaunch(exec_, stream_));
-        else
-            enqueue_chunk(opt, chunk);
-        launches_++;
+        (void)hipGetLastError();
+        if (graph_ok) {
+            exec_precision_ = opt.precision;
+            exec_period_ = opt.period;
+            exec_chunk_ = chunk;
+        }
+    }
+    if (graph_ok)
+        SVX_HIP(hipGraphLaunch(exec_, stream_));
+    else
+        enqueue_chunk(opt, chunk);
+    launches_++;
+    launch_count_running(descs_.p, (int32_t)h_descs_.size(), d_flag, stream_);
+    SVX_HIP(hipMemcpyAsync(h_flag, d_flag, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+}
+
+int Engine::settle_phase_chunk(const EngineOptions& opt, const int32_t* h_flag) {
+    const int running = h_flag[0], live_tiles = h_flag[1];
+    if (running == 0) return 0;
+    // Re-tile once finished instances hold a quarter of the tiles: each of
+    // their workgroups still costs a launch slot and a state load per phase.
+    if (opt.compact && tiled_instances_ > 1 && running < tiled_instances_ && 4 * live_tiles <= 3 * tiled_tiles_) {
+        std::vector<InstScal> sc;
+        read_scalars(sc);
+        std::vector<char> live(h_descs_.size());
+        for (size_t i = 0; i < sc.size(); ++i) live[i] = sc[i].status == ST_RUNNING;
+        build_tiles(live, true);
+    }
+    return running;
+}
+
+int Engine::run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
+    int done_chunks = 0;
+    for (;;) {
+        enqueue_phase_chunk(opt, h_flag, d_flag);
         done_chunks++;
-        launch_count_running(descs_.p, (int32_t)h_descs_.size(), d_flag, stream_);
-        SVX_HIP(hipMemcpyAsync(h_flag, d_flag, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
         if (!opt.wait) return -1;
         SVX_HIP(hipStreamSynchronize(stream_));
-        const int running = h_flag[0], live_tiles = h_flag[1];
+        const int running = settle_phase_chunk(opt, h_flag);
         if (running == 0) return 0;
         if (max_chunks >= 0 && done_chunks >= max_chunks) return running;
-        // Re-tile once finished instances hold a quarter of the tiles: each of
-        // their workgroups still costs a launch slot and a state load per phase.
-        if (opt.compact && tiled_instances_ > 1 && running < tiled_instances_ && 4 * live_tiles <= 3 * tiled_tiles_) {
-            std::vector<InstScal> sc;
-            read_scalars(sc);
-            std::vector<char> live(h_descs_.size());
-            for (size_t i = 0; i < sc.size(); ++i) live[i] = sc[i].status == ST_RUNNING;
-            build_tiles(live, true);
+    }
+}
+
+void run_engines(const std::vector<EngineLane>& lanes, const EngineOptions& opt) {
+    struct State {
+        EngineOptions opt;
+        bool done = false, fallback = false;
+    };
+    std::vector<State> st(lanes.size());
+    for (size_t k = 0; k < lanes.size(); ++k) {
+        st[k].opt = opt;
+        st[k].opt.wait = true;
+        st[k].done = lanes[k].engine->n_instances() == 0;
+    }
+    for (;;) {
+        bool any = false;
+        // one chunk on every lane's stream before any lane is polled, so the
+        // lanes' kernels run side by side
+        for (size_t k = 0; k < lanes.size(); ++k)
+            if (!st[k].done) {
+                lanes[k].engine->enqueue_phase_chunk(st[k].opt, lanes[k].h_flag, lanes[k].d_flag);
+                any = true;
+            }
+        if (!any) return;
+        for (size_t k = 0; k < lanes.size(); ++k) {
+            if (st[k].done) continue;
+            SVX_HIP(hipStreamSynchronize(lanes[k].engine->stream()));
+            if (lanes[k].engine->settle_phase_chunk(st[k].opt, lanes[k].h_flag) != 0) continue;
+            if (st[k].opt.precision != SVX_FP64 && !st[k].fallback && lanes[k].engine->begin_fallback()) {
+                st[k].fallback = true;
+                st[k].opt.precision = SVX_FP64;
+            } else {
+                st[k].done = true;
+            }
         }
     }
 }

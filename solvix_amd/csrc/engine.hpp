@@ -32,6 +32,7 @@ struct EngineShape {
     int rows_per_tile[4] = {0, 0, 0, 0};  // 0: per-instance default; block sizes 2, 3: whole 64-node-row chunks
     int xcd[4] = {1, 1, 1, 1};  // gathering kernels: XCD-contiguous tile order (1) or tile = workgroup (0)
     int nt[4] = {0, 0, 0, 0};   // FP64 ELL SpMV: nontemporal matrix loads (1) or cached (0)
+    int nt32[4] = {0, 0, 0, 0}; // FP32 ELL SpMV (FP32-IR): the same for the FP32 copy
     // Assembled instances with more row tiles than this take their FP64
     // pending direction step (r.r and r.z reduced over the instance's tiles,
     // convergence, budget, beta) in the step kernel, once per instance,
@@ -93,6 +94,15 @@ class Engine {
     int run_chunks(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks) {
         return run_phase(opt, h_flag, d_flag, max_chunks);
     }
+    // The steps of one chunk for a caller that drives several engines: enqueue
+    // it (with the running count copied to h_flag) without waiting; after the
+    // caller has synchronized the stream, settle it (re-tile as run_phase
+    // does between chunks; returns the running count). begin_fallback moves
+    // the FP32-IR instances that fell back into an FP64 phase (false: none).
+    void enqueue_phase_chunk(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag);
+    int settle_phase_chunk(const EngineOptions& opt, const int32_t* h_flag);
+    bool begin_fallback();
+    hipStream_t stream() const { return stream_; }
 
    private:
     int run_phase(const EngineOptions& opt, int32_t* h_flag, int32_t* d_flag, int max_chunks);
@@ -126,6 +136,20 @@ class Engine {
     template <typename T>
     void place(DevBuf<T>& b, size_t count);
 };
+
+// One engine of a set that runs side by side, each on its own stream with
+// its own running-count flags.
+struct EngineLane {
+    Engine* engine;
+    int32_t* h_flag;  // pinned host, 2 x int32
+    int32_t* d_flag;  // device, 2 x int32
+};
+// Run every lane's instances to the end (opt.precision, then the FP64 phase
+// of FP32-IR instances that fell back), enqueueing one chunk on every lane
+// before polling any, so that one lane's latency-bound products overlap
+// another lane's bandwidth-bound updates. Each instance runs exactly the
+// operations of a single-engine run, so its results are the same bits.
+void run_engines(const std::vector<EngineLane>& lanes, const EngineOptions& opt);
 
 int default_lanes(double avg_row_len);
 // Default tile of an instance (node-block size bs, n_rows rows) and ELL

@@ -779,10 +779,15 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P
     const int fs = cap_elem + 1;
     const int tid = threadIdx.x;
     const bool owner = tid < P.n_owned;
-    uint32_t code[MAXA];
+    // the owned node's incidence codes, two 16-bit codes per register
+    uint32_t code2[MAXA / 2];
 #pragma unroll
-    for (int k = 0; k < MAXA; ++k)
-        code[k] = (k < P.n_adj && owner) ? (uint32_t)M.adj[P.adj_off + k * kBlock + tid] : (uint32_t)(P.n_elem * 4);
+    for (int k = 0; k < MAXA; k += 2) {
+        const uint32_t lo = (k < P.n_adj && owner) ? (uint32_t)M.adj[P.adj_off + k * kBlock + tid] : (uint32_t)(P.n_elem * 4);
+        const uint32_t hi =
+            (k + 1 < P.n_adj && owner) ? (uint32_t)M.adj[P.adj_off + (k + 1) * kBlock + tid] : (uint32_t)(P.n_elem * 4);
+        code2[k / 2] = lo | (hi << 16);
+    }
     uint64_t ec[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
@@ -848,7 +853,8 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P
         double a0 = 0.0, a1 = 0.0;
 #pragma unroll
         for (int k = 0; k < MAXA; ++k) {
-            const uint32_t sl = code[k] >> 2, cr = code[k] & 3u;  // the sentinel reads the zero slot
+            const uint32_t ck = (code2[k / 2] >> (16 * (k & 1))) & 0xffffu;
+            const uint32_t sl = ck >> 2, cr = ck & 3u;  // the sentinel reads the zero slot
             a0 += fe[(2 * cr) * fs + sl];
             a1 += fe[(2 * cr + 1) * fs + sl];
         }
@@ -900,7 +906,7 @@ __global__ __launch_bounds__(kBlock) void cg64_mf_step_kernel(const Tile* __rest
 // mf_tiles[base + xcd_tile(xcd)]; its p.q partial goes to slot P_MPQ, column
 // base + xcd_tile(xcd).
 template <int NEN>
-__global__ __launch_bounds__(kBlock) void cg64_mf_spmv_kernel(const Tile* __restrict__ mf_tiles, int base,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void cg64_mf_spmv_kernel(const Tile* __restrict__ mf_tiles, int base,
                                                               const InstDesc* __restrict__ descs,
                                                               double* __restrict__ partials, int pstride, int par,
                                                               int force_dir, int cap_nodes, int cap_elem, int xcd) {
@@ -1201,7 +1207,7 @@ __global__ __launch_bounds__(kBlock) void cg64_chunk_end_kernel(const Tile* __re
 // times its maximum since the last refresh, reached tol, or went
 // kForcedRefresh iterations unchecked; at the budget the instance holds
 // (FL_HOLD) for the final FP64 check.
-template <int BS, int W>
+template <int BS, int W, bool NT>
 __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restrict__ tiles,
                                                            const InstDesc* __restrict__ descs,
                                                            double* __restrict__ partials, int pstride, int par,
@@ -1215,7 +1221,7 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
     const gptr<float> nxt = par ? D.p32 : D.p32_alt;
     double v[1];
     if (!force_dir && S->dir) {
-        v[0] = tile_spmv<BS, W, float, false>(t, D, D.ell_val32, D.bval, Plain<float>{cur}, nxt, D.q32);
+        v[0] = tile_spmv<BS, W, float, NT>(t, D, D.ell_val32, D.bval, Plain<float>{cur}, nxt, D.q32);
     } else {
         float beta = 1.0f;
         if (!force_dir) {
@@ -1243,7 +1249,7 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
             if (hold) return;
             beta = (float)beta_d;
         }
-        v[0] = tile_spmv<BS, W, float, false>(t, D, D.ell_val32, D.bval, Step<float>{D.z32, cur, beta}, nxt, D.q32);
+        v[0] = tile_spmv<BS, W, float, NT>(t, D, D.ell_val32, D.bval, Step<float>{D.z32, cur, beta}, nxt, D.q32);
     }
     const int slot[1] = {P_PQ};
     write_partials<1>(v, slot, partials, pstride, xcd_tile(xcd));
@@ -1662,17 +1668,31 @@ void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir)
 void launch_ir32_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {
     if (k.bs == 3) {
 #define L(U) \
-    ir32_spmv_kernel<3, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
-        SVX_UNROLL_DISPATCH(k.unroll, L)
+    ir32_spmv_kernel<3, U, false><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
+#define LN(U) \
+    ir32_spmv_kernel<3, U, true><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
+        if (k.nt32) {
+            SVX_UNROLL_DISPATCH(k.unroll, LN)
+        } else {
+            SVX_UNROLL_DISPATCH(k.unroll, L)
+        }
+#undef LN
 #undef L
     } else if (k.bs == 2) {
 #define L(U) \
-    ir32_spmv_kernel<2, U><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
-        SVX_UNROLL_DISPATCH(k.unroll, L)
+    ir32_spmv_kernel<2, U, false><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
+#define LN(U) \
+    ir32_spmv_kernel<2, U, true><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
+        if (k.nt32) {
+            SVX_UNROLL_DISPATCH(k.unroll, LN)
+        } else {
+            SVX_UNROLL_DISPATCH(k.unroll, L)
+        }
+#undef LN
 #undef L
     } else {
 #define L(LN) \
-    ir32_spmv_kernel<1, LN><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
+    ir32_spmv_kernel<1, LN, false><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par, force_dir, k.xcd)
         SVX_LANES_DISPATCH(k.lanes, L)
 #undef L
     }

@@ -17,7 +17,12 @@ struct MfMaterial {
 };
 
 // CPS4: sum over the Gauss points of t detJ B^T D B u with B = B~ / detJ
-// (quad_b), evaluated as (t / detJ) B~^T D (B~ u).
+// (quad_b), evaluated in contracted form: with the shape-function
+// derivatives dN_i = (dxi_i, deta_i), the Jacobian J = sum_i dN_i x_i^T and
+// the natural-coordinate gradient Gn = sum_i dN_i u_i^T, the rows of B~ are
+// adj(J) dN_i, so B~ u = adj(J) Gn (the strains times detJ) and the nodal
+// forces are dN_i^T (adj(J)^T S) with S the stress times t / detJ. The same
+// linear map as the stiffness btdb assembles, up to rounding.
 __host__ __device__ inline void mf_cps4_forces(const double (&x)[4][2], const double (&u)[4][2], const MfMaterial& m,
                                                double (&f)[4][2]) {
     const double g = 1.0 / __builtin_sqrt(3.0);
@@ -26,42 +31,36 @@ __host__ __device__ inline void mf_cps4_forces(const double (&x)[4][2], const do
 #pragma unroll
     for (int gp = 0; gp < 4; ++gp) {
         const double xi = (gp & 2) ? g : -g, eta = (gp & 1) ? g : -g;
-        double dxi[4], deta[4];
+        double j00 = 0.0, j01 = 0.0, j10 = 0.0, j11 = 0.0, gx0 = 0.0, gx1 = 0.0, ge0 = 0.0, ge1 = 0.0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            dxi[i] = 0.25 * cx[i] * (1.0 + eta * cy[i]);
-            deta[i] = 0.25 * cy[i] * (1.0 + xi * cx[i]);
-        }
-        double j00 = 0.0, j01 = 0.0, j10 = 0.0, j11 = 0.0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            j00 += dxi[i] * x[i][0];
-            j01 += dxi[i] * x[i][1];
-            j10 += deta[i] * x[i][0];
-            j11 += deta[i] * x[i][1];
+            const double dxi = 0.25 * cx[i] * (1.0 + eta * cy[i]), deta = 0.25 * cy[i] * (1.0 + xi * cx[i]);
+            j00 += dxi * x[i][0];
+            j01 += dxi * x[i][1];
+            j10 += deta * x[i][0];
+            j11 += deta * x[i][1];
+            gx0 += dxi * u[i][0];
+            gx1 += dxi * u[i][1];
+            ge0 += deta * u[i][0];
+            ge1 += deta * u[i][1];
         }
         const double det = j00 * j11 - j01 * j10;
-        double bx[4], by[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            bx[i] = j11 * dxi[i] - j01 * deta[i];
-            by[i] = j00 * deta[i] - j10 * dxi[i];
-        }
-        double exx = 0.0, eyy = 0.0, gxy = 0.0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            exx += bx[i] * u[i][0];
-            eyy += by[i] * u[i][1];
-            gxy += by[i] * u[i][0] + bx[i] * u[i][1];
-        }
+        // B~ u: rows of B~ are (j11 dxi - j01 deta, j00 deta - j10 dxi)
+        const double exx = j11 * gx0 - j01 * ge0;
+        const double eyy = j00 * ge1 - j10 * gx1;
+        const double gxy = (j00 * ge0 - j10 * gx0) + (j11 * gx1 - j01 * ge1);
         const double sc = m.t / det;
         const double sxx = sc * (m.c11 * exx + m.c12 * eyy);
         const double syy = sc * (m.c12 * exx + m.c11 * eyy);
         const double sxy = sc * (m.c33 * gxy);
+        // f_i += B~_i^T s = dxi (j11 sxx - j10 sxy, j11 sxy - j10 syy) + deta (j00 sxy - j01 sxx, j00 syy - j01 sxy)
+        const double t00 = j11 * sxx - j10 * sxy, t01 = j11 * sxy - j10 * syy;
+        const double t10 = j00 * sxy - j01 * sxx, t11 = j00 * syy - j01 * sxy;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            f[i][0] += bx[i] * sxx + by[i] * sxy;
-            f[i][1] += by[i] * syy + bx[i] * sxy;
+            const double dxi = 0.25 * cx[i] * (1.0 + eta * cy[i]), deta = 0.25 * cy[i] * (1.0 + xi * cx[i]);
+            f[i][0] += dxi * t00 + deta * t10;
+            f[i][1] += dxi * t01 + deta * t11;
         }
     }
 }

@@ -44,6 +44,42 @@ def test_batch_equals_solo_bitwise(precision, ctx):
         assert np.array_equal(u.view(np.uint64), u0.view(np.uint64)), m.name
 
 
+@pytest.mark.parametrize("precision,operator", [(Precision.FP64, "assembled"), (Precision.FP32_IR, "assembled"),
+                                                (Precision.FP64, "matrix_free"), (Precision.FP32_IR, "matrix_free")],
+                         ids=["fp64", "fp32ir", "fp64-mf", "fp32ir-mf"])
+def test_batch_streams_equal_one_stream_bitwise(precision, operator):
+    """A batch split over 3 engines on 3 streams (svx_set_batch_streams), 2D
+    and 3D instances of uneven sizes, gives every instance the bits of the
+    one-engine solve, in both precisions and operators (FP32-IR instances
+    that fall back finish in FP64 within their own lane)."""
+    from solvix_amd.core import Context
+    models = small_models() + [decks.plane_instance(decks.SplitMix64(0x53170005 ^ i), 2e3, 3e4) for i in range(7)]
+    out = {}
+    for streams in (1, 3):
+        ctx = Context(0)
+        try:
+            ctx.set_batch_streams(streams)
+            ctx.set_operator(operator)
+            keep = []
+            b = Batch(ctx)
+            for m in models:
+                mesh, s = _system(m, ctx)
+                keep.append((mesh, s))
+                b.add(s)
+            res = b.solve(precision)
+            out[streams] = [(r, s.solution()) for r, (mesh, s) in zip(res, keep)]
+            b.close()
+            for mesh, s in keep:
+                s.close()
+                mesh.close()
+        finally:
+            ctx.close()
+    for m, ((r1, u1), (r3, u3)) in zip(models, zip(out[1], out[3])):
+        assert r1[2] and r3[2], m.name
+        assert r1[0] == r3[0] and r1[1] == r3[1], m.name
+        assert np.array_equal(u1.view(np.uint64), u3.view(np.uint64)), m.name
+
+
 def test_batch_matches_oracle(ctx):
     models = [decks.plane_instance(decks.SplitMix64(0x53170003 ^ i), 2e3, 1.5e4) for i in range(8)]
     models.append(decks.cfg2(nx=12, ny=3, nz=3))
