@@ -497,13 +497,15 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol,
                     int64_t max_iter, int precision, double* U,
                     int64_t* iterations, double* residuals, int32_t* converged,
                     double* elapsed_ms, svx_error* err);
-/* Timing of the FP64 block solve's SpMV (with its fused direction step) for
- * `nrhs` load cases on this system, in the launch shape svx_solve_multi
- * uses: `reps` launches on the context's stream after one warm launch,
- * *avg_us the average per launch (HIP events), *bytes its algorithmic bytes
- * (8 nnz + 4 nblk + 32 n nrhs + 8 n: matrix values, block columns, r and the
- * previous direction in, the diagonal, p' and q out). No reference
- * counterpart: a measurement entry. */
+/* Timing of the FP64 block solve's SpMV kernel for `nrhs` load cases on
+ * this system, in the launch shape svx_solve_multi uses: `reps` launches on
+ * the context's stream after one warm iteration's SpMV, *avg_us the average
+ * per launch (HIP events), *bytes its algorithmic bytes. With node groups
+ * (svx_set_multi_group) the kernel is the group product of a direction the
+ * step pass already wrote: 8 nnz + 4 nblk + 16 n nrhs (matrix values, block
+ * columns, p' in, q out). The gathering SpMV evaluates the step itself:
+ * 8 nnz + 4 nblk + 32 n nrhs + 8 n (r and the previous direction in, the
+ * diagonal, p' and q out). No reference counterpart: a measurement entry. */
 int svx_multi_spmv_timing(svx_system* sys, int32_t nrhs, int32_t reps, double* avg_us, double* bytes);
 
 /* ---------------------------------------------------------------------------

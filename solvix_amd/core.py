@@ -355,7 +355,8 @@ class System:
 
     def multi_spmv_timing(self, nrhs: int = 64, reps: int = 100):
         """(average us per launch, algorithmic bytes per launch) of the FP64
-        block solve's SpMV for `nrhs` load cases (svx_multi_spmv_timing)."""
+        block solve's SpMV kernel for `nrhs` load cases (svx_multi_spmv_timing:
+        with node groups the product of the direction the step pass wrote)."""
         us, by = C.c_double(), C.c_double()
         _raise(lib().svx_multi_spmv_timing(self._h, int(nrhs), int(reps), C.byref(us), C.byref(by)))
         return us.value, by.value

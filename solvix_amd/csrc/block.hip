@@ -7,16 +7,16 @@
 // iteration for every load case. Each load case runs the recurrence of
 // solver.rs:82-149 with its own alpha, beta, residual and convergence mask.
 //
-// One iteration is six launches:
-//  1. SpMV with the pending direction step fused in: p' = z + beta p with
-//     z = r / diag evaluated from r wherever a row of p' is needed (p is
-//     double-buffered: iteration parity `par` reads P[par] and the tile that
-//     owns a row stores its p' into P[1 - par]), q = K p', partial p'.q per
-//     tile and load case;
-//  2-3. a first-level reduction over groups of tiles, then one wavefront:
+// One iteration is seven launches (six for the gathering SpMV):
+//  1. the pending direction step p' = z + beta p with z = r / diag (p is
+//     double-buffered: iteration parity `par` reads P[par] and writes p'
+//     into P[1 - par]); the gathering SpMV evaluates it on the fly instead,
+//     wherever a row of p' is needed, and the owner of a row stores it;
+//  2. SpMV q = K p', partial p'.q per tile and load case;
+//  3-4. a first-level reduction over groups of tiles, then one wavefront:
 //     alpha, breakdown (solver.rs:118-123);
-//  4. the update of u and r (z is not stored), partials r.r and r.z;
-//  5-6. a first-level reduction, then one wavefront: residual, convergence,
+//  5. the update of u and r (z is not stored), partials r.r and r.z;
+//  6-7. a first-level reduction, then one wavefront: residual, convergence,
 //     budget and beta (solver.rs:126-148).
 // Reductions are per load case in a fixed order (first-level groups of
 // consecutive tiles with the group's waves taking tiles in turn, then the
@@ -151,23 +151,102 @@ __global__ void blk_init_finish_kernel(BlockArgs A) {
 }
 
 // ---------------------------------------------------------------- group SpMV
+// With a group plan an iteration takes the direction step in its own pass
+// (blk_step_kernel: p' = z + beta p over the vector tiles, p double-buffered)
+// and the group SpMV gathers p' alone: one row of p' per union column and
+// dof, where evaluating the step on the fly would gather r and the previous
+// direction (twice the vector loads, and vector-load return is what bounds
+// the product).
+__global__ __launch_bounds__(kBlock) void blk_step_kernel(BlockArgs A, int par) {
+    if (!A.S->any_running) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
+    if (lane >= A.R) return;
+    const double beta = A.S->status[lane] == ST_RUNNING ? A.S->beta[lane] : 0.0;
+    const double* __restrict__ Pold = par ? A.P2 : A.P;
+    double* __restrict__ Pnew = par ? A.P : A.P2;
+    const double* __restrict__ Rv = A.Rv;
+    for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
+        const size_t k = (size_t)i * A.R + lane;
+        Pnew[k] = dir_step(beta, Pold[k], Rv[k], A.dinv[i]);
+    }
+}
+
+// Products of one union column's members OFF + i for the set bits i of the
+// 4-bit mask M (their blocks consecutive at v, in ascending position): every
+// block load is issued before the first product, so a column costs one
+// scalar-load round trip per nibble of its member mask.
+template <int BS, int G, int OFF, uint32_t M>
+__device__ __forceinline__ void col_products(const double* __restrict__ v, const double (&p)[BS],
+                                             double (&acc)[G][BS]) {
+    constexpr int NM = __builtin_popcount(M);
+    double kv[NM][BS * BS];
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+#pragma unroll
+        for (int e = 0; e < BS * BS; ++e) kv[k][e] = v[k * BS * BS + e];
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if ((M >> i) & 1u) {
+#pragma unroll
+            for (int a = 0; a < BS; ++a)
+#pragma unroll
+                for (int b = 0; b < BS; ++b) acc[OFF + i][a] += kv[k][a * BS + b] * p[b];
+            ++k;
+        }
+    }
+}
+
+// Members OFF..OFF + 3 of one column (mask nibble m); then the block pointer
+// moves past their blocks.
+template <int BS, int G, int OFF>
+__device__ __forceinline__ void col_nibble(const double* __restrict__& v, uint32_t m, const double (&p)[BS],
+                                           double (&acc)[G][BS]) {
+    switch (m) {
+#define SVX_COL(M)                                 \
+    case M:                                        \
+        col_products<BS, G, OFF, M>(v, p, acc);    \
+        break;
+        SVX_COL(1) SVX_COL(2) SVX_COL(3) SVX_COL(4) SVX_COL(5) SVX_COL(6) SVX_COL(7) SVX_COL(8)
+        SVX_COL(9) SVX_COL(10) SVX_COL(11) SVX_COL(12) SVX_COL(13) SVX_COL(14) SVX_COL(15)
+#undef SVX_COL
+        default:
+            break;
+    }
+    v += __builtin_popcount(m) * BS * BS;
+}
+
+// One union column with member mask m (groups of 4 or 8 nodes).
+template <int BS, int G>
+__device__ __forceinline__ void col_step(const double* __restrict__& v, uint32_t m, const double (&p)[BS],
+                                         double (&acc)[G][BS]) {
+    static_assert(G == 4 || G == 8, "groups of 4 or 8 nodes");
+    col_nibble<BS, G, 0>(v, m & 15u, p, acc);
+    if constexpr (G == 8) col_nibble<BS, G, 4>(v, m >> 4, p, acc);
+}
+
+template <int BS>
+__device__ __forceinline__ void gather_rows(const double* __restrict__ Pd, int32_t node, int R, int ln,
+                                            double (&p)[BS]) {
+#pragma unroll
+    for (int b = 0; b < BS; ++b) p[b] = Pd[(size_t)(node * BS + b) * R + ln];
+}
+
 // One wavefront per node group (consecutive nodes in Morton order, see
 // build_blk_plan): it walks the union of the members' block columns in
-// ascending order, evaluates each column's rows of p' once (gathered from r
-// and the previous direction, one load case per lane) and applies them to
-// every member block in that column. Per block row the blocks are added in
-// ascending column order, the order of the CSR row, so each row's sum is the
-// gathering SpMV's. The union column, its member mask, the diagonal and the
-// packed values are wave-uniform (scalar loads, one contiguous run of blocks
-// per group); the gathers are branch-free (lanes past the last load case
-// read that case's entries and discard them), so a column's gathers are in
-// flight together.
+// ascending order, gathers each column's rows of p' once (one load case per
+// lane) and applies them to every member block in that column. Per block row
+// the blocks are added in ascending column order, the order of the CSR row,
+// so each row's sum is the gathering SpMV's. The union column, its member
+// mask and the packed values are wave-uniform (scalar loads, one contiguous
+// run of blocks per group); the gathers are branch-free (lanes past the last
+// load case read that case's entries and discard them).
 template <int BS, int G>
-__device__ __forceinline__ double group_rows(int gi, int lane, int R, bool on, double beta,
-                                             const int32_t* __restrict__ grp, const int32_t* __restrict__ gnodes,
-                                             const int2* __restrict__ ucol, const double* __restrict__ pval,
-                                             const double* __restrict__ dinv, const double* __restrict__ Rv,
-                                             const double* __restrict__ Pold, double* __restrict__ Pnew,
+__device__ __forceinline__ double group_rows(int gi, int lane, int R, bool on, const int32_t* __restrict__ grp,
+                                             const int32_t* __restrict__ gnodes, const int2* __restrict__ ucol,
+                                             const double* __restrict__ pval, const double* __restrict__ Pd,
                                              double* __restrict__ Q) {
     const int32_t c0 = grp[4 * gi], c1 = grp[4 * gi + 1], t0 = grp[4 * gi + 2];
     const int ln = on ? lane : R - 1;
@@ -177,30 +256,24 @@ __device__ __forceinline__ double group_rows(int gi, int lane, int R, bool on, d
 #pragma unroll
         for (int a = 0; a < BS; ++a) acc[g][a] = 0.0;
     const double* __restrict__ v = pval + (size_t)t0 * BS * BS;
-    for (int32_t c = c0; c < c1; ++c) {
-        const int2 um = ucol[c];
-        double po[BS], rv[BS], dv[BS];
-#pragma unroll
-        for (int b = 0; b < BS; ++b) {
-            const int32_t row = um.x * BS + b;
-            const size_t o = (size_t)row * R + ln;
-            po[b] = Pold[o];
-            rv[b] = Rv[o];
-            dv[b] = dinv[row];
-        }
-        double p[BS];
-#pragma unroll
-        for (int b = 0; b < BS; ++b) p[b] = dir_step(beta, po[b], rv[b], dv[b]);
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            if ((um.y >> g) & 1) {
-#pragma unroll
-                for (int a = 0; a < BS; ++a)
-#pragma unroll
-                    for (int b = 0; b < BS; ++b) acc[g][a] += v[a * BS + b] * p[b];
-                v += BS * BS;
-            }
-        }
+    // Software pipeline over the union columns, two per trip with two row
+    // buffers: column c's products run while column c + 1's rows of p' and
+    // column c + 2's entry are in flight. Past the group's end the last
+    // column is loaded again, unused.
+    int2 ua = ucol[c0], ub = ucol[min(c0 + 1, c1 - 1)];
+    double pa[BS], pb[BS];
+    gather_rows<BS>(Pd, ua.x, R, ln, pa);
+    for (int32_t c = c0;; c += 2) {
+        gather_rows<BS>(Pd, ub.x, R, ln, pb);
+        const int2 ua2 = ucol[min(c + 2, c1 - 1)];
+        col_step<BS, G>(v, (uint32_t)ua.y, pa, acc);
+        if (c + 1 >= c1) break;
+        gather_rows<BS>(Pd, ua2.x, R, ln, pa);
+        const int2 ub2 = ucol[min(c + 3, c1 - 1)];
+        col_step<BS, G>(v, (uint32_t)ub.y, pb, acc);
+        if (c + 2 >= c1) break;
+        ua = ua2;
+        ub = ub2;
     }
     double pq = 0.0;
 #pragma unroll
@@ -209,11 +282,9 @@ __device__ __forceinline__ double group_rows(int gi, int lane, int R, bool on, d
         if (node >= 0) {
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
-                const int32_t row = node * BS + a;
-                const size_t o = (size_t)row * R + ln;
-                const double own = dir_step(beta, Pold[o], Rv[o], dinv[row]);
+                const size_t o = (size_t)(node * BS + a) * R + ln;
+                const double own = Pd[o];
                 if (on) {
-                    Pnew[o] = own;
                     Q[o] = acc[g][a];
                     pq += own * acc[g][a];
                 }
@@ -230,11 +301,10 @@ __global__ __launch_bounds__(kBlock) void blk_spmv_group_kernel(BlockArgs A, int
     const int tile = spmv_tile(A.xcd);
     const int gi = tile * kWavesPerBlock + w;
     const bool on = lane < A.R;
-    const double beta = (on && A.S->status[lane] == ST_RUNNING) ? A.S->beta[lane] : 0.0;
     double pq = 0.0;
     if (gi < A.n_groups)
-        pq = group_rows<BS, G>(gi, lane, A.R, on, beta, A.grp, A.gnodes, (const int2*)A.ucol, A.pval, A.dinv, A.Rv,
-                               par ? A.P2 : A.P, par ? A.P : A.P2, A.Q);
+        pq = group_rows<BS, G>(gi, lane, A.R, on, A.grp, A.gnodes, (const int2*)A.ucol, A.pval, par ? A.P : A.P2,
+                               A.Q);
     write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
 }
 
@@ -444,7 +514,9 @@ __global__ void blk_unpack_kernel(const double* __restrict__ in, double* __restr
     out[t] = in[(size_t)i * R + j];
 }
 
-void launch_blk_spmv(const BlockArgs& a, int par, hipStream_t s) {
+// The SpMV kernel of iteration parity `par` (with a group plan, the product
+// of the p' that blk_step_kernel wrote).
+void launch_blk_product(const BlockArgs& a, int par, hipStream_t s) {
     if (a.group > 0) {
         const int bs = a.bs, g = a.group;
         if (bs == 3 && g == 8)
@@ -466,6 +538,11 @@ void launch_blk_spmv(const BlockArgs& a, int par, hipStream_t s) {
     } else {
         blk_spmv_kernel<<<a.n_tiles, kBlock, 0, s>>>(a, par);
     }
+}
+
+void launch_blk_spmv(const BlockArgs& a, int par, hipStream_t s) {
+    if (a.group > 0) blk_step_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a, par);
+    launch_blk_product(a, par, s);
 }
 
 }  // namespace
@@ -521,6 +598,11 @@ void blk_count_running(const BlockArgs& a, int32_t* out, hipStream_t s) {
 
 void blk_spmv_only(const BlockArgs& a, hipStream_t s) {
     launch_blk_spmv(a, 0, s);
+    SVX_HIP(hipGetLastError());
+}
+
+void blk_product_only(const BlockArgs& a, hipStream_t s) {
+    launch_blk_product(a, 0, s);
     SVX_HIP(hipGetLastError());
 }
 

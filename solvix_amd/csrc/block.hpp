@@ -59,7 +59,12 @@ void blk_init(const BlockArgs& a, hipStream_t s);
 // Iteration `it` of a solve (its parity picks the direction buffers).
 void blk_iteration(const BlockArgs& a, int it, hipStream_t s);
 void blk_count_running(const BlockArgs& a, int32_t* out, hipStream_t s);
+// One SpMV of iteration 0 (with a group plan: the direction step, then the
+// product).
 void blk_spmv_only(const BlockArgs& a, hipStream_t s);
+// The SpMV kernel alone (with a group plan, the product of the p' the last
+// step wrote; the gathering SpMV evaluates the step itself).
+void blk_product_only(const BlockArgs& a, hipStream_t s);
 void blk_pack(const double* in, double* out, int32_t n, int32_t R, const uint8_t* mask, hipStream_t s);
 void blk_unpack(const double* in, double* out, int32_t n, int32_t R, hipStream_t s);
 // Packs the CSR values block by block in group order (BlockPlan's pk_src /

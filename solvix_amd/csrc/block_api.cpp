@@ -301,12 +301,12 @@ int svx_multi_spmv_timing(svx_system* sys, int32_t nrhs, int32_t reps, double* a
         W.Fin.upload(F.data(), F.size(), s);
         blk_pack(W.Fin.p, W.Fp.p, n, nrhs, W.dmask.p, s);
         blk_init(W.a, s);
-        blk_spmv_only(W.a, s);  // warm
+        blk_spmv_only(W.a, s);  // warm; with a group plan it also writes p
         hipEvent_t e0, e1;
         SVX_HIP(hipEventCreate(&e0));
         SVX_HIP(hipEventCreate(&e1));
         SVX_HIP(hipEventRecord(e0, s));
-        for (int32_t r = 0; r < reps; ++r) blk_spmv_only(W.a, s);
+        for (int32_t r = 0; r < reps; ++r) blk_product_only(W.a, s);
         SVX_HIP(hipEventRecord(e1, s));
         SVX_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;
@@ -314,10 +314,13 @@ int svx_multi_spmv_timing(svx_system* sys, int32_t nrhs, int32_t reps, double* a
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         *avg_us = 1e3 * ms / reps;
-        // matrix values, block columns (patch positions or node columns), r
-        // and the previous direction in, the diagonal, p' and q out
+        // group SpMV: matrix values, block columns, p' in and q out; the
+        // gathering SpMV also reads r, the previous direction and the
+        // diagonal and writes p'
         if (bytes)
-            *bytes = 8.0 * (double)m->nnz + 4.0 * (double)m->nblk + 32.0 * (double)n * nrhs + 8.0 * (double)n;
+            *bytes = W.a.group > 0 ? 8.0 * (double)m->nnz + 4.0 * (double)m->nblk + 16.0 * (double)n * nrhs
+                                   : 8.0 * (double)m->nnz + 4.0 * (double)m->nblk + 32.0 * (double)n * nrhs +
+                                         8.0 * (double)n;
         return SVX_OK;
     } catch (const std::exception&) {
         return SVX_E_DEVICE;
