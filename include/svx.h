@@ -488,10 +488,15 @@ int svx_batch_time_spmv(svx_batch* batch, int reps, double* avg_ms,
  * the solutions; iterations / residuals / converged are per load case;
  * elapsed_ms is device time from the upload of F to the download of U.
  * SVX_FP64: one block PCG, vectors [n][R], the matrix read once per
- * iteration for all cases. SVX_FP32_IR: every case runs the FP32-IR solve of
- * svx_solve (FP32 inner iterations, FP64 refinement, residual and final
- * test) as one instance of a batched engine over shared matrix copies; each
- * case's result equals its own svx_solve in SVX_FP32_IR bit for bit.
+ * iteration for all cases. SVX_FP32_IR on a mesh with node groups
+ * (svx_set_multi_group): a block FP32 phase (FP32 matrix and vectors, FP64
+ * dots) from u = 0 with an FP64 refresh every 64 iterations (u += the FP32
+ * correction, r = f - K u in FP64); a case leaves it at the first refresh
+ * whose true residual reaches tol or does not halve its best, or on a
+ * breakdown, and the FP64 block PCG finishes every case from its u to tol on
+ * the recursive residual (the FP32 phase never converges a case). Without
+ * node groups every case runs the FP32-IR solve of svx_solve as one instance
+ * of a batched engine over shared matrix copies, bit for bit that solve.
  * ------------------------------------------------------------------------- */
 int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol,
                     int64_t max_iter, int precision, double* U,

@@ -143,6 +143,7 @@ __global__ void blk_init_finish_kernel(BlockArgs A) {
         A.S->rz[lane] = rz;
         A.S->beta[lane] = 0.0;
         A.S->it[lane] = 0;
+        A.S->best[lane] = 1.0;  // ||f - K 0|| / ||f||
         A.S->status[lane] = (ff == 0.0) ? ST_CONVERGED : ST_RUNNING;  // solver.rs:93-95
         A.S->res[lane] = 0.0;
     }
@@ -177,11 +178,10 @@ __global__ __launch_bounds__(kBlock) void blk_step_kernel(BlockArgs A, int par) 
 // 4-bit mask M (their blocks consecutive at v, in ascending position): every
 // block load is issued before the first product, so a column costs one
 // scalar-load round trip per nibble of its member mask.
-template <int BS, int G, int OFF, uint32_t M>
-__device__ __forceinline__ void col_products(const double* __restrict__ v, const double (&p)[BS],
-                                             double (&acc)[G][BS]) {
+template <int BS, int G, int OFF, uint32_t M, typename T>
+__device__ __forceinline__ void col_products(const T* __restrict__ v, const T (&p)[BS], T (&acc)[G][BS]) {
     constexpr int NM = __builtin_popcount(M);
-    double kv[NM][BS * BS];
+    T kv[NM][BS * BS];
 #pragma unroll
     for (int k = 0; k < NM; ++k)
 #pragma unroll
@@ -201,13 +201,12 @@ __device__ __forceinline__ void col_products(const double* __restrict__ v, const
 
 // Members OFF..OFF + 3 of one column (mask nibble m); then the block pointer
 // moves past their blocks.
-template <int BS, int G, int OFF>
-__device__ __forceinline__ void col_nibble(const double* __restrict__& v, uint32_t m, const double (&p)[BS],
-                                           double (&acc)[G][BS]) {
+template <int BS, int G, int OFF, typename T>
+__device__ __forceinline__ void col_nibble(const T* __restrict__& v, uint32_t m, const T (&p)[BS], T (&acc)[G][BS]) {
     switch (m) {
 #define SVX_COL(M)                                 \
     case M:                                        \
-        col_products<BS, G, OFF, M>(v, p, acc);    \
+        col_products<BS, G, OFF, M, T>(v, p, acc); \
         break;
         SVX_COL(1) SVX_COL(2) SVX_COL(3) SVX_COL(4) SVX_COL(5) SVX_COL(6) SVX_COL(7) SVX_COL(8)
         SVX_COL(9) SVX_COL(10) SVX_COL(11) SVX_COL(12) SVX_COL(13) SVX_COL(14) SVX_COL(15)
@@ -219,17 +218,15 @@ __device__ __forceinline__ void col_nibble(const double* __restrict__& v, uint32
 }
 
 // One union column with member mask m (groups of 4 or 8 nodes).
-template <int BS, int G>
-__device__ __forceinline__ void col_step(const double* __restrict__& v, uint32_t m, const double (&p)[BS],
-                                         double (&acc)[G][BS]) {
+template <int BS, int G, typename T>
+__device__ __forceinline__ void col_step(const T* __restrict__& v, uint32_t m, const T (&p)[BS], T (&acc)[G][BS]) {
     static_assert(G == 4 || G == 8, "groups of 4 or 8 nodes");
-    col_nibble<BS, G, 0>(v, m & 15u, p, acc);
-    if constexpr (G == 8) col_nibble<BS, G, 4>(v, m >> 4, p, acc);
+    col_nibble<BS, G, 0, T>(v, m & 15u, p, acc);
+    if constexpr (G == 8) col_nibble<BS, G, 4, T>(v, m >> 4, p, acc);
 }
 
-template <int BS>
-__device__ __forceinline__ void gather_rows(const double* __restrict__ Pd, int32_t node, int R, int ln,
-                                            double (&p)[BS]) {
+template <int BS, typename T>
+__device__ __forceinline__ void gather_rows(const T* __restrict__ Pd, int32_t node, int R, int ln, T (&p)[BS]) {
 #pragma unroll
     for (int b = 0; b < BS; ++b) p[b] = Pd[(size_t)(node * BS + b) * R + ln];
 }
@@ -243,34 +240,36 @@ __device__ __forceinline__ void gather_rows(const double* __restrict__ Pd, int32
 // mask and the packed values are wave-uniform (scalar loads, one contiguous
 // run of blocks per group); the gathers are branch-free (lanes past the last
 // load case read that case's entries and discard them).
-template <int BS, int G>
+// T is the type of the values, the direction and q (double, or float for
+// FP32-IR's inner iterations); p'.q is accumulated in FP64.
+template <int BS, int G, typename T>
 __device__ __forceinline__ double group_rows(int gi, int lane, int R, bool on, const int32_t* __restrict__ grp,
                                              const int32_t* __restrict__ gnodes, const int2* __restrict__ ucol,
-                                             const double* __restrict__ pval, const double* __restrict__ Pd,
-                                             double* __restrict__ Q) {
+                                             const T* __restrict__ pval, const T* __restrict__ Pd,
+                                             T* __restrict__ Q) {
     const int32_t c0 = grp[4 * gi], c1 = grp[4 * gi + 1], t0 = grp[4 * gi + 2];
     const int ln = on ? lane : R - 1;
-    double acc[G][BS];
+    T acc[G][BS];
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int a = 0; a < BS; ++a) acc[g][a] = 0.0;
-    const double* __restrict__ v = pval + (size_t)t0 * BS * BS;
+        for (int a = 0; a < BS; ++a) acc[g][a] = T(0);
+    const T* __restrict__ v = pval + (size_t)t0 * BS * BS;
     // Software pipeline over the union columns, two per trip with two row
     // buffers: column c's products run while column c + 1's rows of p' and
     // column c + 2's entry are in flight. Past the group's end the last
     // column is loaded again, unused.
     int2 ua = ucol[c0], ub = ucol[min(c0 + 1, c1 - 1)];
-    double pa[BS], pb[BS];
-    gather_rows<BS>(Pd, ua.x, R, ln, pa);
+    T pa[BS], pb[BS];
+    gather_rows<BS, T>(Pd, ua.x, R, ln, pa);
     for (int32_t c = c0;; c += 2) {
-        gather_rows<BS>(Pd, ub.x, R, ln, pb);
+        gather_rows<BS, T>(Pd, ub.x, R, ln, pb);
         const int2 ua2 = ucol[min(c + 2, c1 - 1)];
-        col_step<BS, G>(v, (uint32_t)ua.y, pa, acc);
+        col_step<BS, G, T>(v, (uint32_t)ua.y, pa, acc);
         if (c + 1 >= c1) break;
-        gather_rows<BS>(Pd, ua2.x, R, ln, pa);
+        gather_rows<BS, T>(Pd, ua2.x, R, ln, pa);
         const int2 ub2 = ucol[min(c + 3, c1 - 1)];
-        col_step<BS, G>(v, (uint32_t)ub.y, pb, acc);
+        col_step<BS, G, T>(v, (uint32_t)ub.y, pb, acc);
         if (c + 2 >= c1) break;
         ua = ua2;
         ub = ub2;
@@ -283,10 +282,10 @@ __device__ __forceinline__ double group_rows(int gi, int lane, int R, bool on, c
 #pragma unroll
             for (int a = 0; a < BS; ++a) {
                 const size_t o = (size_t)(node * BS + a) * R + ln;
-                const double own = Pd[o];
+                const T own = Pd[o];
                 if (on) {
                     Q[o] = acc[g][a];
-                    pq += own * acc[g][a];
+                    pq += (double)own * (double)acc[g][a];
                 }
             }
         }
@@ -294,8 +293,11 @@ __device__ __forceinline__ double group_rows(int gi, int lane, int R, bool on, c
     return pq;
 }
 
-template <int BS, int G>
-__global__ __launch_bounds__(kBlock) void blk_spmv_group_kernel(BlockArgs A, int par) {
+// q = K p' over every group (values pval, direction Pd), partial p'.q per
+// tile and load case.
+template <int BS, int G, typename T>
+__global__ __launch_bounds__(kBlock) void blk_spmv_group_kernel(BlockArgs A, const T* __restrict__ Pd,
+                                                                  T* __restrict__ Q, const T* __restrict__ pval) {
     if (!A.S->any_running) return;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
     const int tile = spmv_tile(A.xcd);
@@ -303,23 +305,22 @@ __global__ __launch_bounds__(kBlock) void blk_spmv_group_kernel(BlockArgs A, int
     const bool on = lane < A.R;
     double pq = 0.0;
     if (gi < A.n_groups)
-        pq = group_rows<BS, G>(gi, lane, A.R, on, A.grp, A.gnodes, (const int2*)A.ucol, A.pval, par ? A.P : A.P2,
-                               A.Q);
+        pq = group_rows<BS, G, T>(gi, lane, A.R, on, A.grp, A.gnodes, (const int2*)A.ucol, pval, Pd, Q);
     write_rhs_partials(pq, 0.0, A.partials, tile, A.n_tiles, 1);
 }
 
 // Block values in group order: entry (a, b) of packed block t is
 // val[src + a * stride + b], or, for a diagonal-only row (stride 0), its
 // diagonal val[src + a] when a == b and 0 otherwise.
-template <int BS>
+template <int BS, typename T>
 __global__ void blk_pack_values_kernel(const double* __restrict__ val, const int32_t* __restrict__ src,
-                                       const int32_t* __restrict__ stride, int64_t nblk, double* __restrict__ out) {
+                                       const int32_t* __restrict__ stride, int64_t nblk, T* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nblk * BS * BS) return;
     const int64_t blk = t / (BS * BS);
     const int e = (int)(t % (BS * BS)), a = e / BS, b = e % BS;
     const int32_t st = stride[blk], s0 = src[blk];
-    out[t] = st == 0 ? (a == b ? val[s0 + a] : 0.0) : val[(int64_t)s0 + (int64_t)a * st + b];
+    out[t] = (T)(st == 0 ? (a == b ? val[s0 + a] : 0.0) : val[(int64_t)s0 + (int64_t)a * st + b]);
 }
 
 // ------------------------------------------------------------ gathering SpMV
@@ -498,6 +499,216 @@ __global__ void blk_flag_kernel(BlockArgs A, int32_t* running_out) {
     }
 }
 
+// ------------------------------------------- FP32-IR: FP32 phase, hand-over
+// The FP32 phase runs the block iteration above on the FP32 packed values
+// and FP32 vectors (x32 the correction since the last refresh, r32, p32 in
+// two buffers, q32), with dots accumulated in FP64 (z = r / diag in FP32).
+// A refresh folds x32 into the FP64 u, recomputes r = f - K u in FP64 and
+// hands a load case over to FP64 (status ST_FALLBACK) once its true residual
+// reaches tol or fails to halve its best so far; otherwise CG continues with
+// the replaced residual (beta from its r.z). A breakdown also hands over
+// (from the u and r of the last refresh). The FP32 phase never converges a
+// case: the FP64 block iteration finishes every case (blk_warm_start).
+
+__global__ __launch_bounds__(kBlock) void blk32_init_kernel(BlockArgs A) {
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
+    const bool on = lane < A.R;
+    double ff = 0.0, rz = 0.0;
+    for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
+        if (!on) continue;
+        const size_t k = (size_t)i * A.R + lane;
+        const double fi = A.F[k];
+        const float r = (float)fi;
+        const float z = r * (float)A.dinv[i];
+        A.U[k] = 0.0;
+        A.Rv[k] = fi;
+        A.X32[k] = 0.0f;
+        A.R32[k] = r;
+        A.P32[k] = 0.0f;  // with beta = 0 the first step gives p' = z
+        ff += fi * fi;
+        rz += (double)r * (double)z;
+    }
+    write_rhs_partials(ff, rz, A.partials, tile, A.n_vtiles, 2);
+}
+
+__global__ __launch_bounds__(kBlock) void blk32_step_kernel(BlockArgs A, int par) {
+    if (!A.S->any_running) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
+    if (lane >= A.R) return;
+    const float beta = A.S->status[lane] == ST_RUNNING ? (float)A.S->beta[lane] : 0.0f;
+    const float* __restrict__ Pold = par ? A.P32b : A.P32;
+    float* __restrict__ Pnew = par ? A.P32 : A.P32b;
+    const float* __restrict__ R32 = A.R32;
+    for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
+        const size_t k = (size_t)i * A.R + lane;
+        Pnew[k] = fmaf(beta, Pold[k], R32[k] * (float)A.dinv[i]);
+    }
+}
+
+// One wavefront: alpha; a breakdown hands the case over to FP64.
+__global__ void blk32_alpha_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
+    const int lane = threadIdx.x;
+    const double pq = l1_sum(A.l1, 0, lane);
+    if (lane >= A.R || A.S->status[lane] != ST_RUNNING) return;
+    if (pq <= 0.0 || pq != pq) {
+        A.S->status[lane] = ST_FALLBACK;
+        A.S->alpha[lane] = 0.0;
+    } else {
+        A.S->alpha[lane] = A.S->rz[lane] / pq;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void blk32_update_kernel(BlockArgs A, const float* __restrict__ Pn) {
+    if (!A.S->any_running) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
+    const bool on = lane < A.R && A.S->status[lane] == ST_RUNNING;
+    const float alpha = on ? (float)A.S->alpha[lane] : 0.0f;
+    double rr = 0.0, rz = 0.0;
+    if (on)
+        for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
+            const size_t k = (size_t)i * A.R + lane;
+            A.X32[k] = fmaf(alpha, Pn[k], A.X32[k]);
+            const float r = fmaf(-alpha, A.Q32[k], A.R32[k]);
+            A.R32[k] = r;
+            const float z = r * (float)A.dinv[i];
+            rr += (double)r * (double)r;
+            rz += (double)r * (double)z;
+        }
+    write_rhs_partials(rr, rz, A.partials, tile, A.n_vtiles, 2);
+}
+
+// One wavefront: iteration count, budget and beta (no convergence test on
+// the FP32 recursive residual).
+__global__ void blk32_beta_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
+    const int lane = threadIdx.x;
+    const double rr = l1_sum(A.l1, 0, lane);
+    const double rz = l1_sum(A.l1, 1, lane);
+    bool run = false;
+    if (lane < A.R && A.S->status[lane] == ST_RUNNING) {
+        A.S->rr[lane] = rr;
+        A.S->it[lane] += 1;
+        if (A.S->it[lane] >= A.S->max_iter) {
+            A.S->res[lane] = sqrt(rr) / A.S->bnorm[lane];
+            A.S->status[lane] = ST_MAXITER;
+        } else {
+            A.S->beta[lane] = rz / A.S->rz[lane];
+            A.S->rz_old[lane] = A.S->rz[lane];
+            A.S->rz[lane] = rz;
+            run = true;
+        }
+    }
+    unsigned long long m = __ballot(run);
+    if (lane == 0) A.S->any_running = m != 0ull;
+}
+
+// Refresh, first pass: u += x32, x32 = 0 for the running cases.
+__global__ __launch_bounds__(kBlock) void blk32_fold_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
+    if (lane >= A.R || A.S->status[lane] != ST_RUNNING) return;
+    for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
+        const size_t k = (size_t)i * A.R + lane;
+        A.U[k] += (double)A.X32[k];
+        A.X32[k] = 0.0f;
+    }
+}
+
+// Refresh, after q = K u in FP64: r = f - q (FP64 and FP32 copies) for the
+// running cases, partials r.r (FP64) and r32.z32.
+__global__ __launch_bounds__(kBlock) void blk32_residual_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
+    const bool on = lane < A.R && A.S->status[lane] == ST_RUNNING;
+    double rr = 0.0, rz = 0.0;
+    if (on)
+        for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
+            const size_t k = (size_t)i * A.R + lane;
+            const double r = A.F[k] - A.Q[k];
+            const float r32 = (float)r;
+            const float z = r32 * (float)A.dinv[i];
+            A.Rv[k] = r;
+            A.R32[k] = r32;
+            rr += r * r;
+            rz += (double)r32 * (double)z;
+        }
+    write_rhs_partials(rr, rz, A.partials, tile, A.n_vtiles, 2);
+}
+
+// Refresh, one wavefront: true residual and the hand-over test per case.
+__global__ void blk32_decide_kernel(BlockArgs A) {
+    if (!A.S->any_running) return;
+    const int lane = threadIdx.x;
+    const double rr = l1_sum(A.l1, 0, lane);
+    const double rz = l1_sum(A.l1, 1, lane);
+    bool run = false;
+    if (lane < A.R && A.S->status[lane] == ST_RUNNING) {
+        const double res = sqrt(rr) / A.S->bnorm[lane];
+        A.S->res[lane] = res;
+        A.S->rr[lane] = rr;
+        if (res <= A.S->tol || !(res <= 0.5 * A.S->best[lane])) {
+            A.S->status[lane] = ST_FALLBACK;
+        } else {
+            A.S->best[lane] = res;
+            A.S->beta[lane] = rz / A.S->rz_old[lane];
+            A.S->rz[lane] = rz;
+            run = true;
+        }
+    }
+    unsigned long long m = __ballot(run);
+    if (lane == 0) A.S->any_running = m != 0ull;
+}
+
+// FP64 set-up from the handed-over cases' u and r = f - K u: p = 0 (the
+// first FP64 step gives p' = z), partials r.r and r.z.
+__global__ __launch_bounds__(kBlock) void blk_warm_kernel(BlockArgs A) {
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tile = blockIdx.x;
+    const int32_t r0 = tile * A.vrows_per_tile, r1 = min(A.n, r0 + A.vrows_per_tile);
+    const bool on = lane < A.R;
+    double rr = 0.0, rz = 0.0;
+    for (int32_t i = r0 + w; i < r1; i += kWavesPerBlock) {
+        if (!on) continue;
+        const size_t k = (size_t)i * A.R + lane;
+        const double ri = A.Rv[k];
+        const double zi = ri * A.dinv[i];
+        A.P[k] = 0.0;
+        rr += ri * ri;
+        rz += ri * zi;
+    }
+    write_rhs_partials(rr, rz, A.partials, tile, A.n_vtiles, 2);
+}
+
+// One wavefront: handed-over cases resume as FP64 iterations, or converge
+// at once when their true residual is already at tol.
+__global__ void blk_warm_finish_kernel(BlockArgs A) {
+    const int lane = threadIdx.x;
+    const double rr = l1_sum(A.l1, 0, lane);
+    const double rz = l1_sum(A.l1, 1, lane);
+    if (lane < A.R && A.S->status[lane] == ST_FALLBACK) {
+        const double res = sqrt(rr) / A.S->bnorm[lane];
+        A.S->rr[lane] = rr;
+        A.S->rz[lane] = rz;
+        A.S->beta[lane] = 0.0;
+        A.S->res[lane] = res;
+        A.S->status[lane] = res <= A.S->tol ? ST_CONVERGED : ST_RUNNING;
+    }
+    const int run = lane < A.R && A.S->status[lane] == ST_RUNNING;
+    unsigned long long m = __ballot(run);
+    if (lane == 0) A.S->any_running = m != 0ull;
+}
+
 // [R][n] <-> [n][R] (lane-major), zeroing the constrained rows on the way in.
 __global__ void blk_pack_kernel(const double* __restrict__ in, double* __restrict__ out, int32_t n, int32_t R,
                                 const uint8_t* __restrict__ mask) {
@@ -516,21 +727,26 @@ __global__ void blk_unpack_kernel(const double* __restrict__ in, double* __restr
 
 // The SpMV kernel of iteration parity `par` (with a group plan, the product
 // of the p' that blk_step_kernel wrote).
+template <typename T>
+void launch_group_product(const BlockArgs& a, const T* Pd, T* Q, const T* pval, hipStream_t s) {
+    const int bs = a.bs, g = a.group;
+    if (bs == 3 && g == 8)
+        blk_spmv_group_kernel<3, 8, T><<<a.n_tiles, kBlock, 0, s>>>(a, Pd, Q, pval);
+    else if (bs == 3)
+        blk_spmv_group_kernel<3, 4, T><<<a.n_tiles, kBlock, 0, s>>>(a, Pd, Q, pval);
+    else if (bs == 2 && g == 8)
+        blk_spmv_group_kernel<2, 8, T><<<a.n_tiles, kBlock, 0, s>>>(a, Pd, Q, pval);
+    else if (bs == 2)
+        blk_spmv_group_kernel<2, 4, T><<<a.n_tiles, kBlock, 0, s>>>(a, Pd, Q, pval);
+    else if (g == 8)
+        blk_spmv_group_kernel<1, 8, T><<<a.n_tiles, kBlock, 0, s>>>(a, Pd, Q, pval);
+    else
+        blk_spmv_group_kernel<1, 4, T><<<a.n_tiles, kBlock, 0, s>>>(a, Pd, Q, pval);
+}
+
 void launch_blk_product(const BlockArgs& a, int par, hipStream_t s) {
     if (a.group > 0) {
-        const int bs = a.bs, g = a.group;
-        if (bs == 3 && g == 8)
-            blk_spmv_group_kernel<3, 8><<<a.n_tiles, kBlock, 0, s>>>(a, par);
-        else if (bs == 3)
-            blk_spmv_group_kernel<3, 4><<<a.n_tiles, kBlock, 0, s>>>(a, par);
-        else if (bs == 2 && g == 8)
-            blk_spmv_group_kernel<2, 8><<<a.n_tiles, kBlock, 0, s>>>(a, par);
-        else if (bs == 2)
-            blk_spmv_group_kernel<2, 4><<<a.n_tiles, kBlock, 0, s>>>(a, par);
-        else if (g == 8)
-            blk_spmv_group_kernel<1, 8><<<a.n_tiles, kBlock, 0, s>>>(a, par);
-        else
-            blk_spmv_group_kernel<1, 4><<<a.n_tiles, kBlock, 0, s>>>(a, par);
+        launch_group_product<double>(a, par ? a.P : a.P2, a.Q, a.pval, s);
     } else if (a.bs == 3) {
         blk_spmv_node_kernel<3><<<a.n_tiles, kBlock, 0, s>>>(a, par);
     } else if (a.bs == 2) {
@@ -547,18 +763,31 @@ void launch_blk_spmv(const BlockArgs& a, int par, hipStream_t s) {
 
 }  // namespace
 
-void blk_pack_values(const double* val, const int32_t* pk_src, const int32_t* pk_stride, int64_t nblk, int bs,
-                     double* pval, hipStream_t s) {
+namespace {
+template <typename T>
+void pack_values(const double* val, const int32_t* pk_src, const int32_t* pk_stride, int64_t nblk, int bs, T* pval,
+                 hipStream_t s) {
     const int64_t tot = nblk * bs * bs;
     if (tot == 0) return;
     const unsigned grid = (unsigned)((tot + kBlock - 1) / kBlock);
     if (bs == 3)
-        blk_pack_values_kernel<3><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
+        blk_pack_values_kernel<3, T><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
     else if (bs == 2)
-        blk_pack_values_kernel<2><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
+        blk_pack_values_kernel<2, T><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
     else
-        blk_pack_values_kernel<1><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
+        blk_pack_values_kernel<1, T><<<grid, kBlock, 0, s>>>(val, pk_src, pk_stride, nblk, pval);
     SVX_HIP(hipGetLastError());
+}
+}  // namespace
+
+void blk_pack_values(const double* val, const int32_t* pk_src, const int32_t* pk_stride, int64_t nblk, int bs,
+                     double* pval, hipStream_t s) {
+    pack_values<double>(val, pk_src, pk_stride, nblk, bs, pval, s);
+}
+
+void blk_pack_values32(const double* val, const int32_t* pk_src, const int32_t* pk_stride, int64_t nblk, int bs,
+                       float* pval, hipStream_t s) {
+    pack_values<float>(val, pk_src, pk_stride, nblk, bs, pval, s);
 }
 
 void blk_pack(const double* in, double* out, int32_t n, int32_t R, const uint8_t* mask, hipStream_t s) {
@@ -588,6 +817,41 @@ void blk_iteration(const BlockArgs& a, int it, hipStream_t s) {
     blk_update_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a, par ? a.P : a.P2);
     blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_vtiles, 2, a.l1, a.S);
     blk_beta_kernel<<<1, kWave, 0, s>>>(a);
+    SVX_HIP(hipGetLastError());
+}
+
+void blk32_init(const BlockArgs& a, hipStream_t s) {
+    blk32_init_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
+    blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_vtiles, 2, a.l1, nullptr);
+    blk_init_finish_kernel<<<1, kWave, 0, s>>>(a);
+    SVX_HIP(hipGetLastError());
+}
+
+void blk32_iteration(const BlockArgs& a, int it, hipStream_t s) {
+    const int par = it & 1;
+    blk32_step_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a, par);
+    launch_group_product<float>(a, par ? a.P32 : a.P32b, a.Q32, a.pval32, s);
+    blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_tiles, 1, a.l1, a.S);
+    blk32_alpha_kernel<<<1, kWave, 0, s>>>(a);
+    blk32_update_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a, par ? a.P32 : a.P32b);
+    blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_vtiles, 2, a.l1, a.S);
+    blk32_beta_kernel<<<1, kWave, 0, s>>>(a);
+    SVX_HIP(hipGetLastError());
+}
+
+void blk32_refresh(const BlockArgs& a, hipStream_t s) {
+    blk32_fold_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
+    launch_group_product<double>(a, a.U, a.Q, a.pval, s);
+    blk32_residual_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
+    blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_vtiles, 2, a.l1, a.S);
+    blk32_decide_kernel<<<1, kWave, 0, s>>>(a);
+    SVX_HIP(hipGetLastError());
+}
+
+void blk_warm_start(const BlockArgs& a, hipStream_t s) {
+    blk_warm_kernel<<<a.n_vtiles, kBlock, 0, s>>>(a);
+    blk_reduce1_kernel<<<kReduceGroups, kBlock, 0, s>>>(a.partials, a.n_vtiles, 2, a.l1, nullptr);
+    blk_warm_finish_kernel<<<1, kWave, 0, s>>>(a);
     SVX_HIP(hipGetLastError());
 }
 

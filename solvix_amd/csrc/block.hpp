@@ -17,6 +17,8 @@ struct alignas(16) BlockScal {
     double res[kMaxRhs];
     int64_t it[kMaxRhs];
     int32_t status[kMaxRhs];
+    double best[kMaxRhs];    // FP32-IR: best true residual of the FP32 phase so far
+    double rz_old[kMaxRhs];  // FP32-IR: r.z before the last beta (beta after a refresh)
     double tol;
     int64_t max_iter;
     int32_t any_running;
@@ -50,6 +52,14 @@ struct BlockArgs {
     double* P;   // direction, even iterations read it and the odd ones write it
     double* P2;  // the other direction buffer
     double* Q;
+    // FP32-IR inner iterations (group SpMV only): FP32 packed values,
+    // correction since the last refresh, residual, direction (two buffers), q
+    const float* pval32;
+    float* X32;
+    float* R32;
+    float* P32;
+    float* P32b;
+    float* Q32;
     double* partials;  // slot-major per-tile, per-load-case sums: [1][n_tiles][64] (p.q), [2][n_vtiles][64] (r.r, r.z)
     double* l1;        // first-level sums [2][kReduceGroups][64]
     BlockScal* S;
@@ -65,11 +75,23 @@ void blk_spmv_only(const BlockArgs& a, hipStream_t s);
 // The SpMV kernel alone (with a group plan, the product of the p' the last
 // step wrote; the gathering SpMV evaluates the step itself).
 void blk_product_only(const BlockArgs& a, hipStream_t s);
+// FP32-IR block solve (a mesh with a group plan): the FP32 phase starts from
+// u = 0 (blk32_init), runs FP32 iterations (blk32_iteration) and refreshes
+// (blk32_refresh: u += x32, r = f - K u in FP64, hand-over test per load
+// case); blk_warm_start then sets up the FP64 block iterations from every
+// handed-over case's u and r.
+void blk32_init(const BlockArgs& a, hipStream_t s);
+void blk32_iteration(const BlockArgs& a, int it, hipStream_t s);
+void blk32_refresh(const BlockArgs& a, hipStream_t s);
+void blk_warm_start(const BlockArgs& a, hipStream_t s);
 void blk_pack(const double* in, double* out, int32_t n, int32_t R, const uint8_t* mask, hipStream_t s);
 void blk_unpack(const double* in, double* out, int32_t n, int32_t R, hipStream_t s);
 // Packs the CSR values block by block in group order (BlockPlan's pk_src /
 // pk_stride) into pval [nblk][bs][bs].
 void blk_pack_values(const double* val, const int32_t* pk_src, const int32_t* pk_stride, int64_t nblk, int bs,
                      double* pval, hipStream_t s);
+// The same, rounded to FP32.
+void blk_pack_values32(const double* val, const int32_t* pk_src, const int32_t* pk_stride, int64_t nblk, int bs,
+                       float* pval, hipStream_t s);
 
 }  // namespace svx

@@ -10,12 +10,14 @@ using namespace svx;
 
 namespace {
 
-// FP32-IR over the load cases: each case is an instance of the Krylov
-// engine whose descriptor shares the system's matrix copies (CSR, FP64 and
-// FP32 ELL, preconditioner) and owns its vectors, so every case runs the
-// FP32-IR recurrence of a single solve (FP32 inner iterations, FP64
-// residual replacement and final test, FP64 finish from the evaluation
-// floor or a stall) with its own scalars, in one launch per phase.
+// FP32-IR over the load cases of a mesh without a group plan (scalar
+// systems, or the gathering SpMV selected): each case is an instance of the
+// Krylov engine whose descriptor shares the system's matrix copies (CSR,
+// FP64 and FP32 ELL, preconditioner) and owns its vectors, so every case runs
+// the FP32-IR recurrence of a single solve (FP32 inner iterations, FP64
+// residual replacement and final test, FP64 finish from the evaluation floor
+// or a stall) with its own scalars, in one launch per phase. With a group
+// plan the block solve runs an FP32 block phase instead (blk32_*).
 int solve_multi_ir32(svx_system* sys, int32_t R, const double* F, double tol, int64_t max_iter, double* U,
                      int64_t* iterations, double* residuals, int32_t* converged, double* elapsed_ms, svx_error* err) {
     svx_mesh* m = sys->mesh;
@@ -181,6 +183,78 @@ struct BlockWork {
     }
 };
 
+// Runs `body` (one chunk of iterations, enqueued on s) as a captured graph
+// when the context allows graphs, else kernel by kernel, until no load case
+// runs; the host polls the running count once per chunk.
+template <typename Body>
+void run_chunks(svx_ctx* ctx, const BlockArgs& a, hipStream_t s, Body body) {
+    hipGraphExec_t exec = nullptr;
+    if (ctx->tune.use_graph) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            bool ok = true;
+            try {
+                body();
+            } catch (const DeviceError&) {
+                ok = false;
+            }
+            if (hipStreamEndCapture(s, &g) == hipSuccess && g && ok) {
+                if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+            }
+            if (g) (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+    }
+    try {
+        for (;;) {
+            if (exec)
+                SVX_HIP(hipGraphLaunch(exec, s));
+            else
+                body();
+            blk_count_running(a, ctx->d_flag, s);
+            SVX_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            SVX_HIP(hipStreamSynchronize(s));
+            if (*ctx->h_flag == 0) break;
+        }
+    } catch (...) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        throw;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+}
+
+// FP64 block iterations until every load case has stopped.
+void run_fp64(svx_ctx* ctx, const BlockArgs& a, hipStream_t s) {
+    const int chunk = (ctx->tune.chunk + 1) / 2 * 2;  // even: every chunk starts on the same direction buffer
+    run_chunks(ctx, a, s, [&] {
+        for (int i = 0; i < chunk; ++i) blk_iteration(a, i, s);
+    });
+}
+
+// FP32-IR refresh interval of the block solve (iterations between true
+// residuals; the single solve's FP32-IR checks at least this often).
+constexpr int kBlockRefresh = 64;
+
+// Per-case results of a block solve (scalars downloaded) and the error.
+int finish_multi(const BlockScal& hs, int32_t R, int64_t* iterations, double* residuals, int32_t* converged,
+                 svx_error* err) {
+    int failed = 0;
+    for (int j = 0; j < R; ++j) {
+        if (iterations) iterations[j] = hs.it[j];
+        if (residuals) residuals[j] = hs.res[j];
+        if (converged) converged[j] = hs.status[j] == ST_CONVERGED;
+        failed += hs.status[j] != ST_CONVERGED;
+    }
+    if (failed) {
+        if (err) {
+            err->code = SVX_E_NOT_CONVERGED;
+            snprintf(err->message, sizeof err->message, "%d of %d load cases did not converge", failed, R);
+        }
+        return SVX_E_NOT_CONVERGED;
+    }
+    return SVX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -196,7 +270,10 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
     if (precision != SVX_FP64 && precision != SVX_FP32_IR) return SVX_E_ARGUMENT;
     if (precision == SVX_FP32_IR) {
         try {
-            return solve_multi_ir32(sys, nrhs, F, tol, max_iter, U, iterations, residuals, converged, elapsed_ms, err);
+            // without a group plan each case runs the single FP32-IR solve
+            if (ensure_blk_plan(sys->mesh, sys->mesh->ctx->tune, sys->mesh->ctx->stream) == 0)
+                return solve_multi_ir32(sys, nrhs, F, tol, max_iter, U, iterations, residuals, converged, elapsed_ms,
+                                        err);
         } catch (const std::exception& e) {
             if (err) {
                 err->code = SVX_E_DEVICE;
@@ -215,44 +292,35 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
         SVX_HIP(hipEventCreate(&e1));
         BlockWork W(sys, R, tol, max_iter);
         const size_t nr = (size_t)n * R;
+        // FP32-IR (group plan in force): FP32 copies of the packed values and
+        // the FP32 phase's vectors
+        DevBuf<float> pval32, X32, R32, P32, P32b, Q32;
+        BlockArgs& a = W.a;
+        if (precision == SVX_FP32_IR) {
+            pval32.alloc((size_t)m->nblk * a.bs * a.bs);
+            blk_pack_values32(sys->vals.p, m->blk_pk_src.p, m->blk_pk_stride.p, m->nblk, a.bs, pval32.p, s);
+            for (DevBuf<float>* b : {&X32, &R32, &P32, &P32b, &Q32}) b->alloc(nr);
+            a.pval32 = pval32.p;
+            a.X32 = X32.p, a.R32 = R32.p, a.P32 = P32.p, a.P32b = P32b.p, a.Q32 = Q32.p;
+        }
         SVX_HIP(hipEventRecord(e0, s));
         W.Fin.upload(F, nr, s);
         blk_pack(W.Fin.p, W.Fp.p, n, R, W.dmask.p, s);
-        const BlockArgs& a = W.a;
         DevBuf<double>& Fin = W.Fin;
         DevBuf<double>& Uv = W.Uv;
         DevBuf<BlockScal>& S = W.S;
         BlockScal hs{};
-        blk_init(a, s);
-        const int chunk = (ctx->tune.chunk + 1) / 2 * 2;  // even: every chunk starts on the same direction buffer
-        hipGraphExec_t exec = nullptr;
-        if (ctx->tune.use_graph) {
-            hipGraph_t g = nullptr;
-            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                bool ok = true;
-                try {
-                    for (int i = 0; i < chunk; ++i) blk_iteration(a, i, s);
-                } catch (const DeviceError&) {
-                    ok = false;
-                }
-                if (hipStreamEndCapture(s, &g) == hipSuccess && g && ok) {
-                    if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
-                }
-                if (g) (void)hipGraphDestroy(g);
-            }
-            (void)hipGetLastError();
+        if (precision == SVX_FP32_IR) {
+            blk32_init(a, s);
+            run_chunks(ctx, a, s, [&] {
+                for (int i = 0; i < kBlockRefresh; ++i) blk32_iteration(a, i, s);
+                blk32_refresh(a, s);
+            });
+            blk_warm_start(a, s);
+        } else {
+            blk_init(a, s);
         }
-        for (;;) {
-            if (exec)
-                SVX_HIP(hipGraphLaunch(exec, s));
-            else
-                for (int i = 0; i < chunk; ++i) blk_iteration(a, i, s);
-            blk_count_running(a, ctx->d_flag, s);
-            SVX_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            SVX_HIP(hipStreamSynchronize(s));
-            if (*ctx->h_flag == 0) break;
-        }
-        if (exec) (void)hipGraphExecDestroy(exec);
+        run_fp64(ctx, a, s);
         blk_unpack(Uv.p, Fin.p, n, R, s);
         if (U) Fin.download(U, nr, s);
         SVX_HIP(hipEventRecord(e1, s));
@@ -263,21 +331,7 @@ int svx_solve_multi(svx_system* sys, int32_t nrhs, const double* F, double tol, 
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (elapsed_ms) *elapsed_ms = ms;
-        int failed = 0;
-        for (int j = 0; j < R; ++j) {
-            if (iterations) iterations[j] = hs.it[j];
-            if (residuals) residuals[j] = hs.res[j];
-            if (converged) converged[j] = hs.status[j] == ST_CONVERGED;
-            failed += hs.status[j] != ST_CONVERGED;
-        }
-        if (failed) {
-            if (err) {
-                err->code = SVX_E_NOT_CONVERGED;
-                snprintf(err->message, sizeof err->message, "%d of %d load cases did not converge", failed, R);
-            }
-            return SVX_E_NOT_CONVERGED;
-        }
-        return SVX_OK;
+        return finish_multi(hs, R, iterations, residuals, converged, err);
     } catch (const std::exception& e) {
         if (err) {
             err->code = SVX_E_DEVICE;

@@ -35,33 +35,67 @@ def test_multi_rhs_matches_single_solves(ctx):
         assert e <= 1e-8
 
 
-def test_multi_rhs_fp32_ir_matches_oracle_and_single_solves(ctx):
-    """SVX_FP32_IR: every load case runs the FP32-IR solve of a single
-    svx_solve (FP32 inner iterations, FP64 refinement and final test) over
-    the shared matrix; each result equals that case's own FP32-IR solve bit
-    for bit and the oracle within 1e-8."""
-    model, cases = decks.cfg4(nrhs=12, nx=24, ny=4, nz=4)
+def _cfg4_small(ctx, nrhs=12):
+    model, cases = decks.cfg4(nrhs=nrhs, nx=24, ny=4, nz=4)
     mesh, sys_ = assemble(model, ctx)
     rows, _, _ = constrained_dofs(model)
     sys_.apply_dirichlet(rows)
-    n = mesh.n
     pos = {int(nid): i for i, nid in enumerate(model.node_ids)}
-    F = np.zeros((len(cases), n))
+    F = np.zeros((len(cases), mesh.n))
     for k, (nodes, dofs, vals) in enumerate(cases):
         for nid, d, v in zip(nodes, dofs, vals):
             F[k, pos[int(nid)] * 3 + int(d) - 1] += v
+    return model, cases, sys_, F
+
+
+def test_multi_rhs_fp32_ir_block_matches_oracle_and_fp64(ctx):
+    """SVX_FP32_IR with node groups: a block FP32 phase with FP64 refreshes,
+    then the FP64 block PCG finishes every case to the recursive 1e-10. Each
+    case lands within 1e-8 of the oracle and within 1e-9 of the FP64 block
+    solve. The FP32 phase does not shorten the FP64 work on these systems,
+    so the total is the FP64 block solve's count plus the FP32 phase (at
+    most a few 64-iteration refresh intervals)."""
+    model, cases, sys_, F = _cfg4_small(ctx)
     U, its, res, ok, ms = sys_.solve_multi(F, precision=Precision.FP32_IR)
     assert ok.all() and (res <= 1e-10).all()
+    U64, its64, _, ok64, _ = sys_.solve_multi(F)
+    assert ok64.all()
     for k in range(len(cases)):
-        sys_.set_rhs(F[k])
-        u1, its1, _ = sys_.solve(1e-10, 0, Precision.FP32_IR)
         m = decks.cfg4(nrhs=12, nx=24, ny=4, nz=4)[0]
         m.cload_node, m.cload_dof, m.cload_value = cases[k]
         ref = oracle.solve(m)
         e = rel_err(U[k], ref.u)
-        print(f"case {k}: its {its[k]} (single {its1}, oracle {ref.iterations}) res {res[k]:.2e} err {e:.1e}")
-        assert e <= 1e-8 and its[k] == its1
-        assert np.array_equal(U[k].view(np.uint64), u1.view(np.uint64))
+        e64 = rel_err(U[k], U64[k])
+        print(f"case {k}: its {its[k]} (FP64 block {its64[k]}, oracle {ref.iterations}) res {res[k]:.2e} "
+              f"err {e:.1e} vs FP64 {e64:.1e}")
+        assert e <= 1e-8 and e64 <= 1e-9
+        assert its[k] <= its64[k] + 4 * 64
+
+
+def test_multi_rhs_fp32_ir_without_groups_equals_single_solves():
+    """SVX_FP32_IR without node groups (the gathering SpMV selected): every
+    load case runs the FP32-IR solve of a single svx_solve over the shared
+    matrix; each result equals that case's own FP32-IR solve bit for bit and
+    the oracle within 1e-8."""
+    from solvix_amd.core import Context
+    ctx = Context(0)
+    try:
+        ctx.set_multi_group(-1)
+        model, cases, sys_, F = _cfg4_small(ctx, nrhs=6)
+        U, its, res, ok, ms = sys_.solve_multi(F, precision=Precision.FP32_IR)
+        assert ok.all() and (res <= 1e-10).all()
+        for k in range(len(cases)):
+            sys_.set_rhs(F[k])
+            u1, its1, _ = sys_.solve(1e-10, 0, Precision.FP32_IR)
+            m = decks.cfg4(nrhs=6, nx=24, ny=4, nz=4)[0]
+            m.cload_node, m.cload_dof, m.cload_value = cases[k]
+            ref = oracle.solve(m)
+            e = rel_err(U[k], ref.u)
+            print(f"case {k}: its {its[k]} (single {its1}, oracle {ref.iterations}) res {res[k]:.2e} err {e:.1e}")
+            assert e <= 1e-8 and its[k] == its1
+            assert np.array_equal(U[k].view(np.uint64), u1.view(np.uint64))
+    finally:
+        ctx.close()
 
 
 
