@@ -490,7 +490,7 @@ int svx_batch_time_spmv(svx_batch* batch, int reps, double* avg_ms,
  * SVX_FP64: one block PCG, vectors [n][R], the matrix read once per
  * iteration for all cases. SVX_FP32_IR on a mesh with node groups
  * (svx_set_multi_group): a block FP32 phase (FP32 matrix and vectors, FP64
- * dots) from u = 0 with an FP64 refresh every 64 iterations (u += the FP32
+ * dots) from u = 0 with an FP64 refresh every 32 iterations (u += the FP32
  * correction, r = f - K u in FP64); a case leaves it at the first refresh
  * whose true residual reaches tol or does not halve its best, or on a
  * breakdown, and the FP64 block PCG finishes every case from its u to tol on

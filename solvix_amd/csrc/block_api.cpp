@@ -232,8 +232,8 @@ void run_fp64(svx_ctx* ctx, const BlockArgs& a, hipStream_t s) {
 }
 
 // FP32-IR refresh interval of the block solve (iterations between true
-// residuals; the single solve's FP32-IR checks at least this often).
-constexpr int kBlockRefresh = 64;
+// residuals; the single solve's FP32-IR checks at least this often too).
+constexpr int kBlockRefresh = 32;
 
 // Per-case results of a block solve (scalars downloaded) and the error.
 int finish_multi(const BlockScal& hs, int32_t R, int64_t* iterations, double* residuals, int32_t* converged,

@@ -54,7 +54,7 @@ def test_multi_rhs_fp32_ir_block_matches_oracle_and_fp64(ctx):
     case lands within 1e-8 of the oracle and within 1e-9 of the FP64 block
     solve. The FP32 phase does not shorten the FP64 work on these systems,
     so the total is the FP64 block solve's count plus the FP32 phase (at
-    most a few 64-iteration refresh intervals)."""
+    most a few 32-iteration refresh intervals)."""
     model, cases, sys_, F = _cfg4_small(ctx)
     U, its, res, ok, ms = sys_.solve_multi(F, precision=Precision.FP32_IR)
     assert ok.all() and (res <= 1e-10).all()
@@ -69,7 +69,7 @@ def test_multi_rhs_fp32_ir_block_matches_oracle_and_fp64(ctx):
         print(f"case {k}: its {its[k]} (FP64 block {its64[k]}, oracle {ref.iterations}) res {res[k]:.2e} "
               f"err {e:.1e} vs FP64 {e64:.1e}")
         assert e <= 1e-8 and e64 <= 1e-9
-        assert its[k] <= its64[k] + 4 * 64
+        assert its[k] <= its64[k] + 8 * 32
 
 
 def test_multi_rhs_fp32_ir_without_groups_equals_single_solves():
