@@ -194,3 +194,26 @@ def test_multi_rhs_tile_sizes():
     U0, its0 = out[0]
     for rpt, (U, its) in out.items():
         assert (np.abs(its - its0) <= np.maximum(2, 0.02 * its0)).all() and rel_err(U, U0) <= 1e-8, rpt
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
+def test_multi_rhs_zero_case_odd_count_and_budget(precision, ctx):
+    """Five load cases (an odd count) with an all-zero one: the zero case
+    converges at once with u = 0 (solver.rs:93-95) and the others match the
+    oracle; with a budget of 50 iterations every loaded case stops on the
+    budget (NotConverged) after exactly 50 iterations, in both precisions."""
+    model, cases, sys_, F = _cfg4_small(ctx, nrhs=5)
+    F[2] = 0.0
+    U, its, res, ok, _ = sys_.solve_multi(F, precision=precision)
+    assert ok.all()
+    assert not U[2].any() and its[2] == 0
+    for k in (0, 1, 3, 4):
+        m = decks.cfg4(nrhs=5, nx=24, ny=4, nz=4)[0]
+        m.cload_node, m.cload_dof, m.cload_value = cases[k]
+        e = rel_err(U[k], oracle.solve(m).u)
+        print(f"case {k}: its {its[k]} res {res[k]:.2e} err {e:.1e}")
+        assert res[k] <= 1e-10 and e <= 1e-8
+    U, its, res, ok, _ = sys_.solve_multi(F, max_iter=50, precision=precision, raise_on_failure=False)
+    print("budget 50:", its.tolist(), ok.tolist())
+    assert ok.tolist() == [False, False, True, False, False]
+    assert its[[0, 1, 3, 4]].tolist() == [50] * 4
