@@ -240,7 +240,10 @@ int svx_set_load_hint(svx_ctx* ctx, int hint);
  * take the pending direction step of each iteration (r.r and r.z over all
  * their tiles, the convergence and budget tests of solver.rs:129-148, beta)
  * in one workgroup per system, where smaller ones take it in every SpMV tile
- * (whose reduction grows with the square of the tile count). 0: the default,
+ * (whose reduction grows with the square of the tile count). Systems with
+ * more than four times this many p.q partials (row tiles, or matrix-free
+ * patches) also take alpha and the breakdown test of solver.rs:116-123 in
+ * one workgroup per system instead of in every update tile. 0: the default,
  * 1024; negative: never. Either way gives the same results bit for bit. No
  * reference counterpart: a launch knob. */
 int svx_set_step_kernel_tiles(svx_ctx* ctx, int32_t tiles);

@@ -55,7 +55,7 @@ void Engine::place(DevBuf<T>& b, size_t count) {
 size_t Engine::arena_bytes_for_csr(int64_t n) {
     const size_t tiles = (size_t)std::max<int64_t>(1, (n + 255) / 256);
     auto r = [](size_t b) { return (b + 255) / 256 * 256; };
-    return r(tiles * sizeof(Tile)) + 3 * r(sizeof(Tile)) + r(sizeof(InstDesc)) +
+    return r(tiles * sizeof(Tile)) + 4 * r(sizeof(Tile)) + r(sizeof(InstDesc)) +
            r(std::max<size_t>(kPartialSlots, kPartialSlots * tiles) * sizeof(double)) + r(sizeof(InstScal));
 }
 
@@ -182,6 +182,21 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         if (d.step_kernel && d.bs == 3)
             mf_leaders.push_back(Tile{(int32_t)i, 0, 0, d.tile_begin, d.tile_count, P_PQ, d.tile_begin, d.tile_count, d.s});
     }
+    // leaders whose instance has more than four times the step-kernel
+    // threshold of p.q partials also take alpha once per instance
+    // (cg64_alpha_kernel). Measured on plane cantilevers, update with alpha
+    // against the per-tile reduction: 6,267 tiles 41.7 against 53.7 us;
+    // 1,571 tiles 15.0 against 13.1 us (the extra launch costs more there).
+    std::vector<Tile> alpha_leaders;
+    int32_t n_alpha2 = 0;
+    for (InstDesc& d : h_descs_) d.alpha_kernel = 0;
+    for (size_t j = 0; j < mf_leaders.size(); ++j) {
+        const Tile& t = mf_leaders[j];
+        if (shape_.step_tiles < 0 || t.pq_count <= 4 * shape_.step_tiles) continue;
+        h_descs_[t.inst].alpha_kernel = 1;
+        alpha_leaders.push_back(t);
+        if ((int32_t)j < n_lead2) n_alpha2 = (int32_t)alpha_leaders.size();
+    }
     tiled_tiles_ = (int)tiles.size();
     int stride[4] = {0, 0, 0, 0};
     size_t off[4] = {0, 0, 0, 0}, total = 0;
@@ -210,6 +225,8 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
     mf_tiles_.upload(mf_tiles.data(), mf_tiles.size(), stream_);
     place(mf_leaders_, std::max<size_t>(1, mf_leaders.size()));
     mf_leaders_.upload(mf_leaders.data(), mf_leaders.size(), stream_);
+    place(alpha_leaders_, std::max<size_t>(1, alpha_leaders.size()));
+    alpha_leaders_.upload(alpha_leaders.data(), alpha_leaders.size(), stream_);
     place(mf3_tiles_, std::max<size_t>(1, mf3_tiles.size()));
     mf3_tiles_.upload(mf3_tiles.data(), mf3_tiles.size(), stream_);
     place(descs_, std::max<size_t>(1, h_descs_.size()));
@@ -242,6 +259,8 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         k.mf_tiles = bs == 2 ? mf_tiles_.p : nullptr;
         k.mf_leaders = bs == 2 ? mf_leaders_.p : bs == 3 ? mf_leaders_.p + n_lead2 : nullptr;
         k.n_mf_inst = bs == 2 ? n_lead2 : bs == 3 ? (int32_t)mf_leaders.size() - n_lead2 : 0;
+        k.alpha_leaders = bs == 2 ? alpha_leaders_.p : bs == 3 ? alpha_leaders_.p + n_alpha2 : nullptr;
+        k.n_alpha = bs == 2 ? n_alpha2 : bs == 3 ? (int32_t)alpha_leaders.size() - n_alpha2 : 0;
         k.mf3_tiles = bs == 3 ? mf3_tiles_.p : nullptr;
         k.n_mf3_tiles = bs == 3 ? (int32_t)mf3_tiles.size() : 0;
         k.n_mf4 = bs == 2 ? n_mf[0] : 0;

@@ -120,6 +120,7 @@ class Engine {
     DevBuf<Tile> tiles_;
     DevBuf<Tile> mf_tiles_;
     DevBuf<Tile> mf_leaders_;
+    DevBuf<Tile> alpha_leaders_;
     DevBuf<Tile> mf3_tiles_;
     DevBuf<InstDesc> descs_;
     DevBuf<double> partials_;

@@ -108,7 +108,8 @@ __device__ __forceinline__ T load_hint(gptr<const T> p) {
 
 struct Dev {
     int32_t n;
-    int32_t stepk;  // InstDesc::step_kernel
+    int32_t stepk;   // InstDesc::step_kernel
+    int32_t alphak;  // InstDesc::alpha_kernel
     gptr<const int32_t> row_ptr, col, brow_ptr, bcol;
     gptr<const int32_t> ell_off, ell_nb, ell_col;
     gptr<const double> val, ell_val, dinv, f;
@@ -121,6 +122,7 @@ __device__ __forceinline__ Dev view(const InstDesc& d) {
     Dev v;
     v.n = d.n;
     v.stepk = d.step_kernel;
+    v.alphak = d.alpha_kernel;
     v.row_ptr = global(d.row_ptr);
     v.col = global(d.col);
     v.brow_ptr = global(d.brow_ptr);
@@ -1090,15 +1092,42 @@ __device__ __forceinline__ void cg64_update_rows(int32_t b, int32_t e, double al
     }
 }
 
-__global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restrict__ tiles,
-                                                             const InstDesc* __restrict__ descs,
-                                                             double* __restrict__ partials, int pstride, int par) {
-    const Tile t = tiles[blockIdx.x];
+// alpha of one FP64 iteration for the instances with many p.q partials, one
+// workgroup per instance (its leader tile): the p.q reduction of the
+// instance's SpMV tiles and the breakdown test (solver.rs:116-123), with the
+// scalars the update's leader tile records otherwise. The instance's update
+// tiles then read alpha instead of each reducing the partials, so the
+// reduction's cost no longer grows with the square of the tile count.
+__global__ __launch_bounds__(kBlock) void cg64_alpha_kernel(const Tile* __restrict__ leaders,
+                                                            const double* __restrict__ partials, int pstride) {
+    const Tile t = leaders[blockIdx.x];
     const gptr<InstScal> S = global(t.s);
-    const int32_t status = S->status, flags = S->flags;
-    const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
-    issue_before_test(D.p, D.p_alt, D.q, D.dinv, D.u, D.r, D.z, par);
-    if (status != ST_RUNNING || (flags & FL_FRESH)) return;
+    if (S->status != ST_RUNNING || (S->flags & FL_FRESH)) return;
+    const double rz = S->rz;
+    const int sp[1] = {t.pq_slot};
+    double pq[1];
+    reduce_tiles<1>(partials, pstride, t.pq_begin, t.pq_count, sp, pq);
+    if (threadIdx.x != 0) return;
+    S->pap = pq[0];
+    if (pq[0] <= 0.0 || pq[0] != pq[0]) {  // solver.rs:118-123
+        S->it += 1;
+        S->res = sqrt(S->rr) / S->bnorm;
+        S->status = ST_BREAKDOWN;
+        return;
+    }
+    S->alpha = rz / pq[0];
+    S->snap_rz = rz;
+    S->snap_it = S->it;
+    S->dir = 0;  // the next SpMV takes the direction step
+}
+
+// One row tile of the FP64 update; TAKEN: cg64_alpha_kernel took alpha (and
+// the breakdown test) for the instance, otherwise the tile reduces the p.q
+// partials itself. Two instantiations, so each path keeps its own schedule
+// (every preload ahead of the reduction).
+template <bool TAKEN>
+__device__ __forceinline__ void update_tile(const Tile& t, gptr<InstScal> S, const Dev& D,
+                                            double* __restrict__ partials, int pstride, int par) {
     const gptr<double> pd = par ? D.p : D.p_alt;
     const int32_t b = t.row_begin, e = t.row_end;
     double p0[kPre], q0[kPre], d0[kPre], u0[kPre], r0[kPre];
@@ -1107,26 +1136,31 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
     preload<double>(D.dinv, b, e, d0);
     preload<double>(D.u, b, e, u0);
     preload<double>(D.r, b, e, r0);
-    const double rz = S->rz;  // written by the previous phases' leaders only
-    const int sp[1] = {t.pq_slot};  // P_PQ over the row tiles, or P_MPQ over the matrix-free tiles
-    double pq[1];
-    reduce_tiles<1>(partials, pstride, t.pq_begin, t.pq_count, sp, pq);
-    if (pq[0] <= 0.0 || pq[0] != pq[0]) {  // solver.rs:118-123
+    double alpha;
+    if constexpr (TAKEN) {
+        alpha = S->alpha;
+    } else {
+        const double rz = S->rz;  // written by the previous phases' leaders only
+        const int sp[1] = {t.pq_slot};  // P_PQ over the row tiles, or P_MPQ over the matrix-free tiles
+        double pq[1];
+        reduce_tiles<1>(partials, pstride, t.pq_begin, t.pq_count, sp, pq);
+        if (pq[0] <= 0.0 || pq[0] != pq[0]) {  // solver.rs:118-123
+            if (is_leader(t)) {
+                S->pap = pq[0];
+                S->it += 1;
+                S->res = sqrt(S->rr) / S->bnorm;
+                S->status = ST_BREAKDOWN;
+            }
+            return;
+        }
+        alpha = rz / pq[0];
         if (is_leader(t)) {
             S->pap = pq[0];
-            S->it += 1;
-            S->res = sqrt(S->rr) / S->bnorm;
-            S->status = ST_BREAKDOWN;
+            S->alpha = alpha;
+            S->snap_rz = rz;
+            S->snap_it = S->it;
+            S->dir = 0;  // the next SpMV takes the direction step
         }
-        return;
-    }
-    const double alpha = rz / pq[0];
-    if (is_leader(t)) {
-        S->pap = pq[0];
-        S->alpha = alpha;
-        S->snap_rz = rz;
-        S->snap_it = S->it;
-        S->dir = 0;  // the next SpMV takes the direction step
     }
     double v[2] = {0.0, 0.0};
 #pragma unroll
@@ -1145,6 +1179,21 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
     cg64_update_rows(b + kPre * kBlock, e, alpha, pd, D.q, D.dinv, D.u, D.r, D.z, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restrict__ tiles,
+                                                             const InstDesc* __restrict__ descs,
+                                                             double* __restrict__ partials, int pstride, int par) {
+    const Tile t = tiles[blockIdx.x];
+    const gptr<InstScal> S = global(t.s);
+    const int32_t status = S->status, flags = S->flags;
+    const Dev D = view(descs[t.inst]);  // read with the state, so the loads overlap
+    issue_before_test(D.p, D.p_alt, D.q, D.dinv, D.u, D.r, D.z, par, D.alphak);
+    if (status != ST_RUNNING || (flags & FL_FRESH)) return;
+    if (D.alphak)
+        update_tile<true>(t, S, D, partials, pstride, par);
+    else
+        update_tile<false>(t, S, D, partials, pstride, par);
 }
 
 // End of a chunk (an even number of iterations, so the current direction is
@@ -1820,6 +1869,7 @@ void cg64_init(const KrylovSet& k, hipStream_t s) {
 void cg64_iteration(const KrylovSet& k, hipStream_t s, int par) {
     if (k.n_tiles <= 0) return;
     launch_cg64_spmv(k, s, par, 0);
+    if (k.n_alpha > 0) cg64_alpha_kernel<<<k.n_alpha, kBlock, 0, s>>>(k.alpha_leaders, k.partials, k.pstride);
     cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
     SVX_HIP(hipGetLastError());
 }
@@ -1888,8 +1938,10 @@ void launch_set_running(const InstDesc* descs, int32_t n_inst, hipStream_t s, in
 
 void update_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0) return;
-    if (precision == SVX_FP64)
+    if (precision == SVX_FP64) {
+        if (k.n_alpha > 0) cg64_alpha_kernel<<<k.n_alpha, kBlock, 0, s>>>(k.alpha_leaders, k.partials, k.pstride);
         cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, 0);
+    }
     else
         ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, 0);
     SVX_HIP(hipGetLastError());

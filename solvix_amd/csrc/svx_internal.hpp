@@ -141,6 +141,10 @@ struct InstDesc {
     // reads the pending direction step that cg64_mf_step_kernel took once
     // for it, instead of every tile reducing the instance's partial sums
     int32_t step_kernel;
+    // set by the engine: the FP64 update of this (large) instance reads the
+    // alpha that cg64_alpha_kernel took once for it (the p.q reduction and
+    // the breakdown test), instead of every row tile reducing the partials
+    int32_t alpha_kernel;
     const int32_t* row_ptr;
     const int32_t* col;
     const double* val;
@@ -308,6 +312,10 @@ struct KrylovSet {
     // the matrix-free instances, then the large assembled ones
     const Tile* mf_leaders = nullptr;
     int32_t n_mf_inst = 0;
+    // the leaders of the instances whose alpha runs in cg64_alpha_kernel
+    // (more than four times the step-kernel threshold of p.q partials)
+    const Tile* alpha_leaders = nullptr;
+    int32_t n_alpha = 0;
     const Tile* mf3_tiles = nullptr;  // C3D8 matrix-free element tiles (row_begin / row_end: elements)
     int32_t n_mf3_tiles = 0;
     int32_t mf_cap_nodes[2] = {0, 0};  // CPS4, CPS3: largest patch of the set

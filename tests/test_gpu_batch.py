@@ -159,15 +159,18 @@ def test_many_tiles_equal_solo(precision, rows_per_tile, ctx):
 
 def test_step_kernel_changes_no_bits():
     """Instances with more row tiles than the threshold take each pending
-    direction step in the per-instance step kernel: the same reduction over
-    the same partial sums as every SpMV tile takes otherwise, so the bits and
-    iteration counts do not change (one-chunk tiles give the 265,923-DOF block
-    1,385 tiles and a 1000x80 plane cantilever 1,267)."""
+    direction step in the per-instance step kernel (and, past four times the
+    threshold, each alpha in the per-instance alpha kernel): the same
+    reductions over the same partial sums as every SpMV and update tile takes
+    otherwise, so the bits and iteration counts do not change (one-chunk
+    tiles give the 265,923-DOF block 1,385 tiles and a 1000x80 plane
+    cantilever 1,267; a threshold of 300 puts both past the alpha kernel's
+    1,200)."""
     from solvix_amd.core import Context
     models = [decks.cfg2(), decks.cfg1(nx=1000, ny=80)]
     runs = [(m, Precision.FP64) for m in models] + [(models[0], Precision.FP32_IR)]
     out = {}
-    for tiles in (0, -1):
+    for tiles in (0, 300, -1):
         ctx = Context(0)
         ctx.set_tuning(rows_per_tile=1)  # one ELL chunk per tile
         ctx.set_step_kernel_tiles(tiles)
@@ -182,10 +185,43 @@ def test_step_kernel_changes_no_bits():
             out[tiles] = res
         finally:
             ctx.close()
-    for (ua, ia, ra), (ub, ib, rb), (m, prec) in zip(out[0], out[-1], runs):
-        print(f"{m.name} [{prec}]: {ia} / {ib} iterations, residual {ra:.3e} / {rb:.3e}")
-        assert ia == ib and ra == rb, m.name
-        assert np.array_equal(ua.view(np.uint64), ub.view(np.uint64)), m.name
+    for tiles in (0, 300):
+        for (ua, ia, ra), (ub, ib, rb), (m, prec) in zip(out[tiles], out[-1], runs):
+            print(f"{m.name} [{prec}] threshold {tiles}: {ia} / {ib} iterations, residual {ra:.3e} / {rb:.3e}")
+            assert ia == ib and ra == rb, m.name
+            assert np.array_equal(ua.view(np.uint64), ub.view(np.uint64)), m.name
+
+
+def test_alpha_kernel_changes_no_bits_on_the_matrix_free_operator():
+    """Matrix-free instances with more patches than four times the threshold
+    (4 here) take alpha in the per-instance alpha kernel: the same bits and iteration
+    counts as every update tile reducing the patch partials itself, for a
+    plane cantilever and a jittered plane instance, in FP64 and in FP32-IR's
+    FP64 phase."""
+    from solvix_amd.core import Context
+    models = [decks.cfg1(nx=400, ny=40), decks.plane_instance(decks.SplitMix64(0x91), 2e4, 6e4)]
+    out = {}
+    for tiles in (4, -1):
+        ctx = Context(0)
+        ctx.set_operator("matrix_free")
+        ctx.set_step_kernel_tiles(tiles)
+        try:
+            res = []
+            for m in models:
+                for prec in (Precision.FP64, Precision.FP32_IR):
+                    mesh, s = _system(m, ctx)
+                    u, its, r = s.solve(1e-10, 0, prec)
+                    assert s.operator == "matrix_free"
+                    res.append((u, its, r))
+                    s.close()
+                    mesh.close()
+            out[tiles] = res
+        finally:
+            ctx.close()
+    for (ua, ia, ra), (ub, ib, rb) in zip(out[4], out[-1]):
+        print(f"{ia} / {ib} iterations, residual {ra:.3e} / {rb:.3e}")
+        assert ia == ib and ra == rb
+        assert np.array_equal(ua.view(np.uint64), ub.view(np.uint64))
 
 
 @pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
