@@ -108,6 +108,10 @@ enum { SVX_TILES_LINEAR = 0, SVX_TILES_XCD = 1 };
  * SVX_LOADS_AUTO:      streaming when the matrix values one launch reads in
  *                      its precision (all its instances of one node-block
  *                      size) exceed 256 MiB, else cached.
+ * The hint also covers the vectors of the PCG updates (x, r, z written; p,
+ * q, the diagonal, x, r read): STREAMING and CACHED set them alike, AUTO
+ * streams them once the FP64 update's vectors (64 bytes per row over the
+ * launch's instances of one node-block size) exceed 256 MiB.
  * Every hint gives the same results bit for bit. */
 enum { SVX_LOADS_AUTO = 0, SVX_LOADS_CACHED = 1, SVX_LOADS_STREAMING = 2 };
 

@@ -515,6 +515,7 @@ void system_engine_setup(svx_system* sys, double tol, int64_t max_iter, int prec
     shape.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
     shape.nt[bs] = streaming_loads({sys}, bs, tp, 8);
     shape.nt32[bs] = streaming_loads({sys}, bs, tp, 4);
+    shape.ntv[bs] = streaming_vectors({sys}, bs, tp);
     if (tp.step_tiles != 0) shape.step_tiles = tp.step_tiles;
     sys->engine->set_instances(descs, {1}, shape);
     sys->engine_precision = precision;

@@ -162,6 +162,11 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
 // that precision exceed kStreamBytes.
 constexpr int64_t kStreamBytes = int64_t(256) << 20;
 int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp, int value_bytes);
+// The vector hint (EngineShape::ntv) of the FP64 and FP32 updates over
+// `systems`' instances of node-block size bs: tp.load_hint, or for
+// SVX_LOADS_AUTO streaming once the FP64 update's vectors (five reads and
+// three writes of 8 bytes per row) exceed kStreamBytes.
+int streaming_vectors(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp);
 
 // A model validated and resolved on the host (lib.rs:593-632 up to the
 // element pass): node positions, per-element material constants, the load

@@ -57,6 +57,14 @@ int streaming_loads(const std::vector<svx_system*>& systems, int bs, const TuneP
     return bytes > kStreamBytes;
 }
 
+int streaming_vectors(const std::vector<svx_system*>& systems, int bs, const TuneParams& tp) {
+    if (tp.load_hint != SVX_LOADS_AUTO) return tp.load_hint == SVX_LOADS_STREAMING;
+    int64_t bytes = 0;
+    for (const svx_system* s : systems)
+        if (s->mesh->ndof == bs) bytes += s->mesh->n * int64_t(64);
+    return bytes > kStreamBytes;
+}
+
 // Launch shape for a set of systems: the per-block-size shape every solve
 // uses, so batched results equal solo ones bit for bit.
 EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParams& tp, int precision) {
@@ -68,6 +76,7 @@ EngineShape batch_shape(const std::vector<svx_system*>& systems, const TuneParam
         sh.xcd[bs] = tp.tile_order == SVX_TILES_XCD;
         sh.nt[bs] = streaming_loads(systems, bs, tp, 8);
         sh.nt32[bs] = streaming_loads(systems, bs, tp, 4);
+        sh.ntv[bs] = streaming_vectors(systems, bs, tp);
     }
     if (tp.step_tiles != 0) sh.step_tiles = tp.step_tiles;
     sh.matrix_free = tp.matrix_free;  // FP32-IR: the operator of its FP64 phase

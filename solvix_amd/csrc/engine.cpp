@@ -256,6 +256,7 @@ void Engine::build_tiles(const std::vector<char>& running, bool carry) {
         k.xcd = shape_.xcd[bs];
         k.nt = shape_.nt[bs];
         k.nt32 = shape_.nt32[bs];
+        k.ntv = shape_.ntv[bs];
         k.mf_tiles = bs == 2 ? mf_tiles_.p : nullptr;
         k.mf_leaders = bs == 2 ? mf_leaders_.p : bs == 3 ? mf_leaders_.p + n_lead2 : nullptr;
         k.n_mf_inst = bs == 2 ? n_lead2 : bs == 3 ? (int32_t)mf_leaders.size() - n_lead2 : 0;

@@ -33,6 +33,7 @@ struct EngineShape {
     int xcd[4] = {1, 1, 1, 1};  // gathering kernels: XCD-contiguous tile order (1) or tile = workgroup (0)
     int nt[4] = {0, 0, 0, 0};   // FP64 ELL SpMV: nontemporal matrix loads (1) or cached (0)
     int nt32[4] = {0, 0, 0, 0}; // FP32 ELL SpMV (FP32-IR): the same for the FP32 copy
+    int ntv[4] = {0, 0, 0, 0};  // FP64 and FP32 updates: nontemporal vector loads and stores (1) or cached (0)
     // Assembled instances with more row tiles than this take their FP64
     // pending direction step (r.r and r.z reduced over the instance's tiles,
     // convergence, budget, beta) in the step kernel, once per instance,

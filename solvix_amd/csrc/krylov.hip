@@ -105,6 +105,17 @@ __device__ __forceinline__ T load_hint(gptr<const T> p) {
     else
         return *p;
 }
+// The store counterpart. The update kernels stream their vectors with both
+// hints once a launch's vectors exceed the last-level cache: on 16 M rows of
+// five FP64 reads and three writes, 6.28 TB/s against 5.50 TB/s without
+// hints and 5.6 TB/s with either hint alone.
+template <bool NT, typename T>
+__device__ __forceinline__ void store_hint(gptr<T> p, T v) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
 
 struct Dev {
     int32_t n;
@@ -647,12 +658,12 @@ __device__ __forceinline__ double tile_spmv(const Tile& t, const Dev& D, gptr<co
 // longer than the default shapes) go through the row loops afterwards.
 constexpr int kPre = 3;
 
-template <typename T>
+template <typename T, bool NT = false>
 __device__ __forceinline__ void preload(gptr<const T> a, int32_t b, int32_t e, T (&v)[kPre]) {
 #pragma unroll
     for (int j = 0; j < kPre; ++j) {
         const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
-        v[j] = i < e ? a[i] : T(0);
+        v[j] = i < e ? load_hint<NT>(a + i) : T(0);
     }
 }
 
@@ -1075,18 +1086,20 @@ __global__ __launch_bounds__(kBlock) void cg64_mf3_node_kernel(const Tile* __res
 }
 
 // u += alpha p, r -= alpha q, z = D^-1 r over [b, e); this thread's r.r, r.z.
+template <bool NT>
 __device__ __forceinline__ void cg64_update_rows(int32_t b, int32_t e, double alpha, gptr<const double> __restrict__ p,
                                                  gptr<const double> __restrict__ q,
                                                  gptr<const double> __restrict__ dinv, gptr<double> __restrict__ u,
                                                  gptr<double> __restrict__ r, gptr<double> __restrict__ z,
                                                  double (&v)[2]) {
     for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
-        const double pi = p[i], qi = q[i], di = dinv[i], ui = u[i], r0 = r[i];
-        u[i] = ui + alpha * pi;
+        const double pi = load_hint<NT>(p + i), qi = load_hint<NT>(q + i), di = load_hint<NT>(dinv + i);
+        const double ui = load_hint<NT>(gptr<const double>(u + i)), r0 = load_hint<NT>(gptr<const double>(r + i));
+        store_hint<NT>(u + i, ui + alpha * pi);
         const double ri = r0 - alpha * qi;
-        r[i] = ri;
+        store_hint<NT>(r + i, ri);
         const double zi = ri * di;
-        z[i] = zi;
+        store_hint<NT>(z + i, zi);
         v[0] += ri * ri;
         v[1] += ri * zi;
     }
@@ -1124,18 +1137,19 @@ __global__ __launch_bounds__(kBlock) void cg64_alpha_kernel(const Tile* __restri
 // One row tile of the FP64 update; TAKEN: cg64_alpha_kernel took alpha (and
 // the breakdown test) for the instance, otherwise the tile reduces the p.q
 // partials itself. Two instantiations, so each path keeps its own schedule
-// (every preload ahead of the reduction).
-template <bool TAKEN>
+// (every preload ahead of the reduction). NT: the vectors are loaded and
+// stored with the nontemporal hint (KrylovSet::ntv).
+template <bool TAKEN, bool NT>
 __device__ __forceinline__ void update_tile(const Tile& t, gptr<InstScal> S, const Dev& D,
                                             double* __restrict__ partials, int pstride, int par) {
     const gptr<double> pd = par ? D.p : D.p_alt;
     const int32_t b = t.row_begin, e = t.row_end;
     double p0[kPre], q0[kPre], d0[kPre], u0[kPre], r0[kPre];
-    preload<double>(pd, b, e, p0);
-    preload<double>(D.q, b, e, q0);
-    preload<double>(D.dinv, b, e, d0);
-    preload<double>(D.u, b, e, u0);
-    preload<double>(D.r, b, e, r0);
+    preload<double, NT>(pd, b, e, p0);
+    preload<double, NT>(D.q, b, e, q0);
+    preload<double, NT>(D.dinv, b, e, d0);
+    preload<double, NT>(D.u, b, e, u0);
+    preload<double, NT>(D.r, b, e, r0);
     double alpha;
     if constexpr (TAKEN) {
         alpha = S->alpha;
@@ -1167,20 +1181,21 @@ __device__ __forceinline__ void update_tile(const Tile& t, gptr<InstScal> S, con
     for (int j = 0; j < kPre; ++j) {
         const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
         if (i < e) {
-            D.u[i] = u0[j] + alpha * p0[j];
+            store_hint<NT>(D.u + i, u0[j] + alpha * p0[j]);
             const double ri = r0[j] - alpha * q0[j];
-            D.r[i] = ri;
+            store_hint<NT>(D.r + i, ri);
             const double zi = ri * d0[j];
-            D.z[i] = zi;
+            store_hint<NT>(D.z + i, zi);
             v[0] += ri * ri;
             v[1] += ri * zi;
         }
     }
-    cg64_update_rows(b + kPre * kBlock, e, alpha, pd, D.q, D.dinv, D.u, D.r, D.z, v);
+    cg64_update_rows<NT>(b + kPre * kBlock, e, alpha, pd, D.q, D.dinv, D.u, D.r, D.z, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
 
+template <bool NT>
 __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
                                                              double* __restrict__ partials, int pstride, int par) {
@@ -1191,9 +1206,9 @@ __global__ __launch_bounds__(kBlock) void cg64_update_kernel(const Tile* __restr
     issue_before_test(D.p, D.p_alt, D.q, D.dinv, D.u, D.r, D.z, par, D.alphak);
     if (status != ST_RUNNING || (flags & FL_FRESH)) return;
     if (D.alphak)
-        update_tile<true>(t, S, D, partials, pstride, par);
+        update_tile<true, NT>(t, S, D, partials, pstride, par);
     else
-        update_tile<false>(t, S, D, partials, pstride, par);
+        update_tile<false, NT>(t, S, D, partials, pstride, par);
 }
 
 // End of a chunk (an even number of iterations, so the current direction is
@@ -1305,18 +1320,20 @@ __global__ __launch_bounds__(kBlock) void ir32_spmv_kernel(const Tile* __restric
     write_partials<1>(v, slot, partials, pstride, xcd_tile(xcd));
 }
 
+template <bool NT>
 __device__ __forceinline__ void ir32_update_rows(int32_t b, int32_t e, float alpha, gptr<const float> __restrict__ p,
                                                  gptr<const float> __restrict__ q,
                                                  gptr<const float> __restrict__ dinv, gptr<float> __restrict__ x,
                                                  gptr<float> __restrict__ r, gptr<float> __restrict__ z,
                                                  double (&v)[2]) {
     for (int32_t i = b + threadIdx.x; i < e; i += kBlock) {
-        const float pi = p[i], qi = q[i], di = dinv[i], xi = x[i], r0 = r[i];
-        x[i] = xi + alpha * pi;
+        const float pi = load_hint<NT>(p + i), qi = load_hint<NT>(q + i), di = load_hint<NT>(dinv + i);
+        const float xi = load_hint<NT>(gptr<const float>(x + i)), r0 = load_hint<NT>(gptr<const float>(r + i));
+        store_hint<NT>(x + i, xi + alpha * pi);
         const float ri = r0 - alpha * qi;
-        r[i] = ri;
+        store_hint<NT>(r + i, ri);
         const float zi = ri * di;
-        z[i] = zi;
+        store_hint<NT>(z + i, zi);
         v[0] += (double)ri * (double)ri;
         v[1] += (double)ri * (double)zi;
     }
@@ -1341,7 +1358,9 @@ __device__ __forceinline__ void fold_rows(int32_t b, int32_t e, gptr<float> __re
 }
 
 // x32 += alpha p32, r32 -= alpha q32, z32 = D^-1 r32 and the partials, with
-// p32 the direction the SpMV of this iteration stored (P[1 - par]).
+// p32 the direction the SpMV of this iteration stored (P[1 - par]). NT: the
+// vectors are loaded and stored with the nontemporal hint (KrylovSet::ntv).
+template <bool NT>
 __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restrict__ tiles,
                                                              const InstDesc* __restrict__ descs,
                                                              double* __restrict__ partials, int pstride, int par) {
@@ -1353,11 +1372,11 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
     const gptr<float> pd = par ? D.p32 : D.p32_alt;
     const int32_t b = t.row_begin, e = t.row_end;
     float p0[kPre], q0[kPre], d0[kPre], x0[kPre], r0[kPre];
-    preload<float>(pd, b, e, p0);
-    preload<float>(D.q32, b, e, q0);
-    preload<float>(D.dinv32, b, e, d0);
-    preload<float>(D.x32, b, e, x0);
-    preload<float>(D.r32, b, e, r0);
+    preload<float, NT>(pd, b, e, p0);
+    preload<float, NT>(D.q32, b, e, q0);
+    preload<float, NT>(D.dinv32, b, e, d0);
+    preload<float, NT>(D.x32, b, e, x0);
+    preload<float, NT>(D.r32, b, e, r0);
     const double rz = S->rz;  // written by the previous phases' leaders only
     const int sp[1] = {P_PQ};
     double pq[1];
@@ -1387,16 +1406,16 @@ __global__ __launch_bounds__(kBlock) void ir32_update_kernel(const Tile* __restr
     for (int j = 0; j < kPre; ++j) {
         const int32_t i = b + (int32_t)threadIdx.x + j * kBlock;
         if (i < e) {
-            D.x32[i] = x0[j] + alpha * p0[j];
+            store_hint<NT>(D.x32 + i, x0[j] + alpha * p0[j]);
             const float ri = r0[j] - alpha * q0[j];
-            D.r32[i] = ri;
+            store_hint<NT>(D.r32 + i, ri);
             const float zi = ri * d0[j];
-            D.z32[i] = zi;
+            store_hint<NT>(D.z32 + i, zi);
             v[0] += (double)ri * (double)ri;
             v[1] += (double)ri * (double)zi;
         }
     }
-    ir32_update_rows(b + kPre * kBlock, e, alpha, pd, D.q32, D.dinv32, D.x32, D.r32, D.z32, v);
+    ir32_update_rows<NT>(b + kPre * kBlock, e, alpha, pd, D.q32, D.dinv32, D.x32, D.r32, D.z32, v);
     const int slot[2] = {P_RR, P_RZ};
     write_partials<2>(v, slot, partials, pstride, blockIdx.x);
 }
@@ -1669,6 +1688,20 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
         default: MACRO(4); break;          \
     }
 
+static void launch_cg64_update(const KrylovSet& k, hipStream_t s, int par) {
+    if (k.ntv)
+        cg64_update_kernel<true><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
+    else
+        cg64_update_kernel<false><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
+}
+
+static void launch_ir32_update(const KrylovSet& k, hipStream_t s, int par) {
+    if (k.ntv)
+        ir32_update_kernel<true><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
+    else
+        ir32_update_kernel<false><<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
+}
+
 void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {
     const int g = k.n_ell_tiles;
     // the pending step of the matrix-free and the large assembled instances,
@@ -1870,7 +1903,7 @@ void cg64_iteration(const KrylovSet& k, hipStream_t s, int par) {
     if (k.n_tiles <= 0) return;
     launch_cg64_spmv(k, s, par, 0);
     if (k.n_alpha > 0) cg64_alpha_kernel<<<k.n_alpha, kBlock, 0, s>>>(k.alpha_leaders, k.partials, k.pstride);
-    cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
+    launch_cg64_update(k, s, par);
     SVX_HIP(hipGetLastError());
 }
 
@@ -1890,7 +1923,7 @@ void ir32_init(const KrylovSet& k, hipStream_t s) {
 void ir32_iteration(const KrylovSet& k, hipStream_t s, int par) {
     if (k.n_tiles <= 0) return;
     launch_ir32_spmv(k, s, par, 0);
-    ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, par);
+    launch_ir32_update(k, s, par);
     SVX_HIP(hipGetLastError());
 }
 
@@ -1940,10 +1973,10 @@ void update_phase(const KrylovSet& k, int precision, hipStream_t s) {
     if (k.n_tiles <= 0) return;
     if (precision == SVX_FP64) {
         if (k.n_alpha > 0) cg64_alpha_kernel<<<k.n_alpha, kBlock, 0, s>>>(k.alpha_leaders, k.partials, k.pstride);
-        cg64_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, 0);
+        launch_cg64_update(k, s, 0);
     }
     else
-        ir32_update_kernel<<<k.n_tiles, kBlock, 0, s>>>(k.tiles, k.descs, k.partials, k.pstride, 0);
+        launch_ir32_update(k, s, 0);
     SVX_HIP(hipGetLastError());
 }
 

@@ -302,6 +302,7 @@ struct KrylovSet {
     int xcd = 1;                 // gathering kernels: 1 = XCD-contiguous tile order, 0 = tile = workgroup
     int nt = 0;                  // FP64 ELL SpMV: matrix values loaded nontemporal (1) or cached (0)
     int nt32 = 0;                // FP32 ELL SpMV: the same
+    int ntv = 0;                 // FP64 and FP32 updates: vectors loaded and stored nontemporal (1) or cached (0)
     // FP64 SpMV: row tiles [0, n_ell_tiles) multiply the assembled matrix;
     // instances with matrix-free tiles (mf_tiles, CPS4 first, then CPS3) own
     // the row tiles after them. FP32 SpMV launches cover every row tile.
