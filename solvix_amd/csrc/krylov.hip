@@ -778,8 +778,10 @@ inline size_t mf_lds_bytes(int nen, int cap_nodes, int cap_elem) {
 }
 
 // One patch of q = K p' (the pending direction step evaluated by x); stores
-// p' and q of the owned nodes; returns this thread's p'.q.
-template <int NEN, typename X>
+// p' and q of the owned nodes; returns this thread's p'.q. NT: the patch's
+// own lists (incidence codes, element connectivity, node list), read once
+// per product, are loaded with the nontemporal hint.
+template <int NEN, bool NT, typename X>
 __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P, const X& x, gptr<double> nxt,
                                                 gptr<double> q, int cap_nodes, int cap_elem) {
     constexpr int MAXA = NEN == 4 ? 4 : 8;
@@ -797,16 +799,16 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P
     uint32_t code2[MAXA / 2];
 #pragma unroll
     for (int k = 0; k < MAXA; k += 2) {
-        const uint32_t lo = (k < P.n_adj && owner) ? (uint32_t)M.adj[P.adj_off + k * kBlock + tid] : (uint32_t)(P.n_elem * 4);
+        const uint32_t lo = (k < P.n_adj && owner) ? (uint32_t)load_hint<NT>(M.adj + P.adj_off + k * kBlock + tid) : (uint32_t)(P.n_elem * 4);
         const uint32_t hi =
-            (k + 1 < P.n_adj && owner) ? (uint32_t)M.adj[P.adj_off + (k + 1) * kBlock + tid] : (uint32_t)(P.n_elem * 4);
+            (k + 1 < P.n_adj && owner) ? (uint32_t)load_hint<NT>(M.adj + P.adj_off + (k + 1) * kBlock + tid) : (uint32_t)(P.n_elem * 4);
         code2[k / 2] = lo | (hi << 16);
     }
     uint64_t ec[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const int sl = tid + r * kBlock;
-        ec[r] = sl < P.n_elem ? M.econn[P.elem_off + sl] : 0ull;
+        ec[r] = sl < P.n_elem ? load_hint<NT>(M.econn + P.elem_off + sl) : 0ull;
     }
     // every global load of the staging phase is issued before its LDS
     // stores: the node list (clamped to the patch, so no branch), then the
@@ -814,7 +816,7 @@ __device__ __forceinline__ double mf_patch_spmv(const MfDev& M, const MfPatch& P
     constexpr int NR = kMfMaxNodes / kBlock;
     int32_t g[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) g[r] = M.nodes[P.node_off + min(tid + r * kBlock, P.n_nodes - 1)];
+    for (int r = 0; r < NR; ++r) g[r] = load_hint<NT>(M.nodes + P.node_off + min(tid + r * kBlock, P.n_nodes - 1));
     double cx[NR], cy[NR], va[NR], vb[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -918,8 +920,9 @@ __global__ __launch_bounds__(kBlock) void cg64_mf_step_kernel(const Tile* __rest
 // operator (cg64_spmv_kernel with node patches for row tiles), after
 // cg64_mf_step_kernel has tested convergence and set beta. Tile t is
 // mf_tiles[base + xcd_tile(xcd)]; its p.q partial goes to slot P_MPQ, column
-// base + xcd_tile(xcd).
-template <int NEN>
+// base + xcd_tile(xcd). NT: the patch lists loaded with the nontemporal
+// hint (KrylovSet::ntv).
+template <int NEN, bool NT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void cg64_mf_spmv_kernel(const Tile* __restrict__ mf_tiles, int base,
                                                               const InstDesc* __restrict__ descs,
                                                               double* __restrict__ partials, int pstride, int par,
@@ -947,10 +950,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     const gptr<double> nxt = par ? D.p : D.p_alt;
     double v[1];
     if (!force_dir && dir) {
-        v[0] = mf_patch_spmv<NEN>(M, P, Plain<double>{cur}, nxt, D.q, cap_nodes, cap_elem);
+        v[0] = mf_patch_spmv<NEN, NT>(M, P, Plain<double>{cur}, nxt, D.q, cap_nodes, cap_elem);
     } else {
         const double beta = force_dir ? 1.0 : sbeta;
-        v[0] = mf_patch_spmv<NEN>(M, P, Step<double>{D.z, cur, beta}, nxt, D.q, cap_nodes, cap_elem);
+        v[0] = mf_patch_spmv<NEN, NT>(M, P, Step<double>{D.z, cur, beta}, nxt, D.q, cap_nodes, cap_elem);
     }
     const int slot[1] = {P_MPQ};
     write_partials<1>(v, slot, partials, pstride, base + xcd_tile(xcd));
@@ -1740,12 +1743,22 @@ void launch_cg64_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir)
         cg64_mf3_node_kernel<<<k.n_tiles - k.n_ell_tiles, kBlock, 0, s>>>(k.tiles, k.n_ell_tiles, k.descs,
                                                                             k.partials, k.pstride, par, force_dir);
     }
-    if (k.n_mf4 > 0)
-        cg64_mf_spmv_kernel<4><<<k.n_mf4, kBlock, mf_lds_bytes(4, k.mf_cap_nodes[0], k.mf_cap_elem[0]), s>>>(
-            k.mf_tiles, 0, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[0], k.mf_cap_elem[0], k.xcd);
-    if (k.n_mf3 > 0)
-        cg64_mf_spmv_kernel<3><<<k.n_mf3, kBlock, mf_lds_bytes(3, k.mf_cap_nodes[1], k.mf_cap_elem[1]), s>>>(
-            k.mf_tiles, k.n_mf4, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[1], k.mf_cap_elem[1], k.xcd);
+#define L(NEN, NT, CNT, BASE, I)                                                                                    \
+    cg64_mf_spmv_kernel<NEN, NT><<<CNT, kBlock, mf_lds_bytes(NEN, k.mf_cap_nodes[I], k.mf_cap_elem[I]), s>>>(        \
+        k.mf_tiles, BASE, k.descs, k.partials, k.pstride, par, force_dir, k.mf_cap_nodes[I], k.mf_cap_elem[I], k.xcd)
+    if (k.n_mf4 > 0) {
+        if (k.ntv)
+            L(4, true, k.n_mf4, 0, 0);
+        else
+            L(4, false, k.n_mf4, 0, 0);
+    }
+    if (k.n_mf3 > 0) {
+        if (k.ntv)
+            L(3, true, k.n_mf3, k.n_mf4, 1);
+        else
+            L(3, false, k.n_mf3, k.n_mf4, 1);
+    }
+#undef L
 }
 
 void launch_ir32_spmv(const KrylovSet& k, hipStream_t s, int par, int force_dir) {

@@ -224,12 +224,14 @@ def test_alpha_kernel_changes_no_bits_on_the_matrix_free_operator():
         assert np.array_equal(ua.view(np.uint64), ub.view(np.uint64))
 
 
+@pytest.mark.parametrize("operator", ["assembled", "matrix_free"])
 @pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32_IR], ids=["fp64", "fp32ir"])
-def test_load_hint_changes_no_bits(precision):
-    """Nontemporal and ordinary matrix loads, and the update kernels' vector
-    loads and stores, move the same values in the same order: single solves
-    and a batch of 2D and 3D systems on the assembled operator give identical
-    bits and iteration counts under either hint."""
+def test_load_hint_changes_no_bits(precision, operator):
+    """Nontemporal and ordinary matrix loads, the update kernels' vector loads
+    and stores, and the matrix-free products' patch-list loads move the same
+    values in the same order: single solves and a batch of 2D and 3D systems
+    on either operator give identical bits and iteration counts under either
+    hint."""
     from solvix_amd.core import Context
     models = small_models() + [decks.plane_instance(decks.SplitMix64(0x78 ^ i), 2e4, 6e4) for i in range(2)]
     models.append(decks.cfg2(nx=40, ny=6, nz=6))
@@ -237,6 +239,7 @@ def test_load_hint_changes_no_bits(precision):
     for hint in ("cached", "streaming"):
         ctx = Context(0)
         ctx.set_load_hint(hint)
+        ctx.set_operator(operator)
         try:
             solo = []
             for m in models:
